@@ -1,0 +1,319 @@
+/*
+ * sicp.h — C ABI of the MI355X-native dense SLAM hot path.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers, sizes and POD
+ * descriptors, runs on the caller's HIP stream (`stream`, may be NULL for the
+ * null stream) and returns an int status (SICP_OK = 0) instead of exiting the
+ * process the way the reference's `SafeCall` does (Utility/SafeCall.h:13-21).
+ *
+ * Image descriptors (`sicp_image`) describe pitched device buffers the same way
+ * the reference's `DeviceArray2D<T>` / `PtrStepSz<T>` do
+ * (Utility/DeviceArray.h:31-41,153-162): row y starts at
+ * `(char*)ptr + y * pitch`.
+ *
+ * Poses follow the reference convention (Core/Frame.cc:626-645): a camera to
+ * world transform with row-major rotation R, its transpose Rinv and
+ * translation t, all float. Entry points that fuse, raycast or track also
+ * accept the pose through a device pointer (`sicp_pose_f`), so a whole frame
+ * can run with no host round trip.
+ */
+#ifndef SICP_H_
+#define SICP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    SICP_OK = 0,
+    SICP_ERR_INVALID_ARGUMENT = 1, /* shape / pitch / config mismatch        */
+    SICP_ERR_HIP = 2,              /* a HIP runtime call failed               */
+    SICP_ERR_CAPACITY = 3,         /* heap, request set or buffer exhausted   */
+    SICP_ERR_NO_VISIBLE = 4,       /* no visible block (FuseMap.cu:1117-1120)  */
+    SICP_ERR_NOTHING_TO_RENDER = 5,/* CreateRenderingBlocks returned false    */
+    SICP_ERR_NO_DEVICE = 6         /* no HIP device present                   */
+};
+
+/* Last HIP / argument error message of the calling thread (never NULL). */
+const char* sicp_last_error(void);
+
+/* ---- POD descriptors ---------------------------------------------------- */
+typedef struct sicp_image {
+    void* ptr;      /* device pointer to row 0                           */
+    size_t pitch;   /* bytes between rows                                */
+    int cols;       /* width in elements                                 */
+    int rows;       /* height in rows                                    */
+} sicp_image;
+
+typedef struct sicp_intrinsics {
+    float fx, fy, cx, cy; /* Utility/Intrinsics.h:3-18 */
+} sicp_intrinsics;
+
+/* Float view of one pose: R row-major, Rinv = R^T, t. */
+typedef struct sicp_pose_f {
+    float R[9];
+    float Rinv[9];
+    float t[3];
+} sicp_pose_f;
+
+/* ---- map layout (public so callers can inspect a downloaded map) -------- */
+
+/* Geometry plane, one record per voxel, 8 B: what the raycast reads.
+ * Fields of the reference's `Voxel` sdf / weight / color
+ * (Mapping/DeviceMap.h:109-111). */
+typedef struct sicp_voxel_geom {
+    float sdf;        /* normalised by the truncation, NaN when empty      */
+    uint8_t weight;   /* 0 = empty                                          */
+    uint8_t r, g, b;  /* running-mean colour                                */
+} sicp_voxel_geom;
+
+/* Semantic plane, one record per voxel, 12 B: the reference's label fields
+ * (Mapping/DeviceMap.h:113-118) with the counts widened to uint16. */
+typedef struct sicp_voxel_sem {
+    uint16_t count, count2;
+    uint8_t label, label2;
+    uint8_t r2, g2, b2;
+    uint8_t pad[3];
+} sicp_voxel_sem;
+
+/* One hash-table entry, 16 B. `key` packs the block coordinate
+ * (21 bits per axis, two's complement); `slot` is the heap block (-1 =
+ * empty); `next` links the bucket's overflow chain (-1 = end). The entry
+ * replaces the reference's `HashEntry {int3 pos; int ptr; int offset}`
+ * (Mapping/DeviceMap.h:20-60). */
+typedef struct sicp_hash_entry {
+    uint64_t key;
+    int32_t slot;
+    int32_t next;
+} sicp_hash_entry;
+
+/* One allocated / visible block: block coordinate and heap slot. Plays the
+ * role of the reference's `visibleEntries` records (DeviceMap.h:198). */
+typedef struct sicp_block {
+    int32_t x, y, z, slot;
+} sicp_block;
+
+/* Runtime map configuration. The reference hard-codes these as
+ * `DeviceMap` constants (Mapping/DeviceMap.h:166-189). */
+typedef struct sicp_map_config {
+    float voxel_size;       /* metres (VoxelSize, DeviceMap.h:183)          */
+    float truncation;       /* metres (TruncateDist, DeviceMap.h:184)       */
+    float depth_min;        /* metres (DepthMin, DeviceMap.h:168)           */
+    float depth_max;        /* metres (DepthMax, DeviceMap.h:169)           */
+    int32_t num_buckets;    /* buckets of 2 entries each                    */
+    int32_t num_blocks;     /* heap capacity in 8^3 blocks                  */
+    int32_t max_weight;     /* TSDF weight cap, 255 in the reference        */
+    int32_t max_label_count;/* label vote cap: 255 reference, 65535 widened */
+    int32_t request_capacity; /* per-frame new-block request set, pow2     */
+    int32_t max_cols;       /* largest image width the map is used with     */
+    int32_t max_rows;       /* largest image height                         */
+} sicp_map_config;
+
+/* Fills `cfg` with the reference's constants scaled to the given voxel size
+ * and truncation; buckets/blocks as given. */
+void sicp_map_config_default(sicp_map_config* cfg, float voxel_size, float truncation,
+                             int32_t num_buckets, int32_t num_blocks);
+
+typedef struct sicp_map_stats {
+    uint32_t allocated_blocks;  /* heap slots handed out so far              */
+    uint32_t visible_blocks;    /* last visibility pass                      */
+    uint32_t new_blocks;        /* blocks created by the last fusion         */
+    uint32_t dropped_heap;      /* requests lost to heap exhaustion (total)  */
+    uint32_t dropped_requests;  /* requests lost to request-set overflow     */
+    uint32_t dropped_range;     /* block coordinates outside 21-bit range    */
+    uint32_t updated_voxels;    /* voxels written by the last fusion         */
+    uint32_t render_tiles;      /* blocks that produced tiles, last prepass  */
+    uint32_t pad;
+    uint64_t updated_total;     /* voxels written, summed since reset        */
+    uint64_t visible_total;     /* visible blocks fused, summed since reset  */
+    uint64_t fusions;           /* fusion passes since reset                 */
+} sicp_map_stats;
+
+typedef struct sicp_map sicp_map; /* opaque; owns every map buffer (Mapping.cc:11-51) */
+
+/* ---- device / library --------------------------------------------------- */
+int sicp_device_count(int* count);
+const char* sicp_version(void);
+
+/* ---- depth / intensity pyramid (Tracking/Reduction.h:18-42) -------------- */
+
+/* FilterDepth (Tracking/Pyrdown.cu:72-121): uint16 depth -> metres in `raw`
+ * (NaN at or beyond the cutoff) and the 5x5 bilateral `filtered`. */
+int sicp_filter_depth(sicp_image depth_u16, sicp_image raw_f32, sicp_image filtered_f32,
+                      float depth_scale, float depth_cutoff, void* stream);
+
+/* ComputeVMap (Pyrdown.cu:123-158): depth -> camera-frame float4 vertices. */
+int sicp_compute_vmap(sicp_image depth_f32, sicp_image vmap_f4, float fx, float fy,
+                      float cx, float cy, float depth_cutoff, void* stream);
+
+/* ComputeNMap (Pyrdown.cu:160-192): forward-difference normals. */
+int sicp_compute_nmap(sicp_image vmap_f4, sicp_image nmap_f4, void* stream);
+
+/* PyrDownGauss (Pyrdown.cu:194-248), float and uint8 variants. */
+int sicp_pyrdown_gauss_f32(sicp_image src, sicp_image dst, void* stream);
+int sicp_pyrdown_gauss_u8(sicp_image src, sicp_image dst, void* stream);
+
+/* ImageToIntensity (Pyrdown.cu:250-272): uchar3 RGB -> uint8. */
+int sicp_image_to_intensity(sicp_image rgb_u8x3, sicp_image gray_u8, void* stream);
+
+/* ComputeDerivativeImage (Pyrdown.cu:274-316): flipped Sobel -> int16 dx, dy. */
+int sicp_compute_derivative_image(sicp_image gray_u8, sicp_image dx_i16, sicp_image dy_i16,
+                                  void* stream);
+
+/* ResizeMap (Pyrdown.cu:318-358): 2x2 mean of vertex / normal maps. */
+int sicp_resize_map(sicp_image vsrc, sicp_image nsrc, sicp_image vdst, sicp_image ndst,
+                    void* stream);
+
+/* ---- ICP reduction (Tracking/Reduction.h:74-79) -------------------------- */
+
+/* ICPStep (Tracking/Reduction.cu:328-385). Computes the 29 sums (27 upper
+ * A|b entries, sum r^2, count) of the point-to-plane Gauss-Newton system into
+ * `sums_dev` (29 floats, device). `partials_dev` is zero-initialised device
+ * scratch of `sicp_icp_partials_size()` bytes (it must stay zeroed between
+ * calls, which the call itself guarantees). When `host_sums` is non-NULL the call
+ * synchronises `stream` and copies the 29 sums to the host; when `A_host`,
+ * `b_host`, `residual_host` are non-NULL it also expands them exactly like
+ * the reference's CreateMatrix<6,7> (Reduction.cu:6-16): A row-major 6x6
+ * (mirrored), b[6], residual = {sum r^2, count}. */
+int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nlast,
+                  const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
+                  void* partials_dev, float* sums_dev, float* host_sums,
+                  double* A_host, double* b_host, float* residual_host, void* stream);
+size_t sicp_icp_partials_size(void);
+
+/* FP64 6x6 solve + SE(3) update of one Gauss-Newton iteration, on the host
+ * arrays of Tracking/Tracking.cc:493-498: x = A^-1 b (Cholesky),
+ * delta = exp(x) * delta, next = last * delta^-1. Poses are 4x4 row-major. */
+int sicp_se3_update_host(const double* A, const double* b, double* delta,
+                         const double* last_pose, double* next_pose, double* x_out);
+
+/* ---- map (Mapping/RenderScene.h:3-55) ------------------------------------ */
+int sicp_map_create(const sicp_map_config* cfg, sicp_map** out);
+int sicp_map_destroy(sicp_map* map);
+int sicp_map_get_config(const sicp_map* map, sicp_map_config* cfg);
+
+/* ResetMap (FuseMap.cu:1291-1304). */
+int sicp_map_reset(sicp_map* map, void* stream);
+
+/* FuseMapColor (FuseMap.cu:1039-1162): allocate the blocks of the +-T/2 band
+ * around every valid raw-depth pixel, compact the visible blocks, then fuse
+ * TSDF + colour and, when num_detection > 0, the label votes from `mask`.
+ * Pose through `pose_host` (by value) or `pose_dev` (device pointer, used when
+ * non-NULL). `visible_host`, when non-NULL, receives the visible count (the
+ * call then synchronises `stream`). Returns SICP_ERR_NO_VISIBLE when that
+ * count is zero and `visible_host` was given. */
+int sicp_fuse_map_color(sicp_map* map, sicp_image depth_f32, sicp_image rgb_u8x3,
+                        sicp_image nmap_f4, sicp_image mask_u8, int num_detection,
+                        const sicp_pose_f* pose_host, const sicp_pose_f* pose_dev,
+                        sicp_intrinsics K, uint32_t* visible_host, void* stream);
+
+/* CheckBlockVisibility (FuseMap.cu:991-1037): visibility compaction only. */
+int sicp_check_block_visibility(sicp_map* map, const sicp_pose_f* pose_host,
+                                const sicp_pose_f* pose_dev, int cols, int rows,
+                                sicp_intrinsics K, float depth_max, float depth_min,
+                                uint32_t* visible_host, void* stream);
+
+/* CreateRenderingBlocks (RenderScene.cu:210-289): per-8x8-tile depth range of
+ * the visible blocks into zmin/zmax (ceil(W/8) x ceil(H/8) float images).
+ * `nonempty_host`, when non-NULL, receives 1 if any tile was touched (the
+ * call then synchronises). */
+int sicp_create_rendering_blocks(sicp_map* map, sicp_image zmin, sicp_image zmax,
+                                 float depth_max, float depth_min,
+                                 const sicp_pose_f* pose_host, const sicp_pose_f* pose_dev,
+                                 sicp_intrinsics K, int* nonempty_host, void* stream);
+
+/* Raycast (RenderScene.cu:495-537): march every pixel through the TSDF and
+ * write camera-frame vertices / normals (NaN where no surface). Takes fx, fy
+ * like its siblings; the reference's invfx = 1/fx is derived inside. */
+int sicp_raycast(sicp_map* map, sicp_image vmap_f4, sicp_image nmap_f4, sicp_image zmin,
+                 sicp_image zmax, const sicp_pose_f* pose_host, const sicp_pose_f* pose_dev,
+                 sicp_intrinsics K, void* stream);
+
+int sicp_map_get_stats(sicp_map* map, sicp_map_stats* out, void* stream);
+
+/* Map inspection (synchronous device->host copies). Buffers sized by the
+ * config: table = 2*num_buckets entries, excess / blocks = num_blocks,
+ * voxels = num_blocks*512. Any pointer may be NULL. */
+int sicp_map_download(sicp_map* map, sicp_hash_entry* table, sicp_hash_entry* excess,
+                      sicp_block* blocks, sicp_voxel_geom* geom, sicp_voxel_sem* sem,
+                      uint32_t n_blocks_limit);
+/* Visible list of the last visibility pass (up to `cap` records). */
+int sicp_map_download_visible(sicp_map* map, sicp_block* visible, uint32_t cap,
+                              uint32_t* count);
+
+/* ---- per-frame system (Core/System.cc:131-172) --------------------------- */
+typedef struct sicp_system_config {
+    int32_t cols, rows;
+    sicp_intrinsics K;       /* level-0 intrinsics (SysDesc, MainTum.cc:40-51) */
+    float depth_scale;       /* raw units per metre (Frame::mDepthScale)       */
+    float depth_cutoff;      /* metres (Frame::mDepthCutoff)                   */
+    int32_t icp_iters[3];    /* ITERATIONS_SE3 = {10, 5, 3} (Tracking.h:110)   */
+    int32_t icp_min_count[3];/* MIN_ICP_COUNT = {2000,1000,100} (Tracking.h:112) */
+    float icp_thresh;        /* THRESH_ICP_SE3 = 1e-4 (Tracking.h:117)         */
+    sicp_map_config map;
+} sicp_system_config;
+
+typedef struct sicp_system sicp_system;
+
+typedef struct sicp_frame_result {
+    int32_t frame_index;
+    int32_t tracked;        /* 1 tracked (or first frame), 0 lost              */
+    int32_t icp_count;      /* valid correspondences of the last iteration     */
+    int32_t iterations;     /* Gauss-Newton iterations actually applied        */
+    float icp_error;        /* sqrt(sum r^2) / count                           */
+    uint32_t visible_blocks;
+    uint32_t allocated_blocks;
+    double pose[16];        /* camera -> world, row-major                      */
+} sicp_frame_result;
+
+void sicp_system_config_default(sicp_system_config* cfg, int cols, int rows, float fx,
+                                float fy, float cx, float cy, float voxel_size,
+                                float truncation, int32_t num_buckets, int32_t num_blocks);
+int sicp_system_create(const sicp_system_config* cfg, sicp_system** out);
+int sicp_system_destroy(sicp_system* sys);
+/* GrabImage: track the frame against the model, fuse it, raycast the model.
+ * Inputs are device buffers (uint16 depth, uchar3 RGB, uint8 labels or NULL).
+ * Asynchronous on `stream`; results through sicp_system_last_result. */
+int sicp_system_grab_image(sicp_system* sys, sicp_image depth_u16, sicp_image rgb_u8x3,
+                           sicp_image labels_u8, int num_detection, void* stream);
+int sicp_system_last_result(sicp_system* sys, sicp_frame_result* out, void* stream);
+sicp_map* sicp_system_map(sicp_system* sys);
+/* Stage timing with HIP events on the frame's stream (no counterpart in the
+ * reference, whose only timing is std::clock around GrabImage,
+ * Core/System.cc:142-146). Stages: 0 pyramid, 1 track, 2 allocate,
+ * 3 visibility, 4 fuse, 5 render prepass, 6 raycast. read_timing synchronises
+ * `stream`, returns per-stage summed milliseconds and instance counts of the
+ * frames grabbed since set_timing(1), and clears them. */
+int sicp_system_set_timing(sicp_system* sys, int enable);
+int sicp_system_read_timing(sicp_system* sys, double* total_ms, int* counts, int n_stages,
+                            void* stream);
+/* Model / frame buffers of the system, for inspection: which = 0 vmap, 1 nmap
+ * of the last frame at pyramid `level`; 2 raw depth; 3 filtered depth L0. */
+int sicp_system_frame_image(sicp_system* sys, int which, int level, sicp_image* out);
+
+/* ---- seeded synthetic room (test and benchmark input) -------------------- */
+typedef struct sicp_synth_config {
+    int32_t cols, rows;
+    float fx, fy, cx, cy;
+    uint64_t seed;
+    float depth_units_per_m; /* 1000: 1 mm per unit */
+    float dropout;           /* fraction of pixels set to 0 */
+    int32_t noise;           /* 0 = noiseless depth */
+} sicp_synth_config;
+typedef struct sicp_synth sicp_synth;
+int sicp_synth_create(const sicp_synth_config* cfg, sicp_synth** out);
+int sicp_synth_destroy(sicp_synth* s);
+/* Camera->world pose of frame i (row-major 4x4 double). */
+int sicp_synth_pose(const sicp_synth* s, int frame, double pose[16]);
+/* Renders frame i into host buffers (depth rows*cols, rgb rows*cols*3, labels). */
+int sicp_synth_render(const sicp_synth* s, int frame, uint16_t* depth, uint8_t* rgb,
+                      uint8_t* labels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SICP_H_ */

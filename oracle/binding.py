@@ -1,0 +1,341 @@
+"""ctypes binding of the oracle (liboracle.so) — TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of
+bench.py, as the checker or the timed CPU baseline; never by the product.
+All arrays are contiguous numpy host arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(_HERE))
+from semantic_icpslam_amd.binding import (  # noqa: E402
+    BLOCK, HASH_ENTRY, VOXEL_GEOM, VOXEL_SEM, FrameResult, Intrinsics, MapConfig, MapStats,
+    PoseF, SystemConfig)
+
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+_P = C.c_void_p
+
+_SIGS = {
+    "oracle_set_threads": (C.c_int, [C.c_int]),
+    "oracle_max_threads": (C.c_int, []),
+    "oracle_expf": (C.c_float, [C.c_float]),
+    "oracle_filter_depth": (None, [_P, C.c_int, C.c_int, C.c_float, C.c_float, _P, _P]),
+    "oracle_compute_vmap": (None, [_P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                   C.c_float, C.c_float, _P]),
+    "oracle_compute_nmap": (None, [_P, C.c_int, C.c_int, _P]),
+    "oracle_pyrdown_f32": (None, [_P, C.c_int, C.c_int, _P, C.c_int, C.c_int]),
+    "oracle_pyrdown_u8": (None, [_P, C.c_int, C.c_int, _P, C.c_int, C.c_int]),
+    "oracle_image_to_intensity": (None, [_P, C.c_int, C.c_int, _P]),
+    "oracle_derivative_image": (None, [_P, C.c_int, C.c_int, _P, _P]),
+    "oracle_resize_map": (None, [_P, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int]),
+    "oracle_icp_sums": (None, [_P, _P, _P, _P, C.c_int, C.c_int, C.POINTER(PoseF),
+                               C.POINTER(PoseF), Intrinsics, _P, _P]),
+    "oracle_icp_sums_fp64": (None, [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
+    "oracle_create_matrix": (None, [_P, _P, _P]),
+    "oracle_cholesky_solve6": (C.c_int, [_P, _P, _P]),
+    "oracle_se3_exp": (None, [_P, _P]),
+    "oracle_se3_update": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "oracle_euler_sin_norm": (None, [_P, _P, _P, _P]),
+    "oracle_map_create": (_P, [C.POINTER(MapConfig), C.c_int]),
+    "oracle_map_destroy": (None, [_P]),
+    "oracle_map_reset": (None, [_P]),
+    "oracle_map_hash": (C.c_uint32, [_P, C.c_int, C.c_int, C.c_int]),
+    "oracle_fuse_map_color": (C.c_uint32, [_P, _P, _P, _P, _P, C.c_int, C.c_int, C.c_int,
+                                           C.POINTER(PoseF), Intrinsics]),
+    "oracle_fuse_map_color_fp64": (C.c_uint32, [_P, _P, _P, _P, _P, C.c_int, C.c_int, C.c_int,
+                                                _P, _P]),
+    "oracle_check_visibility": (C.c_uint32, [_P, C.POINTER(PoseF), C.c_int, C.c_int, Intrinsics,
+                                             C.c_float, C.c_float]),
+    "oracle_create_rendering_blocks": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_float,
+                                                 C.c_float, C.POINTER(PoseF), Intrinsics]),
+    "oracle_raycast": (None, [_P, _P, _P, C.c_int, C.c_int, _P, _P, C.c_int, C.POINTER(PoseF),
+                              Intrinsics]),
+    "oracle_map_get_stats": (None, [_P, C.POINTER(MapStats)]),
+    "oracle_map_export": (None, [_P, _P, _P, _P, _P, _P]),
+    "oracle_map_visible": (C.c_uint32, [_P, _P, C.c_uint32]),
+    "oracle_map_counters": (None, [_P, _P]),
+    "oracle_system_create": (_P, [C.POINTER(SystemConfig), C.c_int]),
+    "oracle_system_destroy": (None, [_P]),
+    "oracle_system_grab": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(FrameResult)]),
+    "oracle_system_model": (None, [_P, C.c_int, _P, _P]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def set_threads(n):
+    return lib().oracle_set_threads(n)
+
+
+def expf(x):
+    return lib().oracle_expf(float(x))
+
+
+def filter_depth(depth_u16, scale, cutoff):
+    d = _c(depth_u16, np.uint16)
+    rows, cols = d.shape
+    raw = np.empty((rows, cols), np.float32)
+    filt = np.empty((rows, cols), np.float32)
+    lib().oracle_filter_depth(_p(d), cols, rows, scale, cutoff, _p(raw), _p(filt))
+    return raw, filt
+
+
+def compute_vmap(depth, fx, fy, cx, cy, cutoff):
+    d = _c(depth, np.float32)
+    rows, cols = d.shape
+    v = np.empty((rows, cols, 4), np.float32)
+    lib().oracle_compute_vmap(_p(d), cols, rows, fx, fy, cx, cy, cutoff, _p(v))
+    return v
+
+
+def compute_nmap(vmap):
+    v = _c(vmap, np.float32)
+    rows, cols = v.shape[:2]
+    n = np.empty_like(v)
+    lib().oracle_compute_nmap(_p(v), cols, rows, _p(n))
+    return n
+
+
+def pyrdown_f32(src):
+    s = _c(src, np.float32)
+    rows, cols = s.shape
+    d = np.empty((rows // 2, cols // 2), np.float32)
+    lib().oracle_pyrdown_f32(_p(s), cols, rows, _p(d), cols // 2, rows // 2)
+    return d
+
+
+def pyrdown_u8(src):
+    s = _c(src, np.uint8)
+    rows, cols = s.shape
+    d = np.empty((rows // 2, cols // 2), np.uint8)
+    lib().oracle_pyrdown_u8(_p(s), cols, rows, _p(d), cols // 2, rows // 2)
+    return d
+
+
+def image_to_intensity(rgb):
+    s = _c(rgb, np.uint8)
+    rows, cols = s.shape[:2]
+    g = np.empty((rows, cols), np.uint8)
+    lib().oracle_image_to_intensity(_p(s), cols, rows, _p(g))
+    return g
+
+
+def derivative_image(gray):
+    s = _c(gray, np.uint8)
+    rows, cols = s.shape
+    dx = np.empty((rows, cols), np.int16)
+    dy = np.empty((rows, cols), np.int16)
+    lib().oracle_derivative_image(_p(s), cols, rows, _p(dx), _p(dy))
+    return dx, dy
+
+
+def resize_map(vsrc, nsrc):
+    v = _c(vsrc, np.float32)
+    n = _c(nsrc, np.float32)
+    rows, cols = v.shape[:2]
+    vd = np.empty((rows // 2, cols // 2, 4), np.float32)
+    nd = np.empty_like(vd)
+    lib().oracle_resize_map(_p(v), _p(n), cols, rows, _p(vd), _p(nd), cols // 2, rows // 2)
+    return vd, nd
+
+
+def icp_sums(vcurr, vlast, ncurr, nlast, pose_curr, pose_last, K, rows_out=False):
+    a = [_c(x, np.float32) for x in (vcurr, vlast, ncurr, nlast)]
+    rows, cols = a[0].shape[:2]
+    sums = np.zeros(29, np.float64)
+    ro = np.zeros((rows * cols, 29), np.float32) if rows_out else None
+    lib().oracle_icp_sums(_p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), cols, rows,
+                          C.byref(pose_curr), C.byref(pose_last), K, _p(sums), _p(ro))
+    return (sums, ro) if rows_out else sums
+
+
+def icp_sums_fp64(vcurr, vlast, ncurr, nlast, T_curr, T_last, K4):
+    a = [_c(x, np.float64) for x in (vcurr, vlast, ncurr, nlast)]
+    rows, cols = a[0].shape[:2]
+    sums = np.zeros(29, np.float64)
+    Tc = _c(T_curr, np.float64)
+    Tl = _c(T_last, np.float64)
+    k = _c(K4, np.float64)
+    lib().oracle_icp_sums_fp64(_p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), cols, rows, _p(Tc),
+                               _p(Tl), _p(k), _p(sums))
+    return sums
+
+
+def create_matrix(sums29):
+    s = _c(sums29, np.float32)
+    A = np.zeros(36, np.float64)
+    b = np.zeros(6, np.float64)
+    lib().oracle_create_matrix(_p(s), _p(A), _p(b))
+    return A.reshape(6, 6), b
+
+
+def solve6(A, b):
+    A = _c(A, np.float64)
+    b = _c(b, np.float64)
+    x = np.zeros(6, np.float64)
+    ok = lib().oracle_cholesky_solve6(_p(A), _p(b), _p(x))
+    return x, bool(ok)
+
+
+def se3_exp(xi):
+    xi = _c(xi, np.float64)
+    T = np.zeros(16, np.float64)
+    lib().oracle_se3_exp(_p(xi), _p(T))
+    return T.reshape(4, 4)
+
+
+def se3_update(A, b, delta, last_pose):
+    A = _c(A, np.float64)
+    b = _c(b, np.float64)
+    delta = _c(delta, np.float64).copy()
+    lp = _c(last_pose, np.float64)
+    nxt = np.zeros(16, np.float64)
+    x = np.zeros(6, np.float64)
+    lib().oracle_se3_update(_p(A), _p(b), _p(delta), _p(lp), _p(nxt), _p(x))
+    return delta.reshape(4, 4), nxt.reshape(4, 4), x
+
+
+def euler_sin_norm(Ta, Tb):
+    a = _c(Ta, np.float64)
+    b = _c(Tb, np.float64)
+    rn = C.c_double()
+    tn = C.c_double()
+    lib().oracle_euler_sin_norm(_p(a), _p(b), C.byref(rn), C.byref(tn))
+    return rn.value, tn.value
+
+
+class OracleMap:
+    def __init__(self, cfg: MapConfig, fp64=False):
+        self.cfg = cfg
+        self.fp64 = fp64
+        self.h = lib().oracle_map_create(C.byref(cfg), 1 if fp64 else 0)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().oracle_map_destroy(self.h)
+        except Exception:
+            pass
+
+    def reset(self):
+        lib().oracle_map_reset(self.h)
+
+    def hash(self, x, y, z):
+        return lib().oracle_map_hash(self.h, x, y, z)
+
+    def fuse(self, depth, rgb, nmap, mask, num_detection, pose, K):
+        d = _c(depth, np.float32)
+        rows, cols = d.shape
+        r = _c(rgb, np.uint8)
+        n = _c(nmap, np.float32) if nmap is not None else None
+        m = _c(mask, np.uint8) if mask is not None else None
+        return lib().oracle_fuse_map_color(self.h, _p(d), _p(r), _p(n), _p(m), cols, rows,
+                                           num_detection, C.byref(pose), K)
+
+    def visibility(self, pose, cols, rows, K, dmax, dmin):
+        return lib().oracle_check_visibility(self.h, C.byref(pose), cols, rows, K, dmax, dmin)
+
+    def rendering_blocks(self, zcols, zrows, dmax, dmin, pose, K):
+        zmin = np.empty((zrows, zcols), np.float32)
+        zmax = np.empty((zrows, zcols), np.float32)
+        any_ = lib().oracle_create_rendering_blocks(self.h, _p(zmin), _p(zmax), zcols, zrows,
+                                                    dmax, dmin, C.byref(pose), K)
+        return zmin, zmax, bool(any_)
+
+    def raycast(self, cols, rows, zmin, zmax, pose, K):
+        v = np.empty((rows, cols, 4), np.float32)
+        n = np.empty((rows, cols, 4), np.float32)
+        zmin = _c(zmin, np.float32)
+        zmax = _c(zmax, np.float32)
+        lib().oracle_raycast(self.h, _p(v), _p(n), cols, rows, _p(zmin), _p(zmax),
+                             zmin.shape[1], C.byref(pose), K)
+        return v, n
+
+    def stats(self):
+        s = MapStats()
+        lib().oracle_map_get_stats(self.h, C.byref(s))
+        return {n: getattr(s, n) for n, _ in MapStats._fields_}
+
+    def export(self):
+        cfg = self.cfg
+        table = np.zeros(2 * cfg.num_buckets, HASH_ENTRY)
+        excess = np.zeros(cfg.num_blocks, HASH_ENTRY)
+        blocks = np.zeros(cfg.num_blocks, BLOCK)
+        geom = np.zeros(cfg.num_blocks * 512, VOXEL_GEOM)
+        sem = np.zeros(cfg.num_blocks * 512, VOXEL_SEM)
+        lib().oracle_map_export(self.h, _p(table), _p(excess), _p(blocks), _p(geom), _p(sem))
+        return dict(table=table, excess=excess, blocks=blocks, geom=geom, sem=sem)
+
+    def visible(self):
+        n = lib().oracle_map_visible(self.h, None, 0)
+        out = np.zeros(n, BLOCK)
+        if n:
+            lib().oracle_map_visible(self.h, _p(out), n)
+        return out
+
+    def counters(self):
+        v = np.zeros(8, np.uint64)
+        lib().oracle_map_counters(self.h, _p(v))
+        names = ("alloc_probes", "ray_lookups", "march_steps", "ray_norange", "ray_back",
+                 "ray_nocross", "ray_nonormal", "ray_hit")
+        return {k: int(x) for k, x in zip(names, v)}
+
+
+class OracleSystem:
+    def __init__(self, cfg: SystemConfig, fp64=False):
+        self.cfg = cfg
+        self.h = lib().oracle_system_create(C.byref(cfg), 1 if fp64 else 0)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().oracle_system_destroy(self.h)
+        except Exception:
+            pass
+
+    def grab(self, depth, rgb, labels=None):
+        d = _c(depth, np.uint16)
+        r = _c(rgb, np.uint8)
+        lab = _c(labels, np.uint8) if labels is not None else None
+        res = FrameResult()
+        lib().oracle_system_grab(self.h, _p(d), _p(r), _p(lab), 1 if labels is not None else 0,
+                                 C.byref(res))
+        return dict(frame_index=res.frame_index, tracked=res.tracked, icp_count=res.icp_count,
+                    iterations=res.iterations, icp_error=res.icp_error,
+                    visible_blocks=res.visible_blocks, allocated_blocks=res.allocated_blocks,
+                    pose=np.array(list(res.pose)).reshape(4, 4))
+
+    def model(self, level=0):
+        cols = self.cfg.cols >> level
+        rows = self.cfg.rows >> level
+        v = np.empty((rows, cols, 4), np.float32)
+        n = np.empty((rows, cols, 4), np.float32)
+        lib().oracle_system_model(self.h, level, _p(v), _p(n))
+        return v, n

@@ -1,0 +1,107 @@
+/*
+ * oracle.h — CPU restatement of the reference's per-frame dense SLAM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load this library, and only as the checker or
+ * the CPU baseline; the product library (libsicp.so) never links it.
+ *
+ * Every function is a scalar restatement of the reference file:line it cites,
+ * written from reading that source. Images are contiguous row-major host
+ * arrays (no pitch); float4 maps are rows*cols*4 floats; RGB is rows*cols*3.
+ * The map, fusion, raycast and tracking functions exist in two precisions:
+ * `fp64 = 0` keeps the reference's FP32 operation order (including its FP64
+ * promotions), `fp64 = 1` computes every floating-point value in double.
+ */
+#ifndef SICP_ORACLE_H_
+#define SICP_ORACLE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+#include "../include/sicp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int oracle_set_threads(int n);       /* OpenMP threads, returns the count in use */
+int oracle_max_threads(void);
+
+/* ---- pyramid (Tracking/Pyrdown.cu) ---- */
+float oracle_expf(float x);
+void oracle_filter_depth(const uint16_t* depth, int cols, int rows, float depth_scale,
+                         float depth_cutoff, float* raw, float* filtered);
+void oracle_compute_vmap(const float* depth, int cols, int rows, float fx, float fy, float cx,
+                         float cy, float depth_cutoff, float* vmap);
+void oracle_compute_nmap(const float* vmap, int cols, int rows, float* nmap);
+void oracle_pyrdown_f32(const float* src, int scols, int srows, float* dst, int dcols, int drows);
+void oracle_pyrdown_u8(const uint8_t* src, int scols, int srows, uint8_t* dst, int dcols,
+                       int drows);
+void oracle_image_to_intensity(const uint8_t* rgb, int cols, int rows, uint8_t* gray);
+void oracle_derivative_image(const uint8_t* gray, int cols, int rows, int16_t* dx, int16_t* dy);
+void oracle_resize_map(const float* vsrc, const float* nsrc, int scols, int srows, float* vdst,
+                       float* ndst, int dcols, int drows);
+
+/* ---- ICP (Tracking/Reduction.cu:213-385) ---- */
+/* Per-pixel rows in FP32 (same order); sums accumulated in FP64 into
+ * sums_fp64 (29), and optionally the per-pixel 29-vectors into rows_out
+ * (rows*cols*29 floats). */
+void oracle_icp_sums(const float* vcurr, const float* vlast, const float* ncurr,
+                     const float* nlast, int cols, int rows, const sicp_pose_f* curr,
+                     const sicp_pose_f* last, sicp_intrinsics K, double* sums_fp64,
+                     float* rows_out);
+/* The same computed entirely in double from double poses (4x4 row-major). */
+void oracle_icp_sums_fp64(const double* vcurr, const double* vlast, const double* ncurr,
+                          const double* nlast, int cols, int rows, const double* curr_pose,
+                          const double* last_pose, const double* K4, double* sums);
+void oracle_create_matrix(const float* sums29, double* A, double* b);
+int oracle_cholesky_solve6(const double* A, const double* b, double* x);
+void oracle_se3_exp(const double* xi, double* T);
+int oracle_se3_update(const double* A, const double* b, double* delta, const double* last_pose,
+                      double* next_pose, double* x_out);
+void oracle_euler_sin_norm(const double* pose_a, const double* pose_b, double* rot_norm,
+                           double* trans_norm);
+
+/* ---- map (Mapping/DeviceMap.cu, FuseMap.cu, RenderScene.cu) ---- */
+void* oracle_map_create(const sicp_map_config* cfg, int fp64);
+void oracle_map_destroy(void* map);
+void oracle_map_reset(void* map);
+uint32_t oracle_map_hash(const void* map, int x, int y, int z);
+/* FuseMapColor (FuseMap.cu:1039-1162). Returns the visible count. */
+uint32_t oracle_fuse_map_color(void* map, const float* depth, const uint8_t* rgb,
+                               const float* nmap, const uint8_t* mask, int cols, int rows,
+                               int num_detection, const sicp_pose_f* pose, sicp_intrinsics K);
+/* The same with a double pose for the fp64 map. */
+uint32_t oracle_fuse_map_color_fp64(void* map, const double* depth, const uint8_t* rgb,
+                                    const double* nmap, const uint8_t* mask, int cols, int rows,
+                                    int num_detection, const double* pose, const double* K4);
+uint32_t oracle_check_visibility(void* map, const sicp_pose_f* pose, int cols, int rows,
+                                 sicp_intrinsics K, float depth_max, float depth_min);
+/* CreateRenderingBlocks (RenderScene.cu:210-289). Returns 1 if any tile was touched. */
+int oracle_create_rendering_blocks(void* map, float* zmin, float* zmax, int zcols, int zrows,
+                                   float depth_max, float depth_min, const sicp_pose_f* pose,
+                                   sicp_intrinsics K);
+/* Raycast (RenderScene.cu:380-537). */
+void oracle_raycast(void* map, float* vmap, float* nmap, int cols, int rows, const float* zmin,
+                    const float* zmax, int zcols, const sicp_pose_f* pose, sicp_intrinsics K);
+void oracle_map_get_stats(void* map, sicp_map_stats* out);
+void oracle_map_export(void* map, sicp_hash_entry* table, sicp_hash_entry* excess,
+                       sicp_block* blocks, sicp_voxel_geom* geom, sicp_voxel_sem* sem);
+uint32_t oracle_map_visible(void* map, sicp_block* out, uint32_t cap);
+/* Counters of the last passes: table probes of the allocation, block
+ * lookups and march steps of the raycast, and the raycast outcomes (no depth
+ * range, back-face exit, no zero crossing, normal rejected, surface found). */
+void oracle_map_counters(void* map, uint64_t* out8);
+
+/* ---- per-frame system (Core/System.cc:131-172) ---- */
+void* oracle_system_create(const sicp_system_config* cfg, int fp64);
+void oracle_system_destroy(void* sys);
+int oracle_system_grab(void* sys, const uint16_t* depth, const uint8_t* rgb,
+                       const uint8_t* labels, int num_detection, sicp_frame_result* out);
+void* oracle_system_map(void* sys);
+/* Copies the last frame's level-l model vertex / normal maps (after raycast). */
+void oracle_system_model(void* sys, int level, float* vmap, float* nmap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

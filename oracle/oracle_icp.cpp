@@ -1,0 +1,305 @@
+// oracle_icp.cpp — CPU restatement of the point-to-plane ICP reduction
+// (Tracking/Reduction.cu:6-16,213-385) and of the Gauss-Newton pose update
+// of Tracker::ComputeSE3 (Tracking/Tracking.cc:369-517).
+// TEST INFRASTRUCTURE ONLY (see oracle.h).
+#include "oracle_impl.hpp"
+
+namespace oracle {
+
+// ICPReduce::searchPoint (Reduction.cu:228-258). The projection keeps the
+// reference's FP64 promotions: invz = 1.0 / z in double stored to float, and
+// the +0.5 rounding offset added in double before the truncating cast (Q9).
+template <class R>
+static inline bool search_point(const R* vcurr, const R* vlast, const R* ncurr, const R* nlast,
+                                int cols, int rows, int x, int y, const Pose<R>& curr,
+                                const Pose<R>& last, Intr<R> K, V3<R>& vcurr_g, V3<R>& vlast_g,
+                                V3<R>& nlast_g) {
+    const R* vc = vcurr + ((size_t)y * cols + x) * 4;
+    V3<R> vcurr_c{vc[0], vc[1], vc[2]};
+    if (isnan_(vcurr_c.x) || (double)vcurr_c.z < 1e-3) return false;
+    vcurr_g = add(mv(curr.R_, vcurr_c), curr.t);
+    V3<R> vcurr_p = mv(last.Rinv, sub(vcurr_g, last.t));
+    R invz = (R)(1.0 / (double)vcurr_p.z);
+    int u = f2i((double)(vcurr_p.x * invz * K.fx + K.cx) + 0.5);
+    int v = f2i((double)(vcurr_p.y * invz * K.fy + K.cy) + 0.5);
+    if (u < 0 || v < 0 || u >= cols || v >= rows) return false;
+    const R* vl = vlast + ((size_t)v * cols + u) * 4;
+    V3<R> vlast_c{vl[0], vl[1], vl[2]};
+    vlast_g = add(mv(last.R_, vlast_c), last.t);
+    const R* nc = ncurr + ((size_t)y * cols + x) * 4;
+    V3<R> ncurr_c{nc[0], nc[1], nc[2]};
+    V3<R> ncurr_g = mv(curr.R_, ncurr_c);
+    const R* nl = nlast + ((size_t)v * cols + u) * 4;
+    V3<R> nlast_c{nl[0], nl[1], nl[2]};
+    nlast_g = mv(last.R_, nlast_c);
+    R dist = norm(sub(vlast_g, vcurr_g));
+    R sine = norm(cross(ncurr_g, nlast_g));
+    const R angle_thresh = (R)0.6, dist_thresh = (R)0.1;  // Reduction.cu:361-362
+    return sine < angle_thresh && dist <= dist_thresh && !isnan_(ncurr_c.x) &&
+           !isnan_(nlast_c.x);
+}
+
+// ICPReduce::getRow (Reduction.cu:260-290): J = [-n, n x v_last],
+// r = -n . (v_curr - v_last), all in the last camera's frame; then the 28
+// products row_i * row_j (i <= j) and the valid flag.
+template <class R>
+static inline void get_row(const R* vcurr, const R* vlast, const R* ncurr, const R* nlast,
+                           int cols, int rows, int i, const Pose<R>& curr, const Pose<R>& last,
+                           Intr<R> K, R* out29) {
+    int y = i / cols;
+    int x = i - y * cols;
+    V3<R> vc, vl, nl;
+    bool found = search_point(vcurr, vlast, ncurr, nlast, cols, rows, x, y, curr, last, K, vc,
+                              vl, nl);
+    R row[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (found) {
+        nl = mv(last.Rinv, nl);
+        vc = mv(last.Rinv, sub(vc, last.t));
+        vl = mv(last.Rinv, sub(vl, last.t));
+        V3<R> mn = neg(nl);
+        row[0] = mn.x;
+        row[1] = mn.y;
+        row[2] = mn.z;
+        V3<R> c = cross(nl, vl);
+        row[3] = c.x;
+        row[4] = c.y;
+        row[5] = c.z;
+        row[6] = dot(mn, sub(vc, vl));
+    }
+    int count = 0;
+    for (int a = 0; a < 7; ++a)
+        for (int b = a; b < 7; ++b) out29[count++] = row[a] * row[b];
+    out29[count] = (R)(found ? 1 : 0);
+}
+
+template <class R>
+void icp_sums(const R* vcurr, const R* vlast, const R* ncurr, const R* nlast, int cols, int rows,
+              const Pose<R>& curr, const Pose<R>& last, Intr<R> K, double* sums,
+              float* rows_out) {
+    const int N = cols * rows;
+    for (int j = 0; j < 29; ++j) sums[j] = 0.0;
+#pragma omp parallel
+    {
+        double loc[29] = {0};
+#pragma omp for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            R v[29];
+            get_row(vcurr, vlast, ncurr, nlast, cols, rows, i, curr, last, K, v);
+            for (int j = 0; j < 29; ++j) loc[j] += (double)v[j];
+            if (rows_out)
+                for (int j = 0; j < 29; ++j) rows_out[(size_t)i * 29 + j] = (float)v[j];
+        }
+#pragma omp critical
+        for (int j = 0; j < 29; ++j) sums[j] += loc[j];
+    }
+}
+
+template void icp_sums<float>(const float*, const float*, const float*, const float*, int, int,
+                              const Pose<float>&, const Pose<float>&, Intr<float>, double*,
+                              float*);
+template void icp_sums<double>(const double*, const double*, const double*, const double*, int,
+                               int, const Pose<double>&, const Pose<double>&, Intr<double>,
+                               double*, float*);
+
+// CreateMatrix<6,7> (Reduction.cu:6-16): upper-triangle sums -> mirrored A, b.
+void create_matrix(const double* s, double* A, double* b) {
+    int shift = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 7; ++j) {
+            double value = s[shift++];
+            if (j == 6)
+                b[i] = value;
+            else
+                A[j * 6 + i] = A[i * 6 + j] = value;
+        }
+}
+
+// x = A^-1 b by an LDL^T factorisation without pivoting; a zero pivot gives a
+// zero solution component, as Eigen's LDLT::solve does (Tracking.cc:493).
+// Equal to the Cholesky solution for symmetric positive definite A.
+bool cholesky_solve6(const double* A, const double* b, double* x) {
+    double L[36] = {0}, D[6];
+    for (int j = 0; j < 6; ++j) {
+        double d = A[j * 6 + j];
+        for (int k = 0; k < j; ++k) d -= L[j * 6 + k] * L[j * 6 + k] * D[k];
+        D[j] = d;
+        L[j * 6 + j] = 1.0;
+        for (int i = j + 1; i < 6; ++i) {
+            double s = A[i * 6 + j];
+            for (int k = 0; k < j; ++k) s -= L[i * 6 + k] * L[j * 6 + k] * D[k];
+            L[i * 6 + j] = (d != 0.0) ? s / d : 0.0;
+        }
+    }
+    double z[6];
+    for (int i = 0; i < 6; ++i) {
+        double s = b[i];
+        for (int k = 0; k < i; ++k) s -= L[i * 6 + k] * z[k];
+        z[i] = s;
+    }
+    bool ok = true;
+    for (int i = 0; i < 6; ++i) {
+        if (D[i] != 0.0)
+            z[i] = z[i] / D[i];
+        else {
+            z[i] = 0.0;
+            ok = false;
+        }
+    }
+    for (int i = 5; i >= 0; --i) {
+        double s = z[i];
+        for (int k = i + 1; k < 6; ++k) s -= L[k * 6 + i] * x[k];
+        x[i] = s;
+    }
+    return ok;
+}
+
+// SE(3) exponential with the tangent ordered (translation, rotation), the
+// order of the Jacobian columns [-n, n x v] (Reduction.cu:276-277) that
+// Sophus::SE3d::exp consumes (Tracking.cc:494). Rotation from the unit
+// quaternion (cos(t/2), sin(t/2) w/t), translation V(w) v with
+// V = I + (1-cos t)/t^2 W + (t - sin t)/t^3 W^2; both with the small-angle
+// series below 1e-10.
+void se3_exp(const double* xi, double* T) {
+    const double* v = xi;
+    const double* w = xi + 3;
+    const double eps = 1e-10;
+    double theta_sq = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    double theta = std::sqrt(theta_sq);
+    double half = 0.5 * theta;
+    double imag, real;
+    if (theta_sq < eps * eps) {
+        double t4 = theta_sq * theta_sq;
+        imag = 0.5 - theta_sq / 48.0 + t4 / 3840.0;
+        real = 1.0 - theta_sq / 8.0 + t4 / 384.0;
+    } else {
+        imag = std::sin(half) / theta;
+        real = std::cos(half);
+    }
+    double qw = real, qx = imag * w[0], qy = imag * w[1], qz = imag * w[2];
+    double tx = 2 * qx, ty = 2 * qy, tz = 2 * qz;
+    double twx = tx * qw, twy = ty * qw, twz = tz * qw;
+    double txx = tx * qx, txy = ty * qx, txz = tz * qx;
+    double tyy = ty * qy, tyz = tz * qy, tzz = tz * qz;
+    double Rm[9] = {1 - (tyy + tzz), txy - twz,       txz + twy,  //
+                    txy + twz,       1 - (txx + tzz), tyz - twx,  //
+                    txz - twy,       tyz + twx,       1 - (txx + tyy)};
+    double V[9];
+    if (theta < eps) {
+        for (int i = 0; i < 9; ++i) V[i] = Rm[i];
+    } else {
+        double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+        double W2[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                W2[i * 3 + j] = W[i * 3 + 0] * W[0 * 3 + j] + W[i * 3 + 1] * W[1 * 3 + j] +
+                                W[i * 3 + 2] * W[2 * 3 + j];
+        double c1 = (1.0 - std::cos(theta)) / theta_sq;
+        double c2 = (theta - std::sin(theta)) / (theta_sq * theta);
+        for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0 ? 1.0 : 0.0) + c1 * W[i] + c2 * W2[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        T[i * 4 + 0] = Rm[i * 3 + 0];
+        T[i * 4 + 1] = Rm[i * 3 + 1];
+        T[i * 4 + 2] = Rm[i * 3 + 2];
+        T[i * 4 + 3] = V[i * 3 + 0] * v[0] + V[i * 3 + 1] * v[1] + V[i * 3 + 2] * v[2];
+    }
+    T[12] = T[13] = T[14] = 0.0;
+    T[15] = 1.0;
+}
+
+void mat4_mul(const double* A, const double* B, double* C) {
+    double t[16];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            t[i * 4 + j] = A[i * 4 + 0] * B[0 * 4 + j] + A[i * 4 + 1] * B[1 * 4 + j] +
+                           A[i * 4 + 2] * B[2 * 4 + j] + A[i * 4 + 3] * B[3 * 4 + j];
+    std::memcpy(C, t, sizeof(t));
+}
+
+// Inverse of a rigid transform [R t; 0 1] = [R^T, -R^T t; 0 1].
+void rigid_inverse(const double* T, double* Ti) {
+    double t[16];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) t[i * 4 + j] = T[j * 4 + i];
+    for (int i = 0; i < 3; ++i)
+        t[i * 4 + 3] = -(t[i * 4 + 0] * T[3] + t[i * 4 + 1] * T[7] + t[i * 4 + 2] * T[11]);
+    t[12] = t[13] = t[14] = 0.0;
+    t[15] = 1.0;
+    std::memcpy(Ti, t, sizeof(t));
+}
+
+// Motion gate of ComputeSE3 (Tracking.cc:505-511): the norm of the sines of
+// the XYZ Euler angles of the relative rotation and the norm of the relative
+// translation. Both Euler solutions of a rotation give the same sine norm, so
+// the textbook decomposition R = Rx(a) Ry(b) Rz(c) stands in for Eigen's
+// eulerAngles(0, 1, 2).
+void euler_sin_norm(const double* T, double* rot_norm, double* trans_norm) {
+    double a = std::atan2(-T[6], T[10]);
+    double b = std::atan2(T[2], std::sqrt(T[0] * T[0] + T[1] * T[1]));
+    double c = std::atan2(-T[1], T[0]);
+    double sa = std::sin(a), sb = std::sin(b), sc = std::sin(c);
+    *rot_norm = std::sqrt(sa * sa + sb * sb + sc * sc);
+    *trans_norm = std::sqrt(T[3] * T[3] + T[7] * T[7] + T[11] * T[11]);
+}
+
+}  // namespace oracle
+
+using namespace oracle;
+
+extern "C" {
+
+void oracle_icp_sums(const float* vcurr, const float* vlast, const float* ncurr,
+                     const float* nlast, int cols, int rows, const sicp_pose_f* curr,
+                     const sicp_pose_f* last, sicp_intrinsics K, double* sums_fp64,
+                     float* rows_out) {
+    Pose<float> pc = pose_from_f<float>(curr->R, curr->Rinv, curr->t);
+    Pose<float> pl = pose_from_f<float>(last->R, last->Rinv, last->t);
+    icp_sums<float>(vcurr, vlast, ncurr, nlast, cols, rows, pc, pl,
+                    Intr<float>{K.fx, K.fy, K.cx, K.cy}, sums_fp64, rows_out);
+}
+
+void oracle_icp_sums_fp64(const double* vcurr, const double* vlast, const double* ncurr,
+                          const double* nlast, int cols, int rows, const double* curr_pose,
+                          const double* last_pose, const double* K4, double* sums) {
+    Pose<double> pc = pose_from_d<double>(curr_pose);
+    Pose<double> pl = pose_from_d<double>(last_pose);
+    icp_sums<double>(vcurr, vlast, ncurr, nlast, cols, rows, pc, pl,
+                     Intr<double>{K4[0], K4[1], K4[2], K4[3]}, sums, nullptr);
+}
+
+void oracle_create_matrix(const float* sums29, double* A, double* b) {
+    double s[29];
+    for (int i = 0; i < 29; ++i) s[i] = (double)sums29[i];
+    create_matrix(s, A, b);
+}
+
+int oracle_cholesky_solve6(const double* A, const double* b, double* x) {
+    return cholesky_solve6(A, b, x) ? 1 : 0;
+}
+
+void oracle_se3_exp(const double* xi, double* T) { se3_exp(xi, T); }
+
+// One Gauss-Newton update of ComputeSE3 (Tracking.cc:493-498).
+int oracle_se3_update(const double* A, const double* b, double* delta, const double* last_pose,
+                      double* next_pose, double* x_out) {
+    double x[6];
+    int ok = cholesky_solve6(A, b, x) ? 1 : 0;
+    double dT[16], inv[16];
+    se3_exp(x, dT);
+    mat4_mul(dT, delta, delta);
+    rigid_inverse(delta, inv);
+    mat4_mul(last_pose, inv, next_pose);
+    if (x_out)
+        for (int i = 0; i < 6; ++i) x_out[i] = x[i];
+    return ok;
+}
+
+void oracle_euler_sin_norm(const double* pose_a, const double* pose_b, double* rot_norm,
+                           double* trans_norm) {
+    double inv[16], p[16];
+    rigid_inverse(pose_a, inv);
+    mat4_mul(inv, pose_b, p);
+    euler_sin_norm(p, rot_norm, trans_norm);
+}
+
+}  // extern "C"

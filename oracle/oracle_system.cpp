@@ -1,0 +1,278 @@
+// oracle_system.cpp — CPU restatement of the per-frame sequence of
+// System::GrabImage (Core/System.cc:131-172): Frame::ResizeImages /
+// FillImages (Core/Frame.cc:68-98), Tracker::Track / TrackFrame /
+// ComputeSE3 with icpOnly = true (Tracking/Tracking.cc:65-119,157-173,
+// 369-517), Mapping::FuseColor and Mapping::RayTrace (Mapping/Mapping.cc:
+// 76-97,142-161). TEST INFRASTRUCTURE ONLY (see oracle.h); also the CPU
+// baseline that bench.py times.
+#include <omp.h>
+
+#include "oracle_impl.hpp"
+
+namespace oracle {
+
+template <class R> struct OFrame {
+    int cols[3], rows[3];
+    std::vector<R> range;
+    std::vector<uint8_t> color;
+    std::vector<R> depth[3], vmap[3], nmap[3];
+    std::vector<uint8_t> image[3];
+    std::vector<int16_t> dx[3], dy[3];
+    double pose[16];
+
+    // Frame::Create (Core/Frame.cc:33-62): level i is cols/2^i x rows/2^i.
+    void create(int c, int r) {
+        for (int i = 0; i < 3; ++i) {
+            cols[i] = c / (1 << i);
+            rows[i] = r / (1 << i);
+            size_t n = (size_t)cols[i] * rows[i];
+            depth[i].assign(n, 0);
+            vmap[i].assign(n * 4, 0);
+            nmap[i].assign(n * 4, 0);
+            image[i].assign(n, 0);
+            dx[i].assign(n, 0);
+            dy[i].assign(n, 0);
+        }
+        range.assign((size_t)c * r, 0);
+        color.assign((size_t)c * r * 3, 0);
+        for (int i = 0; i < 16; ++i) pose[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    }
+};
+
+template <class R> struct OSystem {
+    sicp_system_config cfg;
+    OMap<R> map;
+    OFrame<R> fa, fb;
+    OFrame<R>* next = &fa;
+    OFrame<R>* last = &fb;
+    int state = 1;  // 1: initialise on the next frame (Tracking.cc:74-78)
+    int frame_index = 0;
+    int zc, zr;
+    std::vector<R> zmin, zmax;
+    Intr<R> K[3];
+};
+
+template <class R> static void fill_images(OSystem<R>& s, OFrame<R>& f, const uint16_t* depth,
+                                           const uint8_t* rgb) {
+    std::memcpy(f.color.data(), rgb, f.color.size());
+    R scale_inv = (R)(1.0 / (double)s.cfg.depth_scale);
+    R cutoff = (R)s.cfg.depth_cutoff;
+    filter_depth<R>(depth, f.cols[0], f.rows[0], scale_inv, cutoff, f.range.data(),
+                    f.depth[0].data());
+    image_to_intensity(rgb, f.cols[0], f.rows[0], f.image[0].data());
+    for (int i = 1; i < 3; ++i) {
+        pyrdown<R, R, R>(f.depth[i - 1].data(), f.cols[i - 1], f.rows[i - 1], f.depth[i].data(),
+                         f.cols[i], f.rows[i]);
+        pyrdown<uint8_t, uint8_t, float>(f.image[i - 1].data(), f.cols[i - 1], f.rows[i - 1],
+                                         f.image[i].data(), f.cols[i], f.rows[i]);
+    }
+    for (int i = 0; i < 3; ++i) {
+        const Intr<R>& k = s.K[i];
+        compute_vmap<R>(f.depth[i].data(), f.cols[i], f.rows[i], (R)(1.0 / (double)k.fx),
+                        (R)(1.0 / (double)k.fy), k.cx, k.cy, cutoff, f.vmap[i].data());
+        compute_nmap<R>(f.vmap[i].data(), f.cols[i], f.rows[i], f.nmap[i].data());
+        derivative_image(f.image[i].data(), f.cols[i], f.rows[i], f.dx[i].data(),
+                         f.dy[i].data());
+    }
+}
+
+template <class R> static void resize_images(OFrame<R>& f) {
+    for (int i = 1; i < 3; ++i)
+        resize_map<R>(f.vmap[i - 1].data(), f.nmap[i - 1].data(), f.cols[i - 1], f.rows[i - 1],
+                      f.vmap[i].data(), f.nmap[i].data(), f.cols[i], f.rows[i]);
+}
+
+// ICPStep's host side: the 29 sums are float on the device (Reduction.cu:
+// 377-384), so the FP32 restatement rounds its FP64 accumulation to float.
+template <class R> static void round_sums(double* s);
+template <> void round_sums<float>(double* s) {
+    for (int i = 0; i < 29; ++i) s[i] = (double)(float)s[i];
+}
+template <> void round_sums<double>(double*) {}
+
+// Tracker::ComputeSE3 (Tracking.cc:369-517), icpOnly branch (:488-490).
+template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res) {
+    OFrame<R>& nf = *s.next;
+    OFrame<R>& lf = *s.last;
+    double last_pose[16], pose0[16], delta[16], candidate[16];
+    std::memcpy(last_pose, lf.pose, sizeof(last_pose));
+    std::memcpy(pose0, nf.pose, sizeof(pose0));
+    for (int i = 0; i < 16; ++i) delta[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    std::memcpy(candidate, nf.pose, sizeof(candidate));
+    R icp_error = 0, last_error = (R)100.f, curr_error = 0, candidate_icp = 0;
+    int icp_count = 0, break_count = 0, applied = 0;
+    for (int i = 2; i >= 0; --i) {
+        for (int j = 0; j < s.cfg.icp_iters[i]; ++j) {
+            Pose<R> pc = pose_from_d<R>(nf.pose);
+            Pose<R> pl = pose_from_d<R>(lf.pose);
+            double sums[29], A[36], b[6];
+            icp_sums<R>(nf.vmap[i].data(), lf.vmap[i].data(), nf.nmap[i].data(),
+                        lf.nmap[i].data(), nf.cols[i], nf.rows[i], pc, pl, s.K[i], sums, nullptr);
+            round_sums<R>(sums);
+            create_matrix(sums, A, b);
+            R r0 = (R)sums[27], r1 = (R)sums[28];
+            icp_error = std::sqrt(r0) / r1;
+            icp_count = (int)r1;
+            res->icp_count = icp_count;
+            if (isnan_(icp_error) || icp_count < s.cfg.icp_min_count[i]) {
+                std::memcpy(nf.pose, last_pose, sizeof(last_pose));
+                res->icp_error = (float)icp_error;
+                res->iterations = applied;
+                return false;
+            }
+            curr_error = icp_error;
+            if (last_error < curr_error) {
+                break_count++;
+                if (break_count == 1) std::memcpy(candidate, nf.pose, sizeof(candidate));
+                candidate_icp = icp_error;  // brace-less in the reference (Q11)
+            } else {
+                break_count = 0;
+            }
+            if (break_count > 2) {
+                std::memcpy(nf.pose, candidate, sizeof(candidate));
+                icp_error = candidate_icp;
+                break;
+            }
+            last_error = curr_error;
+            double x[6], dT[16], inv[16];
+            cholesky_solve6(A, b, x);
+            se3_exp(x, dT);
+            mat4_mul(dT, delta, delta);
+            rigid_inverse(delta, inv);
+            mat4_mul(last_pose, inv, nf.pose);
+            applied++;
+        }
+        if (break_count > 2) break;
+    }
+    res->icp_error = (float)icp_error;
+    res->iterations = applied;
+    double inv0[16], p[16], rn, tn;
+    rigid_inverse(pose0, inv0);
+    mat4_mul(inv0, nf.pose, p);
+    euler_sin_norm(p, &rn, &tn);
+    if (icp_error < (R)s.cfg.icp_thresh && rn <= 0.1 && tn <= 0.1) return true;
+    std::memcpy(nf.pose, last_pose, sizeof(last_pose));
+    return false;
+}
+
+template <class R> static void system_init(OSystem<R>& s, const sicp_system_config& cfg) {
+    s.cfg = cfg;
+    map_init(s.map, cfg.map);
+    s.fa.create(cfg.cols, cfg.rows);
+    s.fb.create(cfg.cols, cfg.rows);
+    for (int i = 0; i < 3; ++i) {
+        // Frame::SetK (Core/Frame.cc:580-588): K / 2^i in float.
+        float d = (float)(1 << i);
+        s.K[i] = Intr<R>{(R)(cfg.K.fx / d), (R)(cfg.K.fy / d), (R)(cfg.K.cx / d),
+                         (R)(cfg.K.cy / d)};
+    }
+    s.zc = (cfg.cols + 7) / 8;
+    s.zr = (cfg.rows + 7) / 8;
+    s.zmin.assign((size_t)s.zc * s.zr, 0);
+    s.zmax.assign((size_t)s.zc * s.zr, 0);
+}
+
+template <class R>
+static int system_grab(OSystem<R>& s, const uint16_t* depth, const uint8_t* rgb,
+                       const uint8_t* labels, int num_detection, sicp_frame_result* res) {
+    *res = sicp_frame_result{};
+    res->frame_index = s.frame_index++;
+    // Tracker::GrabFrame (Tracking.cc:246-280)
+    resize_images(*s.last);
+    fill_images(s, *s.next, depth, rgb);
+    bool tracked;
+    if (s.state == 1) {
+        // InitTracking: both poses identity, then SwapFrame (Tracking.cc:74-78,233-240)
+        for (int i = 0; i < 16; ++i) s.next->pose[i] = s.last->pose[i] = (i % 5 == 0) ? 1 : 0;
+        std::swap(s.next, s.last);
+        s.state = 0;
+        tracked = true;
+    } else {
+        std::memcpy(s.next->pose, s.last->pose, sizeof(s.next->pose));  // TrackFrame (:166)
+        tracked = compute_se3(s, res);
+        if (tracked) std::swap(s.next, s.last);
+    }
+    res->tracked = tracked ? 1 : 0;
+    if (tracked) {
+        OFrame<R>& f = *s.last;
+        Pose<R> p = pose_from_d<R>(f.pose);
+        uint32_t nv = fuse_map_color(s.map, f.range.data(), f.color.data(), f.nmap[0].data(),
+                                     labels, f.cols[0], f.rows[0], labels ? num_detection : 0, p,
+                                     s.K[0]);
+        res->visible_blocks = nv;
+        if (nv > 0) {
+            int any = create_rendering_blocks(s.map, s.zmin.data(), s.zmax.data(), s.zc, s.zr,
+                                              (R)s.cfg.map.depth_max, (R)s.cfg.map.depth_min, p,
+                                              s.K[0]);
+            if (any)
+                raycast(s.map, f.vmap[0].data(), f.nmap[0].data(), f.cols[0], f.rows[0],
+                        s.zmin.data(), s.zmax.data(), s.zc, p, (R)(1.0 / (double)s.K[0].fx),
+                        (R)(1.0 / (double)s.K[0].fy), s.K[0].cx, s.K[0].cy);
+        }
+    }
+    res->allocated_blocks = s.map.heap_used;
+    std::memcpy(res->pose, s.last->pose, sizeof(res->pose));
+    return SICP_OK;
+}
+
+}  // namespace oracle
+
+using namespace oracle;
+
+struct OracleSystemHandle {
+    int fp64;
+    OSystem<float> f;
+    OSystem<double> d;
+};
+
+extern "C" {
+
+int oracle_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
+
+int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+void* oracle_system_create(const sicp_system_config* cfg, int fp64) {
+    OracleSystemHandle* h = new OracleSystemHandle();
+    h->fp64 = fp64;
+    if (fp64)
+        system_init(h->d, *cfg);
+    else
+        system_init(h->f, *cfg);
+    return h;
+}
+
+void oracle_system_destroy(void* sys) { delete (OracleSystemHandle*)sys; }
+
+int oracle_system_grab(void* sys, const uint16_t* depth, const uint8_t* rgb,
+                       const uint8_t* labels, int num_detection, sicp_frame_result* out) {
+    OracleSystemHandle* h = (OracleSystemHandle*)sys;
+    return h->fp64 ? system_grab(h->d, depth, rgb, labels, num_detection, out)
+                   : system_grab(h->f, depth, rgb, labels, num_detection, out);
+}
+
+void* oracle_system_map(void* sys) {
+    // Only the FP32 system's map is exposed through the map handle API.
+    (void)sys;
+    return nullptr;
+}
+
+void oracle_system_model(void* sys, int level, float* vmap, float* nmap) {
+    OracleSystemHandle* h = (OracleSystemHandle*)sys;
+    auto go = [&](auto& s) {
+        auto& f = *s.last;
+        size_t n = f.vmap[level].size();
+        for (size_t i = 0; i < n; ++i) {
+            vmap[i] = (float)f.vmap[level][i];
+            nmap[i] = (float)f.nmap[level][i];
+        }
+    };
+    if (h->fp64)
+        go(h->d);
+    else
+        go(h->f);
+}
+
+}  // extern "C"

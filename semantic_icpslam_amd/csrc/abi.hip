@@ -1,0 +1,465 @@
+// abi.hip — the extern "C" entry points of include/sicp.h: argument
+// validation, map ownership and the reference-shaped operations.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "se3_math.hpp"
+#include "sicp_internal.hpp"
+
+namespace sicp {
+
+static thread_local std::string g_last_error = "";
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return SICP_ERR_HIP;
+}
+
+int invalid(const char* msg) {
+    set_error(msg);
+    return SICP_ERR_INVALID_ARGUMENT;
+}
+
+bool image_ok(const sicp_image& im, size_t elem, int cols, int rows) {
+    if (!im.ptr || im.cols != cols || im.rows != rows || cols <= 0 || rows <= 0) return false;
+    if (im.pitch < (size_t)cols * elem) return false;
+    return true;
+}
+
+static bool pow2(uint32_t v) { return v && !(v & (v - 1)); }
+
+template <class T> static int dmalloc(sicp_map* m, T** p, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T) + 16);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    m->owned.push_back(q);
+    *p = (T*)q;
+    return SICP_OK;
+}
+
+int map_alloc(const sicp_map_config& cfg, sicp_map** out) {
+    if (cfg.voxel_size <= 0.f || cfg.truncation <= 0.f) return invalid("voxel_size / truncation must be > 0");
+    if (cfg.num_buckets <= 0 || cfg.num_blocks <= 0) return invalid("num_buckets / num_blocks must be > 0");
+    if (cfg.num_blocks > (1 << 22)) return invalid("num_blocks above 4M");
+    if (!pow2((uint32_t)cfg.request_capacity)) return invalid("request_capacity must be a power of two");
+    if (cfg.max_cols <= 0 || cfg.max_rows <= 0) return invalid("max_cols / max_rows must be > 0");
+    if ((size_t)cfg.max_cols * cfg.max_rows > (size_t)4096 * 1024) return invalid("image area above 4M pixels");
+    if (cfg.max_weight < 1 || cfg.max_weight > 255) return invalid("max_weight outside 1..255");
+    if (cfg.max_label_count < 1 || cfg.max_label_count > 65535) return invalid("max_label_count outside 1..65535");
+    sicp_map* m = new sicp_map();
+    m->cfg = cfg;
+    int rc = SICP_OK;
+    const size_t nb = (size_t)cfg.num_buckets, nblk = (size_t)cfg.num_blocks;
+    const size_t q = (size_t)cfg.request_capacity;
+    const size_t px = (size_t)cfg.max_cols * cfg.max_rows;
+    MapDev& d = m->dev;
+    if ((rc = dmalloc(m, &d.table, 2 * nb)) || (rc = dmalloc(m, &d.excess, nblk)) ||
+        (rc = dmalloc(m, &d.blocks, nblk)) || (rc = dmalloc(m, &d.geom, nblk * kBlockVoxels)) ||
+        (rc = dmalloc(m, &d.sem, nblk * kBlockVoxels)) || (rc = dmalloc(m, &m->visible, nblk)) ||
+        (rc = dmalloc(m, &m->ctr, 1))) {
+        map_free(m);
+        return rc;
+    }
+    AllocScratch& a = m->alloc;
+    if ((rc = dmalloc(m, &a.req_keys, q)) || (rc = dmalloc(m, &a.req_first, q)) ||
+        (rc = dmalloc(m, &a.pix_mask, px)) || (rc = dmalloc(m, &a.pix_count, px)) ||
+        (rc = dmalloc(m, &a.pix_base, px)) || (rc = dmalloc(m, &a.scan_partials, 4096 + 64)) ||
+        (rc = dmalloc(m, &a.new_keys, q)) || (rc = dmalloc(m, &a.new_req_slot, q)) ||
+        (rc = dmalloc(m, &a.bucket_leader, nb)) || (rc = dmalloc(m, &a.bucket_head, nb)) ||
+        (rc = dmalloc(m, &a.pending_next, q)) || (rc = dmalloc(m, &a.vis_flags, nblk))) {
+        map_free(m);
+        return rc;
+    }
+    a.req_mask = (uint32_t)(q - 1);
+    a.cap_pixels = (uint32_t)px;
+    d.nb = (uint32_t)nb;
+    d.nb_mask = pow2((uint32_t)nb) ? (uint32_t)nb - 1 : 0u;
+    d.num_blocks = cfg.num_blocks;
+    d.voxel = cfg.voxel_size;
+    d.voxel_inv = (float)(1.0 / (double)cfg.voxel_size);                 // DeviceMap.h:186
+    d.trunc = cfg.truncation;
+    d.step_scale = (float)(0.5 * (double)cfg.truncation * (double)d.voxel_inv);  // DeviceMap.h:189
+    d.block_width = cfg.voxel_size * (float)kBlockSide;                  // DeviceMap.h:187
+    d.dmin = cfg.depth_min;
+    d.dmax = cfg.depth_max;
+    d.max_weight = cfg.max_weight;
+    d.max_count = cfg.max_label_count;
+    rc = launch_map_reset(m->dev, m->ctr, m->alloc, nullptr);
+    if (rc == SICP_OK) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) rc = hip_fail(e, "map reset");
+    }
+    if (rc) {
+        map_free(m);
+        return rc;
+    }
+    *out = m;
+    return SICP_OK;
+}
+
+void map_free(sicp_map* m) {
+    if (!m) return;
+    for (void* p : m->owned) (void)hipFree(p);
+    delete m;
+}
+
+static PoseArg pose_arg(const sicp_pose_f* host, const sicp_pose_f* dev) {
+    return make_pose_arg(host, dev);
+}
+
+}  // namespace sicp
+
+using namespace sicp;
+
+extern "C" {
+
+const char* sicp_last_error(void) { return g_last_error.c_str(); }
+
+const char* sicp_version(void) { return "semantic_icpslam_amd 0.1 (gfx950)"; }
+
+int sicp_device_count(int* count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return hip_fail(e, "hipGetDeviceCount");
+    }
+    *count = n;
+    return SICP_OK;
+}
+
+void sicp_map_config_default(sicp_map_config* cfg, float voxel_size, float truncation,
+                             int32_t num_buckets, int32_t num_blocks) {
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->voxel_size = voxel_size;
+    cfg->truncation = truncation;
+    cfg->depth_min = 0.1f;   // DeviceMap::DepthMin (DeviceMap.h:168)
+    cfg->depth_max = 3.0f;   // DeviceMap::DepthMax (DeviceMap.h:169)
+    cfg->num_buckets = num_buckets;
+    cfg->num_blocks = num_blocks;
+    cfg->max_weight = 255;   // min(255, w + 1) (FuseMap.cu:505)
+    cfg->max_label_count = 65535;
+    cfg->request_capacity = 1 << 20;
+    cfg->max_cols = 640;
+    cfg->max_rows = 480;
+}
+
+// ---------------------------------------------------------------- pyramid --
+int sicp_filter_depth(sicp_image depth_u16, sicp_image raw_f32, sicp_image filtered_f32,
+                      float depth_scale, float depth_cutoff, void* stream) {
+    const int c = depth_u16.cols, r = depth_u16.rows;
+    if (!image_ok(depth_u16, 2, c, r) || !image_ok(raw_f32, 4, c, r) ||
+        !image_ok(filtered_f32, 4, c, r))
+        return invalid("filter_depth: image shape / pitch mismatch");
+    if (!(depth_scale > 0.f)) return invalid("filter_depth: depth_scale must be > 0");
+    // FilterDepth passes 1.0 / depthScale rounded to float (Pyrdown.cu:117).
+    return launch_filter_depth(view<uint16_t>(depth_u16), view<float>(raw_f32),
+                               view<float>(filtered_f32), (float)(1.0 / (double)depth_scale),
+                               depth_cutoff, (hipStream_t)stream);
+}
+
+int sicp_compute_vmap(sicp_image depth_f32, sicp_image vmap_f4, float fx, float fy, float cx,
+                      float cy, float depth_cutoff, void* stream) {
+    const int c = depth_f32.cols, r = depth_f32.rows;
+    if (!image_ok(depth_f32, 4, c, r) || !image_ok(vmap_f4, 16, c, r))
+        return invalid("compute_vmap: image shape / pitch mismatch");
+    return launch_vmap(view<float>(depth_f32), view<float4>(vmap_f4), (float)(1.0 / (double)fx),
+                       (float)(1.0 / (double)fy), cx, cy, depth_cutoff, (hipStream_t)stream);
+}
+
+int sicp_compute_nmap(sicp_image vmap_f4, sicp_image nmap_f4, void* stream) {
+    const int c = vmap_f4.cols, r = vmap_f4.rows;
+    if (!image_ok(vmap_f4, 16, c, r) || !image_ok(nmap_f4, 16, c, r))
+        return invalid("compute_nmap: image shape / pitch mismatch");
+    return launch_nmap(view<float4>(vmap_f4), view<float4>(nmap_f4), (hipStream_t)stream);
+}
+
+static bool half_of(const sicp_image& src, const sicp_image& dst) {
+    return dst.cols == src.cols / 2 && dst.rows == src.rows / 2 && dst.cols > 0 && dst.rows > 0;
+}
+
+int sicp_pyrdown_gauss_f32(sicp_image src, sicp_image dst, void* stream) {
+    if (!image_ok(src, 4, src.cols, src.rows) || !image_ok(dst, 4, dst.cols, dst.rows) ||
+        !half_of(src, dst))
+        return invalid("pyrdown_gauss_f32: dst must be src/2 (integer division)");
+    return launch_pyrdown_f32(view<float>(src), view<float>(dst), (hipStream_t)stream);
+}
+
+int sicp_pyrdown_gauss_u8(sicp_image src, sicp_image dst, void* stream) {
+    if (!image_ok(src, 1, src.cols, src.rows) || !image_ok(dst, 1, dst.cols, dst.rows) ||
+        !half_of(src, dst))
+        return invalid("pyrdown_gauss_u8: dst must be src/2 (integer division)");
+    return launch_pyrdown_u8(view<uint8_t>(src), view<uint8_t>(dst), (hipStream_t)stream);
+}
+
+int sicp_image_to_intensity(sicp_image rgb_u8x3, sicp_image gray_u8, void* stream) {
+    const int c = gray_u8.cols, r = gray_u8.rows;
+    if (!image_ok(rgb_u8x3, 3, c, r) || !image_ok(gray_u8, 1, c, r))
+        return invalid("image_to_intensity: image shape / pitch mismatch");
+    return launch_intensity(view<uint8_t>(rgb_u8x3), view<uint8_t>(gray_u8), (hipStream_t)stream);
+}
+
+int sicp_compute_derivative_image(sicp_image gray_u8, sicp_image dx_i16, sicp_image dy_i16,
+                                  void* stream) {
+    const int c = gray_u8.cols, r = gray_u8.rows;
+    if (!image_ok(gray_u8, 1, c, r) || !image_ok(dx_i16, 2, c, r) || !image_ok(dy_i16, 2, c, r))
+        return invalid("compute_derivative_image: image shape / pitch mismatch");
+    return launch_derivative(view<uint8_t>(gray_u8), view<int16_t>(dx_i16), view<int16_t>(dy_i16),
+                             (hipStream_t)stream);
+}
+
+int sicp_resize_map(sicp_image vsrc, sicp_image nsrc, sicp_image vdst, sicp_image ndst,
+                    void* stream) {
+    const int c = vsrc.cols, r = vsrc.rows;
+    if (!image_ok(vsrc, 16, c, r) || !image_ok(nsrc, 16, c, r) ||
+        !image_ok(vdst, 16, c / 2, r / 2) || !image_ok(ndst, 16, c / 2, r / 2))
+        return invalid("resize_map: dst must be src/2 (integer division)");
+    return launch_resize_map(view<float4>(vsrc), view<float4>(nsrc), view<float4>(vdst),
+                             view<float4>(ndst), (hipStream_t)stream);
+}
+
+// -------------------------------------------------------------------- ICP --
+size_t sicp_icp_partials_size(void) { return sizeof(double) * kIcpMaxBlocks * kIcpSums + 256; }
+
+int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nlast,
+                  const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
+                  void* partials_dev, float* sums_dev, float* host_sums, double* A_host,
+                  double* b_host, float* residual_host, void* stream) {
+    const int c = vcurr.cols, r = vcurr.rows;
+    if (!image_ok(vcurr, 16, c, r) || !image_ok(vlast, 16, c, r) || !image_ok(ncurr, 16, c, r) ||
+        !image_ok(nlast, 16, c, r))
+        return invalid("icp_step: the four maps must share one shape");
+    if (!curr || !last || !partials_dev || !sums_dev) return invalid("icp_step: null argument");
+    // The arrival counter lives after the partial rows; it is zero between
+    // launches (callers allocate the scratch zeroed).
+    IcpScratch sc{(double*)partials_dev,
+                  (unsigned*)((double*)partials_dev + (size_t)kIcpMaxBlocks * kIcpSums)};
+    hipStream_t s = (hipStream_t)stream;
+    int rc = launch_icp_sums(view<float4>(vcurr), view<float4>(vlast), view<float4>(ncurr),
+                             view<float4>(nlast), make_pose_arg(curr, nullptr),
+                             make_pose_arg(last, nullptr), K, sc, sums_dev, s);
+    if (rc) return rc;
+    if (host_sums || A_host || b_host || residual_host) {
+        float h[kIcpSums];
+        SICP_HIP(hipMemcpyAsync(h, sums_dev, sizeof(h), hipMemcpyDeviceToHost, s));
+        SICP_HIP(hipStreamSynchronize(s));
+        if (host_sums) std::memcpy(host_sums, h, sizeof(h));
+        double A[36], b[6];
+        create_matrix(h, A, b);
+        if (A_host) std::memcpy(A_host, A, sizeof(A));
+        if (b_host) std::memcpy(b_host, b, sizeof(b));
+        if (residual_host) {
+            residual_host[0] = h[27];
+            residual_host[1] = h[28];
+        }
+    }
+    return SICP_OK;
+}
+
+int sicp_se3_update_host(const double* A, const double* b, double* delta,
+                         const double* last_pose, double* next_pose, double* x_out) {
+    if (!A || !b || !delta || !last_pose || !next_pose) return invalid("se3_update_host: null argument");
+    se3_update_host(A, b, delta, last_pose, next_pose, x_out);
+    return SICP_OK;
+}
+
+// -------------------------------------------------------------------- map --
+int sicp_map_create(const sicp_map_config* cfg, sicp_map** out) {
+    if (!cfg || !out) return invalid("map_create: null argument");
+    return map_alloc(*cfg, out);
+}
+
+int sicp_map_destroy(sicp_map* map) {
+    map_free(map);
+    return SICP_OK;
+}
+
+int sicp_map_get_config(const sicp_map* map, sicp_map_config* cfg) {
+    if (!map || !cfg) return invalid("map_get_config: null argument");
+    *cfg = map->cfg;
+    return SICP_OK;
+}
+
+int sicp_map_reset(sicp_map* map, void* stream) {
+    if (!map) return invalid("map_reset: null map");
+    return launch_map_reset(map->dev, map->ctr, map->alloc, (hipStream_t)stream);
+}
+
+static int read_u32(const uint32_t* dev, uint32_t* host, hipStream_t s) {
+    SICP_HIP(hipMemcpyAsync(host, dev, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SICP_HIP(hipStreamSynchronize(s));
+    return SICP_OK;
+}
+
+int sicp_fuse_map_color(sicp_map* map, sicp_image depth_f32, sicp_image rgb_u8x3,
+                        sicp_image nmap_f4, sicp_image mask_u8, int num_detection,
+                        const sicp_pose_f* pose_host, const sicp_pose_f* pose_dev,
+                        sicp_intrinsics K, uint32_t* visible_host, void* stream) {
+    if (!map) return invalid("fuse_map_color: null map");
+    const int c = depth_f32.cols, r = depth_f32.rows;
+    if (!image_ok(depth_f32, 4, c, r) || !image_ok(rgb_u8x3, 3, c, r))
+        return invalid("fuse_map_color: depth / rgb shape mismatch");
+    if (num_detection > 0 && (!image_ok(nmap_f4, 16, c, r) || !image_ok(mask_u8, 1, c, r)))
+        return invalid("fuse_map_color: nmap / mask required when num_detection > 0");
+    if (!pose_host && !pose_dev) return invalid("fuse_map_color: no pose");
+    if ((size_t)c * r > (size_t)map->cfg.max_cols * map->cfg.max_rows)
+        return invalid("fuse_map_color: image larger than the map's max_cols x max_rows");
+    hipStream_t s = (hipStream_t)stream;
+    PoseArg pa = pose_arg(pose_host, pose_dev);
+    int rc = launch_allocate(map->dev, map->ctr, map->alloc, view<float>(depth_f32), pa, K,
+                             nullptr, s);
+    if (rc) return rc;
+    // FuseMapColor checks visibility against the map's own depth range
+    // (FuseMap.cu:1080-1081).
+    rc = launch_visibility(map->dev, map->ctr, map->alloc, map->visible, pa, K, c, r,
+                           map->cfg.depth_max, map->cfg.depth_min, nullptr, s);
+    if (rc) return rc;
+    sicp_image nm = nmap_f4, mk = mask_u8;
+    if (num_detection <= 0) {
+        nm = depth_f32;  // unused by the kernel when num_detection <= 0
+        mk = depth_f32;
+    }
+    rc = launch_fuse(map->dev, map->ctr, map->visible, view<float>(depth_f32),
+                     view<uint8_t>(rgb_u8x3), view<float4>(nm), view<uint8_t>(mk), num_detection,
+                     pa, K, nullptr, s);
+    if (rc) return rc;
+    if (visible_host) {
+        rc = read_u32(&map->ctr->visible, visible_host, s);
+        if (rc) return rc;
+        if (*visible_host == 0) {
+            set_error("no visible blocks found");
+            return SICP_ERR_NO_VISIBLE;
+        }
+    }
+    return SICP_OK;
+}
+
+int sicp_check_block_visibility(sicp_map* map, const sicp_pose_f* pose_host,
+                                const sicp_pose_f* pose_dev, int cols, int rows,
+                                sicp_intrinsics K, float depth_max, float depth_min,
+                                uint32_t* visible_host, void* stream) {
+    if (!map) return invalid("check_block_visibility: null map");
+    if (!pose_host && !pose_dev) return invalid("check_block_visibility: no pose");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = launch_visibility(map->dev, map->ctr, map->alloc, map->visible,
+                               pose_arg(pose_host, pose_dev), K, cols, rows, depth_max, depth_min,
+                               nullptr, s);
+    if (rc) return rc;
+    if (visible_host) {
+        rc = read_u32(&map->ctr->visible, visible_host, s);
+        if (rc) return rc;
+        if (*visible_host == 0) {
+            set_error("no visible blocks found");
+            return SICP_ERR_NO_VISIBLE;
+        }
+    }
+    return SICP_OK;
+}
+
+int sicp_create_rendering_blocks(sicp_map* map, sicp_image zmin, sicp_image zmax,
+                                 float depth_max, float depth_min,
+                                 const sicp_pose_f* pose_host, const sicp_pose_f* pose_dev,
+                                 sicp_intrinsics K, int* nonempty_host, void* stream) {
+    if (!map) return invalid("create_rendering_blocks: null map");
+    if (!image_ok(zmin, 4, zmin.cols, zmin.rows) || !image_ok(zmax, 4, zmin.cols, zmin.rows))
+        return invalid("create_rendering_blocks: zmin / zmax shape mismatch");
+    if (!pose_host && !pose_dev) return invalid("create_rendering_blocks: no pose");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = launch_render_prepass(map->dev, map->ctr, map->visible, view<float>(zmin),
+                                   view<float>(zmax), pose_arg(pose_host, pose_dev), K, depth_max,
+                                   depth_min, nullptr, s);
+    if (rc) return rc;
+    if (nonempty_host) {
+        uint32_t n = 0;
+        rc = read_u32(&map->ctr->render_tiles, &n, s);
+        if (rc) return rc;
+        *nonempty_host = n > 0 ? 1 : 0;
+        if (!n) {
+            set_error("nothing to render");
+            return SICP_ERR_NOTHING_TO_RENDER;
+        }
+    }
+    return SICP_OK;
+}
+
+int sicp_raycast(sicp_map* map, sicp_image vmap_f4, sicp_image nmap_f4, sicp_image zmin,
+                 sicp_image zmax, const sicp_pose_f* pose_host, const sicp_pose_f* pose_dev,
+                 sicp_intrinsics K, void* stream) {
+    if (!map) return invalid("raycast: null map");
+    const int c = vmap_f4.cols, r = vmap_f4.rows;
+    if (!image_ok(vmap_f4, 16, c, r) || !image_ok(nmap_f4, 16, c, r))
+        return invalid("raycast: vmap / nmap shape mismatch");
+    if (!image_ok(zmin, 4, (c + 7) / 8, (r + 7) / 8) || !image_ok(zmax, 4, (c + 7) / 8, (r + 7) / 8))
+        return invalid("raycast: zmin / zmax must be ceil(W/8) x ceil(H/8)");
+    if (!pose_host && !pose_dev) return invalid("raycast: no pose");
+    // Mapping::RayTrace passes 1.0 / fx rounded to float (Mapping.cc:158-159).
+    return launch_raycast(map->dev, view<float4>(vmap_f4), view<float4>(nmap_f4),
+                          view<float>(zmin), view<float>(zmax), pose_arg(pose_host, pose_dev),
+                          (float)(1.0 / (double)K.fx), (float)(1.0 / (double)K.fy), K.cx, K.cy,
+                          nullptr, (hipStream_t)stream);
+}
+
+int sicp_map_get_stats(sicp_map* map, sicp_map_stats* out, void* stream) {
+    if (!map || !out) return invalid("map_get_stats: null argument");
+    MapCounters c;
+    hipStream_t s = (hipStream_t)stream;
+    SICP_HIP(hipMemcpyAsync(&c, map->ctr, sizeof(c), hipMemcpyDeviceToHost, s));
+    SICP_HIP(hipStreamSynchronize(s));
+    out->allocated_blocks = c.heap_used;
+    out->visible_blocks = c.visible;
+    out->new_blocks = c.new_blocks;
+    out->dropped_heap = c.dropped_heap;
+    out->dropped_requests = c.dropped_requests;
+    out->dropped_range = c.dropped_range;
+    out->updated_voxels = c.updated_voxels;
+    out->render_tiles = c.render_tiles;
+    out->pad = 0;
+    out->updated_total = c.updated_total;
+    out->visible_total = c.visible_total;
+    out->fusions = c.fusions;
+    return SICP_OK;
+}
+
+int sicp_map_download(sicp_map* map, sicp_hash_entry* table, sicp_hash_entry* excess,
+                      sicp_block* blocks, sicp_voxel_geom* geom, sicp_voxel_sem* sem,
+                      uint32_t n_blocks_limit) {
+    if (!map) return invalid("map_download: null map");
+    SICP_HIP(hipDeviceSynchronize());
+    const size_t nb = (size_t)map->cfg.num_buckets;
+    size_t nblk = (size_t)map->cfg.num_blocks;
+    if (n_blocks_limit && n_blocks_limit < nblk) nblk = n_blocks_limit;
+    if (table)
+        SICP_HIP(hipMemcpy(table, map->dev.table, 2 * nb * sizeof(sicp_hash_entry),
+                           hipMemcpyDeviceToHost));
+    if (excess)
+        SICP_HIP(hipMemcpy(excess, map->dev.excess, nblk * sizeof(sicp_hash_entry),
+                           hipMemcpyDeviceToHost));
+    if (blocks)
+        SICP_HIP(hipMemcpy(blocks, map->dev.blocks, nblk * sizeof(sicp_block),
+                           hipMemcpyDeviceToHost));
+    if (geom)
+        SICP_HIP(hipMemcpy(geom, map->dev.geom, nblk * kBlockVoxels * sizeof(sicp_voxel_geom),
+                           hipMemcpyDeviceToHost));
+    if (sem)
+        SICP_HIP(hipMemcpy(sem, map->dev.sem, nblk * kBlockVoxels * sizeof(sicp_voxel_sem),
+                           hipMemcpyDeviceToHost));
+    return SICP_OK;
+}
+
+int sicp_map_download_visible(sicp_map* map, sicp_block* visible, uint32_t cap,
+                              uint32_t* count) {
+    if (!map) return invalid("map_download_visible: null map");
+    SICP_HIP(hipDeviceSynchronize());
+    MapCounters c;
+    SICP_HIP(hipMemcpy(&c, map->ctr, sizeof(c), hipMemcpyDeviceToHost));
+    if (count) *count = c.visible;
+    uint32_t n = c.visible < cap ? c.visible : cap;
+    if (visible && n)
+        SICP_HIP(hipMemcpy(visible, map->visible, n * sizeof(sicp_block), hipMemcpyDeviceToHost));
+    return SICP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// map_common.hpp — device-side voxel-block hash shared by the allocation,
+// fusion and raycast kernels.
+#pragma once
+#include "sicp_kernels.hpp"
+
+namespace sicp {
+
+// DeviceMap::Hash (Mapping/DeviceMap.cu:3-10): 32-bit wrapping products, XOR,
+// and a remainder taken on the unsigned value (the int XOR converts to the
+// unsigned NumBuckets type before `%`).
+SICP_DEV uint32_t bucket_of(int x, int y, int z, const MapDev& m) {
+    uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349669u) ^ ((uint32_t)z * 83492791u);
+    return m.nb_mask ? (h & m.nb_mask) : (h % m.nb);
+}
+
+SICP_DEV bool key_in_range(int x, int y, int z) {
+    const int lim = 1 << 20;
+    return x >= -lim && x < lim && y >= -lim && y < lim && z >= -lim && z < lim;
+}
+
+SICP_DEV uint64_t pack_key(int x, int y, int z) {
+    const uint64_t msk = (1ull << 21) - 1;
+    return (((uint64_t)(uint32_t)x & msk) << 42) | (((uint64_t)(uint32_t)y & msk) << 21) |
+           ((uint64_t)(uint32_t)z & msk);
+}
+
+SICP_DEV i3 unpack_key(uint64_t k) {
+    auto sx = [](uint64_t v) { return (int)((int64_t)(v << 43) >> 43); };
+    return i3{sx(k >> 42), sx(k >> 21), sx(k)};
+}
+
+// voxelPosToBlockPos (DeviceMap.cu:169-180).
+SICP_DEV i3 voxel_to_block(i3 v) {
+    if (v.x < 0) v.x -= kBlockSide - 1;
+    if (v.y < 0) v.y -= kBlockSide - 1;
+    if (v.z < 0) v.z -= kBlockSide - 1;
+    return i3{v.x / kBlockSide, v.y / kBlockSide, v.z / kBlockSide};
+}
+
+// voxelPosToLocalIdx (DeviceMap.cu:186-201,218-220).
+SICP_DEV int voxel_local_idx(i3 v) {
+    int x = v.x % kBlockSide, y = v.y % kBlockSide, z = v.z % kBlockSide;
+    if (x < 0) x += kBlockSide;
+    if (y < 0) y += kBlockSide;
+    if (z < 0) z += kBlockSide;
+    return z * kBlockSide * kBlockSide + y * kBlockSide + x;
+}
+
+struct EntryView {
+    uint64_t key;
+    int slot, next;
+};
+SICP_DEV EntryView load_entry(const sicp_hash_entry* p) {
+    uint4 w = *(const uint4*)p;
+    return EntryView{(uint64_t)w.x | ((uint64_t)w.y << 32), (int)w.z, (int)w.w};
+}
+
+// FindEntry (DeviceMap.cu:141-154) over the two-entry bucket (one 32-byte
+// read) and its overflow chain. Returns the heap slot or -1.
+SICP_DEV int map_lookup(const MapDev& m, int x, int y, int z) {
+    if (!key_in_range(x, y, z)) return -1;
+    const uint32_t b = bucket_of(x, y, z, m);
+    const uint64_t key = pack_key(x, y, z);
+    EntryView e0 = load_entry(m.table + 2 * (size_t)b);
+    EntryView e1 = load_entry(m.table + 2 * (size_t)b + 1);
+    if (e0.slot >= 0 && e0.key == key) return e0.slot;
+    if (e1.slot >= 0 && e1.key == key) return e1.slot;
+    int n = e1.next;
+    while (n >= 0) {
+        EntryView e = load_entry(m.excess + n);
+        if (e.key == key) return e.slot;
+        n = e.next;
+    }
+    return -1;
+}
+
+}  // namespace sicp

@@ -1,0 +1,45 @@
+// sicp_internal.hpp — host-side objects behind the opaque C ABI handles.
+#pragma once
+
+#include <vector>
+
+#include "sicp_kernels.hpp"
+
+struct sicp_map {
+    sicp_map_config cfg;
+    sicp::MapDev dev;
+    sicp::MapCounters* ctr = nullptr;  // device
+    sicp::AllocScratch alloc{};
+    sicp_block* visible = nullptr;     // device, num_blocks
+    std::vector<void*> owned;          // every device allocation of the map
+};
+
+namespace sicp {
+
+int map_alloc(const sicp_map_config& cfg, sicp_map** out);
+void map_free(sicp_map* m);
+int launch_raycast_gated(MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin,
+                         View<float> zmax, PoseArg pose, float invfx, float invfy, float cx,
+                         float cy, const int* gate, const uint32_t* nonempty, hipStream_t s);
+
+// A pitched device image owned by the library.
+template <class T> struct DevImage {
+    void* ptr = nullptr;
+    size_t pitch = 0;
+    int cols = 0, rows = 0;
+    int create(int c, int r, size_t elem = sizeof(T)) {
+        cols = c;
+        rows = r;
+        hipError_t e = hipMallocPitch(&ptr, &pitch, (size_t)c * elem, (size_t)(r > 0 ? r : 1));
+        if (e != hipSuccess) return hip_fail(e, "hipMallocPitch");
+        return SICP_OK;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+    }
+    View<T> v() const { return View<T>{(char*)ptr, pitch, cols, rows}; }
+    sicp_image img() const { return sicp_image{ptr, pitch, cols, rows}; }
+};
+
+}  // namespace sicp

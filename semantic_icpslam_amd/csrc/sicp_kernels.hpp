@@ -1,0 +1,142 @@
+// sicp_kernels.hpp — internal launch API shared by the HIP translation
+// units, the C ABI (abi.cpp) and the per-frame driver (system.cpp).
+#pragma once
+
+#include "sicp_common.hpp"
+
+namespace sicp {
+
+// ------------------------------------------------------------- pyramid ----
+int launch_filter_depth(View<uint16_t> depth, View<float> raw, View<float> filtered,
+                        float scale_inv, float cutoff, hipStream_t s);
+int launch_vmap(View<float> depth, View<float4> vmap, float invfx, float invfy, float cx,
+                float cy, float cutoff, hipStream_t s);
+int launch_nmap(View<float4> vmap, View<float4> nmap, hipStream_t s);
+int launch_pyrdown_f32(View<float> src, View<float> dst, hipStream_t s);
+int launch_pyrdown_u8(View<uint8_t> src, View<uint8_t> dst, hipStream_t s);
+int launch_intensity(View<uint8_t> rgb, View<uint8_t> gray, hipStream_t s);
+int launch_derivative(View<uint8_t> img, View<int16_t> dx, View<int16_t> dy, hipStream_t s);
+int launch_resize_map(View<float4> vsrc, View<float4> nsrc, View<float4> vdst, View<float4> ndst,
+                      hipStream_t s);
+
+// ---------------------------------------------------------------- ICP -----
+constexpr int kIcpThreads = 256;
+constexpr int kIcpMaxBlocks = 1024;
+constexpr int kIcpSums = 29;
+
+// Pose and Gauss-Newton state of one frame's tracking, resident on the device
+// so every iteration runs without a host round trip.
+struct FrameMeta {
+    double pose[16];   // camera -> world, row-major (Frame::pose, Core/Frame.h)
+    sicp_pose_f pose_f;
+};
+
+struct TrackState {
+    double last_pose[16];
+    double next_pose[16];
+    double pose0[16];
+    double delta[16];
+    double candidate[16];
+    double A[36], b[6], x[6];
+    sicp_pose_f curr_f, last_f;
+    float sums[32];
+    float icp_error, last_error, candidate_icp, thresh;
+    int icp_count, break_count, applied, done;
+    int failed, tracked, frame_index, pad;
+    unsigned counter;  // last-workgroup arrival counter of the reduction
+    unsigned pad2[3];
+};
+
+struct IcpScratch {
+    double* partials;  // kIcpMaxBlocks * kIcpSums
+    unsigned* counter; // 1, zero between launches
+};
+
+// ICPStep: the 29 sums of one level into `sums_out` (device).
+int launch_icp_sums(View<float4> vcurr, View<float4> vlast, View<float4> ncurr,
+                    View<float4> nlast, PoseArg curr, PoseArg last, sicp_intrinsics K,
+                    IcpScratch scratch, float* sums_out, hipStream_t s);
+// One Gauss-Newton iteration of ComputeSE3 driven by `st`.
+int launch_icp_iterate(View<float4> vcurr, View<float4> vlast, View<float4> ncurr,
+                       View<float4> nlast, sicp_intrinsics K, int min_count, TrackState* st,
+                       double* partials, hipStream_t s);
+int launch_track_init(FrameMeta* next, FrameMeta* last, TrackState* st, hipStream_t s);
+int launch_track_begin(FrameMeta* next, FrameMeta* last, TrackState* st, float thresh,
+                       hipStream_t s);
+int launch_track_end(FrameMeta* next, TrackState* st, hipStream_t s);
+// When the frame was not tracked, copy the last frame's level-0 model maps
+// into `next` so the host-side frame swap keeps the last model.
+int launch_rollback_maps(View<float4> vlast, View<float4> nlast, View<float4> vnext,
+                         View<float4> nnext, const TrackState* st, hipStream_t s);
+
+// Host FP64 solve + SE(3) update (same code path as the device).
+void se3_update_host(const double* A, const double* b, double* delta, const double* last_pose,
+                     double* next_pose, double* x_out);
+
+// ---------------------------------------------------------------- map -----
+struct MapDev {
+    sicp_hash_entry* table;   // 2 * nb
+    sicp_hash_entry* excess;  // num_blocks, indexed by heap slot
+    sicp_block* blocks;       // num_blocks
+    sicp_voxel_geom* geom;    // num_blocks * 512
+    sicp_voxel_sem* sem;      // num_blocks * 512
+    uint32_t nb, nb_mask;     // nb_mask = nb-1 when nb is a power of two, else 0
+    int num_blocks;
+    float voxel, voxel_inv, trunc, step_scale, block_width, dmin, dmax;
+    int max_weight, max_count;
+};
+
+struct MapCounters {
+    uint32_t heap_used;
+    uint32_t visible;
+    uint32_t new_total;      // distinct new blocks requested this frame
+    uint32_t new_blocks;     // of which created
+    uint32_t dropped_heap;
+    uint32_t dropped_requests;
+    uint32_t dropped_range;
+    uint32_t updated_voxels;
+    uint32_t render_tiles;
+    uint32_t pad0;
+    unsigned long long updated_total;  // voxels written, summed since reset
+    unsigned long long visible_total;  // visible blocks, summed since reset
+    unsigned long long fusions;        // fusion passes since reset
+};
+
+struct AllocScratch {
+    uint64_t* req_keys;        // request set keys (EMPTY = ~0)
+    unsigned long long* req_first; // first requester id per request-set slot
+    uint32_t req_mask;         // capacity - 1 (power of two)
+    unsigned long long* pix_mask;  // per pixel: steps that missed the table
+    uint32_t* pix_count;       // per pixel: new blocks first seen there
+    uint32_t* pix_base;        // exclusive scan of pix_count
+    uint32_t* scan_partials;   // scan workspace
+    uint64_t* new_keys;        // rank -> key
+    uint32_t* new_req_slot;    // rank -> request-set slot
+    int32_t* bucket_leader;    // per bucket, smallest rank this frame (INT_MAX idle)
+    int32_t* bucket_head;      // per bucket, pending list head (-1 idle)
+    int32_t* pending_next;     // rank -> next pending rank
+    uint8_t* vis_flags;        // per heap slot
+    uint32_t cap_pixels;
+};
+
+int launch_map_reset(MapDev m, MapCounters* ctr, AllocScratch a, hipStream_t s);
+int launch_allocate(MapDev m, MapCounters* ctr, AllocScratch a, View<float> depth, PoseArg pose,
+                    sicp_intrinsics K, const int* gate, hipStream_t s);
+int launch_visibility(MapDev m, MapCounters* ctr, AllocScratch a, sicp_block* visible,
+                      PoseArg pose, sicp_intrinsics K, int cols, int rows, float dmax,
+                      float dmin, const int* gate, hipStream_t s);
+int launch_fuse(MapDev m, MapCounters* ctr, const sicp_block* visible, View<float> depth,
+                View<uint8_t> rgb, View<float4> nmap, View<uint8_t> mask, int num_detection,
+                PoseArg pose, sicp_intrinsics K, const int* gate, hipStream_t s);
+int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible, View<float> zmin,
+                          View<float> zmax, PoseArg pose, sicp_intrinsics K, float dmax,
+                          float dmin, const int* gate, hipStream_t s);
+int launch_raycast(MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin,
+                   View<float> zmax, PoseArg pose, float invfx, float invfy, float cx, float cy,
+                   const int* gate, hipStream_t s);
+
+// Device exclusive scan of `n_dev` (device count, at most `cap`) u32 values.
+int launch_scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t cap,
+                    uint32_t* partials, uint32_t* total_out, hipStream_t s);
+
+}  // namespace sicp

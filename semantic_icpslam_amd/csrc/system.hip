@@ -1,0 +1,376 @@
+// system.hip — the per-frame driver: the C++ counterpart of the reference's
+// Frame / Tracker / Mapping / System callers of the hot path
+// (Core/Frame.cc:33-98, Tracking/Tracking.cc:65-119,157-173,246-280,369-517,
+// Mapping/Mapping.cc:76-97,142-161, Core/System.cc:131-172).
+//
+// A frame is one ordered sequence of launches on one stream with no host
+// synchronisation: the poses, the Gauss-Newton state, the visible count and
+// the tracked / lost decision all stay in device memory, and the fusion /
+// raycast kernels read the decision from there. The host only swaps its
+// frame pointers, unconditionally; when a frame is lost, a device-side copy
+// puts the last model maps and pose into the frame that becomes "last", so
+// the next frame tracks against the same model as in the reference, where
+// the swap is skipped.
+#include <cstring>
+
+#include "se3_math.hpp"
+#include "sicp_internal.hpp"
+
+namespace sicp {
+
+struct FrameBuf {
+    int cols[3], rows[3];
+    DevImage<float> range;  // raw depth in metres, level 0 (Frame::range)
+    DevImage<float> depth[3];
+    DevImage<float4> vmap[3], nmap[3];
+    DevImage<uint8_t> image[3];
+    DevImage<int16_t> dx[3], dy[3];
+    FrameMeta* meta = nullptr;  // device
+
+    int create(int c, int r) {
+        int rc;
+        if ((rc = range.create(c, r))) return rc;
+        for (int i = 0; i < 3; ++i) {
+            cols[i] = c / (1 << i);
+            rows[i] = r / (1 << i);
+            if ((rc = depth[i].create(cols[i], rows[i])) || (rc = vmap[i].create(cols[i], rows[i])) ||
+                (rc = nmap[i].create(cols[i], rows[i])) || (rc = image[i].create(cols[i], rows[i])) ||
+                (rc = dx[i].create(cols[i], rows[i])) || (rc = dy[i].create(cols[i], rows[i])))
+                return rc;
+        }
+        hipError_t e = hipMalloc((void**)&meta, sizeof(FrameMeta));
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(FrameMeta)");
+        FrameMeta h{};
+        set_identity(h.pose);
+        pose_to_float(h.pose, &h.pose_f);
+        e = hipMemcpy(meta, &h, sizeof(h), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpy(FrameMeta)");
+        return SICP_OK;
+    }
+    void release() {
+        range.release();
+        for (int i = 0; i < 3; ++i) {
+            depth[i].release();
+            vmap[i].release();
+            nmap[i].release();
+            image[i].release();
+            dx[i].release();
+            dy[i].release();
+        }
+        if (meta) (void)hipFree(meta);
+        meta = nullptr;
+    }
+};
+
+}  // namespace sicp
+
+struct sicp_system {
+    sicp_system_config cfg;
+    sicp_map* map = nullptr;
+    sicp::FrameBuf fa, fb;
+    sicp::FrameBuf* next = &fa;
+    sicp::FrameBuf* last = &fb;
+    sicp::TrackState* st = nullptr;
+    double* partials = nullptr;
+    sicp::DevImage<float> zmin, zmax;
+    sicp_intrinsics K[3];
+    int frame_index = 0;
+    // Stage timing: when enabled, every stage of every frame is bracketed by
+    // a pair of events recorded on the frame's stream.
+    bool timing = false;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    std::vector<std::pair<int, size_t>> marks;  // (stage, index of the start event)
+};
+
+namespace sicp {
+
+enum Stage { kStagePyramid = 0, kStageTrack, kStageAlloc, kStageVisible, kStageFuse,
+             kStagePrepass, kStageRaycast, kStageCount };
+
+// Returns the index of the start event, or SIZE_MAX when timing is off.
+static size_t stage_begin(sicp_system* s, hipStream_t st) {
+    if (!s->timing) return (size_t)-1;
+    while (s->pool.size() < s->pool_used + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return (size_t)-1;
+        s->pool.push_back(e);
+    }
+    size_t i = s->pool_used;
+    s->pool_used += 2;
+    (void)hipEventRecord(s->pool[i], st);
+    return i;
+}
+
+static void stage_end(sicp_system* s, int stage, size_t i, hipStream_t st) {
+    if (i == (size_t)-1) return;
+    (void)hipEventRecord(s->pool[i + 1], st);
+    s->marks.push_back({stage, i});
+}
+
+// Frame::FillImages (Core/Frame.cc:68-92).
+static int fill_images(sicp_system* s, FrameBuf& f, const sicp_image& depth_u16,
+                       const sicp_image& rgb, hipStream_t st) {
+    int rc;
+    const float cutoff = s->cfg.depth_cutoff;
+    if ((rc = launch_filter_depth(view<uint16_t>(depth_u16), f.range.v(), f.depth[0].v(),
+                                  (float)(1.0 / (double)s->cfg.depth_scale), cutoff, st)))
+        return rc;
+    if ((rc = launch_intensity(view<uint8_t>(rgb), f.image[0].v(), st))) return rc;
+    for (int i = 1; i < 3; ++i) {
+        if ((rc = launch_pyrdown_f32(f.depth[i - 1].v(), f.depth[i].v(), st))) return rc;
+        if ((rc = launch_pyrdown_u8(f.image[i - 1].v(), f.image[i].v(), st))) return rc;
+    }
+    for (int i = 0; i < 3; ++i) {
+        const sicp_intrinsics& k = s->K[i];
+        if ((rc = launch_vmap(f.depth[i].v(), f.vmap[i].v(), (float)(1.0 / (double)k.fx),
+                              (float)(1.0 / (double)k.fy), k.cx, k.cy, cutoff, st)))
+            return rc;
+        if ((rc = launch_nmap(f.vmap[i].v(), f.nmap[i].v(), st))) return rc;
+        if ((rc = launch_derivative(f.image[i].v(), f.dx[i].v(), f.dy[i].v(), st))) return rc;
+    }
+    return SICP_OK;
+}
+
+// Frame::ResizeImages (Core/Frame.cc:94-98).
+static int resize_images(FrameBuf& f, hipStream_t st) {
+    for (int i = 1; i < 3; ++i) {
+        int rc = launch_resize_map(f.vmap[i - 1].v(), f.nmap[i - 1].v(), f.vmap[i].v(),
+                                   f.nmap[i].v(), st);
+        if (rc) return rc;
+    }
+    return SICP_OK;
+}
+
+static void system_free(sicp_system* s) {
+    if (!s) return;
+    for (hipEvent_t e : s->pool) (void)hipEventDestroy(e);
+    s->fa.release();
+    s->fb.release();
+    s->zmin.release();
+    s->zmax.release();
+    if (s->st) (void)hipFree(s->st);
+    if (s->partials) (void)hipFree(s->partials);
+    map_free(s->map);
+    delete s;
+}
+
+}  // namespace sicp
+
+using namespace sicp;
+
+extern "C" {
+
+void sicp_system_config_default(sicp_system_config* cfg, int cols, int rows, float fx, float fy,
+                                float cx, float cy, float voxel_size, float truncation,
+                                int32_t num_buckets, int32_t num_blocks) {
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->cols = cols;
+    cfg->rows = rows;
+    cfg->K = sicp_intrinsics{fx, fy, cx, cy};
+    cfg->depth_scale = 1000.f;  // Frame::mDepthScale (Core/Frame.cc:20)
+    cfg->depth_cutoff = 3.0f;   // Frame::mDepthCutoff (Core/Frame.cc:19)
+    cfg->icp_iters[0] = 10;     // ITERATIONS_SE3 (Tracking.h:110)
+    cfg->icp_iters[1] = 5;
+    cfg->icp_iters[2] = 3;
+    cfg->icp_min_count[0] = 2000;  // MIN_ICP_COUNT (Tracking.h:112)
+    cfg->icp_min_count[1] = 1000;
+    cfg->icp_min_count[2] = 100;
+    cfg->icp_thresh = 1e-4f;    // THRESH_ICP_SE3 (Tracking.h:117)
+    sicp_map_config_default(&cfg->map, voxel_size, truncation, num_buckets, num_blocks);
+    cfg->map.max_cols = cols;
+    cfg->map.max_rows = rows;
+}
+
+int sicp_system_create(const sicp_system_config* cfg, sicp_system** out) {
+    if (!cfg || !out) return invalid("system_create: null argument");
+    if (cfg->cols < 8 || cfg->rows < 8) return invalid("system_create: image too small");
+    if ((size_t)cfg->cols * cfg->rows > (size_t)cfg->map.max_cols * cfg->map.max_rows)
+        return invalid("system_create: map max_cols x max_rows smaller than the image");
+    sicp_system* s = new sicp_system();
+    s->cfg = *cfg;
+    int rc = map_alloc(cfg->map, &s->map);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    if ((rc = s->fa.create(cfg->cols, cfg->rows)) || (rc = s->fb.create(cfg->cols, cfg->rows)) ||
+        (rc = s->zmin.create((cfg->cols + 7) / 8, (cfg->rows + 7) / 8)) ||
+        (rc = s->zmax.create((cfg->cols + 7) / 8, (cfg->rows + 7) / 8))) {
+        system_free(s);
+        return rc;
+    }
+    hipError_t e = hipMalloc((void**)&s->st, sizeof(TrackState));
+    if (e == hipSuccess) e = hipMemset(s->st, 0, sizeof(TrackState));
+    if (e == hipSuccess) e = hipMalloc((void**)&s->partials, sizeof(double) * kIcpMaxBlocks * kIcpSums);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "system_create");
+        system_free(s);
+        return rc;
+    }
+    for (int i = 0; i < 3; ++i) {
+        // Frame::SetK (Core/Frame.cc:580-588): K / 2^i in float.
+        float d = (float)(1 << i);
+        s->K[i] = sicp_intrinsics{cfg->K.fx / d, cfg->K.fy / d, cfg->K.cx / d, cfg->K.cy / d};
+    }
+    *out = s;
+    return SICP_OK;
+}
+
+int sicp_system_destroy(sicp_system* sys) {
+    system_free(sys);
+    return SICP_OK;
+}
+
+sicp_map* sicp_system_map(sicp_system* sys) { return sys ? sys->map : nullptr; }
+
+// System::GrabImage (Core/System.cc:131-172) with Tracker::GrabFrame /
+// Track / TrackFrame / ComputeSE3 (Tracking.cc:65-119,157-173,246-280,369-517).
+int sicp_system_grab_image(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
+                           sicp_image labels_u8, int num_detection, void* stream) {
+    if (!s) return invalid("system_grab_image: null system");
+    const int c = s->cfg.cols, r = s->cfg.rows;
+    if (!image_ok(depth_u16, 2, c, r) || !image_ok(rgb_u8x3, 3, c, r))
+        return invalid("system_grab_image: depth / rgb shape mismatch");
+    const bool labels = num_detection > 0 && labels_u8.ptr;
+    if (labels && !image_ok(labels_u8, 1, c, r))
+        return invalid("system_grab_image: label image shape mismatch");
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    size_t ev;
+    // Tracker::GrabFrame: model pyramid of the last frame, then the new frame.
+    ev = stage_begin(s, st);
+    if ((rc = resize_images(*s->last, st))) return rc;
+    if ((rc = fill_images(s, *s->next, depth_u16, rgb_u8x3, st))) return rc;
+    stage_end(s, kStagePyramid, ev, st);
+    FrameBuf& nf = *s->next;
+    FrameBuf& lf = *s->last;
+    ev = stage_begin(s, st);
+    if (s->frame_index == 0) {
+        // InitTracking (Tracking.cc:233-240): both poses identity.
+        if ((rc = launch_track_init(nf.meta, lf.meta, s->st, st))) return rc;
+    } else {
+        if ((rc = launch_track_begin(nf.meta, lf.meta, s->st, s->cfg.icp_thresh, st))) return rc;
+        for (int l = 2; l >= 0; --l) {
+            for (int j = 0; j < s->cfg.icp_iters[l]; ++j) {
+                rc = launch_icp_iterate(nf.vmap[l].v(), lf.vmap[l].v(), nf.nmap[l].v(),
+                                        lf.nmap[l].v(), s->K[l], s->cfg.icp_min_count[l], s->st,
+                                        s->partials, st);
+                if (rc) return rc;
+            }
+        }
+        if ((rc = launch_track_end(nf.meta, s->st, st))) return rc;
+        if ((rc = launch_rollback_maps(lf.vmap[0].v(), lf.nmap[0].v(), nf.vmap[0].v(),
+                                       nf.nmap[0].v(), s->st, st)))
+            return rc;
+    }
+    stage_end(s, kStageTrack, ev, st);
+    std::swap(s->next, s->last);  // SwapFrame (Tracking.cc:318-320)
+    s->frame_index++;
+    FrameBuf& f = *s->last;
+    const int* gate = &s->st->tracked;
+    PoseArg pose = make_pose_arg(nullptr, &f.meta->pose_f);
+    sicp_map* m = s->map;
+    // Mapping::FuseColor (Mapping.cc:76-97) -> FuseMapColor (FuseMap.cu:1039-1162)
+    ev = stage_begin(s, st);
+    if ((rc = launch_allocate(m->dev, m->ctr, m->alloc, f.range.v(), pose, s->K[0], gate, st)))
+        return rc;
+    stage_end(s, kStageAlloc, ev, st);
+    ev = stage_begin(s, st);
+    if ((rc = launch_visibility(m->dev, m->ctr, m->alloc, m->visible, pose, s->K[0], c, r,
+                               m->cfg.depth_max, m->cfg.depth_min, gate, st)))
+        return rc;
+    stage_end(s, kStageVisible, ev, st);
+    View<uint8_t> mask = labels ? view<uint8_t>(labels_u8) : View<uint8_t>{};
+    ev = stage_begin(s, st);
+    if ((rc = launch_fuse(m->dev, m->ctr, m->visible, f.range.v(), view<uint8_t>(rgb_u8x3),
+                          f.nmap[0].v(), mask, labels ? num_detection : 0, pose, s->K[0], gate,
+                          st)))
+        return rc;
+    stage_end(s, kStageFuse, ev, st);
+    // Mapping::RayTrace (Mapping.cc:142-161): prepass, then the raycast into
+    // the frame's level-0 model maps when any block produced a tile.
+    ev = stage_begin(s, st);
+    if ((rc = launch_render_prepass(m->dev, m->ctr, m->visible, s->zmin.v(), s->zmax.v(), pose,
+                                    s->K[0], m->cfg.depth_max, m->cfg.depth_min, gate, st)))
+        return rc;
+    stage_end(s, kStagePrepass, ev, st);
+    ev = stage_begin(s, st);
+    rc = launch_raycast_gated(m->dev, f.vmap[0].v(), f.nmap[0].v(), s->zmin.v(), s->zmax.v(),
+                              pose, (float)(1.0 / (double)s->K[0].fx),
+                              (float)(1.0 / (double)s->K[0].fy), s->K[0].cx, s->K[0].cy, gate,
+                              &m->ctr->render_tiles, st);
+    stage_end(s, kStageRaycast, ev, st);
+    return rc;
+}
+
+// Stage timing of the frames grabbed since it was enabled: per stage, the
+// summed event time in ms and the number of timed instances. Disables and
+// clears the recorded marks. Stages: 0 pyramid, 1 track, 2 allocate,
+// 3 visibility, 4 fuse, 5 render prepass, 6 raycast.
+int sicp_system_set_timing(sicp_system* s, int enable) {
+    if (!s) return invalid("system_set_timing: null system");
+    s->timing = enable != 0;
+    s->marks.clear();
+    s->pool_used = 0;
+    return SICP_OK;
+}
+
+int sicp_system_read_timing(sicp_system* s, double* total_ms, int* counts, int n_stages,
+                            void* stream) {
+    if (!s || !total_ms || !counts) return invalid("system_read_timing: null argument");
+    SICP_HIP(hipStreamSynchronize((hipStream_t)stream));
+    for (int i = 0; i < n_stages; ++i) {
+        total_ms[i] = 0.0;
+        counts[i] = 0;
+    }
+    for (auto& mk : s->marks) {
+        float ms = 0.f;
+        SICP_HIP(hipEventElapsedTime(&ms, s->pool[mk.second], s->pool[mk.second + 1]));
+        if (mk.first < n_stages) {
+            total_ms[mk.first] += ms;
+            counts[mk.first] += 1;
+        }
+    }
+    s->marks.clear();
+    s->pool_used = 0;
+    return SICP_OK;
+}
+
+int sicp_system_last_result(sicp_system* s, sicp_frame_result* out, void* stream) {
+    if (!s || !out) return invalid("system_last_result: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    TrackState t;
+    MapCounters c;
+    FrameMeta fm;
+    SICP_HIP(hipMemcpyAsync(&t, s->st, sizeof(t), hipMemcpyDeviceToHost, st));
+    SICP_HIP(hipMemcpyAsync(&c, s->map->ctr, sizeof(c), hipMemcpyDeviceToHost, st));
+    SICP_HIP(hipMemcpyAsync(&fm, s->last->meta, sizeof(fm), hipMemcpyDeviceToHost, st));
+    SICP_HIP(hipStreamSynchronize(st));
+    std::memset(out, 0, sizeof(*out));
+    out->frame_index = s->frame_index - 1;
+    out->tracked = t.tracked;
+    out->icp_count = t.icp_count;
+    out->iterations = t.applied;
+    out->icp_error = t.icp_error;
+    out->visible_blocks = c.visible;
+    out->allocated_blocks = c.heap_used;
+    std::memcpy(out->pose, fm.pose, sizeof(out->pose));
+    return SICP_OK;
+}
+
+int sicp_system_frame_image(sicp_system* s, int which, int level, sicp_image* out) {
+    if (!s || !out || level < 0 || level > 2) return invalid("system_frame_image: bad argument");
+    FrameBuf& f = *s->last;
+    switch (which) {
+        case 0: *out = f.vmap[level].img(); break;
+        case 1: *out = f.nmap[level].img(); break;
+        case 2: *out = f.range.img(); break;
+        case 3: *out = f.depth[0].img(); break;
+        default: return invalid("system_frame_image: unknown image");
+    }
+    return SICP_OK;
+}
+
+}  // extern "C"

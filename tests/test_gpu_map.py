@@ -1,0 +1,165 @@
+"""Parity of the map path (Mapping/DeviceMap.cu, FuseMap.cu, RenderScene.cu)
+against the oracle, with the same poses fed to both sides.
+
+Bit-exact: the hash table (both bucket entries and the overflow chains), the
+block list, every voxel record (sdf bits, weight, colour, labels, counts),
+the visible list, the per-tile depth ranges and the raycast vertex / normal
+maps (NaN patterns included).
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import (B, CFG_SMALL, CFG_VGA, K, bits_equal, empty_dev, host, map_cfg,
+                     mismatch_report, synth, to_dev)
+from oracle import binding as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(c, s, i):
+    d, rgb, lab = s.render(i)
+    raw, filt = O.filter_depth(d, 1000.0, 3.0)
+    v = O.compute_vmap(filt, c["f"], c["f"], c["cx"], c["cy"], 3.0)
+    n = O.compute_nmap(v)
+    return raw, rgb, n, lab
+
+
+def _fuse_both(dev, c, frames, gm, om, labels=True, seed=0):
+    s = synth(c, seed=seed)
+    vis = []
+    for i in frames:
+        raw, rgb, n, lab = _inputs(c, s, i)
+        pose = B.pose_f(s.pose(i))
+        nd = 1 if labels else 0
+        vh = B.C.c_uint32(0)
+        # keep the device tensors referenced until the kernels have run
+        tr = [to_dev(a, dev) for a in (raw, rgb, n, lab)]
+        B.check(B.lib().sicp_fuse_map_color(gm.h, *[B.image(t) for t in tr], nd, B.C.byref(pose),
+                                            None, K(c), B.C.byref(vh), B.stream_ptr()))
+        ov = om.fuse(raw, rgb, n, lab, nd, pose, K(c))
+        assert vh.value == ov
+        vis.append(ov)
+    return s, vis
+
+
+def _compare_maps(gm, om):
+    gs, os_ = gm.stats(), om.stats()
+    n = os_["allocated_blocks"]
+    assert gs["allocated_blocks"] == n
+    g = gm.download(n_blocks=0)
+    o = om.export()
+    assert np.array_equal(g["table"], o["table"]), "hash table differs"
+    assert np.array_equal(g["excess"][:n], o["excess"][:n]), "overflow chains differ"
+    assert np.array_equal(g["blocks"][:n], o["blocks"][:n]), "block list differs"
+    gg, og = g["geom"][: n * 512], o["geom"][: n * 512]
+    assert bits_equal(gg["sdf"], og["sdf"]), mismatch_report(gg["sdf"], og["sdf"])
+    for f in ("weight", "r", "g", "b"):
+        assert np.array_equal(gg[f], og[f]), f
+    gsm, osm = g["sem"][: n * 512], o["sem"][: n * 512]
+    for f in ("count", "count2", "label", "label2", "r2", "g2", "b2"):
+        assert np.array_equal(gsm[f], osm[f]), f
+    assert np.array_equal(gm.visible(), om.visible())
+    assert gs["dropped_heap"] == os_["dropped_heap"]
+    assert gs["dropped_range"] == os_["dropped_range"]
+    assert gs["updated_voxels"] == os_["updated_voxels"]
+    return n
+
+
+@pytest.mark.parametrize("labels", [True, False])
+def test_fusion_small_bit_exact(dev, labels):
+    c = CFG_SMALL
+    cfg = map_cfg(c)
+    gm, om = B.Map(cfg), O.OracleMap(cfg)
+    _fuse_both(dev, c, range(0, 12, 2), gm, om, labels=labels)
+    n = _compare_maps(gm, om)
+    assert n > 500
+
+
+def test_fusion_vga_bit_exact(dev):
+    c = CFG_VGA
+    cfg = map_cfg(c)
+    gm, om = B.Map(cfg), O.OracleMap(cfg)
+    _fuse_both(dev, c, [0, 3], gm, om)
+    n = _compare_maps(gm, om)
+    assert n > 2000
+
+
+def test_fusion_overflow_chains_and_heap_exhaustion(dev):
+    """A 2^8-bucket table forces long overflow chains; a 700-block heap runs
+    out, and both sides must drop the same requests and count them."""
+    c = CFG_SMALL
+    cfg = map_cfg(c, num_buckets=1 << 8, num_blocks=700)
+    gm, om = B.Map(cfg), O.OracleMap(cfg)
+    _fuse_both(dev, c, [0, 5], gm, om)
+    _compare_maps(gm, om)
+    assert om.stats()["dropped_heap"] > 0
+    assert (om.export()["excess"]["slot"] >= 0).sum() > 100
+
+
+def test_non_power_of_two_buckets(dev):
+    c = CFG_SMALL
+    cfg = map_cfg(c, num_buckets=1000003 % 200000 + 100000)
+    gm, om = B.Map(cfg), O.OracleMap(cfg)
+    _fuse_both(dev, c, [0, 4], gm, om)
+    _compare_maps(gm, om)
+
+
+def test_label_count_cap(dev):
+    """max_label_count = 255 reproduces the reference's min(255, count + 1)."""
+    c = dict(CFG_SMALL, cols=160, rows=120, f=131.25, cx=79.5, cy=59.5)
+    cfg = map_cfg(c, max_label_count=255)
+    gm, om = B.Map(cfg), O.OracleMap(cfg)
+    s = synth(c, noise=False, dropout=0.0)
+    raw, rgb, n, lab = _inputs(c, s, 0)
+    pose = B.pose_f(np.eye(4))
+    tr = [to_dev(a, dev) for a in (raw, rgb, n, lab)]
+    for _ in range(300):
+        B.check(B.lib().sicp_fuse_map_color(gm.h, *[B.image(t) for t in tr], 1, B.C.byref(pose),
+                                            None, K(c), None, B.stream_ptr()))
+        om.fuse(raw, rgb, n, lab, 1, pose, K(c))
+    _compare_maps(gm, om)
+    sem = om.export()["sem"]
+    assert sem["count"].max() == 255
+
+
+def test_empty_frame_no_visible(dev):
+    c = CFG_SMALL
+    cfg = map_cfg(c)
+    gm = B.Map(cfg)
+    raw = np.zeros((c["rows"], c["cols"]), np.float32)
+    rgb = np.zeros((c["rows"], c["cols"], 3), np.uint8)
+    pose = B.pose_f(np.eye(4))
+    vh = B.C.c_uint32(7)
+    rt, ct = to_dev(raw, dev), to_dev(rgb, dev)
+    rc = B.lib().sicp_fuse_map_color(gm.h, B.image(rt), B.image(ct), B.Image(), B.Image(), 0,
+                                     B.C.byref(pose), None, K(c), B.C.byref(vh), B.stream_ptr())
+    assert rc == B.SICP_ERR_NO_VISIBLE and vh.value == 0
+
+
+@pytest.mark.parametrize("c", [CFG_SMALL, CFG_VGA])
+def test_prepass_and_raycast_bit_exact(dev, c):
+    cfg = map_cfg(c)
+    gm, om = B.Map(cfg), O.OracleMap(cfg)
+    s, _ = _fuse_both(dev, c, [0, 2, 4], gm, om)
+    pose = B.pose_f(s.pose(4))
+    zc, zr = (c["cols"] + 7) // 8, (c["rows"] + 7) // 8
+    zmin = empty_dev((zr, zc), torch.float32, dev)
+    zmax = empty_dev((zr, zc), torch.float32, dev)
+    ne = B.C.c_int(0)
+    B.check(B.lib().sicp_create_rendering_blocks(gm.h, B.image(zmin), B.image(zmax), 3.0, 0.1,
+                                                 B.C.byref(pose), None, K(c), B.C.byref(ne),
+                                                 B.stream_ptr()))
+    ozmin, ozmax, oany = om.rendering_blocks(zc, zr, 3.0, 0.1, pose, K(c))
+    assert ne.value == 1 and oany
+    assert bits_equal(host(zmin), ozmin), mismatch_report(host(zmin), ozmin)
+    assert bits_equal(host(zmax), ozmax), mismatch_report(host(zmax), ozmax)
+    v = empty_dev((c["rows"], c["cols"], 4), torch.float32, dev, 0, 0.0)
+    n = empty_dev((c["rows"], c["cols"], 4), torch.float32, dev, 0, 0.0)
+    B.check(B.lib().sicp_raycast(gm.h, B.image(v), B.image(n), B.image(zmin), B.image(zmax),
+                                 B.C.byref(pose), None, K(c), B.stream_ptr()))
+    ov, on = om.raycast(c["cols"], c["rows"], ozmin, ozmax, pose, K(c))
+    assert bits_equal(host(v), ov), mismatch_report(host(v), ov)
+    assert bits_equal(host(n), on), mismatch_report(host(n), on)
+    hit = ~np.isnan(ov[..., 0])
+    assert hit.mean() > 0.3
