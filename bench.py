@@ -102,9 +102,8 @@ def main():
     for i in range(args.warmup):
         sysg.grab(depth[i], rgb[i], lab[i], stream)
     torch.cuda.synchronize()
-    m = B.Map.__new__(B.Map)
-    m.h, m.cfg = sysg.map_handle(), cfg.map
-    st0 = m.stats()
+    map_h = sysg.map_handle()
+    st0 = B.map_stats(map_h, stream)
 
     # Timed region: K frames, stage events recorded on the same stream.
     B.check(B.lib().sicp_system_set_timing(sysg.h, 1))
@@ -120,7 +119,7 @@ def main():
     B.check(B.lib().sicp_system_read_timing(sysg.h, tot.ctypes.data, cnt.ctypes.data, len(STAGES),
                                             B.stream_ptr(stream)))
     B.check(B.lib().sicp_system_set_timing(sysg.h, 0))
-    st1 = m.stats()
+    st1 = B.map_stats(map_h, stream)
     res = sysg.result(stream)
     gt = syn.pose(n - 1)
     ms_per_frame = 1000.0 * elapsed / args.steps

@@ -230,6 +230,13 @@ def map_config(voxel_size, truncation, num_buckets, num_blocks, **kw):
     return cfg
 
 
+def map_stats(handle, stream=None):
+    """Counters of a map handle (owned by a Map or by a System)."""
+    s = MapStats()
+    check(lib().sicp_map_get_stats(handle, C.byref(s), stream_ptr(stream)), "map_get_stats")
+    return {n: getattr(s, n) for n, _ in MapStats._fields_}
+
+
 class Map:
     """Owner of one device map (the buffers Mapping::Create allocates)."""
 
@@ -254,9 +261,7 @@ class Map:
         check(lib().sicp_map_reset(self.h, stream_ptr(stream)), "sicp_map_reset")
 
     def stats(self, stream=None):
-        s = MapStats()
-        check(lib().sicp_map_get_stats(self.h, C.byref(s), stream_ptr(stream)), "map_get_stats")
-        return {n: getattr(s, n) for n, _ in MapStats._fields_}
+        return map_stats(self.h, stream)
 
     def download(self, n_blocks=0, voxels=True):
         cfg = self.cfg

@@ -105,7 +105,8 @@ SICP_DEV bool icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < kIcpThreads / 64; ++w) s += wave_sums[w][threadIdx.x];
-        partials[blockIdx.x * kIcpSums + threadIdx.x] = s;
+        // column-major: sum j of every workgroup is contiguous
+        partials[threadIdx.x * kIcpMaxBlocks + blockIdx.x] = s;
         __threadfence();
     }
     __shared__ int is_last;
@@ -117,15 +118,20 @@ SICP_DEV bool icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
     }
     __syncthreads();
     if (!is_last) return false;
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
-    if (threadIdx.x < kIcpSums) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // Wavefront w reduces sums w, w+4, ...: lane-strided partial sums over the
+    // workgroups, then a butterfly — a fixed order for a given grid size.
+    __shared__ double totals[kIcpSums];
+    const int nb = (int)gridDim.x;
+    for (int j = wave; j < kIcpSums; j += kIcpThreads / 64) {
         double s = 0.0;
-        for (int b = 0; b < (int)gridDim.x; ++b) {
-            s += __hip_atomic_load(&partials[b * kIcpSums + threadIdx.x], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        }
-        total = s;
+        for (int b = lane; b < nb; b += 64) s += partials[j * kIcpMaxBlocks + b];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) totals[j] = s;
     }
+    __syncthreads();
+    if (threadIdx.x < kIcpSums) total = totals[threadIdx.x];
     if (threadIdx.x == 0) *counter = 0u;  // ready for the next launch on the stream
     return true;
 }
@@ -267,8 +273,9 @@ __global__ __launch_bounds__(256) void rollback_maps_kernel(View<float4> vlast, 
     }
 }
 
+// About four pixels per thread, at most one workgroup per CU.
 inline int icp_blocks(int n) {
-    int b = (int)div_up(n, kIcpThreads * 2);
+    int b = (int)div_up(n, kIcpThreads * 4);
     return b < 1 ? 1 : (b > kIcpMaxBlocks ? kIcpMaxBlocks : b);
 }
 

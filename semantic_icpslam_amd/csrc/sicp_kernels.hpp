@@ -21,7 +21,7 @@ int launch_resize_map(View<float4> vsrc, View<float4> nsrc, View<float4> vdst, V
 
 // ---------------------------------------------------------------- ICP -----
 constexpr int kIcpThreads = 256;
-constexpr int kIcpMaxBlocks = 1024;
+constexpr int kIcpMaxBlocks = 256;
 constexpr int kIcpSums = 29;
 
 // Pose and Gauss-Newton state of one frame's tracking, resident on the device

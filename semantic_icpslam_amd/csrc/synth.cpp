@@ -8,7 +8,8 @@
 // sigma(z) = 0.0012 + 0.0019 (z - 0.4)^2 m, `dropout` of pixels set to 0.
 // RGB: value-noise texture of the 3D hit point (a 4 cm lattice, trilinear)
 // times a per-object colour. Labels: object k -> class k + 1, room -> 0.
-// Trajectory: smooth sinusoids, at most about 1 cm and 0.5 degree per frame.
+// Trajectory: smooth sinusoids, at most 1 cm and 0.5 degree per frame; the yaw
+// sweeps the camera from the front wall to the right wall over 300 frames.
 // Randomness: SplitMix64 streams seeded by (seed) for the scene and by
 // (seed, frame) for the per-frame noise; Gaussians by Box-Muller.
 #include <cmath>
@@ -63,7 +64,8 @@ void room_pose(int i, double* T) {
     double px = 0.5 * std::sin(a * i);
     double py = 0.5 + 0.15 * std::sin(2.0 * a * i);
     double pz = -0.5 + 0.2 * (1.0 - std::cos(a * i));
-    double yaw = 0.35 * std::sin(0.025 * i);
+    // yaw sweeps 0 -> 97 degrees over 300 frames at <= 0.5 degree per frame
+    double yaw = 0.85 * (1.0 - std::cos(6.283185307179586 * i / 600.0));
     double pitch = -0.25 + 0.1 * std::sin(0.05 * i + 1.0);
     double cy = std::cos(yaw), sy = std::sin(yaw), cp = std::cos(pitch), sp = std::sin(pitch);
     // R = Ry(yaw) * Rx(pitch)
