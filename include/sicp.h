@@ -172,9 +172,9 @@ int sicp_resize_map(sicp_image vsrc, sicp_image nsrc, sicp_image vdst, sicp_imag
 
 /* ICPStep (Tracking/Reduction.cu:328-385). Computes the 29 sums (27 upper
  * A|b entries, sum r^2, count) of the point-to-plane Gauss-Newton system into
- * `sums_dev` (29 floats, device). `partials_dev` is zero-initialised device
- * scratch of `sicp_icp_partials_size()` bytes (it must stay zeroed between
- * calls, which the call itself guarantees). When `host_sums` is non-NULL the call
+ * `sums_dev` (29 floats, device). `partials_dev` is device scratch of
+ * `sicp_icp_partials_size()` bytes (per-workgroup partial sums, FP64). The
+ * sums are bitwise reproducible for a given image size. When `host_sums` is non-NULL the call
  * synchronises `stream` and copies the 29 sums to the host; when `A_host`,
  * `b_host`, `residual_host` are non-NULL it also expands them exactly like
  * the reference's CreateMatrix<6,7> (Reduction.cu:6-16): A row-major 6x6

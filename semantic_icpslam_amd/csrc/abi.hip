@@ -69,7 +69,8 @@ int map_alloc(const sicp_map_config& cfg, sicp_map** out) {
         (rc = dmalloc(m, &a.pix_base, px)) || (rc = dmalloc(m, &a.scan_partials, 4096 + 64)) ||
         (rc = dmalloc(m, &a.new_keys, q)) || (rc = dmalloc(m, &a.new_req_slot, q)) ||
         (rc = dmalloc(m, &a.bucket_leader, nb)) || (rc = dmalloc(m, &a.bucket_head, nb)) ||
-        (rc = dmalloc(m, &a.pending_next, q)) || (rc = dmalloc(m, &a.vis_flags, nblk))) {
+        (rc = dmalloc(m, &a.pending_next, q)) || (rc = dmalloc(m, &a.vis_flags, nblk)) ||
+        (rc = dmalloc(m, &a.fuse_counts, (size_t)kFuseBlocks * 4))) {
         map_free(m);
         return rc;
     }
@@ -233,10 +234,7 @@ int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_ima
         !image_ok(nlast, 16, c, r))
         return invalid("icp_step: the four maps must share one shape");
     if (!curr || !last || !partials_dev || !sums_dev) return invalid("icp_step: null argument");
-    // The arrival counter lives after the partial rows; it is zero between
-    // launches (callers allocate the scratch zeroed).
-    IcpScratch sc{(double*)partials_dev,
-                  (unsigned*)((double*)partials_dev + (size_t)kIcpMaxBlocks * kIcpSums)};
+    IcpScratch sc{(double*)partials_dev};
     hipStream_t s = (hipStream_t)stream;
     int rc = launch_icp_sums(view<float4>(vcurr), view<float4>(vlast), view<float4>(ncurr),
                              view<float4>(nlast), make_pose_arg(curr, nullptr),
@@ -324,7 +322,7 @@ int sicp_fuse_map_color(sicp_map* map, sicp_image depth_f32, sicp_image rgb_u8x3
     }
     rc = launch_fuse(map->dev, map->ctr, map->visible, view<float>(depth_f32),
                      view<uint8_t>(rgb_u8x3), view<float4>(nm), view<uint8_t>(mk), num_detection,
-                     pa, K, nullptr, s);
+                     pa, K, map->alloc.fuse_counts, nullptr, s);
     if (rc) return rc;
     if (visible_host) {
         rc = read_u32(&map->ctr->visible, visible_host, s);

@@ -1,15 +1,17 @@
 // icp.hip — point-to-plane ICP reduction and the device-resident
 // Gauss-Newton loop of Tracker::ComputeSE3, for gfx950.
 //
-// One launch per iteration. Each thread accumulates the 29 per-pixel values
-// (27 upper A|b entries, r^2, valid) of its grid-stride pixels in registers;
-// each 64-lane wavefront folds them with cross-lane shuffles; the four
-// wavefronts of a workgroup combine through LDS into one partial row; the
-// last workgroup to arrive (release/acquire counter) sums the partial rows in
-// workgroup order — a fixed order, so results are bitwise reproducible; the
-// accumulation is FP64 throughout —
-// and, in the tracking mode, solves the 6x6 system in FP64 and updates the
-// pose on the device.
+// Two launches per iteration. The pixel kernel gives each thread four pixels
+// with every memory access of a phase in flight together: the current frame's
+// vertex and normal (coalesced), then the projective gathers from the last
+// frame. The 29 per-pixel values (27 upper A|b entries, r^2, valid) are FP32
+// products, as in the reference, accumulated in FP64; a workgroup reduces
+// them by transposition through LDS into one partial row. A one-workgroup
+// kernel then sums the partial rows in a fixed order — results are bitwise
+// reproducible — and, in the tracking mode, solves the 6x6 system in FP64 and
+// updates the pose on the device. (On MI355X the kernel boundary is cheaper
+// than an in-kernel last-workgroup hand-off, whose agent-scope fences write
+// back and invalidate the per-XCD L2 caches.)
 #include "se3_math.hpp"
 #include "sicp_kernels.hpp"
 
@@ -17,36 +19,50 @@ namespace sicp {
 
 namespace {
 
+constexpr int kPixPerThread = 4;
+
 struct IcpGeom {
     int cols, rows, n;
     float fx, fy, cx, cy;
 };
 
-// ICPReduce::searchPoint + getRow (Tracking/Reduction.cu:228-290) for one
-// pixel, in the reference's FP32 order with its FP64 promotions (Q9).
-SICP_DEV bool icp_row(const View<float4>& vcurr, const View<float4>& vlast,
-                      const View<float4>& ncurr, const View<float4>& nlast, int x, int y,
-                      const PoseF& cur, const PoseF& last, const IcpGeom& g, float row[7]) {
-    float4 vc4 = vcurr.at(y, x);
+// ICPReduce::searchPoint up to the gather (Tracking/Reduction.cu:231-243):
+// current vertex to world, into the last camera, projected with the
+// reference's FP64 promotions (invz = 1.0 / z in double stored to float,
+// +0.5 added in double before the truncating cast; Q9).
+struct IcpProj {
+    v3 vcurr_g;
+    int u, v;
+    bool ok;
+};
+SICP_DEV IcpProj icp_project(float4 vc4, const PoseF& cur, const PoseF& last, const IcpGeom& g) {
+    IcpProj p;
     v3 vcurr_c = xyz(vc4);
-    if (isnan_(vcurr_c.x) || (double)vcurr_c.z < 1e-3) return false;
-    v3 vcurr_g = add(mv(cur.R, vcurr_c), cur.t);
-    v3 vcurr_p = mv(last.Rinv, sub(vcurr_g, last.t));
+    p.ok = !(isnan_(vcurr_c.x) || (double)vcurr_c.z < 1e-3);
+    p.vcurr_g = add(mv(cur.R, vcurr_c), cur.t);
+    v3 vcurr_p = mv(last.Rinv, sub(p.vcurr_g, last.t));
     float invz = (float)(1.0 / (double)vcurr_p.z);
-    int u = (int)((double)(vcurr_p.x * invz * g.fx + g.cx) + 0.5);
-    int v = (int)((double)(vcurr_p.y * invz * g.fy + g.cy) + 0.5);
-    if (u < 0 || v < 0 || u >= g.cols || v >= g.rows) return false;
-    v3 vlast_c = xyz(vlast.at(v, u));
-    v3 vlast_g = add(mv(last.R, vlast_c), last.t);
-    v3 ncurr_c = xyz(ncurr.at(y, x));
+    p.u = (int)((double)(vcurr_p.x * invz * g.fx + g.cx) + 0.5);
+    p.v = (int)((double)(vcurr_p.y * invz * g.fy + g.cy) + 0.5);
+    p.ok = p.ok && !(p.u < 0 || p.v < 0 || p.u >= g.cols || p.v >= g.rows);
+    return p;
+}
+
+// The rest of searchPoint (:244-257) and getRow (:260-290): correspondence
+// test, then J = [-n, n x v_last], r = -n . (v_curr - v_last) in the last
+// camera's frame.
+SICP_DEV bool icp_finish(const IcpProj& p, float4 nc4, float4 vl4, float4 nl4, const PoseF& cur,
+                         const PoseF& last, float row[7]) {
+    v3 vlast_g = add(mv(last.R, xyz(vl4)), last.t);
+    v3 ncurr_c = xyz(nc4);
     v3 ncurr_g = mv(cur.R, ncurr_c);
-    v3 nlast_c = xyz(nlast.at(v, u));
+    v3 nlast_c = xyz(nl4);
     v3 nlast_g = mv(last.R, nlast_c);
-    float dist = norm(sub(vlast_g, vcurr_g));
+    float dist = norm(sub(vlast_g, p.vcurr_g));
     float sine = norm(cross(ncurr_g, nlast_g));
     if (!(sine < 0.6f && dist <= 0.1f && !isnan_(ncurr_c.x) && !isnan_(nlast_c.x))) return false;
     v3 nl = mv(last.Rinv, nlast_g);
-    v3 vcl = mv(last.Rinv, sub(vcurr_g, last.t));
+    v3 vcl = mv(last.Rinv, sub(p.vcurr_g, last.t));
     v3 vll = mv(last.Rinv, sub(vlast_g, last.t));
     v3 mn = neg(nl);
     v3 c = cross(nl, vll);
@@ -60,157 +76,197 @@ SICP_DEV bool icp_row(const View<float4>& vcurr, const View<float4>& vlast,
     return true;
 }
 
-// Per-thread accumulation + workgroup reduction into partials[blockIdx.x].
-// Returns true in the one workgroup that arrives last; its thread j < 29 then
-// holds the total of sum j in `total`.
-SICP_DEV bool icp_accumulate(const View<float4>& vcurr, const View<float4>& vlast,
+// Per-thread accumulation + workgroup reduction into this workgroup's
+// partial row (column-major: partials[j * kIcpMaxBlocks + workgroup]).
+SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlast,
                              const View<float4>& ncurr, const View<float4>& nlast,
                              const PoseF& cur, const PoseF& last, const IcpGeom& g,
-                             double* partials, unsigned* counter, double& total) {
-    // The per-pixel products are FP32 (as in the reference); their sums are
-    // carried in FP64, so the totals equal the exact sum of the FP32 products
-    // to double rounding, whatever the reduction order.
+                             double* partials) {
     double acc[kIcpSums];
 #pragma unroll
     for (int j = 0; j < kIcpSums; ++j) acc[j] = 0.0;
     const int stride = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += stride) {
-        int y = i / g.cols;
-        int x = i - y * g.cols;
-        float row[7];
-        if (icp_row(vcurr, vlast, ncurr, nlast, x, y, cur, last, g, row)) {
-            int k = 0;
+    for (int base = blockIdx.x * blockDim.x + threadIdx.x; base < g.n;
+         base += stride * kPixPerThread) {
+        float4 vc[kPixPerThread], nc[kPixPerThread], vl[kPixPerThread], nl[kPixPerThread];
+        IcpProj pr[kPixPerThread];
 #pragma unroll
-            for (int a = 0; a < 7; ++a)
+        for (int k = 0; k < kPixPerThread; ++k) {
+            const int i = base + k * stride;
+            const int ii = i < g.n ? i : 0;
+            const int y = ii / g.cols, x = ii - y * g.cols;
+            vc[k] = vcurr.at(y, x);
+            nc[k] = ncurr.at(y, x);
+            if (i >= g.n) vc[k].x = qnanf();
+        }
 #pragma unroll
-                for (int b = a; b < 7; ++b) acc[k++] += (double)(row[a] * row[b]);
-            acc[28] += 1.0;
+        for (int k = 0; k < kPixPerThread; ++k) {
+            pr[k] = icp_project(vc[k], cur, last, g);
+            const int u = pr[k].ok ? pr[k].u : 0, v = pr[k].ok ? pr[k].v : 0;
+            vl[k] = vlast.at(v, u);
+            nl[k] = nlast.at(v, u);
+        }
+#pragma unroll
+        for (int k = 0; k < kPixPerThread; ++k) {
+            float row[7];
+            if (pr[k].ok && icp_finish(pr[k], nc[k], vl[k], nl[k], cur, last, row)) {
+                int q = 0;
+#pragma unroll
+                for (int a = 0; a < 7; ++a)
+#pragma unroll
+                    for (int b = a; b < 7; ++b) acc[q++] += (double)(row[a] * row[b]);
+                acc[28] += 1.0;
+            }
         }
     }
-    // Wavefront fold: 64 -> 1 lanes.
+    // Workgroup reduction by transposition through LDS: every thread stores its
+    // 29 sums as one row; thread (j, k) then adds rows k, k+8, ... of column j
+    // and eight lanes fold with three butterflies. Fixed order; much cheaper
+    // than folding 29 doubles across 64 lanes with cross-lane permutes.
+    __shared__ double tile[kIcpThreads][kIcpSums + 1];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int j = 0; j < kIcpSums; ++j) acc[j] += __shfl_xor(acc[j], off, 64);
-    }
-    __shared__ double wave_sums[kIcpThreads / 64][kIcpSums];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    if (lane == 0) {
-#pragma unroll
-        for (int j = 0; j < kIcpSums; ++j) wave_sums[wave][j] = acc[j];
-    }
+    for (int j = 0; j < kIcpSums; ++j) tile[threadIdx.x][j] = acc[j];
     __syncthreads();
-    if (threadIdx.x < kIcpSums) {
+    const int t = threadIdx.x;
+    if (t < kIcpSums * 8) {
+        const int j = t >> 3, k = t & 7;
         double s = 0.0;
-#pragma unroll
-        for (int w = 0; w < kIcpThreads / 64; ++w) s += wave_sums[w][threadIdx.x];
+#pragma unroll 8
+        for (int r = k; r < kIcpThreads; r += 8) s += tile[r][j];
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        s += __shfl_xor(s, 4, 64);
         // column-major: sum j of every workgroup is contiguous
-        partials[threadIdx.x * kIcpMaxBlocks + blockIdx.x] = s;
-        __threadfence();
+        if (k == 0) partials[j * kIcpMaxBlocks + blockIdx.x] = s;
     }
-    __shared__ int is_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (prev == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    // Wavefront w reduces sums w, w+4, ...: lane-strided partial sums over the
-    // workgroups, then a butterfly — a fixed order for a given grid size.
-    __shared__ double totals[kIcpSums];
-    const int nb = (int)gridDim.x;
-    for (int j = wave; j < kIcpSums; j += kIcpThreads / 64) {
-        double s = 0.0;
-        for (int b = lane; b < nb; b += 64) s += partials[j * kIcpMaxBlocks + b];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (lane == 0) totals[j] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < kIcpSums) total = totals[threadIdx.x];
-    if (threadIdx.x == 0) *counter = 0u;  // ready for the next launch on the stream
-    return true;
 }
 
-__global__ __launch_bounds__(kIcpThreads) void icp_sums_kernel(
+// Sum of the partial rows of `nb` workgroups into totals (LDS), in a fixed
+// order: thread (j, k) adds rows k, k+8, ... then eight lanes fold.
+SICP_DEV void icp_totals(const double* partials, int nb, double* totals) {
+    const int t = threadIdx.x;
+    if (t < kIcpSums * 8) {
+        const int j = t >> 3, k = t & 7;
+        double s = 0.0;
+        for (int b = k; b < nb; b += 8) s += partials[j * kIcpMaxBlocks + b];
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        s += __shfl_xor(s, 4, 64);
+        if (k == 0) totals[j] = s;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kIcpThreads) void icp_partials_kernel(
     View<float4> vcurr, View<float4> vlast, View<float4> ncurr, View<float4> nlast, PoseArg pc,
-    PoseArg pl, IcpGeom g, double* partials, unsigned* counter, float* sums_out) {
+    PoseArg pl, IcpGeom g, double* partials) {
     const PoseF cur = load_pose(pc);
     const PoseF last = load_pose(pl);
-    double total = 0.0;
-    if (!icp_accumulate(vcurr, vlast, ncurr, nlast, cur, last, g, partials, counter, total))
-        return;
-    if (threadIdx.x < kIcpSums) sums_out[threadIdx.x] = (float)total;
+    icp_accumulate(vcurr, vlast, ncurr, nlast, cur, last, g, partials);
+}
+
+__global__ __launch_bounds__(kIcpThreads) void icp_sums_finalize_kernel(const double* partials,
+                                                                        int nb,
+                                                                        float* sums_out) {
+    __shared__ double totals[kIcpSums];
+    icp_totals(partials, nb, totals);
+    if (threadIdx.x < kIcpSums) sums_out[threadIdx.x] = (float)totals[threadIdx.x];
+}
+
+SICP_DEV void copy16(double* d, const double* s) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) d[i] = s[i];
 }
 
 // One iteration of the inner loop of ComputeSE3 (Tracking.cc:433-514,
-// icpOnly branch :488-490) executed by the last workgroup.
-SICP_DEV void gauss_newton_step(TrackState* st, int min_count) {
-    const float r0 = st->sums[27], r1 = st->sums[28];
+// icpOnly branch :488-490), run by one thread of the last workgroup. The
+// state is read into registers once and written back once.
+SICP_DEV void gauss_newton_step(TrackState* st, const double* totals, int min_count) {
+    float sums[kIcpSums];
+#pragma unroll
+    for (int j = 0; j < kIcpSums; ++j) sums[j] = (float)totals[j];  // the float download
+    double last_pose[16], next_pose[16], delta[16];
+    copy16(last_pose, st->last_pose);
+    copy16(next_pose, st->next_pose);
+    copy16(delta, st->delta);
+    int break_count = st->break_count;
+    const float last_error = st->last_error;
+    const float r0 = sums[27], r1 = sums[28];
     const float icp_error = sqrtf(r0) / r1;
     const int icp_count = (int)r1;
+#pragma unroll
+    for (int j = 0; j < kIcpSums; ++j) st->sums[j] = sums[j];
     st->icp_count = icp_count;
     if (isnan_(icp_error) || icp_count < min_count) {
         // "track icp failed": roll back (Tracking.cc:452-456)
-        for (int i = 0; i < 16; ++i) st->next_pose[i] = st->last_pose[i];
+        copy16(st->next_pose, last_pose);
         st->icp_error = icp_error;
         st->failed = 1;
         st->done = 1;
-        pose_to_float(st->next_pose, &st->curr_f);
+        pose_to_float(last_pose, &st->curr_f);
         return;
     }
     const float curr_error = icp_error;
-    if (st->last_error < curr_error) {
-        st->break_count++;
-        if (st->break_count == 1)
-            for (int i = 0; i < 16; ++i) st->candidate[i] = st->next_pose[i];
+    if (last_error < curr_error) {
+        break_count++;
+        if (break_count == 1) copy16(st->candidate, next_pose);
         st->candidate_icp = icp_error;  // assigned on every increase (Q11)
     } else {
-        st->break_count = 0;
+        break_count = 0;
     }
-    if (st->break_count > 2) {
-        for (int i = 0; i < 16; ++i) st->next_pose[i] = st->candidate[i];
+    st->break_count = break_count;
+    if (break_count > 2) {
+        double cand[16];
+        copy16(cand, st->candidate);
+        copy16(st->next_pose, cand);
         st->icp_error = st->candidate_icp;
         st->done = 1;
-        pose_to_float(st->next_pose, &st->curr_f);
+        pose_to_float(cand, &st->curr_f);
         return;
     }
     st->last_error = curr_error;
     st->icp_error = icp_error;
-    create_matrix(st->sums, st->A, st->b);
-    ldlt_solve6(st->A, st->b, st->x);
-    double dT[16], inv[16];
-    se3_exp(st->x, dT);
-    mat4_mul(dT, st->delta, st->delta);
-    rigid_inverse(st->delta, inv);
-    mat4_mul(st->last_pose, inv, st->next_pose);
+    double A[36], b[6], x[6], dT[16], inv[16];
+    create_matrix(sums, A, b);
+    ldlt_solve6(A, b, x);
+    se3_exp(x, dT);
+    mat4_mul(dT, delta, delta);
+    rigid_inverse(delta, inv);
+    mat4_mul(last_pose, inv, next_pose);
+    copy16(st->delta, delta);
+    copy16(st->next_pose, next_pose);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) st->x[i] = x[i];
     st->applied++;
-    pose_to_float(st->next_pose, &st->curr_f);
+    pose_to_float(next_pose, &st->curr_f);
 }
 
 __global__ __launch_bounds__(kIcpThreads) void icp_iterate_kernel(
     View<float4> vcurr, View<float4> vlast, View<float4> ncurr, View<float4> nlast, IcpGeom g,
-    int min_count, TrackState* st, double* partials) {
-    if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    TrackState* st, double* partials) {
+    if (st->done) return;
     const PoseF cur = to_pose(st->curr_f);
     const PoseF last = to_pose(st->last_f);
-    double total = 0.0;
-    if (!icp_accumulate(vcurr, vlast, ncurr, nlast, cur, last, g, partials, &st->counter, total))
-        return;
-    if (threadIdx.x < kIcpSums) st->sums[threadIdx.x] = (float)total;
-    __syncthreads();
-    if (threadIdx.x == 0) gauss_newton_step(st, min_count);
+    icp_accumulate(vcurr, vlast, ncurr, nlast, cur, last, g, partials);
+}
+
+// The end of one Gauss-Newton iteration: totals of the partial rows, then the
+// update of ComputeSE3 on one thread.
+__global__ __launch_bounds__(kIcpThreads) void icp_solve_kernel(const double* partials, int nb,
+                                                                int min_count, TrackState* st) {
+    if (st->done) return;
+    __shared__ double totals[kIcpSums];
+    icp_totals(partials, nb, totals);
+    if (threadIdx.x == 0) gauss_newton_step(st, totals, min_count);
 }
 
 __global__ void track_init_kernel(FrameMeta* next, FrameMeta* last, TrackState* st) {
-    set_identity(next->pose);
-    set_identity(last->pose);
-    pose_to_float(next->pose, &next->pose_f);
-    pose_to_float(last->pose, &last->pose_f);
+    double I[16];
+    set_identity(I);
+    copy16(next->pose, I);
+    copy16(last->pose, I);
+    pose_to_float(I, &next->pose_f);
+    pose_to_float(I, &last->pose_f);
     st->tracked = 1;
     st->failed = 0;
     st->done = 1;
@@ -222,13 +278,14 @@ __global__ void track_init_kernel(FrameMeta* next, FrameMeta* last, TrackState* 
 // Tracker::TrackFrame (Tracking.cc:157-173) up to the loop of ComputeSE3.
 __global__ void track_begin_kernel(FrameMeta* next, FrameMeta* last, TrackState* st,
                                    float thresh) {
-    for (int i = 0; i < 16; ++i) {
-        st->last_pose[i] = last->pose[i];
-        st->next_pose[i] = last->pose[i];
-        st->pose0[i] = last->pose[i];
-        st->candidate[i] = last->pose[i];
-    }
-    set_identity(st->delta);
+    double lp[16], I[16];
+    copy16(lp, last->pose);
+    set_identity(I);
+    copy16(st->last_pose, lp);
+    copy16(st->next_pose, lp);
+    copy16(st->pose0, lp);
+    copy16(st->candidate, lp);
+    copy16(st->delta, I);
     st->icp_error = 0.f;
     st->last_error = 100.f;
     st->candidate_icp = 0.f;
@@ -239,26 +296,33 @@ __global__ void track_begin_kernel(FrameMeta* next, FrameMeta* last, TrackState*
     st->done = 0;
     st->failed = 0;
     st->tracked = 0;
-    st->counter = 0u;
-    pose_to_float(st->next_pose, &st->curr_f);
-    pose_to_float(st->last_pose, &st->last_f);
+    pose_to_float(lp, &st->curr_f);
+    pose_to_float(lp, &st->last_f);
 }
 
 // Final motion / error gate of ComputeSE3 (Tracking.cc:505-516).
 __global__ void track_end_kernel(FrameMeta* next, TrackState* st) {
+    double p0[16], np_[16], lp[16];
+    copy16(p0, st->pose0);
+    copy16(np_, st->next_pose);
+    copy16(lp, st->last_pose);
+    const int failed = st->failed;
+    const float icp_error = st->icp_error, thresh = st->thresh;
     int ok = 0;
-    if (!st->failed) {
+    if (!failed) {
         double inv0[16], p[16], rn, tn;
-        rigid_inverse(st->pose0, inv0);
-        mat4_mul(inv0, st->next_pose, p);
+        rigid_inverse(p0, inv0);
+        mat4_mul(inv0, np_, p);
         euler_sin_norm(p, &rn, &tn);
-        ok = (st->icp_error < st->thresh && rn <= 0.1 && tn <= 0.1) ? 1 : 0;
-        if (!ok)
-            for (int i = 0; i < 16; ++i) st->next_pose[i] = st->last_pose[i];
+        ok = (icp_error < thresh && rn <= 0.1 && tn <= 0.1) ? 1 : 0;
+        if (!ok) {
+            copy16(np_, lp);
+            copy16(st->next_pose, lp);
+        }
     }
     st->tracked = ok;
-    for (int i = 0; i < 16; ++i) next->pose[i] = st->next_pose[i];
-    pose_to_float(next->pose, &next->pose_f);
+    copy16(next->pose, np_);
+    pose_to_float(np_, &next->pose_f);
 }
 
 __global__ __launch_bounds__(256) void rollback_maps_kernel(View<float4> vlast, View<float4> nlast,
@@ -273,9 +337,9 @@ __global__ __launch_bounds__(256) void rollback_maps_kernel(View<float4> vlast, 
     }
 }
 
-// About four pixels per thread, at most one workgroup per CU.
+// Four pixels per thread, at most kIcpMaxBlocks workgroups.
 inline int icp_blocks(int n) {
-    int b = (int)div_up(n, kIcpThreads * 4);
+    int b = (int)div_up(n, kIcpThreads * kPixPerThread);
     return b < 1 ? 1 : (b > kIcpMaxBlocks ? kIcpMaxBlocks : b);
 }
 
@@ -285,10 +349,12 @@ int launch_icp_sums(View<float4> vcurr, View<float4> vlast, View<float4> ncurr,
                     View<float4> nlast, PoseArg curr, PoseArg last, sicp_intrinsics K,
                     IcpScratch scratch, float* sums_out, hipStream_t s) {
     IcpGeom g{vcurr.cols, vcurr.rows, vcurr.cols * vcurr.rows, K.fx, K.fy, K.cx, K.cy};
-    icp_sums_kernel<<<icp_blocks(g.n), kIcpThreads, 0, s>>>(vcurr, vlast, ncurr, nlast, curr,
-                                                              last, g, scratch.partials,
-                                                              scratch.counter, sums_out);
-    SICP_LAUNCH_CHECK("icp_sums_kernel");
+    const int nb = icp_blocks(g.n);
+    icp_partials_kernel<<<nb, kIcpThreads, 0, s>>>(vcurr, vlast, ncurr, nlast, curr, last, g,
+                                                     scratch.partials);
+    SICP_LAUNCH_CHECK("icp_partials_kernel");
+    icp_sums_finalize_kernel<<<1, kIcpThreads, 0, s>>>(scratch.partials, nb, sums_out);
+    SICP_LAUNCH_CHECK("icp_sums_finalize_kernel");
     return SICP_OK;
 }
 
@@ -296,9 +362,11 @@ int launch_icp_iterate(View<float4> vcurr, View<float4> vlast, View<float4> ncur
                        View<float4> nlast, sicp_intrinsics K, int min_count, TrackState* st,
                        double* partials, hipStream_t s) {
     IcpGeom g{vcurr.cols, vcurr.rows, vcurr.cols * vcurr.rows, K.fx, K.fy, K.cx, K.cy};
-    icp_iterate_kernel<<<icp_blocks(g.n), kIcpThreads, 0, s>>>(vcurr, vlast, ncurr, nlast, g,
-                                                                 min_count, st, partials);
+    const int nb = icp_blocks(g.n);
+    icp_iterate_kernel<<<nb, kIcpThreads, 0, s>>>(vcurr, vlast, ncurr, nlast, g, st, partials);
     SICP_LAUNCH_CHECK("icp_iterate_kernel");
+    icp_solve_kernel<<<1, kIcpThreads, 0, s>>>(partials, nb, min_count, st);
+    SICP_LAUNCH_CHECK("icp_solve_kernel");
     return SICP_OK;
 }
 

@@ -552,11 +552,11 @@ SICP_DEV uint8_t u8i(float v) { return (uint8_t)(uint32_t)(int)v; }
 
 // Fusion::createVoxels (FuseMap.cu:432-507) and, when num_detection > 0,
 // Fusion::analyzeSemantics (FuseMap.cu:509-668) applied once to the same voxel.
-__global__ __launch_bounds__(256) void fuse_kernel(MapDev m, MapCounters* ctr,
+__global__ __launch_bounds__(256) void fuse_kernel(MapDev m, const MapCounters* ctr,
                                                    const sicp_block* visible, View<float> depth,
                                                    View<uint8_t> rgb, View<float4> nmap,
                                                    View<uint8_t> mask, PoseArg pa, FuseArgs g,
-                                                   const int* gate) {
+                                                   uint32_t* wave_counts, const int* gate) {
     if (gate && *gate == 0) return;
     const uint32_t nvis = ctr->visible;
     const PoseF pose = load_pose(pa);
@@ -702,12 +702,30 @@ __global__ __launch_bounds__(256) void fuse_kernel(MapDev m, MapCounters* ctr,
             *gp = gv;
         }
     }
-    // One counter update per wavefront.
+    // One count per wavefront into its own slot (a single device-wide counter
+    // would serialise thousands of atomics on one address).
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) updated += __shfl_xor(updated, off, 64);
-    if (lane == 0 && updated) {
-        atomicAdd(&ctr->updated_voxels, updated);
-        atomicAdd(&ctr->updated_total, (unsigned long long)updated);
+    if (lane == 0) wave_counts[wave0] = updated;
+}
+
+// Sums the per-wavefront counts of fuse_kernel.
+__global__ __launch_bounds__(1024) void fuse_end_kernel(MapCounters* ctr,
+                                                        const uint32_t* wave_counts, int n,
+                                                        const int* gate) {
+    if (gate && *gate == 0) return;
+    __shared__ uint32_t part[16];
+    uint32_t s = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += wave_counts[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+        ctr->updated_voxels = t;
+        ctr->updated_total += t;
     }
 }
 
@@ -790,12 +808,16 @@ int launch_visibility(MapDev m, MapCounters* ctr, AllocScratch a, sicp_block* vi
 
 int launch_fuse(MapDev m, MapCounters* ctr, const sicp_block* visible, View<float> depth,
                 View<uint8_t> rgb, View<float4> nmap, View<uint8_t> mask, int num_detection,
-                PoseArg pose, sicp_intrinsics K, const int* gate, hipStream_t s) {
+                PoseArg pose, sicp_intrinsics K, uint32_t* wave_counts, const int* gate,
+                hipStream_t s) {
     FuseArgs g{K.fx, K.fy, K.cx, K.cy, depth.cols, depth.rows, num_detection};
     fuse_begin_kernel<<<1, 1, 0, s>>>(ctr, gate);
     SICP_LAUNCH_CHECK("fuse_begin_kernel");
-    fuse_kernel<<<2048, 256, 0, s>>>(m, ctr, visible, depth, rgb, nmap, mask, pose, g, gate);
+    fuse_kernel<<<kFuseBlocks, 256, 0, s>>>(m, ctr, visible, depth, rgb, nmap, mask, pose, g,
+                                            wave_counts, gate);
     SICP_LAUNCH_CHECK("fuse_kernel");
+    fuse_end_kernel<<<1, 1024, 0, s>>>(ctr, wave_counts, kFuseBlocks * 4, gate);
+    SICP_LAUNCH_CHECK("fuse_end_kernel");
     return SICP_OK;
 }
 

@@ -111,7 +111,10 @@ __global__ __launch_bounds__(256) void prepass_kernel(MapDev m, MapCounters* ctr
             }
         }
     }
-    if (valid_blocks) atomicAdd(&ctr->render_tiles, valid_blocks);
+    // One atomic per wavefront on the shared counter.
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) valid_blocks += __shfl_xor(valid_blocks, off, 64);
+    if ((threadIdx.x & 63) == 0 && valid_blocks) atomicAdd(&ctr->render_tiles, valid_blocks);
 }
 
 // ------------------------------------------------------------------ raycast --

@@ -21,7 +21,7 @@ int launch_resize_map(View<float4> vsrc, View<float4> nsrc, View<float4> vdst, V
 
 // ---------------------------------------------------------------- ICP -----
 constexpr int kIcpThreads = 256;
-constexpr int kIcpMaxBlocks = 256;
+constexpr int kIcpMaxBlocks = 512;
 constexpr int kIcpSums = 29;
 
 // Pose and Gauss-Newton state of one frame's tracking, resident on the device
@@ -43,13 +43,10 @@ struct TrackState {
     float icp_error, last_error, candidate_icp, thresh;
     int icp_count, break_count, applied, done;
     int failed, tracked, frame_index, pad;
-    unsigned counter;  // last-workgroup arrival counter of the reduction
-    unsigned pad2[3];
 };
 
 struct IcpScratch {
-    double* partials;  // kIcpMaxBlocks * kIcpSums
-    unsigned* counter; // 1, zero between launches
+    double* partials;  // kIcpSums * kIcpMaxBlocks, column-major
 };
 
 // ICPStep: the 29 sums of one level into `sums_out` (device).
@@ -116,8 +113,11 @@ struct AllocScratch {
     int32_t* bucket_head;      // per bucket, pending list head (-1 idle)
     int32_t* pending_next;     // rank -> next pending rank
     uint8_t* vis_flags;        // per heap slot
+    uint32_t* fuse_counts;     // per fusion wavefront: voxels written (kFuseBlocks * 4)
     uint32_t cap_pixels;
 };
+
+constexpr int kFuseBlocks = 2048;  // fusion workgroups (4 wavefronts each), grid-stride
 
 int launch_map_reset(MapDev m, MapCounters* ctr, AllocScratch a, hipStream_t s);
 int launch_allocate(MapDev m, MapCounters* ctr, AllocScratch a, View<float> depth, PoseArg pose,
@@ -127,7 +127,8 @@ int launch_visibility(MapDev m, MapCounters* ctr, AllocScratch a, sicp_block* vi
                       float dmin, const int* gate, hipStream_t s);
 int launch_fuse(MapDev m, MapCounters* ctr, const sicp_block* visible, View<float> depth,
                 View<uint8_t> rgb, View<float4> nmap, View<uint8_t> mask, int num_detection,
-                PoseArg pose, sicp_intrinsics K, const int* gate, hipStream_t s);
+                PoseArg pose, sicp_intrinsics K, uint32_t* wave_counts, const int* gate,
+                hipStream_t s);
 int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible, View<float> zmin,
                           View<float> zmax, PoseArg pose, sicp_intrinsics K, float dmax,
                           float dmin, const int* gate, hipStream_t s);

@@ -285,7 +285,8 @@ int sicp_system_grab_image(sicp_system* s, sicp_image depth_u16, sicp_image rgb_
     View<uint8_t> mask = labels ? view<uint8_t>(labels_u8) : View<uint8_t>{};
     ev = stage_begin(s, st);
     if ((rc = launch_fuse(m->dev, m->ctr, m->visible, f.range.v(), view<uint8_t>(rgb_u8x3),
-                          f.nmap[0].v(), mask, labels ? num_detection : 0, pose, s->K[0], gate,
+                          f.nmap[0].v(), mask, labels ? num_detection : 0, pose, s->K[0],
+                          m->alloc.fuse_counts, gate,
                           st)))
         return rc;
     stage_end(s, kStageFuse, ev, st);
