@@ -291,8 +291,11 @@ sicp_map* sicp_system_map(sicp_system* sys);
 int sicp_system_set_timing(sicp_system* sys, int enable);
 int sicp_system_read_timing(sicp_system* sys, double* total_ms, int* counts, int n_stages,
                             void* stream);
-/* Model / frame buffers of the system, for inspection: which = 0 vmap, 1 nmap
- * of the last frame at pyramid `level`; 2 raw depth; 3 filtered depth L0. */
+/* Frame buffers of the system, for inspection (the public members of Frame,
+ * Core/Frame.h): which = 0 vmap, 1 nmap, 2 raw depth (level 0 only),
+ * 3 filtered depth, 4 intensity, 5 d/dx, 6 d/dy at pyramid `level`, of the
+ * frame just grabbed (its level-0 maps hold the raycast model); add 16 for the
+ * frame before it, whose levels 1-2 hold its model pyramid. */
 int sicp_system_frame_image(sicp_system* sys, int which, int level, sicp_image* out);
 
 /* ---- seeded synthetic room (test and benchmark input) -------------------- */

@@ -76,32 +76,47 @@ SICP_DEV bool icp_finish(const IcpProj& p, float4 nc4, float4 vl4, float4 nl4, c
     return true;
 }
 
+// The current frame's vertices and normals of one thread's pixels: they do
+// not depend on the pose, so their loads are issued before the pose is known.
+struct IcpBatch {
+    float4 vc[kPixPerThread], nc[kPixPerThread];
+};
+SICP_DEV void icp_load_batch(IcpBatch& bt, int base, int stride, const View<float4>& vcurr,
+                             const View<float4>& ncurr, const IcpGeom& g) {
+#pragma unroll
+    for (int k = 0; k < kPixPerThread; ++k) {
+        const int i = base + k * stride;
+        const int ii = i < g.n ? i : 0;
+        const int y = ii / g.cols, x = ii - y * g.cols;
+        bt.vc[k] = vcurr.at(y, x);
+        bt.nc[k] = ncurr.at(y, x);
+        if (i >= g.n) bt.vc[k].x = qnanf();
+    }
+}
+
 // Per-thread accumulation + workgroup reduction into this workgroup's
 // partial row (column-major: partials[j * kIcpMaxBlocks + workgroup]).
+// `get_pose(cur, last)` is called by every thread once the first pixels'
+// loads are in flight; it returns false when there is nothing to do.
+template <class PoseFn>
 SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlast,
                              const View<float4>& ncurr, const View<float4>& nlast,
-                             const PoseF& cur, const PoseF& last, const IcpGeom& g,
-                             double* partials) {
+                             const IcpGeom& g, double* partials, PoseFn get_pose) {
+    const int stride = gridDim.x * blockDim.x;
+    int base = blockIdx.x * blockDim.x + threadIdx.x;
+    IcpBatch bt;
+    icp_load_batch(bt, base, stride, vcurr, ncurr, g);
+    PoseF cur, last;
+    if (!get_pose(cur, last)) return;
     double acc[kIcpSums];
 #pragma unroll
     for (int j = 0; j < kIcpSums; ++j) acc[j] = 0.0;
-    const int stride = gridDim.x * blockDim.x;
-    for (int base = blockIdx.x * blockDim.x + threadIdx.x; base < g.n;
-         base += stride * kPixPerThread) {
-        float4 vc[kPixPerThread], nc[kPixPerThread], vl[kPixPerThread], nl[kPixPerThread];
+    while (base < g.n) {
+        float4 vl[kPixPerThread], nl[kPixPerThread];
         IcpProj pr[kPixPerThread];
 #pragma unroll
         for (int k = 0; k < kPixPerThread; ++k) {
-            const int i = base + k * stride;
-            const int ii = i < g.n ? i : 0;
-            const int y = ii / g.cols, x = ii - y * g.cols;
-            vc[k] = vcurr.at(y, x);
-            nc[k] = ncurr.at(y, x);
-            if (i >= g.n) vc[k].x = qnanf();
-        }
-#pragma unroll
-        for (int k = 0; k < kPixPerThread; ++k) {
-            pr[k] = icp_project(vc[k], cur, last, g);
+            pr[k] = icp_project(bt.vc[k], cur, last, g);
             const int u = pr[k].ok ? pr[k].u : 0, v = pr[k].ok ? pr[k].v : 0;
             vl[k] = vlast.at(v, u);
             nl[k] = nlast.at(v, u);
@@ -109,7 +124,7 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
 #pragma unroll
         for (int k = 0; k < kPixPerThread; ++k) {
             float row[7];
-            if (pr[k].ok && icp_finish(pr[k], nc[k], vl[k], nl[k], cur, last, row)) {
+            if (pr[k].ok && icp_finish(pr[k], bt.nc[k], vl[k], nl[k], cur, last, row)) {
                 int q = 0;
 #pragma unroll
                 for (int a = 0; a < 7; ++a)
@@ -118,6 +133,8 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
                 acc[28] += 1.0;
             }
         }
+        base += stride * kPixPerThread;
+        if (base < g.n) icp_load_batch(bt, base, stride, vcurr, ncurr, g);
     }
     // Workgroup reduction by transposition through LDS: every thread stores its
     // 29 sums as one row; thread (j, k) then adds rows k, k+8, ... of column j
@@ -142,27 +159,46 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
 }
 
 // Sum of the partial rows of `nb` workgroups into totals (LDS), in a fixed
-// order: thread (j, k) adds rows k, k+8, ... then eight lanes fold.
-SICP_DEV void icp_totals(const double* partials, int nb, double* totals) {
+// order: thread (j, k) adds rows k, k+8, ..., k+504 of column j (every load
+// issued before the first add), then eight lanes fold.
+SICP_DEV void icp_totals_issue(const double* partials, int nb, double& s) {
     const int t = threadIdx.x;
+    s = 0.0;
     if (t < kIcpSums * 8) {
         const int j = t >> 3, k = t & 7;
-        double s = 0.0;
-        for (int b = k; b < nb; b += 8) s += partials[j * kIcpMaxBlocks + b];
-        s += __shfl_xor(s, 1, 64);
-        s += __shfl_xor(s, 2, 64);
-        s += __shfl_xor(s, 4, 64);
-        if (k == 0) totals[j] = s;
+        const double* col = partials + j * kIcpMaxBlocks;
+        double v[kIcpMaxBlocks / 8];
+#pragma unroll
+        for (int r = 0; r < kIcpMaxBlocks / 8; ++r) {
+            const int bi = k + 8 * r;
+            v[r] = bi < nb ? col[bi] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < kIcpMaxBlocks / 8; ++r) s += v[r];
     }
+}
+SICP_DEV void icp_totals_fold(double s, double* totals) {
+    const int t = threadIdx.x;
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    s += __shfl_xor(s, 4, 64);
+    if (t < kIcpSums * 8 && (t & 7) == 0) totals[t >> 3] = s;
+}
+SICP_DEV void icp_totals(const double* partials, int nb, double* totals) {
+    double s;
+    icp_totals_issue(partials, nb, s);
+    icp_totals_fold(s, totals);
     __syncthreads();
 }
 
 __global__ __launch_bounds__(kIcpThreads) void icp_partials_kernel(
     View<float4> vcurr, View<float4> vlast, View<float4> ncurr, View<float4> nlast, PoseArg pc,
     PoseArg pl, IcpGeom g, double* partials) {
-    const PoseF cur = load_pose(pc);
-    const PoseF last = load_pose(pl);
-    icp_accumulate(vcurr, vlast, ncurr, nlast, cur, last, g, partials);
+    icp_accumulate(vcurr, vlast, ncurr, nlast, g, partials, [&](PoseF& cur, PoseF& last) {
+        cur = load_pose(pc);
+        last = load_pose(pl);
+        return true;
+    });
 }
 
 __global__ __launch_bounds__(kIcpThreads) void icp_sums_finalize_kernel(const double* partials,
@@ -178,9 +214,10 @@ SICP_DEV void copy16(double* d, const double* s) {
     for (int i = 0; i < 16; ++i) d[i] = s[i];
 }
 
-// One iteration of the inner loop of ComputeSE3 (Tracking.cc:433-514,
-// icpOnly branch :488-490), run by one thread of the last workgroup. The
-// state is read into registers once and written back once.
+// The update of one iteration of the inner loop of ComputeSE3
+// (Tracking.cc:433-514, icpOnly branch :488-490), on one thread, from the
+// iteration's 29 sums. The state lives in LDS; the poses are read into
+// registers once and written back once.
 SICP_DEV void gauss_newton_step(TrackState* st, const double* totals, int min_count) {
     float sums[kIcpSums];
 #pragma unroll
@@ -194,8 +231,6 @@ SICP_DEV void gauss_newton_step(TrackState* st, const double* totals, int min_co
     const float r0 = sums[27], r1 = sums[28];
     const float icp_error = sqrtf(r0) / r1;
     const int icp_count = (int)r1;
-#pragma unroll
-    for (int j = 0; j < kIcpSums; ++j) st->sums[j] = sums[j];
     st->icp_count = icp_count;
     if (isnan_(icp_error) || icp_count < min_count) {
         // "track icp failed": roll back (Tracking.cc:452-456)
@@ -235,49 +270,13 @@ SICP_DEV void gauss_newton_step(TrackState* st, const double* totals, int min_co
     mat4_mul(last_pose, inv, next_pose);
     copy16(st->delta, delta);
     copy16(st->next_pose, next_pose);
-#pragma unroll
-    for (int i = 0; i < 6; ++i) st->x[i] = x[i];
     st->applied++;
     pose_to_float(next_pose, &st->curr_f);
 }
 
-__global__ __launch_bounds__(kIcpThreads) void icp_iterate_kernel(
-    View<float4> vcurr, View<float4> vlast, View<float4> ncurr, View<float4> nlast, IcpGeom g,
-    TrackState* st, double* partials) {
-    if (st->done) return;
-    const PoseF cur = to_pose(st->curr_f);
-    const PoseF last = to_pose(st->last_f);
-    icp_accumulate(vcurr, vlast, ncurr, nlast, cur, last, g, partials);
-}
-
-// The end of one Gauss-Newton iteration: totals of the partial rows, then the
-// update of ComputeSE3 on one thread.
-__global__ __launch_bounds__(kIcpThreads) void icp_solve_kernel(const double* partials, int nb,
-                                                                int min_count, TrackState* st) {
-    if (st->done) return;
-    __shared__ double totals[kIcpSums];
-    icp_totals(partials, nb, totals);
-    if (threadIdx.x == 0) gauss_newton_step(st, totals, min_count);
-}
-
-__global__ void track_init_kernel(FrameMeta* next, FrameMeta* last, TrackState* st) {
-    double I[16];
-    set_identity(I);
-    copy16(next->pose, I);
-    copy16(last->pose, I);
-    pose_to_float(I, &next->pose_f);
-    pose_to_float(I, &last->pose_f);
-    st->tracked = 1;
-    st->failed = 0;
-    st->done = 1;
-    st->applied = 0;
-    st->icp_count = 0;
-    st->icp_error = 0.f;
-}
-
-// Tracker::TrackFrame (Tracking.cc:157-173) up to the loop of ComputeSE3.
-__global__ void track_begin_kernel(FrameMeta* next, FrameMeta* last, TrackState* st,
-                                   float thresh) {
+// ComputeSE3's set-up (Tracking.cc:369-431 with TrackFrame :157-173): the
+// iteration state of a new frame, from the last frame's pose.
+SICP_DEV void track_begin_state(TrackState* st, const FrameMeta* last, float thresh) {
     double lp[16], I[16];
     copy16(lp, last->pose);
     set_identity(I);
@@ -301,40 +300,116 @@ __global__ void track_begin_kernel(FrameMeta* next, FrameMeta* last, TrackState*
 }
 
 // Final motion / error gate of ComputeSE3 (Tracking.cc:505-516).
-__global__ void track_end_kernel(FrameMeta* next, TrackState* st) {
-    double p0[16], np_[16], lp[16];
-    copy16(p0, st->pose0);
-    copy16(np_, st->next_pose);
-    copy16(lp, st->last_pose);
-    const int failed = st->failed;
-    const float icp_error = st->icp_error, thresh = st->thresh;
+SICP_DEV void track_end_state(TrackState* st) {
     int ok = 0;
-    if (!failed) {
-        double inv0[16], p[16], rn, tn;
+    if (!st->failed) {
+        double p0[16], np_[16], inv0[16], p[16], rn, tn;
+        copy16(p0, st->pose0);
+        copy16(np_, st->next_pose);
         rigid_inverse(p0, inv0);
         mat4_mul(inv0, np_, p);
         euler_sin_norm(p, &rn, &tn);
-        ok = (icp_error < thresh && rn <= 0.1 && tn <= 0.1) ? 1 : 0;
-        if (!ok) {
-            copy16(np_, lp);
-            copy16(st->next_pose, lp);
-        }
+        ok = (st->icp_error < st->thresh && rn <= 0.1 && tn <= 0.1) ? 1 : 0;
+        if (!ok) copy16(st->next_pose, st->last_pose);
     }
     st->tracked = ok;
-    copy16(next->pose, np_);
-    pose_to_float(np_, &next->pose_f);
 }
 
-__global__ __launch_bounds__(256) void rollback_maps_kernel(View<float4> vlast, View<float4> nlast,
-                                                            View<float4> vnext, View<float4> nnext,
-                                                            const TrackState* st) {
-    if (st->tracked) return;
+constexpr int kStateWords = (int)(sizeof(TrackState) / sizeof(unsigned long long));
+static_assert(sizeof(TrackState) % sizeof(unsigned long long) == 0, "TrackState padding");
+static_assert(kStateWords <= kIcpThreads, "TrackState copied one word per thread");
+
+// Brings the iteration state into LDS. kFirst: built from the last frame's
+// pose; otherwise the previous iteration's state, advanced by that
+// iteration's Gauss-Newton update from its partial rows. Every workgroup of
+// a launch computes the same state (same inputs, same order), so no
+// workgroup waits for another; workgroup 0 publishes it to `st_out`.
+template <bool kFirst>
+SICP_DEV void icp_state(TrackState* S, double* totals, const TrackState* st_in,
+                        const FrameMeta* last, float thresh, const double* partials_prev,
+                        int nb_prev, int min_count_prev, TrackState* st_out) {
+    if (kFirst) {
+        if (threadIdx.x == 0) track_begin_state(S, last, thresh);
+        __syncthreads();
+    } else {
+        const unsigned long long* src = reinterpret_cast<const unsigned long long*>(st_in);
+        unsigned long long w = threadIdx.x < kStateWords ? src[threadIdx.x] : 0ull;
+        double s;
+        icp_totals_issue(partials_prev, nb_prev, s);
+        if (threadIdx.x < kStateWords) reinterpret_cast<unsigned long long*>(S)[threadIdx.x] = w;
+        icp_totals_fold(s, totals);
+        __syncthreads();
+        if (threadIdx.x == 0 && !S->done) gauss_newton_step(S, totals, min_count_prev);
+        __syncthreads();
+    }
+    if (st_out && blockIdx.x == 0 && threadIdx.x < kStateWords)
+        reinterpret_cast<unsigned long long*>(st_out)[threadIdx.x] =
+            reinterpret_cast<const unsigned long long*>(S)[threadIdx.x];
+}
+
+// One launch per Gauss-Newton iteration: the update of the previous
+// iteration, then this iteration's pixel pass (ICPReduce at the state's pose).
+template <bool kFirst>
+__global__ __launch_bounds__(kIcpThreads) void icp_iteration_kernel(
+    View<float4> vcurr, View<float4> vlast, View<float4> ncurr, View<float4> nlast, IcpGeom g,
+    const TrackState* st_in, const FrameMeta* last, float thresh, const double* partials_prev,
+    int nb_prev, int min_count_prev, TrackState* st_out, double* partials) {
+    __shared__ TrackState S;
+    __shared__ double totals[kIcpSums];
+    icp_accumulate(vcurr, vlast, ncurr, nlast, g, partials, [&](PoseF& cur, PoseF& lst) {
+        icp_state<kFirst>(&S, totals, st_in, last, thresh, partials_prev, nb_prev,
+                          min_count_prev, st_out);
+        if (S.done) return false;
+        cur = to_pose(S.curr_f);
+        lst = to_pose(S.last_f);
+        return true;
+    });
+}
+
+// The end of ComputeSE3 and TrackFrame: the last iteration's update, the
+// motion / error gate, the frame pose; when the frame is not tracked, the
+// last frame's level-0 model maps are copied into the new frame so the frame
+// swap keeps the last model (Tracking.cc:98-117 keeps lastFrame on failure).
+template <bool kFirst>
+__global__ __launch_bounds__(kIcpThreads) void track_finish_kernel(
+    const TrackState* st_in, const FrameMeta* last, float thresh, const double* partials_prev,
+    int nb_prev, int min_count_prev, TrackState* st_final, FrameMeta* next, View<float4> vlast,
+    View<float4> nlast, View<float4> vnext, View<float4> nnext) {
+    __shared__ TrackState S;
+    __shared__ double totals[kIcpSums];
+    icp_state<kFirst>(&S, totals, st_in, last, thresh, partials_prev, nb_prev, min_count_prev,
+                      nullptr);
+    if (threadIdx.x == 0) track_end_state(&S);
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < kStateWords)
+            reinterpret_cast<unsigned long long*>(st_final)[threadIdx.x] =
+                reinterpret_cast<const unsigned long long*>(&S)[threadIdx.x];
+        if (threadIdx.x < 16) next->pose[threadIdx.x] = S.next_pose[threadIdx.x];
+        if (threadIdx.x == 0) pose_to_float(S.next_pose, &next->pose_f);
+    }
+    if (S.tracked) return;
     const int n = vnext.cols * vnext.rows;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int y = i / vnext.cols, x = i - y * vnext.cols;
         vnext.at(y, x) = vlast.at(y, x);
         nnext.at(y, x) = nlast.at(y, x);
     }
+}
+
+__global__ void track_init_kernel(FrameMeta* next, FrameMeta* last, TrackState* st) {
+    double I[16];
+    set_identity(I);
+    copy16(next->pose, I);
+    copy16(last->pose, I);
+    pose_to_float(I, &next->pose_f);
+    pose_to_float(I, &last->pose_f);
+    st->tracked = 1;
+    st->failed = 0;
+    st->done = 1;
+    st->applied = 0;
+    st->icp_count = 0;
+    st->icp_error = 0.f;
 }
 
 // Four pixels per thread, at most kIcpMaxBlocks workgroups.
@@ -358,43 +433,50 @@ int launch_icp_sums(View<float4> vcurr, View<float4> vlast, View<float4> ncurr,
     return SICP_OK;
 }
 
-int launch_icp_iterate(View<float4> vcurr, View<float4> vlast, View<float4> ncurr,
-                       View<float4> nlast, sicp_intrinsics K, int min_count, TrackState* st,
-                       double* partials, hipStream_t s) {
-    IcpGeom g{vcurr.cols, vcurr.rows, vcurr.cols * vcurr.rows, K.fx, K.fy, K.cx, K.cy};
+int launch_icp_iteration(const IcpLevel& cur, const IcpLevel* prev, const FrameMeta* last,
+                         float thresh, const TrackState* st_in, TrackState* st_out,
+                         const double* partials_prev, double* partials, hipStream_t s) {
+    IcpGeom g{cur.vcurr.cols, cur.vcurr.rows, cur.vcurr.cols * cur.vcurr.rows,
+              cur.K.fx,       cur.K.fy,       cur.K.cx,
+              cur.K.cy};
     const int nb = icp_blocks(g.n);
-    icp_iterate_kernel<<<nb, kIcpThreads, 0, s>>>(vcurr, vlast, ncurr, nlast, g, st, partials);
-    SICP_LAUNCH_CHECK("icp_iterate_kernel");
-    icp_solve_kernel<<<1, kIcpThreads, 0, s>>>(partials, nb, min_count, st);
-    SICP_LAUNCH_CHECK("icp_solve_kernel");
+    if (prev) {
+        const int nb_prev = icp_blocks(prev->vcurr.cols * prev->vcurr.rows);
+        icp_iteration_kernel<false><<<nb, kIcpThreads, 0, s>>>(
+            cur.vcurr, cur.vlast, cur.ncurr, cur.nlast, g, st_in, last, thresh, partials_prev,
+            nb_prev, prev->min_count, st_out, partials);
+    } else {
+        icp_iteration_kernel<true><<<nb, kIcpThreads, 0, s>>>(
+            cur.vcurr, cur.vlast, cur.ncurr, cur.nlast, g, nullptr, last, thresh, nullptr, 0, 0,
+            st_out, partials);
+    }
+    SICP_LAUNCH_CHECK("icp_iteration_kernel");
+    return SICP_OK;
+}
+
+int launch_track_finish(const IcpLevel* prev, const FrameMeta* last, float thresh,
+                        const TrackState* st_in, const double* partials_prev,
+                        TrackState* st_final, FrameMeta* next, View<float4> vlast,
+                        View<float4> nlast, View<float4> vnext, View<float4> nnext,
+                        hipStream_t s) {
+    const int n = vnext.cols * vnext.rows;
+    const int blocks = (int)div_up(n, kIcpThreads * 4);
+    if (prev) {
+        const int nb_prev = icp_blocks(prev->vcurr.cols * prev->vcurr.rows);
+        track_finish_kernel<false><<<blocks, kIcpThreads, 0, s>>>(
+            st_in, last, thresh, partials_prev, nb_prev, prev->min_count, st_final, next, vlast,
+            nlast, vnext, nnext);
+    } else {
+        track_finish_kernel<true><<<blocks, kIcpThreads, 0, s>>>(
+            nullptr, last, thresh, nullptr, 0, 0, st_final, next, vlast, nlast, vnext, nnext);
+    }
+    SICP_LAUNCH_CHECK("track_finish_kernel");
     return SICP_OK;
 }
 
 int launch_track_init(FrameMeta* next, FrameMeta* last, TrackState* st, hipStream_t s) {
     track_init_kernel<<<1, 1, 0, s>>>(next, last, st);
     SICP_LAUNCH_CHECK("track_init_kernel");
-    return SICP_OK;
-}
-
-int launch_track_begin(FrameMeta* next, FrameMeta* last, TrackState* st, float thresh,
-                       hipStream_t s) {
-    track_begin_kernel<<<1, 1, 0, s>>>(next, last, st, thresh);
-    SICP_LAUNCH_CHECK("track_begin_kernel");
-    return SICP_OK;
-}
-
-int launch_track_end(FrameMeta* next, TrackState* st, hipStream_t s) {
-    track_end_kernel<<<1, 1, 0, s>>>(next, st);
-    SICP_LAUNCH_CHECK("track_end_kernel");
-    return SICP_OK;
-}
-
-int launch_rollback_maps(View<float4> vlast, View<float4> nlast, View<float4> vnext,
-                         View<float4> nnext, const TrackState* st, hipStream_t s) {
-    int n = vnext.cols * vnext.rows;
-    int blocks = (int)div_up(n, 256 * 4);
-    rollback_maps_kernel<<<blocks, 256, 0, s>>>(vlast, nlast, vnext, nnext, st);
-    SICP_LAUNCH_CHECK("rollback_maps_kernel");
     return SICP_OK;
 }
 

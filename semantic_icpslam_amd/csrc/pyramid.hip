@@ -17,41 +17,57 @@ inline dim3 grid2d(int cols, int rows) { return dim3(div_up(cols, kTX), div_up(r
 // [max(y-2,0), min(y+3, rows-1)), columns likewise (the last row / column is
 // never a tap); zero-depth taps included; sigSpace = 0.5/16, sigRange = 2
 // (Pyrdown.cu:3-4).
-__global__ __launch_bounds__(256) void filter_depth_kernel(View<uint16_t> depth, View<float> raw,
-                                                           View<float> filtered, float scale_inv,
-                                                           float cutoff) {
-    const int x = blockIdx.x * kTX + threadIdx.x;
-    const int y = blockIdx.y * kTY + threadIdx.y;
-    if (x >= depth.cols || y >= depth.rows) return;
+// The 5x5 window is unrolled: every tap's load is issued at a clamped
+// address first, then the taps inside the clipped window are summed in the
+// reference's row-major order.
+SICP_DEV void filter_px(const View<uint16_t>& depth, const View<float>& raw,
+                        const View<float>& filtered, float scale_inv, float cutoff, int x, int y) {
     const float sig_space = 0.03125f, sig_range = 2.0f;
-    float center = (float)depth.at(y, x) * scale_inv;
-    raw.at(y, x) = center < cutoff ? center : qnanf();
     const int tx = min(x + 3, depth.cols - 1);
     const int ty = min(y + 3, depth.rows - 1);
+    uint16_t tap[25];
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy) {
+        const uint16_t* row = &depth.at(min(max(y - 2 + dy, 0), depth.rows - 1), 0);
+#pragma unroll
+        for (int dx = 0; dx < 5; ++dx) tap[dy * 5 + dx] = row[min(max(x - 2 + dx, 0), depth.cols - 1)];
+    }
+    float center = (float)tap[12] * scale_inv;
+    raw.at(y, x) = center < cutoff ? center : qnanf();
     float sum1 = 0.f, sum2 = 0.f;
-    for (int cy = max(y - 2, 0); cy < ty; ++cy) {
-        const uint16_t* row = &depth.at(cy, 0);
-        for (int cx = max(x - 2, 0); cx < tx; ++cx) {
-            float val = (float)row[cx] * scale_inv;
-            float space2 = (float)((x - cx) * (x - cx) + (y - cy) * (y - cy));
-            float color2 = (center - val) * (center - val);
-            float weight = expf_cw(-(space2 * sig_space + color2 * sig_range));
-            sum1 += val * weight;
-            sum2 += weight;
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy) {
+        const int cy = y - 2 + dy;
+#pragma unroll
+        for (int dx = 0; dx < 5; ++dx) {
+            const int cx = x - 2 + dx;
+            if (cy >= 0 && cy < ty && cx >= 0 && cx < tx) {
+                float val = (float)tap[dy * 5 + dx] * scale_inv;
+                float space2 = (float)((x - cx) * (x - cx) + (y - cy) * (y - cy));
+                float color2 = (center - val) * (center - val);
+                float weight = expf_cw(-(space2 * sig_space + color2 * sig_range));
+                sum1 += val * weight;
+                sum2 += weight;
+            }
         }
     }
     float fin = sum1 / sum2;
     filtered.at(y, x) = fin < cutoff ? fin : qnanf();
 }
 
-// ComputeVMapKernel (Pyrdown.cu:123-145). The depth test compares in double
-// (z > 0.1); invalid pixels carry x = NaN.
-__global__ __launch_bounds__(256) void vmap_kernel(View<float> depth, View<float4> vmap,
-                                                   float invfx, float invfy, float cx, float cy,
-                                                   float cutoff) {
+__global__ __launch_bounds__(256) void filter_depth_kernel(View<uint16_t> depth, View<float> raw,
+                                                           View<float> filtered, float scale_inv,
+                                                           float cutoff) {
     const int x = blockIdx.x * kTX + threadIdx.x;
     const int y = blockIdx.y * kTY + threadIdx.y;
     if (x >= depth.cols || y >= depth.rows) return;
+    filter_px(depth, raw, filtered, scale_inv, cutoff, x, y);
+}
+
+// ComputeVMapKernel (Pyrdown.cu:123-145). The depth test compares in double
+// (z > 0.1); invalid pixels carry x = NaN.
+SICP_DEV float4 vmap_px(const View<float>& depth, int x, int y, float invfx, float invfy, float cx,
+                        float cy, float cutoff) {
     float z = depth.at(y, x);
     float4 v;
     if (!isnan_(z) && (double)z > 0.1 && z < cutoff) {
@@ -65,10 +81,28 @@ __global__ __launch_bounds__(256) void vmap_kernel(View<float> depth, View<float
         v.z = z;
         v.w = qnanf();
     }
-    vmap.at(y, x) = v;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void vmap_kernel(View<float> depth, View<float4> vmap,
+                                                   float invfx, float invfy, float cx, float cy,
+                                                   float cutoff) {
+    const int x = blockIdx.x * kTX + threadIdx.x;
+    const int y = blockIdx.y * kTY + threadIdx.y;
+    if (x >= depth.cols || y >= depth.rows) return;
+    vmap.at(y, x) = vmap_px(depth, x, y, invfx, invfy, cx, cy, cutoff);
 }
 
 // ComputeNMapKernel (Pyrdown.cu:160-181).
+SICP_DEV float4 nmap_of(float4 c, float4 r, float4 d) {
+    const float nan = qnanf();
+    if (!isnan_(c.x) && !isnan_(r.x) && !isnan_(d.x)) {
+        v3 n = normalised(cross(sub(xyz(r), xyz(c)), sub(xyz(d), xyz(c))));
+        return make_float4(n.x, n.y, n.z, 1.0f);
+    }
+    return make_float4(nan, nan, nan, nan);
+}
+
 __global__ __launch_bounds__(256) void nmap_kernel(View<float4> vmap, View<float4> nmap) {
     const int x = blockIdx.x * kTX + threadIdx.x;
     const int y = blockIdx.y * kTY + threadIdx.y;
@@ -78,115 +112,290 @@ __global__ __launch_bounds__(256) void nmap_kernel(View<float4> vmap, View<float
         nmap.at(y, x) = make_float4(nan, nan, nan, nan);
         return;
     }
-    float4 c = vmap.at(y, x);
-    float4 r = vmap.at(y, x + 1);
-    float4 d = vmap.at(y + 1, x);
-    if (!isnan_(c.x) && !isnan_(r.x) && !isnan_(d.x)) {
-        v3 n = normalised(cross(sub(xyz(r), xyz(c)), sub(xyz(d), xyz(c))));
-        nmap.at(y, x) = make_float4(n.x, n.y, n.z, 1.0f);
-    } else {
-        nmap.at(y, x) = make_float4(nan, nan, nan, nan);
-    }
+    nmap.at(y, x) = nmap_of(vmap.at(y, x), vmap.at(y, x + 1), vmap.at(y + 1, x));
 }
 
 // PyrDownGaussKernel (Pyrdown.cu:194-225): 5x5 binomial, weight index taken
 // from the clipped window end, NaN taps skipped, integer weight count.
-__constant__ float kGauss[25] = {1, 4, 6, 4, 1, 4, 16, 24, 16, 4, 6, 24, 36,
-                                 24, 6, 4, 16, 24, 16, 4, 1, 4, 6, 4, 1};
+// 5x5 binomial {1,4,6,4,1} x {1,4,6,4,1} of the reference, as a product of
+// its two 1-D taps (exact in float).
+SICP_DEV float binom5(int i) { return i == 2 ? 6.f : (i == 1 || i == 3) ? 4.f : 1.f; }
 
 template <class T> SICP_DEV T out_cast(float v);
 template <> SICP_DEV float out_cast<float>(float v) { return v; }
 template <> SICP_DEV uint8_t out_cast<uint8_t>(float v) { return (uint8_t)(int)v; }
+
+// One output pixel from a source read through `src(y, x)` (global memory or
+// an LDS tile) of size scols x srows.
+template <class T, class Src>
+SICP_DEV T pyrdown_px(const Src& src, int scols, int srows, int x, int y) {
+    const int tx = min(2 * x + 3, scols - 1);
+    const int ty = min(2 * y + 3, srows - 1);
+    T tap[25];
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy) {
+        const int cy = min(max(2 * y - 2 + dy, 0), srows - 1);
+#pragma unroll
+        for (int dx = 0; dx < 5; ++dx) tap[dy * 5 + dx] = src(cy, min(max(2 * x - 2 + dx, 0), scols - 1));
+    }
+    float sum = 0.f;
+    int count = 0;
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy) {
+        const int cy = 2 * y - 2 + dy;
+#pragma unroll
+        for (int dx = 0; dx < 5; ++dx) {
+            const int cx = 2 * x - 2 + dx;
+            if (cy >= 0 && cy < ty && cx >= 0 && cx < tx) {
+                float s = (float)tap[dy * 5 + dx];
+                if (!isnan_(s)) {
+                    float g = binom5(ty - cy - 1) * binom5(tx - cx - 1);
+                    sum += s * g;
+                    count = (int)((float)count + g);
+                }
+            }
+        }
+    }
+    return out_cast<T>(sum / (float)count);
+}
+
+template <class T> struct GlobalSrc {
+    View<T> v;
+    SICP_DEV T operator()(int y, int x) const { return v.at(y, x); }
+};
 
 template <class T>
 __global__ __launch_bounds__(256) void pyrdown_kernel(View<T> src, View<T> dst) {
     const int x = blockIdx.x * kTX + threadIdx.x;
     const int y = blockIdx.y * kTY + threadIdx.y;
     if (x >= dst.cols || y >= dst.rows) return;
-    const int tx = min(2 * x + 3, src.cols - 1);
-    const int ty = min(2 * y + 3, src.rows - 1);
-    float sum = 0.f;
-    int count = 0;
-    for (int cy = max(0, 2 * y - 2); cy < ty; ++cy) {
-        const T* row = &src.at(cy, 0);
-        for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
-            float s = (float)row[cx];
-            if (!isnan_(s)) {
-                float g = kGauss[(ty - cy - 1) * 5 + (tx - cx - 1)];
-                sum += s * g;
-                count = (int)((float)count + g);
-            }
-        }
-    }
-    dst.at(y, x) = out_cast<T>(sum / (float)count);
+    dst.at(y, x) = pyrdown_px<T>(GlobalSrc<T>{src}, src.cols, src.rows, x, y);
 }
 
 // ImageToIntensityKernel (Pyrdown.cu:250-260): double weights of G, R, B.
-__global__ __launch_bounds__(256) void intensity_kernel(View<uint8_t> rgb, View<uint8_t> gray) {
-    const int x = blockIdx.x * kTX + threadIdx.x;
-    const int y = blockIdx.y * kTY + threadIdx.y;
-    if (x >= gray.cols || y >= gray.rows) return;
+SICP_DEV void intensity_px(const View<uint8_t>& rgb, const View<uint8_t>& gray, int x, int y) {
     const uint8_t* p = &rgb.at(y, 3 * x);
     int value = (int)(0.2125 * (double)p[1] + 0.7154 * (double)p[0] + 0.0721 * (double)p[2]);
     gray.at(y, x) = (uint8_t)value;
 }
 
-// ComputeDerivativeImageKernel (Pyrdown.cu:280-304): the tap index counts
-// down from 8 over the clipped 3x3 window.
-__constant__ float kSobelX[9] = {1, 0, -1, 2, 0, -2, 1, 0, -1};
-__constant__ float kSobelY[9] = {1, 2, 1, 0, 0, 0, -1, -2, -1};
-
-__global__ __launch_bounds__(256) void derivative_kernel(View<uint8_t> img, View<int16_t> dx,
-                                                         View<int16_t> dy) {
+__global__ __launch_bounds__(256) void intensity_kernel(View<uint8_t> rgb, View<uint8_t> gray) {
     const int x = blockIdx.x * kTX + threadIdx.x;
     const int y = blockIdx.y * kTY + threadIdx.y;
-    if (x >= img.cols || y >= img.rows) return;
+    if (x >= gray.cols || y >= gray.rows) return;
+    intensity_px(rgb, gray, x, y);
+}
+
+// ComputeDerivativeImageKernel (Pyrdown.cu:280-304): the tap index counts
+// down from 8 over the clipped 3x3 window.
+// Sobel pair of the reference, {1,0,-1, 2,0,-2, 1,0,-1} and its transpose,
+// written as closed forms of the tap index below.
+
+SICP_DEV void derivative_px(const View<uint8_t>& img, const View<int16_t>& dx,
+                            const View<int16_t>& dy, int x, int y) {
+    uint8_t tap[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const uint8_t* row = &img.at(min(max(y - 1 + j, 0), img.rows - 1), 0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tap[j * 3 + i] = row[min(max(x - 1 + i, 0), img.cols - 1)];
+    }
     float dxv = 0.f, dyv = 0.f;
     int k = 8;
-    for (int j = max(y - 1, 0); j <= min(y + 1, img.rows - 1); ++j) {
-        for (int i = max(x - 1, 0); i <= min(x + 1, img.cols - 1); ++i) {
-            float p = (float)img.at(j, i);
-            dxv += p * kSobelX[k];
-            dyv += p * kSobelY[k];
-            --k;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int cy = y - 1 + j, cx = x - 1 + i;
+            if (cy >= 0 && cy < img.rows && cx >= 0 && cx < img.cols) {
+                float p = (float)tap[j * 3 + i];
+                const int r = k / 3, c = k - 3 * r;
+                dxv += p * ((float)(1 - c) * (r == 1 ? 2.f : 1.f));
+                dyv += p * ((float)(1 - r) * (c == 1 ? 2.f : 1.f));
+                --k;
+            }
         }
     }
     dx.at(y, x) = (int16_t)(int)dxv;
     dy.at(y, x) = (int16_t)(int)dyv;
 }
 
+__global__ __launch_bounds__(256) void derivative_kernel(View<uint8_t> img, View<int16_t> dx,
+                                                         View<int16_t> dy) {
+    const int x = blockIdx.x * kTX + threadIdx.x;
+    const int y = blockIdx.y * kTY + threadIdx.y;
+    if (x >= img.cols || y >= img.rows) return;
+    derivative_px(img, dx, dy, x, y);
+}
+
 // ResizeMapKernel (Pyrdown.cu:318-346), guarded on the destination size.
+// The 2x2 block of `v`/`n` (row-major 00, 01, 10, 11) to one coarser pixel.
+SICP_DEV void resize_px(const float4 v[4], const float4 n[4], float4& vo, float4& no) {
+    const float nan = qnanf();
+    if (isnan_(v[0].x) || isnan_(v[1].x) || isnan_(v[2].x) || isnan_(v[3].x)) {
+        vo = make_float4(nan, nan, nan, nan);
+    } else {
+        vo.x = (((v[0].x + v[1].x) + v[2].x) + v[3].x) / 4.0f;
+        vo.y = (((v[0].y + v[1].y) + v[2].y) + v[3].y) / 4.0f;
+        vo.z = (((v[0].z + v[1].z) + v[2].z) + v[3].z) / 4.0f;
+        vo.w = (((v[0].w + v[1].w) + v[2].w) + v[3].w) / 4.0f;
+    }
+    if (isnan_(n[0].x) || isnan_(n[1].x) || isnan_(n[2].x) || isnan_(n[3].x)) {
+        no = make_float4(nan, nan, nan, nan);
+    } else {
+        float mx = (((n[0].x + n[1].x) + n[2].x) + n[3].x) / 4.0f;
+        float my = (((n[0].y + n[1].y) + n[2].y) + n[3].y) / 4.0f;
+        float mz = (((n[0].z + n[1].z) + n[2].z) + n[3].z) / 4.0f;
+        float mw = (((n[0].w + n[1].w) + n[2].w) + n[3].w) / 4.0f;
+        float nn = sqrtf(mx * mx + my * my + mz * mz + mw * mw);
+        no = make_float4(mx / nn, my / nn, mz / nn, mw / nn);
+    }
+}
+
+SICP_DEV void resize_from(const View<float4>& vsrc, const View<float4>& nsrc, int x, int y,
+                          float4& vo, float4& no) {
+    float4 v[4], n[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = vsrc.at(2 * y + (k >> 1), 2 * x + (k & 1));
+        n[k] = nsrc.at(2 * y + (k >> 1), 2 * x + (k & 1));
+    }
+    resize_px(v, n, vo, no);
+}
+
 __global__ __launch_bounds__(256) void resize_map_kernel(View<float4> vsrc, View<float4> nsrc,
                                                          View<float4> vdst, View<float4> ndst) {
     const int x = blockIdx.x * kTX + threadIdx.x;
     const int y = blockIdx.y * kTY + threadIdx.y;
     if (x >= vdst.cols || y >= vdst.rows) return;
-    const float nan = qnanf();
-    float4 v00 = vsrc.at(2 * y, 2 * x), v01 = vsrc.at(2 * y, 2 * x + 1);
-    float4 v10 = vsrc.at(2 * y + 1, 2 * x), v11 = vsrc.at(2 * y + 1, 2 * x + 1);
-    float4 n00 = nsrc.at(2 * y, 2 * x), n01 = nsrc.at(2 * y, 2 * x + 1);
-    float4 n10 = nsrc.at(2 * y + 1, 2 * x), n11 = nsrc.at(2 * y + 1, 2 * x + 1);
-    if (isnan_(v00.x) || isnan_(v01.x) || isnan_(v10.x) || isnan_(v11.x)) {
-        vdst.at(y, x) = make_float4(nan, nan, nan, nan);
-    } else {
-        float4 o;
-        o.x = (((v00.x + v01.x) + v10.x) + v11.x) / 4.0f;
-        o.y = (((v00.y + v01.y) + v10.y) + v11.y) / 4.0f;
-        o.z = (((v00.z + v01.z) + v10.z) + v11.z) / 4.0f;
-        o.w = (((v00.w + v01.w) + v10.w) + v11.w) / 4.0f;
-        vdst.at(y, x) = o;
+    float4 vo, no;
+    resize_from(vsrc, nsrc, x, y, vo, no);
+    vdst.at(y, x) = vo;
+    ndst.at(y, x) = no;
+}
+
+// ------------------------------------------------ whole-frame pyramid ----
+// Frame::Create / FillImages / ResizeImages (Core/Frame.cc:33-98) in three
+// launches instead of seventeen. Each launch runs the same per-pixel
+// functions as the single-image kernels above, so the results are
+// identical; what changes is only which launch computes which pixel.
+//
+//  1. level 0: bilateral filter + intensity of the new frame, and the model
+//     pyramid of the last frame (levels 1 and 2, the latter straight from
+//     level 0 through the same 2x2 means);
+//  2. PyrDown to levels 1 and 2 of depth and intensity: a workgroup owns an
+//     8x8 tile of level 2, computes the 20x20 level-1 window it needs into
+//     LDS (writing the level-1 pixels it owns), then its level-2 pixels;
+//  3. vertex map, normal map (the two neighbour vertices recomputed from
+//     depth) and derivatives for every level.
+
+// Workgroup b of a launch that covers several 64x4-tiled images in turn.
+SICP_DEV bool take_tile(int& b, int cols, int rows, int& x, int& y) {
+    const int nx = (cols + kTX - 1) / kTX, n = nx * ((rows + kTY - 1) / kTY);
+    if (b >= n) {
+        b -= n;
+        return false;
     }
-    if (isnan_(n00.x) || isnan_(n01.x) || isnan_(n10.x) || isnan_(n11.x)) {
-        ndst.at(y, x) = make_float4(nan, nan, nan, nan);
-    } else {
-        float mx = (((n00.x + n01.x) + n10.x) + n11.x) / 4.0f;
-        float my = (((n00.y + n01.y) + n10.y) + n11.y) / 4.0f;
-        float mz = (((n00.z + n01.z) + n10.z) + n11.z) / 4.0f;
-        float mw = (((n00.w + n01.w) + n10.w) + n11.w) / 4.0f;
-        float nn = sqrtf(mx * mx + my * my + mz * mz + mw * mw);
-        ndst.at(y, x) = make_float4(mx / nn, my / nn, mz / nn, mw / nn);
+    x = (b % nx) * kTX + threadIdx.x;
+    y = (b / nx) * kTY + threadIdx.y;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void frame_level0_kernel(View<uint16_t> d16, View<uint8_t> rgb,
+                                                           View<float> range, View<float> depth0,
+                                                           View<uint8_t> gray0, float scale_inv,
+                                                           float cutoff, ModelPyramid mp) {
+    int b = blockIdx.x, x, y;
+    if (take_tile(b, d16.cols, d16.rows, x, y)) {
+        if (x < d16.cols && y < d16.rows) {
+            filter_px(d16, range, depth0, scale_inv, cutoff, x, y);
+            intensity_px(rgb, gray0, x, y);
+        }
+        return;
+    }
+    if (take_tile(b, mp.v[1].cols, mp.v[1].rows, x, y)) {
+        if (x < mp.v[1].cols && y < mp.v[1].rows) {
+            float4 vo, no;
+            resize_from(mp.v[0], mp.n[0], x, y, vo, no);
+            mp.v[1].at(y, x) = vo;
+            mp.n[1].at(y, x) = no;
+        }
+        return;
+    }
+    if (take_tile(b, mp.v[2].cols, mp.v[2].rows, x, y)) {
+        if (x < mp.v[2].cols && y < mp.v[2].rows) {
+            float4 v[4], n[4], vo, no;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                resize_from(mp.v[0], mp.n[0], 2 * x + (k & 1), 2 * y + (k >> 1), v[k], n[k]);
+            resize_px(v, n, vo, no);
+            mp.v[2].at(y, x) = vo;
+            mp.n[2].at(y, x) = no;
+        }
     }
 }
+
+template <class T> struct TileSrc {
+    const T* p;
+    int x0, y0;
+    SICP_DEV T operator()(int y, int x) const { return p[(y - y0) * 20 + (x - x0)]; }
+};
+
+__global__ __launch_bounds__(256) void pyrdown_levels_kernel(View<float> d0, View<uint8_t> g0,
+                                                             View<float> d1, View<uint8_t> g1,
+                                                             View<float> d2, View<uint8_t> g2) {
+    __shared__ float rd[400];
+    __shared__ uint8_t rg[400];
+    const int X0 = blockIdx.x * 8, Y0 = blockIdx.y * 8;  // level-2 tile origin
+    const int rx0 = 2 * X0 - 2, ry0 = 2 * Y0 - 2;        // level-1 window origin
+    // Level-1 pixels owned here: [2*X0, 2*X0 + 16), the last tile column
+    // (row) also the odd remainder up to the image edge.
+    const int ox1 = blockIdx.x == gridDim.x - 1 ? d1.cols : 2 * X0 + 16;
+    const int oy1 = blockIdx.y == gridDim.y - 1 ? d1.rows : 2 * Y0 + 16;
+    for (int i = threadIdx.x; i < 400; i += blockDim.x) {
+        const int rx = rx0 + i % 20, ry = ry0 + i / 20;
+        if (rx < 0 || ry < 0 || rx >= d1.cols || ry >= d1.rows) continue;
+        const float v = pyrdown_px<float>(GlobalSrc<float>{d0}, d0.cols, d0.rows, rx, ry);
+        const uint8_t u = pyrdown_px<uint8_t>(GlobalSrc<uint8_t>{g0}, g0.cols, g0.rows, rx, ry);
+        rd[i] = v;
+        rg[i] = u;
+        if (rx >= 2 * X0 && rx < ox1 && ry >= 2 * Y0 && ry < oy1) {
+            d1.at(ry, rx) = v;
+            g1.at(ry, rx) = u;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int x = X0 + (threadIdx.x & 7), y = Y0 + (threadIdx.x >> 3);
+        if (x < d2.cols && y < d2.rows) {
+            d2.at(y, x) = pyrdown_px<float>(TileSrc<float>{rd, rx0, ry0}, d1.cols, d1.rows, x, y);
+            g2.at(y, x) =
+                pyrdown_px<uint8_t>(TileSrc<uint8_t>{rg, rx0, ry0}, g1.cols, g1.rows, x, y);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void frame_maps_kernel(FramePyramid P, float cutoff) {
+    int b = blockIdx.x, x = 0, y = 0, l = 0;
+    for (; l < 3; ++l)
+        if (take_tile(b, P.depth[l].cols, P.depth[l].rows, x, y)) break;
+    if (l == 3) return;
+    const View<float>& d = P.depth[l];
+    if (x >= d.cols || y >= d.rows) return;
+    const float fx = P.invfx[l], fy = P.invfy[l], cx = P.cx[l], cy = P.cy[l];
+    const float4 c = vmap_px(d, x, y, fx, fy, cx, cy, cutoff);
+    P.vmap[l].at(y, x) = c;
+    if (x == d.cols - 1 || y == d.rows - 1) {
+        const float nan = qnanf();
+        P.nmap[l].at(y, x) = make_float4(nan, nan, nan, nan);
+    } else {
+        P.nmap[l].at(y, x) = nmap_of(c, vmap_px(d, x + 1, y, fx, fy, cx, cy, cutoff),
+                                     vmap_px(d, x, y + 1, fx, fy, cx, cy, cutoff));
+    }
+    derivative_px(P.image[l], P.dx[l], P.dy[l], x, y);
+}
+
+inline int tiles(int cols, int rows) { return (int)(div_up(cols, kTX) * div_up(rows, kTY)); }
 
 }  // namespace
 
@@ -241,6 +450,26 @@ int launch_resize_map(View<float4> vsrc, View<float4> nsrc, View<float4> vdst, V
     resize_map_kernel<<<grid2d(vdst.cols, vdst.rows), dim3(kTX, kTY), 0, s>>>(vsrc, nsrc, vdst,
                                                                               ndst);
     SICP_LAUNCH_CHECK("resize_map_kernel");
+    return SICP_OK;
+}
+
+int launch_frame_pyramid(View<uint16_t> depth_u16, View<uint8_t> rgb, View<float> range,
+                         const FramePyramid& P, float scale_inv, float cutoff,
+                         const ModelPyramid& last_model, hipStream_t s) {
+    const ModelPyramid& mp = last_model;
+    const int n0 = tiles(depth_u16.cols, depth_u16.rows) + tiles(mp.v[1].cols, mp.v[1].rows) +
+                   tiles(mp.v[2].cols, mp.v[2].rows);
+    frame_level0_kernel<<<n0, dim3(kTX, kTY), 0, s>>>(depth_u16, rgb, range, P.depth[0],
+                                                      P.image[0], scale_inv, cutoff, mp);
+    SICP_LAUNCH_CHECK("frame_level0_kernel");
+    const dim3 g2((unsigned)div_up(P.depth[2].cols, 8), (unsigned)div_up(P.depth[2].rows, 8));
+    pyrdown_levels_kernel<<<g2, 256, 0, s>>>(P.depth[0], P.image[0], P.depth[1], P.image[1],
+                                             P.depth[2], P.image[2]);
+    SICP_LAUNCH_CHECK("pyrdown_levels_kernel");
+    int n2 = 0;
+    for (int l = 0; l < 3; ++l) n2 += tiles(P.depth[l].cols, P.depth[l].rows);
+    frame_maps_kernel<<<n2, dim3(kTX, kTY), 0, s>>>(P, cutoff);
+    SICP_LAUNCH_CHECK("frame_maps_kernel");
     return SICP_OK;
 }
 

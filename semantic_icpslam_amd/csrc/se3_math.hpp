@@ -28,19 +28,21 @@ SE3_HD void create_matrix(const float* s, double* A, double* b) {
 
 // x = A^-1 b by LDL^T without pivoting (a zero pivot gives a zero solution
 // component, as Eigen's LDLT::solve, Tracking.cc:493). Returns false on a
-// zero pivot.
+// zero pivot. One reciprocal per pivot: the solve runs on one GPU thread per
+// iteration, where every FP64 division is a long dependent sequence.
 SE3_HD bool ldlt_solve6(const double* A, const double* b, double* x) {
-    double L[36], D[6];
+    double L[36], D[6], Dinv[6];
     for (int i = 0; i < 36; ++i) L[i] = 0.0;
     for (int j = 0; j < 6; ++j) {
         double d = A[j * 6 + j];
         for (int k = 0; k < j; ++k) d -= L[j * 6 + k] * L[j * 6 + k] * D[k];
         D[j] = d;
+        Dinv[j] = (d != 0.0) ? 1.0 / d : 0.0;
         L[j * 6 + j] = 1.0;
         for (int i = j + 1; i < 6; ++i) {
             double s = A[i * 6 + j];
             for (int k = 0; k < j; ++k) s -= L[i * 6 + k] * L[j * 6 + k] * D[k];
-            L[i * 6 + j] = (d != 0.0) ? s / d : 0.0;
+            L[i * 6 + j] = s * Dinv[j];
         }
     }
     double z[6];
@@ -51,12 +53,8 @@ SE3_HD bool ldlt_solve6(const double* A, const double* b, double* x) {
     }
     bool ok = true;
     for (int i = 0; i < 6; ++i) {
-        if (D[i] != 0.0) {
-            z[i] = z[i] / D[i];
-        } else {
-            z[i] = 0.0;
-            ok = false;
-        }
+        z[i] = z[i] * Dinv[i];
+        if (D[i] == 0.0) ok = false;
     }
     for (int i = 5; i >= 0; --i) {
         double s = z[i];
@@ -70,6 +68,8 @@ SE3_HD bool ldlt_solve6(const double* A, const double* b, double* x) {
 // (cos(t/2), sin(t/2) w / t), translation V v with
 // V = I + (1 - cos t)/t^2 W + (t - sin t)/t^3 W^2 (small-angle series below
 // 1e-10) — the closed form Sophus::SE3d::exp evaluates (Tracking.cc:494).
+// One sincos of t/2 serves all four trigonometric values:
+// 1 - cos t = 2 sin^2(t/2) and sin t = 2 sin(t/2) cos(t/2).
 SE3_HD void se3_exp(const double* xi, double* T) {
     const double* v = xi;
     const double* w = xi + 3;
@@ -77,14 +77,16 @@ SE3_HD void se3_exp(const double* xi, double* T) {
     double theta_sq = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
     double theta = sqrt(theta_sq);
     double half = 0.5 * theta;
-    double imag, real;
+    double imag, real, sh = 0.0, ch = 1.0, inv_t = 0.0;
     if (theta_sq < eps * eps) {
         double t4 = theta_sq * theta_sq;
         imag = 0.5 - theta_sq / 48.0 + t4 / 3840.0;
         real = 1.0 - theta_sq / 8.0 + t4 / 384.0;
     } else {
-        imag = sin(half) / theta;
-        real = cos(half);
+        sincos(half, &sh, &ch);
+        inv_t = 1.0 / theta;
+        imag = sh * inv_t;
+        real = ch;
     }
     double qw = real, qx = imag * w[0], qy = imag * w[1], qz = imag * w[2];
     double tx = 2 * qx, ty = 2 * qy, tz = 2 * qz;
@@ -104,8 +106,13 @@ SE3_HD void se3_exp(const double* xi, double* T) {
             for (int j = 0; j < 3; ++j)
                 W2[i * 3 + j] = W[i * 3 + 0] * W[0 * 3 + j] + W[i * 3 + 1] * W[1 * 3 + j] +
                                 W[i * 3 + 2] * W[2 * 3 + j];
-        double c1 = (1.0 - cos(theta)) / theta_sq;
-        double c2 = (theta - sin(theta)) / (theta_sq * theta);
+        if (inv_t == 0.0) {  // theta^2 rounded below eps^2 but theta did not
+            sincos(half, &sh, &ch);
+            inv_t = 1.0 / theta;
+        }
+        const double st = sh * inv_t;
+        double c1 = 2.0 * st * st;                                    // (1 - cos t) / t^2
+        double c2 = (theta - 2.0 * sh * ch) * (inv_t * inv_t * inv_t);  // (t - sin t) / t^3
         for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0 ? 1.0 : 0.0) + c1 * W[i] + c2 * W2[i];
     }
     for (int i = 0; i < 3; ++i) {

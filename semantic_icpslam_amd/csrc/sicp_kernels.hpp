@@ -19,6 +19,24 @@ int launch_derivative(View<uint8_t> img, View<int16_t> dx, View<int16_t> dy, hip
 int launch_resize_map(View<float4> vsrc, View<float4> nsrc, View<float4> vdst, View<float4> ndst,
                       hipStream_t s);
 
+// The images of one frame at three pyramid levels (Frame, Core/Frame.h).
+struct FramePyramid {
+    View<float> depth[3];
+    View<float4> vmap[3], nmap[3];
+    View<uint8_t> image[3];
+    View<int16_t> dx[3], dy[3];
+    float invfx[3], invfy[3], cx[3], cy[3];
+};
+// A frame's model maps (raycast at level 0, 2x2 means below).
+struct ModelPyramid {
+    View<float4> v[3], n[3];
+};
+// FillImages of the new frame and ResizeImages of the last one, in three
+// launches; pixel for pixel the same as the single-image launches above.
+int launch_frame_pyramid(View<uint16_t> depth_u16, View<uint8_t> rgb, View<float> range,
+                         const FramePyramid& P, float scale_inv, float cutoff,
+                         const ModelPyramid& last_model, hipStream_t s);
+
 // ---------------------------------------------------------------- ICP -----
 constexpr int kIcpThreads = 256;
 constexpr int kIcpMaxBlocks = 512;
@@ -37,9 +55,7 @@ struct TrackState {
     double pose0[16];
     double delta[16];
     double candidate[16];
-    double A[36], b[6], x[6];
     sicp_pose_f curr_f, last_f;
-    float sums[32];
     float icp_error, last_error, candidate_icp, thresh;
     int icp_count, break_count, applied, done;
     int failed, tracked, frame_index, pad;
@@ -53,18 +69,30 @@ struct IcpScratch {
 int launch_icp_sums(View<float4> vcurr, View<float4> vlast, View<float4> ncurr,
                     View<float4> nlast, PoseArg curr, PoseArg last, sicp_intrinsics K,
                     IcpScratch scratch, float* sums_out, hipStream_t s);
-// One Gauss-Newton iteration of ComputeSE3 driven by `st`.
-int launch_icp_iterate(View<float4> vcurr, View<float4> vlast, View<float4> ncurr,
-                       View<float4> nlast, sicp_intrinsics K, int min_count, TrackState* st,
-                       double* partials, hipStream_t s);
+
+// The maps of one pyramid level and its correspondence minimum.
+struct IcpLevel {
+    View<float4> vcurr, vlast, ncurr, nlast;
+    sicp_intrinsics K;
+    int min_count;
+};
+// One Gauss-Newton iteration of ComputeSE3: the update of the previous
+// iteration (`prev` = its level; null for the first iteration of a frame,
+// whose state is built from `last`'s pose), then this iteration's pixel pass
+// into `partials`. The state moves from `st_in` to `st_out` (two buffers,
+// alternating).
+int launch_icp_iteration(const IcpLevel& cur, const IcpLevel* prev, const FrameMeta* last,
+                         float thresh, const TrackState* st_in, TrackState* st_out,
+                         const double* partials_prev, double* partials, hipStream_t s);
+// The last iteration's update, the final gate, the new frame's pose and,
+// when the frame is not tracked, the copy of the last frame's level-0 model
+// maps into the new frame. The frame's final state goes to `st_final`.
+int launch_track_finish(const IcpLevel* prev, const FrameMeta* last, float thresh,
+                        const TrackState* st_in, const double* partials_prev,
+                        TrackState* st_final, FrameMeta* next, View<float4> vlast,
+                        View<float4> nlast, View<float4> vnext, View<float4> nnext,
+                        hipStream_t s);
 int launch_track_init(FrameMeta* next, FrameMeta* last, TrackState* st, hipStream_t s);
-int launch_track_begin(FrameMeta* next, FrameMeta* last, TrackState* st, float thresh,
-                       hipStream_t s);
-int launch_track_end(FrameMeta* next, TrackState* st, hipStream_t s);
-// When the frame was not tracked, copy the last frame's level-0 model maps
-// into `next` so the host-side frame swap keeps the last model.
-int launch_rollback_maps(View<float4> vlast, View<float4> nlast, View<float4> vnext,
-                         View<float4> nnext, const TrackState* st, hipStream_t s);
 
 // Host FP64 solve + SE(3) update (same code path as the device).
 void se3_update_host(const double* A, const double* b, double* delta, const double* last_pose,

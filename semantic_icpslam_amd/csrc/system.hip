@@ -70,8 +70,8 @@ struct sicp_system {
     sicp::FrameBuf fa, fb;
     sicp::FrameBuf* next = &fa;
     sicp::FrameBuf* last = &fb;
-    sicp::TrackState* st = nullptr;
-    double* partials = nullptr;
+    sicp::TrackState* st = nullptr;    // [0] the frame's final state, [1..2] iterations
+    double* partials = nullptr;        // two sets of ICP partial rows, alternating
     sicp::DevImage<float> zmin, zmax;
     sicp_intrinsics K[3];
     int frame_index = 0;
@@ -109,37 +109,30 @@ static void stage_end(sicp_system* s, int stage, size_t i, hipStream_t st) {
 }
 
 // Frame::FillImages (Core/Frame.cc:68-92).
-static int fill_images(sicp_system* s, FrameBuf& f, const sicp_image& depth_u16,
-                       const sicp_image& rgb, hipStream_t st) {
-    int rc;
-    const float cutoff = s->cfg.depth_cutoff;
-    if ((rc = launch_filter_depth(view<uint16_t>(depth_u16), f.range.v(), f.depth[0].v(),
-                                  (float)(1.0 / (double)s->cfg.depth_scale), cutoff, st)))
-        return rc;
-    if ((rc = launch_intensity(view<uint8_t>(rgb), f.image[0].v(), st))) return rc;
-    for (int i = 1; i < 3; ++i) {
-        if ((rc = launch_pyrdown_f32(f.depth[i - 1].v(), f.depth[i].v(), st))) return rc;
-        if ((rc = launch_pyrdown_u8(f.image[i - 1].v(), f.image[i].v(), st))) return rc;
-    }
+// Frame::Create / FillImages of the new frame (Core/Frame.cc:33-92) and
+// ResizeImages of the last frame's model maps (:94-98).
+static int frame_pyramid(sicp_system* s, FrameBuf& f, FrameBuf& last,
+                         const sicp_image& depth_u16, const sicp_image& rgb, hipStream_t st) {
+    FramePyramid P;
+    ModelPyramid mp;
     for (int i = 0; i < 3; ++i) {
         const sicp_intrinsics& k = s->K[i];
-        if ((rc = launch_vmap(f.depth[i].v(), f.vmap[i].v(), (float)(1.0 / (double)k.fx),
-                              (float)(1.0 / (double)k.fy), k.cx, k.cy, cutoff, st)))
-            return rc;
-        if ((rc = launch_nmap(f.vmap[i].v(), f.nmap[i].v(), st))) return rc;
-        if ((rc = launch_derivative(f.image[i].v(), f.dx[i].v(), f.dy[i].v(), st))) return rc;
+        P.depth[i] = f.depth[i].v();
+        P.vmap[i] = f.vmap[i].v();
+        P.nmap[i] = f.nmap[i].v();
+        P.image[i] = f.image[i].v();
+        P.dx[i] = f.dx[i].v();
+        P.dy[i] = f.dy[i].v();
+        P.invfx[i] = (float)(1.0 / (double)k.fx);
+        P.invfy[i] = (float)(1.0 / (double)k.fy);
+        P.cx[i] = k.cx;
+        P.cy[i] = k.cy;
+        mp.v[i] = last.vmap[i].v();
+        mp.n[i] = last.nmap[i].v();
     }
-    return SICP_OK;
-}
-
-// Frame::ResizeImages (Core/Frame.cc:94-98).
-static int resize_images(FrameBuf& f, hipStream_t st) {
-    for (int i = 1; i < 3; ++i) {
-        int rc = launch_resize_map(f.vmap[i - 1].v(), f.nmap[i - 1].v(), f.vmap[i].v(),
-                                   f.nmap[i].v(), st);
-        if (rc) return rc;
-    }
-    return SICP_OK;
+    return launch_frame_pyramid(view<uint16_t>(depth_u16), view<uint8_t>(rgb), f.range.v(), P,
+                                (float)(1.0 / (double)s->cfg.depth_scale), s->cfg.depth_cutoff,
+                                mp, st);
 }
 
 static void system_free(sicp_system* s) {
@@ -200,9 +193,10 @@ int sicp_system_create(const sicp_system_config* cfg, sicp_system** out) {
         system_free(s);
         return rc;
     }
-    hipError_t e = hipMalloc((void**)&s->st, sizeof(TrackState));
-    if (e == hipSuccess) e = hipMemset(s->st, 0, sizeof(TrackState));
-    if (e == hipSuccess) e = hipMalloc((void**)&s->partials, sizeof(double) * kIcpMaxBlocks * kIcpSums);
+    hipError_t e = hipMalloc((void**)&s->st, 3 * sizeof(TrackState));
+    if (e == hipSuccess) e = hipMemset(s->st, 0, 3 * sizeof(TrackState));
+    if (e == hipSuccess)
+        e = hipMalloc((void**)&s->partials, 2 * sizeof(double) * kIcpMaxBlocks * kIcpSums);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         rc = hip_fail(e, "system_create");
@@ -241,8 +235,7 @@ int sicp_system_grab_image(sicp_system* s, sicp_image depth_u16, sicp_image rgb_
     size_t ev;
     // Tracker::GrabFrame: model pyramid of the last frame, then the new frame.
     ev = stage_begin(s, st);
-    if ((rc = resize_images(*s->last, st))) return rc;
-    if ((rc = fill_images(s, *s->next, depth_u16, rgb_u8x3, st))) return rc;
+    if ((rc = frame_pyramid(s, *s->next, *s->last, depth_u16, rgb_u8x3, st))) return rc;
     stage_end(s, kStagePyramid, ev, st);
     FrameBuf& nf = *s->next;
     FrameBuf& lf = *s->last;
@@ -251,18 +244,28 @@ int sicp_system_grab_image(sicp_system* s, sicp_image depth_u16, sicp_image rgb_
         // InitTracking (Tracking.cc:233-240): both poses identity.
         if ((rc = launch_track_init(nf.meta, lf.meta, s->st, st))) return rc;
     } else {
-        if ((rc = launch_track_begin(nf.meta, lf.meta, s->st, s->cfg.icp_thresh, st))) return rc;
+        // Every launch runs the previous iteration's update and its own pixel
+        // pass; states and partial rows alternate between two buffers.
+        TrackState* iter = s->st + 1;
+        const size_t psz = (size_t)kIcpMaxBlocks * kIcpSums;
+        IcpLevel prev{};
+        int k = 0;
         for (int l = 2; l >= 0; --l) {
-            for (int j = 0; j < s->cfg.icp_iters[l]; ++j) {
-                rc = launch_icp_iterate(nf.vmap[l].v(), lf.vmap[l].v(), nf.nmap[l].v(),
-                                        lf.nmap[l].v(), s->K[l], s->cfg.icp_min_count[l], s->st,
-                                        s->partials, st);
+            const IcpLevel cur{nf.vmap[l].v(), lf.vmap[l].v(), nf.nmap[l].v(), lf.nmap[l].v(),
+                               s->K[l], s->cfg.icp_min_count[l]};
+            for (int j = 0; j < s->cfg.icp_iters[l]; ++j, ++k) {
+                rc = launch_icp_iteration(cur, k ? &prev : nullptr, lf.meta, s->cfg.icp_thresh,
+                                          iter + ((k + 1) & 1), iter + (k & 1),
+                                          s->partials + psz * ((k + 1) & 1),
+                                          s->partials + psz * (k & 1), st);
                 if (rc) return rc;
+                prev = cur;
             }
         }
-        if ((rc = launch_track_end(nf.meta, s->st, st))) return rc;
-        if ((rc = launch_rollback_maps(lf.vmap[0].v(), lf.nmap[0].v(), nf.vmap[0].v(),
-                                       nf.nmap[0].v(), s->st, st)))
+        if ((rc = launch_track_finish(k ? &prev : nullptr, lf.meta, s->cfg.icp_thresh,
+                                      iter + ((k + 1) & 1), s->partials + psz * ((k + 1) & 1),
+                                      s->st, nf.meta, lf.vmap[0].v(), lf.nmap[0].v(),
+                                      nf.vmap[0].v(), nf.nmap[0].v(), st)))
             return rc;
     }
     stage_end(s, kStageTrack, ev, st);
@@ -363,12 +366,15 @@ int sicp_system_last_result(sicp_system* s, sicp_frame_result* out, void* stream
 
 int sicp_system_frame_image(sicp_system* s, int which, int level, sicp_image* out) {
     if (!s || !out || level < 0 || level > 2) return invalid("system_frame_image: bad argument");
-    FrameBuf& f = *s->last;
-    switch (which) {
+    FrameBuf& f = (which & 16) ? *s->next : *s->last;
+    switch (which & 15) {
         case 0: *out = f.vmap[level].img(); break;
         case 1: *out = f.nmap[level].img(); break;
         case 2: *out = f.range.img(); break;
-        case 3: *out = f.depth[0].img(); break;
+        case 3: *out = f.depth[level].img(); break;
+        case 4: *out = f.image[level].img(); break;
+        case 5: *out = f.dx[level].img(); break;
+        case 6: *out = f.dy[level].img(); break;
         default: return invalid("system_frame_image: unknown image");
     }
     return SICP_OK;

@@ -98,3 +98,19 @@ def mismatch_report(a, b):
     d = np.abs(a[both] - b[both])
     return (f"nan-pattern diff {int((na != nb).sum())}, value diff {int((d > 0).sum())}, "
             f"max {d.max() if d.size else 0}")
+
+
+def download(im, rows, cols, dtype, channels=1):
+    """Copies a pitched device image (a binding.Image) into a numpy array."""
+    import ctypes
+    import torch
+    torch.cuda.synchronize()
+    dt = np.dtype(dtype)
+    shape = (rows, cols, channels) if channels > 1 else (rows, cols)
+    buf = np.zeros(shape, dt)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy2D.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int]
+    width = cols * channels * dt.itemsize
+    assert hip.hipMemcpy2D(buf.ctypes.data, width, im.ptr, im.pitch, width, rows, 2) == 0
+    return buf

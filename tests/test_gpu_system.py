@@ -60,6 +60,61 @@ def test_system_first_frame_model_bit_exact(dev):
     assert bits_equal(buf, ov), mismatch_report(buf, ov)
 
 
+@pytest.mark.parametrize("c", [CFG_SMALL, dict(CFG_SMALL, cols=317, rows=239, cx=158.0, cy=119.0)])
+def test_system_frame_pyramid_bit_exact(dev, c):
+    """The three-launch frame pyramid (filter, intensity, PyrDown to levels
+    1-2, vertex / normal maps, derivatives) and the model pyramid of the
+    previous frame (ResizeMap to levels 1-2) equal the single-image oracle
+    functions applied level by level. The odd 317x239 size exercises the
+    clipped windows and the odd remainder rows / columns."""
+    from helpers import bits_equal, download, mismatch_report
+    cfg = sys_cfg(c)
+    g = B.System(cfg)
+    s = synth(c)
+    frames = [s.render(i) for i in range(2)]
+    for d, rgb, lab in frames:
+        g.grab(to_dev(d, dev), to_dev(rgb, dev), to_dev(lab, dev))
+    d, rgb, _ = frames[1]
+    raw, filt = O.filter_depth(d, 1000.0, 3.0)
+    depth, gray = [filt], [O.image_to_intensity(rgb)]
+    for _ in range(2):
+        depth.append(O.pyrdown_f32(depth[-1]))
+        gray.append(O.pyrdown_u8(gray[-1]))
+    rows0, cols0 = c["rows"], c["cols"]
+    got = download(g.frame_image(2, 0), rows0, cols0, np.float32)
+    assert bits_equal(got, raw), mismatch_report(got, raw)
+    for l in range(3):
+        rows, cols = rows0 >> l, cols0 >> l
+        assert depth[l].shape == (rows, cols)
+        s_ = float(1 << l)
+        f, cx, cy = c["f"] / s_, c["cx"] / s_, c["cy"] / s_
+        want = {
+            3: depth[l],
+            4: gray[l],
+        }
+        dx, dy = O.derivative_image(gray[l])
+        want[5], want[6] = dx, dy
+        if l > 0:  # level 0 of the frame now holds its raycast model
+            v = O.compute_vmap(depth[l], f, f, cx, cy, 3.0)
+            want[0], want[1] = v, O.compute_nmap(v)
+        for which, ref in want.items():
+            ch = 4 if which in (0, 1) else 1
+            got = download(g.frame_image(which, l), rows, cols, ref.dtype, ch)
+            assert bits_equal(got, ref), (l, which, mismatch_report(got, ref))
+    # The previous frame: level 0 = its raycast model, levels 1-2 = ResizeMap.
+    v0 = download(g.frame_image(16 + 0, 0), rows0, cols0, np.float32, 4)
+    n0 = download(g.frame_image(16 + 1, 0), rows0, cols0, np.float32, 4)
+    v, n = v0, n0
+    for l in (1, 2):
+        v, n = O.resize_map(v, n)
+        rows, cols = rows0 >> l, cols0 >> l
+        gv = download(g.frame_image(16 + 0, l), rows, cols, np.float32, 4)
+        gn = download(g.frame_image(16 + 1, l), rows, cols, np.float32, 4)
+        assert bits_equal(gv, v), (l, mismatch_report(gv, v))
+        assert bits_equal(gn, n), (l, mismatch_report(gn, n))
+    assert np.isfinite(v0[..., 2]).sum() > 1000  # the model is not empty
+
+
 def test_system_lost_frame_keeps_model(dev):
     """An all-zero depth frame cannot be tracked (count below MIN_ICP_COUNT):
     the pose rolls back, nothing is fused, and the next good frame tracks
