@@ -232,6 +232,15 @@ int sicp_create_rendering_blocks(sicp_map* map, sicp_image zmin, sicp_image zmax
 int sicp_raycast(sicp_map* map, sicp_image vmap_f4, sicp_image nmap_f4, sicp_image zmin,
                  sicp_image zmax, const sicp_pose_f* pose_host, const sicp_pose_f* pose_dev,
                  sicp_intrinsics K, void* stream);
+/* Raycast with the reference's own argument set (Mapping/RenderScene.h:13-18):
+ * the inverse focal lengths exactly as the caller computed them. */
+int sicp_raycast_inv(sicp_map* map, sicp_image vmap_f4, sicp_image nmap_f4, sicp_image zmin,
+                     sicp_image zmax, const sicp_pose_f* pose_host, const sicp_pose_f* pose_dev,
+                     float invfx, float invfy, float cx, float cy, void* stream);
+/* Device pointer and capacity (records) of the map's visible list, which
+ * CheckBlockVisibility / FuseMapColor fill and CreateRenderingBlocks reads
+ * (the reference's DeviceMap::visibleEntries, Mapping/DeviceMap.h:198). */
+int sicp_map_visible_list(sicp_map* map, sicp_block** list_dev, uint32_t* capacity);
 
 int sicp_map_get_stats(sicp_map* map, sicp_map_stats* out, void* stream);
 

@@ -123,6 +123,9 @@ _SIGS = {
                                                C.POINTER(C.c_int), _P]),
     "sicp_raycast": (C.c_int, [_P, Image, Image, Image, Image, C.POINTER(PoseF), _P, Intrinsics,
                                _P]),
+    "sicp_raycast_inv": (C.c_int, [_P, Image, Image, Image, Image, C.POINTER(PoseF), _P,
+                                   C.c_float, C.c_float, C.c_float, C.c_float, _P]),
+    "sicp_map_visible_list": (C.c_int, [_P, C.POINTER(_P), C.POINTER(C.c_uint32)]),
     "sicp_map_get_stats": (C.c_int, [_P, C.POINTER(MapStats), _P]),
     "sicp_map_download": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_uint32]),
     "sicp_map_download_visible": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),

@@ -383,9 +383,10 @@ int sicp_create_rendering_blocks(sicp_map* map, sicp_image zmin, sicp_image zmax
     return SICP_OK;
 }
 
-int sicp_raycast(sicp_map* map, sicp_image vmap_f4, sicp_image nmap_f4, sicp_image zmin,
-                 sicp_image zmax, const sicp_pose_f* pose_host, const sicp_pose_f* pose_dev,
-                 sicp_intrinsics K, void* stream) {
+static int raycast_common(sicp_map* map, sicp_image vmap_f4, sicp_image nmap_f4,
+                          sicp_image zmin, sicp_image zmax, const sicp_pose_f* pose_host,
+                          const sicp_pose_f* pose_dev, float invfx, float invfy, float cx,
+                          float cy, void* stream) {
     if (!map) return invalid("raycast: null map");
     const int c = vmap_f4.cols, r = vmap_f4.rows;
     if (!image_ok(vmap_f4, 16, c, r) || !image_ok(nmap_f4, 16, c, r))
@@ -393,11 +394,32 @@ int sicp_raycast(sicp_map* map, sicp_image vmap_f4, sicp_image nmap_f4, sicp_ima
     if (!image_ok(zmin, 4, (c + 7) / 8, (r + 7) / 8) || !image_ok(zmax, 4, (c + 7) / 8, (r + 7) / 8))
         return invalid("raycast: zmin / zmax must be ceil(W/8) x ceil(H/8)");
     if (!pose_host && !pose_dev) return invalid("raycast: no pose");
-    // Mapping::RayTrace passes 1.0 / fx rounded to float (Mapping.cc:158-159).
     return launch_raycast(map->dev, view<float4>(vmap_f4), view<float4>(nmap_f4),
                           view<float>(zmin), view<float>(zmax), pose_arg(pose_host, pose_dev),
+                          invfx, invfy, cx, cy, nullptr, (hipStream_t)stream);
+}
+
+int sicp_raycast(sicp_map* map, sicp_image vmap_f4, sicp_image nmap_f4, sicp_image zmin,
+                 sicp_image zmax, const sicp_pose_f* pose_host, const sicp_pose_f* pose_dev,
+                 sicp_intrinsics K, void* stream) {
+    // Mapping::RayTrace passes 1.0 / fx rounded to float (Mapping.cc:158-159).
+    return raycast_common(map, vmap_f4, nmap_f4, zmin, zmax, pose_host, pose_dev,
                           (float)(1.0 / (double)K.fx), (float)(1.0 / (double)K.fy), K.cx, K.cy,
-                          nullptr, (hipStream_t)stream);
+                          stream);
+}
+
+int sicp_raycast_inv(sicp_map* map, sicp_image vmap_f4, sicp_image nmap_f4, sicp_image zmin,
+                     sicp_image zmax, const sicp_pose_f* pose_host, const sicp_pose_f* pose_dev,
+                     float invfx, float invfy, float cx, float cy, void* stream) {
+    return raycast_common(map, vmap_f4, nmap_f4, zmin, zmax, pose_host, pose_dev, invfx, invfy,
+                          cx, cy, stream);
+}
+
+int sicp_map_visible_list(sicp_map* map, sicp_block** list_dev, uint32_t* capacity) {
+    if (!map || !list_dev || !capacity) return invalid("map_visible_list: null argument");
+    *list_dev = map->visible;
+    *capacity = (uint32_t)map->cfg.num_blocks;
+    return SICP_OK;
 }
 
 int sicp_map_get_stats(sicp_map* map, sicp_map_stats* out, void* stream) {

@@ -75,3 +75,15 @@ def test_synthetic_room_is_deterministic():
     d, rgb, lab = a
     assert (d > 0).mean() > 0.9 and lab.max() >= 1
     assert abs(s1.pose(0) - __import__("numpy").eye(4)).max() == 0
+
+
+def test_reference_named_layer_builds_and_loads():
+    """The reference-named C++ layer compiles against the C ABI alone (the
+    test driver links only libsicp.so and the HIP runtime) and loads without
+    a GPU."""
+    import ctypes
+    lib = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "libmirror_check.so")
+    assert os.path.exists(lib), "run build() first"
+    h = ctypes.CDLL(lib)
+    for name in ("mirror_frame", "mirror_icp", "mirror_map"):
+        assert hasattr(h, name), name
