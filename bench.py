@@ -21,12 +21,24 @@ sys.path.insert(0, ROOT)
 
 WORKLOAD = dict(cols=640, rows=480, f=525.0, cx=319.5, cy=239.5, voxel=0.005, trunc=0.015,
                 buckets=1 << 20, blocks=400000, seed=0)
-STAGES = ["pyramid", "track", "allocate", "visibility", "fuse", "prepass", "raycast"]
+STAGES = ["pyramid", "track", "allocate", "visibility", "fuse", "prepass", "raycast",
+          "fuse_count", "frame"]
 HBM_PEAK_GBS = 8000.0
 # Bytes of one fused voxel: 8 B geometry + 12 B semantic record read and
 # written, plus its depth (4), RGB (3), normal (16) and label (1) gathers.
 FUSE_BYTES_PER_VOXEL = 2 * (8 + 12) + 4 + 3 + 16 + 1
 FUSE_BYTES_PER_BLOCK = 16  # visible-list record
+# Raycast: per ray the zRange pair (8) and the vmap + nmap writes (32); per
+# march step the voxel's sdf + weight (5); per interpolated read eight sdf
+# values (32); per block lookup one two-entry bucket (32).
+RAY_BYTES_PER_RAY = 40
+RAY_BYTES_PER_STEP = 5
+RAY_BYTES_PER_INTERP = 32
+RAY_BYTES_PER_LOOKUP = 32
+# ICP: per pixel of a level, current vertex + normal and the gathered last
+# vertex + normal (float4 each).
+ICP_BYTES_PER_PIXEL = 64
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_vga_5mm_sequence.json")
 
 
 def log(*a):
@@ -42,9 +54,11 @@ def make_frames(n, w):
     return syn, frames
 
 
-def cpu_baseline(w, frames, seconds=20.0):
+def cpu_baseline(w, frames, first_timed, seconds=20.0):
     """The oracle (scalar C++ restatement, OpenMP over pixels and blocks) on the
-    same frames, on all host cores, for a bounded prefix of the sequence."""
+    same frames, on all host cores, for a bounded prefix of the sequence; also
+    its work counters over the GPU-timed frames (for algorithmic bytes) and a
+    one-thread time on a short sample."""
     from oracle import binding as O
     from semantic_icpslam_amd import binding as B
     cores = O.set_threads(0)
@@ -53,18 +67,86 @@ def cpu_baseline(w, frames, seconds=20.0):
     sysc = O.OracleSystem(cfg)
     t0 = time.perf_counter()
     times = []
+    work0 = None
     for i, (d, rgb, lab) in enumerate(frames):
+        if i == first_timed:
+            work0 = sysc.work()
         t = time.perf_counter()
         sysc.grab(d, rgb, lab)
         times.append(time.perf_counter() - t)
         if time.perf_counter() - t0 > seconds:
             break
     n = len(times)
+    work1 = sysc.work()
+    work = None
+    if work0 is not None and n > first_timed:
+        work = {k: work1[k] - work0[k] for k in work1}
+        work["frames"] = n - first_timed
+    # one thread, a few frames of the same sequence (fresh map)
+    O.set_threads(1)
+    sys1 = O.OracleSystem(cfg)
+    t1 = []
+    for i, (d, rgb, lab) in enumerate(frames[:4]):
+        t = time.perf_counter()
+        sys1.grab(d, rgb, lab)
+        t1.append(time.perf_counter() - t)
+    O.set_threads(cores)
     # frame 0 has no tracking; report the mean over the tracked frames
     ms = 1000.0 * (np.mean(times[1:]) if n > 1 else times[0])
-    return {"value": round(float(ms), 3), "unit": "ms/frame", "cores": int(cores), "kind": "port",
-            "sample": f"frames 0-{n - 1} of the same 640x480 sequence (ms/frame over frames "
-                      f"1-{n - 1}), oracle with {cores} OpenMP threads"}
+    out = {"value": round(float(ms), 3), "unit": "ms/frame", "cores": int(cores), "kind": "port",
+           "sample": f"frames 0-{n - 1} of the same 640x480 sequence (ms/frame over frames "
+                     f"1-{n - 1}), oracle with {cores} OpenMP threads",
+           "nproc": os.cpu_count(),
+           "value_1_thread": round(float(1000.0 * np.mean(t1[1:])), 3),
+           "sample_1_thread": "frames 1-3 of the same sequence, one thread"}
+    return out, work
+
+
+def pose_error_vs_fp64(w, frames, cfg, m=30):
+    """Per-frame pose of the GPU system against the FP64 oracle on the first m
+    frames (an untimed pass): max rotation angle (rad) of T_fp64^-1 T and max
+    translation difference (m)."""
+    import torch
+    from oracle import binding as O
+    from semantic_icpslam_amd import binding as B
+    dev = torch.device("cuda:0")
+    g = B.System(cfg)
+    o = O.OracleSystem(cfg, fp64=True)
+    rot, tr = [], []
+    for i, (d, rgb, lab) in enumerate(frames[:m]):
+        g.grab(torch.from_numpy(d).to(dev), torch.from_numpy(rgb).to(dev),
+               torch.from_numpy(lab).to(dev))
+        a = g.result()["pose"]
+        b = o.grab(d, rgb, lab)["pose"]
+        dR = b[:3, :3].T @ a[:3, :3]
+        rot.append(float(np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1))))
+        tr.append(float(np.linalg.norm(a[:3, 3] - b[:3, 3])))
+    g.close()
+    return {"frames": m, "max_rotation_rad": max(rot), "max_translation_m": max(tr),
+            "last_translation_m": tr[-1]}
+
+
+def roofline_entry(kernel, nbytes, seconds, launches, traffic_kib=None):
+    achieved = nbytes / max(seconds, 1e-12) / 1e9
+    e = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+         "bytes_per_launch": int(nbytes / max(launches, 1)),
+         "avg_launch_us": round(1e6 * seconds / max(launches, 1), 2)}
+    if traffic_kib is not None:
+        e["traffic"] = int(traffic_kib * 1024)
+    return e
+
+
+def load_traffic():
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes, scaled by the same counters on a plain copy."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return {k: v["fetch_kib"] * t["copy_scale_fetch"] + v["write_kib"] * t["copy_scale_write"]
+                for k, v in t["kernels"].items()}
+    except (OSError, KeyError, ValueError):
+        return {}
 
 
 def main():
@@ -74,6 +156,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run one plain device copy of MIB MiB first (counter calibration)")
+    ap.add_argument("--no-pose-check", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -83,6 +168,15 @@ def main():
 
     import torch
     from semantic_icpslam_amd import binding as B
+
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: {args.copy_calibration} MiB device copy")
+        del src, dst
 
     w = WORKLOAD
     dev = torch.device("cuda:0")
@@ -116,6 +210,11 @@ def main():
     elapsed = time.perf_counter() - t0
     tot = np.zeros(len(STAGES))
     cnt = np.zeros(len(STAGES), np.int32)
+    frame_ms = np.zeros(args.steps)
+    nf = B.C.c_int(0)
+    B.check(B.lib().sicp_system_read_timing_detail(sysg.h, STAGES.index("frame"),
+                                                   frame_ms.ctypes.data, args.steps,
+                                                   B.C.byref(nf), B.stream_ptr(stream)))
     B.check(B.lib().sicp_system_read_timing(sysg.h, tot.ctypes.data, cnt.ctypes.data, len(STAGES),
                                             B.stream_ptr(stream)))
     B.check(B.lib().sicp_system_set_timing(sysg.h, 0))
@@ -125,26 +224,51 @@ def main():
     ms_per_frame = 1000.0 * elapsed / args.steps
     stage_ms = {s: round(float(tot[k] / max(cnt[k], 1)), 4) for k, s in enumerate(STAGES)}
     log("[bench] per-stage ms/frame: " + json.dumps(stage_ms))
+    fm = frame_ms[: nf.value]
+    sysg.close()
 
-    # Roofline of the fusion kernel: algorithmic bytes of the timed frames
-    # over its summed event time.
+    traffic = load_traffic()
+    kernels = {}
+    # Fusion: algorithmic bytes of the timed frames (the map's own counters)
+    # over the summed fuse-kernel event time.
     upd = st1["updated_total"] - st0["updated_total"]
     vis = st1["visible_total"] - st0["visible_total"]
     fus = max(st1["fusions"] - st0["fusions"], 1)
     fuse_bytes = upd * FUSE_BYTES_PER_VOXEL + vis * FUSE_BYTES_PER_BLOCK
-    fuse_s = tot[STAGES.index("fuse")] / 1000.0
-    achieved = fuse_bytes / max(fuse_s, 1e-12) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "fuse_kernel",
-                "bytes_per_launch": int(fuse_bytes / fus),
-                "updated_voxels_per_launch": int(upd / fus),
-                "visible_blocks_per_launch": int(vis / fus)}
+    kernels["fuse_kernel"] = roofline_entry("fuse_kernel", fuse_bytes,
+                                            tot[STAGES.index("fuse")] / 1000.0, fus,
+                                            traffic.get("fuse_kernel"))
+    kernels["fuse_kernel"]["updated_voxels_per_launch"] = int(upd / fus)
+    kernels["fuse_kernel"]["visible_blocks_per_launch"] = int(vis / fus)
 
-    cpu = None
+    cpu, work = None, None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle)")
-        cpu = cpu_baseline(w, frames, args.cpu_seconds)
+        cpu, work = cpu_baseline(w, frames, args.warmup, args.cpu_seconds)
+    if work and work["raycasts"]:
+        # Raycast: the oracle's counts over the same frames (deterministic;
+        # scaled by frames when the CPU leg covered only a prefix of them).
+        scale = args.steps / work["frames"]
+        ray_bytes = scale * (RAY_BYTES_PER_RAY * work["rays"] +
+                             RAY_BYTES_PER_STEP * work["march_steps"] +
+                             RAY_BYTES_PER_INTERP * work["interp_reads"] +
+                             RAY_BYTES_PER_LOOKUP * work["ray_lookups"])
+        kernels["raycast_kernel"] = roofline_entry(
+            "raycast_kernel", ray_bytes, tot[STAGES.index("raycast")] / 1000.0,
+            cnt[STAGES.index("raycast")], traffic.get("raycast_kernel"))
+        kernels["raycast_kernel"]["work_per_ray"] = {
+            k: round(work[k] / max(work["rays"], 1), 3)
+            for k in ("march_steps", "interp_reads", "ray_lookups")}
+        kernels["raycast_kernel"]["work_frames"] = work["frames"]
+    # The dominant kernel: the single-launch-per-frame kernel with the longest
+    # launch.
+    dom = max(kernels, key=lambda k: kernels[k]["avg_launch_us"])
+    roofline = kernels[dom]
+
+    pose_check = None
+    if not args.no_pose_check:
+        log("[bench] pose check against the FP64 oracle")
+        pose_check = pose_error_vs_fp64(w, frames, cfg)
 
     out = {
         "metric": "ms/frame track+fuse+raycast 640x480 (plain FP32)",
@@ -167,7 +291,11 @@ def main():
         "last_frame_tracked": res["tracked"],
         "last_frame_translation_vs_ground_truth_m": round(
             float(np.linalg.norm(res["pose"][:3, 3] - gt[:3, 3])), 5),
+        "frame_ms_p50": round(float(np.percentile(fm, 50)), 4) if fm.size else None,
+        "frame_ms_p99": round(float(np.percentile(fm, 99)), 4) if fm.size else None,
         "roofline": roofline,
+        "roofline_kernels": kernels,
+        "pose_vs_fp64_oracle": pose_check,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

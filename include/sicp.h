@@ -127,7 +127,7 @@ typedef struct sicp_map_stats {
     uint32_t dropped_range;     /* block coordinates outside 21-bit range    */
     uint32_t updated_voxels;    /* voxels written by the last fusion         */
     uint32_t render_tiles;      /* blocks that produced tiles, last prepass  */
-    uint32_t pad;
+    uint32_t ray_handover;      /* rays the last raycast marched in its second pass */
     uint64_t updated_total;     /* voxels written, summed since reset        */
     uint64_t visible_total;     /* visible blocks fused, summed since reset  */
     uint64_t fusions;           /* fusion passes since reset                 */
@@ -294,12 +294,16 @@ sicp_map* sicp_system_map(sicp_system* sys);
 /* Stage timing with HIP events on the frame's stream (no counterpart in the
  * reference, whose only timing is std::clock around GrabImage,
  * Core/System.cc:142-146). Stages: 0 pyramid, 1 track, 2 allocate,
- * 3 visibility, 4 fuse, 5 render prepass, 6 raycast. read_timing synchronises
- * `stream`, returns per-stage summed milliseconds and instance counts of the
- * frames grabbed since set_timing(1), and clears them. */
+ * 3 visibility, 4 fuse kernel, 5 render prepass, 6 raycast kernel, 7 fusion
+ * count, 8 whole frame. read_timing synchronises `stream`, returns per-stage
+ * summed milliseconds and instance counts of the frames grabbed since
+ * set_timing(1), and clears them; read_timing_detail (call it first) gives
+ * one stage's per-instance milliseconds in grab order without clearing. */
 int sicp_system_set_timing(sicp_system* sys, int enable);
 int sicp_system_read_timing(sicp_system* sys, double* total_ms, int* counts, int n_stages,
                             void* stream);
+int sicp_system_read_timing_detail(sicp_system* sys, int stage, double* ms_out, int cap,
+                                   int* count, void* stream);
 /* Frame buffers of the system, for inspection (the public members of Frame,
  * Core/Frame.h): which = 0 vmap, 1 nmap, 2 raw depth (level 0 only),
  * 3 filtered depth, 4 intensity, 5 d/dx, 6 d/dy at pyramid `level`, of the

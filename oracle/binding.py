@@ -65,6 +65,7 @@ _SIGS = {
     "oracle_system_destroy": (None, [_P]),
     "oracle_system_grab": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(FrameResult)]),
     "oracle_system_model": (None, [_P, C.c_int, _P, _P]),
+    "oracle_system_work": (None, [_P, _P]),
 }
 
 
@@ -331,6 +332,13 @@ class OracleSystem:
                     iterations=res.iterations, icp_error=res.icp_error,
                     visible_blocks=res.visible_blocks, allocated_blocks=res.allocated_blocks,
                     pose=np.array(list(res.pose)).reshape(4, 4))
+
+    def work(self):
+        v = np.zeros(8, np.uint64)
+        lib().oracle_system_work(self.h, _p(v))
+        names = ("raycasts", "rays", "march_steps", "interp_reads", "ray_lookups",
+                 "alloc_probes", "updated_voxels", "visible_blocks")
+        return {k: int(x) for k, x in zip(names, v)}
 
     def model(self, level=0):
         cols = self.cfg.cols >> level

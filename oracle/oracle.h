@@ -98,6 +98,10 @@ void oracle_system_destroy(void* sys);
 int oracle_system_grab(void* sys, const uint16_t* depth, const uint8_t* rgb,
                        const uint8_t* labels, int num_detection, sicp_frame_result* out);
 void* oracle_system_map(void* sys);
+/* Work summed over the frames grabbed: raycasts, rays, march steps,
+ * interpolated reads, raycast block lookups, allocation probes, voxels fused,
+ * visible blocks fused. */
+void oracle_system_work(void* sys, uint64_t* out8);
 /* Copies the last frame's level-l model vertex / normal maps (after raycast). */
 void oracle_system_model(void* sys, int level, float* vmap, float* nmap);
 

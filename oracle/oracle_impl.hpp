@@ -65,6 +65,7 @@ struct MapCounters {
     // raycast outcomes: no depth range, back-face exit, no zero crossing,
     // normal rejected, surface found
     uint64_t ray_norange = 0, ray_back = 0, ray_nocross = 0, ray_nonormal = 0, ray_hit = 0;
+    uint64_t interp_reads = 0;  // readSdfInterped calls of the last raycast
 };
 
 template <class R> struct OMap {

@@ -623,13 +623,14 @@ static inline R read_sdf_interped(const OMap<R>& m, V3<R> pt, Cache<R>& cache, b
 // Rendering::readNormal (RenderScene.cu:345-378).
 template <class R>
 static inline bool read_normal(const OMap<R>& m, V3<R> pt, Cache<R>& cache, const Pose<R>& pose,
-                               V3<R>& n, uint64_t& probes) {
+                               V3<R>& n, uint64_t& probes, int& reads) {
     bool valid;
     static const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
     R s[6];
     for (int i = 0; i < 6; ++i) {
         s[i] = read_sdf_interped(m, add(pt, V3<R>{(R)off[i][0], (R)off[i][1], (R)off[i][2]}),
                                  cache, valid, probes);
+        ++reads;
         if (isnan_(s[i]) || s[i] == (R)1.0f || !valid) return false;
     }
     n = V3<R>{s[0] - s[1], s[2] - s[3], s[4] - s[5]};
@@ -642,9 +643,9 @@ template <class R>
 void raycast(OMap<R>& m, R* vmap, R* nmap, int cols, int rows, const R* zmin, const R* zmax,
              int zcols, const Pose<R>& pose, R invfx, R invfy, R cx, R cy) {
     uint64_t probes = 0, steps = 0, c_norange = 0, c_back = 0, c_nocross = 0, c_nonormal = 0,
-             c_hit = 0;
+             c_hit = 0, c_interp = 0;
 #pragma omp parallel for schedule(dynamic, 4) \
-    reduction(+ : probes, steps, c_norange, c_back, c_nocross, c_nonormal, c_hit)
+    reduction(+ : probes, steps, c_norange, c_back, c_nocross, c_nonormal, c_hit, c_interp)
     for (int y = 0; y < rows; ++y) {
         for (int x = 0; x < cols; ++x) {
             R* vo = vmap + ((size_t)y * cols + x) * 4;
@@ -691,8 +692,10 @@ void raycast(OMap<R>& m, R* vmap, R* nmap, int cols, int rows, const R* zmin, co
                 if (!valid_sdf) {
                     step = (R)kBS;
                 } else {
-                    if (sdf <= (R)0.1f && sdf >= (R)-0.5f)
+                    if (sdf <= (R)0.1f && sdf >= (R)-0.5f) {
+                        ++c_interp;
                         sdf = read_sdf_interped(m, result, cache, valid_sdf, probes);
+                    }
                     if (prev_sdf < (R)0.f && sdf >= (R)0.f) {
                         back_face = true;
                         break;
@@ -712,13 +715,16 @@ void raycast(OMap<R>& m, R* vmap, R* nmap, int cols, int rows, const R* zmin, co
                 step = sdf * m.step_scale;
                 result = add(result, mul(step, dir));
                 sdf = read_sdf_interped(m, result, cache, valid_sdf, probes);
+                ++c_interp;
                 step = sdf * m.step_scale;
                 result = add(result, mul(step, dir));
                 found_pt = true;
             }
             if (found_pt) {
                 V3<R> normal;
-                bool has_normal = read_normal(m, result, cache, pose, normal, probes);
+                int reads = 0;
+                bool has_normal = read_normal(m, result, cache, pose, normal, probes, reads);
+                c_interp += reads;
                 if (has_normal) ++c_hit; else ++c_nonormal;
                 if (has_normal) {
                     V3<R> r = mv(pose.Rinv, sub(mul(result, m.voxel), pose.t));
@@ -741,6 +747,7 @@ void raycast(OMap<R>& m, R* vmap, R* nmap, int cols, int rows, const R* zmin, co
     m.ctr.ray_nocross = c_nocross;
     m.ctr.ray_nonormal = c_nonormal;
     m.ctr.ray_hit = c_hit;
+    m.ctr.interp_reads = c_interp;
 }
 
 template void map_init<float>(OMap<float>&, const sicp_map_config&);

@@ -50,6 +50,11 @@ template <class R> struct OSystem {
     int zc, zr;
     std::vector<R> zmin, zmax;
     Intr<R> K[3];
+    // Work of the path summed over the frames grabbed (inputs of the
+    // algorithmic byte counts): raycasts, rays, march steps, interpolated
+    // reads, raycast block lookups, allocation probes, voxels fused, visible
+    // blocks fused.
+    uint64_t work[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 template <class R> static void fill_images(OSystem<R>& s, OFrame<R>& f, const uint16_t* depth,
@@ -204,11 +209,20 @@ static int system_grab(OSystem<R>& s, const uint16_t* depth, const uint8_t* rgb,
             int any = create_rendering_blocks(s.map, s.zmin.data(), s.zmax.data(), s.zc, s.zr,
                                               (R)s.cfg.map.depth_max, (R)s.cfg.map.depth_min, p,
                                               s.K[0]);
-            if (any)
+            if (any) {
                 raycast(s.map, f.vmap[0].data(), f.nmap[0].data(), f.cols[0], f.rows[0],
                         s.zmin.data(), s.zmax.data(), s.zc, p, (R)(1.0 / (double)s.K[0].fx),
                         (R)(1.0 / (double)s.K[0].fy), s.K[0].cx, s.K[0].cy);
+                s.work[0] += 1;
+                s.work[1] += (uint64_t)f.cols[0] * f.rows[0];
+                s.work[2] += s.map.ctr.steps;
+                s.work[3] += s.map.ctr.interp_reads;
+                s.work[4] += s.map.ctr.interp;
+            }
         }
+        s.work[5] += s.map.ctr.probes;
+        s.work[6] += s.map.stats.updated_voxels;
+        s.work[7] += nv;
     }
     res->allocated_blocks = s.map.heap_used;
     std::memcpy(res->pose, s.last->pose, sizeof(res->pose));
@@ -251,6 +265,11 @@ int oracle_system_grab(void* sys, const uint16_t* depth, const uint8_t* rgb,
     OracleSystemHandle* h = (OracleSystemHandle*)sys;
     return h->fp64 ? system_grab(h->d, depth, rgb, labels, num_detection, out)
                    : system_grab(h->f, depth, rgb, labels, num_detection, out);
+}
+
+void oracle_system_work(void* sys, uint64_t* out8) {
+    OracleSystemHandle* h = (OracleSystemHandle*)sys;
+    std::memcpy(out8, h->fp64 ? h->d.work : h->f.work, 8 * sizeof(uint64_t));
 }
 
 void* oracle_system_map(void* sys) {

@@ -49,7 +49,7 @@ class MapConfig(C.Structure):
 class MapStats(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in (
         "allocated_blocks", "visible_blocks", "new_blocks", "dropped_heap",
-        "dropped_requests", "dropped_range", "updated_voxels", "render_tiles", "pad")] + [
+        "dropped_requests", "dropped_range", "updated_voxels", "render_tiles", "ray_handover")] + [
         (n, C.c_uint64) for n in ("updated_total", "visible_total", "fusions")]
 
 
@@ -139,6 +139,8 @@ _SIGS = {
     "sicp_system_map": (_P, [_P]),
     "sicp_system_set_timing": (C.c_int, [_P, C.c_int]),
     "sicp_system_read_timing": (C.c_int, [_P, _P, _P, C.c_int, _P]),
+    "sicp_system_read_timing_detail": (C.c_int, [_P, C.c_int, _P, C.c_int, C.POINTER(C.c_int),
+                                                 _P]),
     "sicp_system_frame_image": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Image)]),
     "sicp_synth_create": (C.c_int, [C.POINTER(SynthConfig), C.POINTER(_P)]),
     "sicp_synth_destroy": (C.c_int, [_P]),

@@ -70,12 +70,14 @@ int map_alloc(const sicp_map_config& cfg, sicp_map** out) {
         (rc = dmalloc(m, &a.new_keys, q)) || (rc = dmalloc(m, &a.new_req_slot, q)) ||
         (rc = dmalloc(m, &a.bucket_leader, nb)) || (rc = dmalloc(m, &a.bucket_head, nb)) ||
         (rc = dmalloc(m, &a.pending_next, q)) || (rc = dmalloc(m, &a.vis_flags, nblk)) ||
-        (rc = dmalloc(m, &a.fuse_counts, (size_t)kFuseBlocks * 4))) {
+        (rc = dmalloc(m, &a.fuse_counts, (size_t)kFuseBlocks * 4)) ||
+        (rc = dmalloc(m, &d.rq, px)) || (rc = dmalloc(m, &d.rq_count, 1))) {
         map_free(m);
         return rc;
     }
     a.req_mask = (uint32_t)(q - 1);
     a.cap_pixels = (uint32_t)px;
+    d.rq_cap = (uint32_t)px;
     d.nb = (uint32_t)nb;
     d.nb_mask = pow2((uint32_t)nb) ? (uint32_t)nb - 1 : 0u;
     d.num_blocks = cfg.num_blocks;
@@ -324,6 +326,8 @@ int sicp_fuse_map_color(sicp_map* map, sicp_image depth_f32, sicp_image rgb_u8x3
                      view<uint8_t>(rgb_u8x3), view<float4>(nm), view<uint8_t>(mk), num_detection,
                      pa, K, map->alloc.fuse_counts, nullptr, s);
     if (rc) return rc;
+    rc = launch_fuse_count(map->ctr, map->alloc.fuse_counts, nullptr, s);
+    if (rc) return rc;
     if (visible_host) {
         rc = read_u32(&map->ctr->visible, visible_host, s);
         if (rc) return rc;
@@ -436,7 +440,9 @@ int sicp_map_get_stats(sicp_map* map, sicp_map_stats* out, void* stream) {
     out->dropped_range = c.dropped_range;
     out->updated_voxels = c.updated_voxels;
     out->render_tiles = c.render_tiles;
-    out->pad = 0;
+    uint32_t handover = 0;
+    SICP_HIP(hipMemcpy(&handover, map->dev.rq_count, sizeof(handover), hipMemcpyDeviceToHost));
+    out->ray_handover = handover;
     out->updated_total = c.updated_total;
     out->visible_total = c.visible_total;
     out->fusions = c.fusions;

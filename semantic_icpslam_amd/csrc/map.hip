@@ -709,11 +709,15 @@ __global__ __launch_bounds__(256) void fuse_kernel(MapDev m, const MapCounters* 
     if (lane == 0) wave_counts[wave0] = updated;
 }
 
-// Sums the per-wavefront counts of fuse_kernel.
+// Sums the per-wavefront counts of fuse_kernel into the map's counters; a
+// gated-off frame fused nothing.
 __global__ __launch_bounds__(1024) void fuse_end_kernel(MapCounters* ctr,
                                                         const uint32_t* wave_counts, int n,
                                                         const int* gate) {
-    if (gate && *gate == 0) return;
+    if (gate && *gate == 0) {
+        if (threadIdx.x == 0) ctr->updated_voxels = 0u;
+        return;
+    }
     __shared__ uint32_t part[16];
     uint32_t s = 0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) s += wave_counts[i];
@@ -726,14 +730,9 @@ __global__ __launch_bounds__(1024) void fuse_end_kernel(MapCounters* ctr,
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
         ctr->updated_voxels = t;
         ctr->updated_total += t;
+        ctr->visible_total += ctr->visible;
+        ctr->fusions += 1ull;
     }
-}
-
-__global__ void fuse_begin_kernel(MapCounters* ctr, const int* gate) {
-    ctr->updated_voxels = 0u;
-    if (gate && *gate == 0) return;
-    ctr->visible_total += ctr->visible;
-    ctr->fusions += 1ull;
 }
 
 }  // namespace
@@ -811,11 +810,14 @@ int launch_fuse(MapDev m, MapCounters* ctr, const sicp_block* visible, View<floa
                 PoseArg pose, sicp_intrinsics K, uint32_t* wave_counts, const int* gate,
                 hipStream_t s) {
     FuseArgs g{K.fx, K.fy, K.cx, K.cy, depth.cols, depth.rows, num_detection};
-    fuse_begin_kernel<<<1, 1, 0, s>>>(ctr, gate);
-    SICP_LAUNCH_CHECK("fuse_begin_kernel");
     fuse_kernel<<<kFuseBlocks, 256, 0, s>>>(m, ctr, visible, depth, rgb, nmap, mask, pose, g,
                                             wave_counts, gate);
     SICP_LAUNCH_CHECK("fuse_kernel");
+    return SICP_OK;
+}
+
+int launch_fuse_count(MapCounters* ctr, const uint32_t* wave_counts, const int* gate,
+                      hipStream_t s) {
     fuse_end_kernel<<<1, 1024, 0, s>>>(ctr, wave_counts, kFuseBlocks * 4, gate);
     SICP_LAUNCH_CHECK("fuse_end_kernel");
     return SICP_OK;

@@ -57,21 +57,31 @@ SICP_DEV EntryView load_entry(const sicp_hash_entry* p) {
 
 // FindEntry (DeviceMap.cu:141-154) over the two-entry bucket (one 32-byte
 // read) and its overflow chain. Returns the heap slot or -1.
-SICP_DEV int map_lookup(const MapDev& m, int x, int y, int z) {
+SICP_DEV int table_lookup(const sicp_hash_entry* table, const sicp_hash_entry* excess,
+                          uint32_t nb, uint32_t nb_mask, int x, int y, int z) {
     if (!key_in_range(x, y, z)) return -1;
-    const uint32_t b = bucket_of(x, y, z, m);
+    const uint32_t h =
+        ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349669u) ^ ((uint32_t)z * 83492791u);
+    const uint32_t b = nb_mask ? (h & nb_mask) : (h % nb);
     const uint64_t key = pack_key(x, y, z);
-    EntryView e0 = load_entry(m.table + 2 * (size_t)b);
-    EntryView e1 = load_entry(m.table + 2 * (size_t)b + 1);
-    if (e0.slot >= 0 && e0.key == key) return e0.slot;
-    if (e1.slot >= 0 && e1.key == key) return e1.slot;
+    // Both entries are read and compared without a branch between them, so
+    // the two 16-byte loads are in flight together (one memory round trip).
+    EntryView e0 = load_entry(table + 2 * (size_t)b);
+    EntryView e1 = load_entry(table + 2 * (size_t)b + 1);
+    const int s1 = (e1.slot >= 0 && e1.key == key) ? e1.slot : -1;
+    const int s0 = (e0.slot >= 0 && e0.key == key) ? e0.slot : s1;
+    if (s0 >= 0) return s0;
     int n = e1.next;
     while (n >= 0) {
-        EntryView e = load_entry(m.excess + n);
+        EntryView e = load_entry(excess + n);
         if (e.key == key) return e.slot;
         n = e.next;
     }
     return -1;
+}
+
+SICP_DEV int map_lookup(const MapDev& m, int x, int y, int z) {
+    return table_lookup(m.table, m.excess, m.nb, m.nb_mask, x, y, z);
 }
 
 }  // namespace sicp

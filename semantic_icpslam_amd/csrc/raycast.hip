@@ -118,65 +118,173 @@ __global__ __launch_bounds__(256) void prepass_kernel(MapDev m, MapCounters* ctr
 }
 
 // ------------------------------------------------------------------ raycast --
+// The march is bound by instruction issue (about five wavefronts share each
+// SIMD for the whole kernel), so voxel addressing is written for few
+// instructions: floor once per distinct float coordinate, block = v >> 3 and
+// local = v & 7 (the same values as voxelPosToBlockPos / voxelPosToLocalIdx,
+// DeviceMap.cu:169-201, for every int), and one cache compare per block.
 struct BlockCache {
     int x, y, z, slot;  // slot -1: empty
 };
 
-// DeviceMap::FindVoxel(float3, cache, valid) (DeviceMap.cu:116-133); the cache
-// starts empty (the reference's default entry would match block (0,0,0)).
-SICP_DEV long find_voxel(const MapDev& m, v3 pt, BlockCache& c, bool& valid) {
-    i3 p = floor_i3(pt);
-    i3 bp = voxel_to_block(p);
-    int idx = voxel_local_idx(p);
-    if (c.slot >= 0 && bp.x == c.x && bp.y == c.y && bp.z == c.z) {
+// make_int3 of one coordinate (VectorMath.h:36-42), as floor_i3.
+SICP_DEV int floor_i(float a) {
+    const int b = (int)a;
+    return ((float)b > a) ? b - 1 : b;
+}
+
+SICP_DEV int local_of(int vx, int vy, int vz) { return ((vz & 7) << 6) | ((vy & 7) << 3) | (vx & 7); }
+
+SICP_DEV bool cache_has(const BlockCache& c, int bx, int by, int bz) {
+    return c.slot >= 0 && bx == c.x && by == c.y && bz == c.z;
+}
+
+// DeviceMap::FindVoxel(int3, cache, valid) (DeviceMap.cu:116-133) on integer
+// voxel coordinates; the cache starts empty (the reference's default entry
+// would match block (0,0,0)).
+SICP_DEV long find_voxel_i(const MapDev& m, int vx, int vy, int vz, BlockCache& c, bool& valid) {
+    const int bx = vx >> 3, by = vy >> 3, bz = vz >> 3;
+    if (cache_has(c, bx, by, bz)) {
         valid = true;
-        return (long)c.slot * kBlockVoxels + idx;
+        return (long)c.slot * kBlockVoxels + local_of(vx, vy, vz);
     }
-    int slot = map_lookup(m, bp.x, bp.y, bp.z);
+    const int slot = map_lookup(m, bx, by, bz);
     if (slot < 0) {
         valid = false;
         return -1;
     }
     valid = true;
-    c = BlockCache{bp.x, bp.y, bp.z, slot};
-    return (long)slot * kBlockVoxels + idx;
+    c = BlockCache{bx, by, bz, slot};
+    return (long)slot * kBlockVoxels + local_of(vx, vy, vz);
 }
 
 SICP_DEV float sdf_at(const MapDev& m, long k) { return k < 0 ? qnanf() : m.geom[k].sdf; }
 
-// Rendering::readSdfInterped (RenderScene.cu:310-343): `valid` is that of the
-// last of the eight lookups.
-SICP_DEV float read_sdf_interped(const MapDev& m, v3 pt, BlockCache& c, bool& valid) {
-    v3 xyz{pt.x - floorf(pt.x), pt.y - floorf(pt.y), pt.z - floorf(pt.z)};
-    float s0, s1, r0, r1, r2, r3;
-    s0 = sdf_at(m, find_voxel(m, pt, c, valid));
-    s1 = sdf_at(m, find_voxel(m, add(pt, mk(1, 0, 0)), c, valid));
-    r0 = (1.0f - xyz.x) * s0 + xyz.x * s1;
-    s0 = sdf_at(m, find_voxel(m, add(pt, mk(0, 1, 0)), c, valid));
-    s1 = sdf_at(m, find_voxel(m, add(pt, mk(1, 1, 0)), c, valid));
-    r1 = (1.0f - xyz.x) * s0 + xyz.x * s1;
-    r2 = (1.0f - xyz.y) * r0 + xyz.y * r1;
-    s0 = sdf_at(m, find_voxel(m, add(pt, mk(0, 0, 1)), c, valid));
-    s1 = sdf_at(m, find_voxel(m, add(pt, mk(1, 0, 1)), c, valid));
-    r0 = (1.0f - xyz.x) * s0 + xyz.x * s1;
-    s0 = sdf_at(m, find_voxel(m, add(pt, mk(0, 1, 1)), c, valid));
-    s1 = sdf_at(m, find_voxel(m, add(pt, mk(1, 1, 1)), c, valid));
-    r1 = (1.0f - xyz.x) * s0 + xyz.x * s1;
-    r3 = (1.0f - xyz.y) * r0 + xyz.y * r1;
-    return (1.0f - xyz.z) * r2 + xyz.z * r3;
+// One axis of an interpolated read centred at coordinate q: the voxel
+// coordinates of q and q + 1 (the reference adds 1.0f to the float point)
+// and the fraction q - floor(q).
+struct AxisPair {
+    int v0, v1;
+    float frac;
+};
+SICP_DEV AxisPair axis_pair(float q) { return AxisPair{floor_i(q), floor_i(q + 1.0f), q - floorf(q)}; }
+
+// Slots of the (up to) eight blocks the corners of an interpolated read touch:
+// per axis the blocks of v0 and v1 (equal, or neighbours). One lookup site,
+// looped, instead of one per corner.
+struct CornerBlocks {
+    int s[8];  // [dx + 2 dy + 4 dz]; dx = 1 only when the x blocks differ, etc.
+};
+SICP_DEV void corner_blocks(const MapDev& m, const AxisPair& X, const AxisPair& Y,
+                            const AxisPair& Z, const BlockCache& c, CornerBlocks& cb) {
+    const int bx0 = X.v0 >> 3, by0 = Y.v0 >> 3, bz0 = Z.v0 >> 3;
+    const int nx = (X.v1 >> 3) != bx0, ny = (Y.v1 >> 3) != by0, nz = (Z.v1 >> 3) != bz0;
+    int s0 = -1, s1 = -1, s2 = -1, s3 = -1, s4 = -1, s5 = -1, s6 = -1, s7 = -1;
+#pragma unroll 1
+    for (int b = 0; b < 8; ++b) {
+        const int dx = b & 1, dy = (b >> 1) & 1, dz = b >> 2;
+        if (dx > nx || dy > ny || dz > nz) continue;
+        const int bx = dx ? (X.v1 >> 3) : bx0, by = dy ? (Y.v1 >> 3) : by0,
+                  bz = dz ? (Z.v1 >> 3) : bz0;
+        const int sl = cache_has(c, bx, by, bz) ? c.slot : map_lookup(m, bx, by, bz);
+        s0 = b == 0 ? sl : s0;
+        s1 = b == 1 ? sl : s1;
+        s2 = b == 2 ? sl : s2;
+        s3 = b == 3 ? sl : s3;
+        s4 = b == 4 ? sl : s4;
+        s5 = b == 5 ? sl : s5;
+        s6 = b == 6 ? sl : s6;
+        s7 = b == 7 ? sl : s7;
+    }
+    cb.s[0] = s0;
+    cb.s[1] = s1;
+    cb.s[2] = s2;
+    cb.s[3] = s3;
+    cb.s[4] = s4;
+    cb.s[5] = s5;
+    cb.s[6] = s6;
+    cb.s[7] = s7;
 }
 
-// Rendering::readNormal (RenderScene.cu:345-378).
+// The slot of corner (cx, cy, cz) in {0,1}^3 (selected without indexing, so
+// the slots stay in registers).
+SICP_DEV int corner_slot(const CornerBlocks& cb, const AxisPair& X, const AxisPair& Y,
+                         const AxisPair& Z, int cx, int cy, int cz) {
+    const int dx = cx && ((X.v1 >> 3) != (X.v0 >> 3));
+    const int dy = cy && ((Y.v1 >> 3) != (Y.v0 >> 3));
+    const int dz = cz && ((Z.v1 >> 3) != (Z.v0 >> 3));
+    const int a0 = dx ? cb.s[1] : cb.s[0], a1 = dx ? cb.s[3] : cb.s[2];
+    const int a2 = dx ? cb.s[5] : cb.s[4], a3 = dx ? cb.s[7] : cb.s[6];
+    const int b0 = dy ? a1 : a0, b1 = dy ? a3 : a2;
+    return dz ? b1 : b0;
+}
+
+// Rendering::readSdfInterped (RenderScene.cu:310-343) from per-axis pairs.
+// Corner order, NaN for a missing block, `valid` of the last corner's lookup
+// and the blend order are the reference's. When all eight corners lie in the
+// cached block (the common case) no lookup is made.
+SICP_DEV float interp_axes(const MapDev& m, AxisPair X, AxisPair Y, AxisPair Z, BlockCache& c,
+                           bool& valid) {
+    float sd[8];
+    if (cache_has(c, X.v0 >> 3, Y.v0 >> 3, Z.v0 >> 3) && (X.v1 >> 3) == c.x &&
+        (Y.v1 >> 3) == c.y && (Z.v1 >> 3) == c.z) {
+        const sicp_voxel_geom* g = m.geom + (long)c.slot * kBlockVoxels;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            sd[i] = g[local_of((i & 1) ? X.v1 : X.v0, (i & 2) ? Y.v1 : Y.v0,
+                               (i & 4) ? Z.v1 : Z.v0)].sdf;
+        valid = true;
+    } else {
+        CornerBlocks cb;
+        corner_blocks(m, X, Y, Z, c, cb);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int sl = corner_slot(cb, X, Y, Z, i & 1, (i >> 1) & 1, i >> 2);
+            sd[i] = sl < 0 ? qnanf()
+                           : m.geom[(long)sl * kBlockVoxels +
+                                    local_of((i & 1) ? X.v1 : X.v0, (i & 2) ? Y.v1 : Y.v0,
+                                             (i & 4) ? Z.v1 : Z.v0)]
+                                 .sdf;
+        }
+        valid = corner_slot(cb, X, Y, Z, 1, 1, 1) >= 0;
+    }
+    const float fx = X.frac, fy = Y.frac, fz = Z.frac;
+    float r0, r1, r2, r3;
+    r0 = (1.0f - fx) * sd[0] + fx * sd[1];
+    r1 = (1.0f - fx) * sd[2] + fx * sd[3];
+    r2 = (1.0f - fy) * r0 + fy * r1;
+    r0 = (1.0f - fx) * sd[4] + fx * sd[5];
+    r1 = (1.0f - fx) * sd[6] + fx * sd[7];
+    r3 = (1.0f - fy) * r0 + fy * r1;
+    return (1.0f - fz) * r2 + fz * r3;
+}
+
+SICP_DEV float read_sdf_interped(const MapDev& m, v3 pt, BlockCache& c, bool& valid) {
+    return interp_axes(m, axis_pair(pt.x), axis_pair(pt.y), axis_pair(pt.z), c, valid);
+}
+
+// Rendering::readNormal (RenderScene.cu:345-378): six interpolated reads at
+// pt +- 1 voxel along each axis (pt.a + 1.0f, pt.a + -1.0f), in order, the
+// first failing one deciding. The three centre values per axis are
+// decomposed once and shared by the six reads.
 SICP_DEV bool read_normal(const MapDev& m, v3 pt, BlockCache& c, const PoseF& pose, v3& n) {
+    const AxisPair xm = axis_pair(pt.x + -1.0f), x0 = axis_pair(pt.x), xp = axis_pair(pt.x + 1.0f);
+    const AxisPair ym = axis_pair(pt.y + -1.0f), y0 = axis_pair(pt.y), yp = axis_pair(pt.y + 1.0f);
+    const AxisPair zm = axis_pair(pt.z + -1.0f), z0 = axis_pair(pt.z), zp = axis_pair(pt.z + 1.0f);
     bool valid;
     float s[6];
-    const v3 off[6] = {mk(1, 0, 0), mk(-1, 0, 0), mk(0, 1, 0),
-                       mk(0, -1, 0), mk(0, 0, 1), mk(0, 0, -1)};
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        s[i] = read_sdf_interped(m, add(pt, off[i]), c, valid);
-        if (isnan_(s[i]) || s[i] == 1.0f || !valid) return false;
-    }
+    s[0] = interp_axes(m, xp, y0, z0, c, valid);
+    if (isnan_(s[0]) || s[0] == 1.0f || !valid) return false;
+    s[1] = interp_axes(m, xm, y0, z0, c, valid);
+    if (isnan_(s[1]) || s[1] == 1.0f || !valid) return false;
+    s[2] = interp_axes(m, x0, yp, z0, c, valid);
+    if (isnan_(s[2]) || s[2] == 1.0f || !valid) return false;
+    s[3] = interp_axes(m, x0, ym, z0, c, valid);
+    if (isnan_(s[3]) || s[3] == 1.0f || !valid) return false;
+    s[4] = interp_axes(m, x0, y0, zp, c, valid);
+    if (isnan_(s[4]) || s[4] == 1.0f || !valid) return false;
+    s[5] = interp_axes(m, x0, y0, zm, c, valid);
+    if (isnan_(s[5]) || s[5] == 1.0f || !valid) return false;
     n = normalised(mv(pose.Rinv, mk(s[0] - s[1], s[2] - s[3], s[4] - s[5])));
     return true;
 }
@@ -186,6 +294,40 @@ struct RayArgs {
     int cols, rows;
 };
 
+// The ray of pixel (x, y) (RenderScene.cu:393-421): start point and unit
+// direction in voxel units, start and end distances along it.
+struct Ray {
+    v3 start, dir;
+    float dist_s, dist_e;
+};
+SICP_DEV bool ray_setup(const MapDev& m, const View<float>& zmin, const View<float>& zmax,
+                        const PoseF& pose, const RayArgs& g, int x, int y, Ray& r) {
+    const float z0 = zmin.at(y >> 3, x >> 3);
+    const float z1 = zmax.at(y >> 3, x >> 3);
+    if ((double)z1 < 1e-3 || (double)z0 < 1e-3 || isnan_(z0) || isnan_(z1)) return false;
+    v3 p;
+    p.z = z0;
+    p.x = p.z * ((float)x - g.cx) * g.invfx;
+    p.y = p.z * ((float)y - g.cy) * g.invfy;
+    r.dist_s = norm(p) * m.voxel_inv;
+    const v3 block_s = scale(add(mv(pose.R, p), pose.t), m.voxel_inv);
+    p.z = z1;
+    p.x = p.z * ((float)x - g.cx) * g.invfx;
+    p.y = p.z * ((float)y - g.cy) * g.invfy;
+    r.dist_e = norm(p) * m.voxel_inv;
+    const v3 block_e = scale(add(mv(pose.R, p), pose.t), m.voxel_inv);
+    r.dir = normalised(sub(block_e, block_s));
+    r.start = block_s;
+    return true;
+}
+
+// Samples one ray marches in the first pass; rays still marching then are
+// handed to the second pass. Most rays end well before; the few that graze a
+// surface for hundreds of one-voxel steps would otherwise set the length of
+// the whole launch.
+constexpr int kRayIters = 48;
+
+// First pass: one lane per ray, 8x8-pixel wavefronts (the zRange tile).
 __global__ __launch_bounds__(64) void raycast_kernel(MapDev m, View<float4> vmap,
                                                      View<float4> nmap, View<float> zmin,
                                                      View<float> zmax, PoseArg pa, RayArgs g,
@@ -199,29 +341,26 @@ __global__ __launch_bounds__(64) void raycast_kernel(MapDev m, View<float4> vmap
     float4 vout = make_float4(nan, nan, nan, nan);
     float4 nout = vout;
     const PoseF pose = load_pose(pa);
-    const float z0 = zmin.at(y >> 3, x >> 3);
-    const float z1 = zmax.at(y >> 3, x >> 3);
-    if (!((double)z1 < 1e-3 || (double)z0 < 1e-3 || isnan_(z0) || isnan_(z1))) {
-        float sdf = 1.0f;
-        v3 p;
-        p.z = z0;
-        p.x = p.z * ((float)x - g.cx) * g.invfx;
-        p.y = p.z * ((float)y - g.cy) * g.invfy;
-        float dist_s = norm(p) * m.voxel_inv;
-        v3 block_s = scale(add(mv(pose.R, p), pose.t), m.voxel_inv);
-        p.z = z1;
-        p.x = p.z * ((float)x - g.cx) * g.invfx;
-        p.y = p.z * ((float)y - g.cy) * g.invfy;
-        float dist_e = norm(p) * m.voxel_inv;
-        v3 block_e = scale(add(mv(pose.R, p), pose.t), m.voxel_inv);
-        v3 dir = normalised(sub(block_e, block_s));
-        v3 result = block_s;
+    Ray ray;
+    bool deferred = false;
+    v3 result{0.f, 0.f, 0.f};
+    float dist_s = 0.f, sdf = 1.0f;
+    if (ray_setup(m, zmin, zmax, pose, g, x, y, ray)) {
+        result = ray.start;
+        dist_s = ray.dist_s;
         bool valid_sdf = false, back_face = false;
         float step, prev_sdf;
         BlockCache cache{0, 0, 0, -1};
-        while (dist_s < dist_e) {
+        int iters = 0;
+        while (dist_s < ray.dist_e) {
+            if (iters == kRayIters) {
+                deferred = true;
+                break;
+            }
+            ++iters;
             prev_sdf = sdf;
-            long k = find_voxel(m, result, cache, valid_sdf);
+            long k = find_voxel_i(m, floor_i(result.x), floor_i(result.y), floor_i(result.z), cache,
+                                  valid_sdf);
             if (k < 0) {
                 sdf = nan;
             } else {
@@ -240,15 +379,15 @@ __global__ __launch_bounds__(64) void raycast_kernel(MapDev m, View<float4> vmap
                 if (prev_sdf > 0.f && sdf <= 0.f) break;
                 step = 1.0f;
             }
-            result = add(result, scale(step, dir));
+            result = add(result, scale(step, ray.dir));
             dist_s += step;
         }
-        if (!back_face && sdf <= 0.0f) {
+        if (!deferred && !back_face && sdf <= 0.0f) {
             step = sdf * m.step_scale;
-            result = add(result, scale(step, dir));
+            result = add(result, scale(step, ray.dir));
             sdf = read_sdf_interped(m, result, cache, valid_sdf);
             step = sdf * m.step_scale;
-            result = add(result, scale(step, dir));
+            result = add(result, scale(step, ray.dir));
             v3 normal;
             if (read_normal(m, result, cache, pose, normal)) {
                 v3 r = mv(pose.Rinv, sub(scale(result, m.voxel), pose.t));
@@ -257,9 +396,231 @@ __global__ __launch_bounds__(64) void raycast_kernel(MapDev m, View<float4> vmap
             }
         }
     }
+    // Hand-over: one queue reservation per wavefront.
+    const unsigned long long mask = __ballot(deferred);
+    if (mask) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((long long)mask) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(m.rq_count, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (deferred) {
+            const uint32_t q = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            if (q < m.rq_cap)
+                m.rq[q] = RayState{y * g.cols + x, result.x, result.y, result.z, dist_s, sdf, {0, 0}};
+        }
+        if (deferred) return;  // the second pass writes this pixel
+    }
     vmap.at(y, x) = vout;
     nmap.at(y, x) = nout;
 }
+
+// ---- second pass: kRayGroup lanes per ray --------------------------------
+// Lane k of a group evaluates the sample the scalar loop reaches after k more
+// one-voxel steps (result + dir added k times, as the loop adds it). A ballot
+// finds the first sample where the scalar loop would do anything but a plain
+// one-voxel step — out of range, missing or empty voxel, the interpolation
+// band, a sign change — and the group applies the loop body there exactly;
+// the samples before it are one-voxel steps taken in one go. Interpolated
+// reads and the normal split their corners over the group's lanes.
+constexpr int kRayGroup = 8;
+static_assert(48 % kRayGroup == 0 && kRayGroup >= 8, "corner split of the normal read");
+
+template <class T> SICP_DEV T gshfl(T v, int src) { return __shfl(v, src, kRayGroup); }
+SICP_DEV v3 gshfl3(v3 v, int src) { return v3{gshfl(v.x, src), gshfl(v.y, src), gshfl(v.z, src)}; }
+
+// Rendering::readSdfInterped with corner c on lane c of the group; every lane
+// returns the blended value and the validity of the last lookup.
+SICP_DEV float coop_interp(const MapDev& m, v3 pt, int k, BlockCache& c, bool& valid) {
+    const AxisPair X = axis_pair(pt.x), Y = axis_pair(pt.y), Z = axis_pair(pt.z);
+    float mine = qnanf();
+    bool ok = false;
+    if (k < 8) {
+        long idx = find_voxel_i(m, (k & 1) ? X.v1 : X.v0, (k & 2) ? Y.v1 : Y.v0,
+                                (k & 4) ? Z.v1 : Z.v0, c, ok);
+        mine = sdf_at(m, idx);
+    }
+    float sd[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) sd[i] = gshfl(mine, i);
+    valid = gshfl((int)ok, 7) != 0;
+    const float fx = X.frac, fy = Y.frac, fz = Z.frac;
+    float r0, r1, r2, r3;
+    r0 = (1.0f - fx) * sd[0] + fx * sd[1];
+    r1 = (1.0f - fx) * sd[2] + fx * sd[3];
+    r2 = (1.0f - fy) * r0 + fy * r1;
+    r0 = (1.0f - fx) * sd[4] + fx * sd[5];
+    r1 = (1.0f - fx) * sd[6] + fx * sd[7];
+    r3 = (1.0f - fy) * r0 + fy * r1;
+    return (1.0f - fz) * r2 + fz * r3;
+}
+
+// Rendering::readNormal with its 48 corners spread over the group's lanes.
+SICP_DEV bool coop_normal(const MapDev& m, v3 pt, int k, BlockCache& c, const PoseF& pose,
+                          v3& n) {
+    const AxisPair xm = axis_pair(pt.x + -1.0f), x0 = axis_pair(pt.x), xp = axis_pair(pt.x + 1.0f);
+    const AxisPair ym = axis_pair(pt.y + -1.0f), y0 = axis_pair(pt.y), yp = axis_pair(pt.y + 1.0f);
+    const AxisPair zm = axis_pair(pt.z + -1.0f), z0 = axis_pair(pt.z), zp = axis_pair(pt.z + 1.0f);
+    constexpr int kPer = 48 / kRayGroup;  // corners per lane
+    float val[kPer];
+    int ok[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int q = k + kRayGroup * j;  // corner q of the 48: read q / 8, corner q % 8
+        const int i = q >> 3, cc = q & 7;
+        const AxisPair X = i == 0 ? xp : (i == 1 ? xm : x0);
+        const AxisPair Y = i == 2 ? yp : (i == 3 ? ym : y0);
+        const AxisPair Z = i == 4 ? zp : (i == 5 ? zm : z0);
+        bool v;
+        long idx = find_voxel_i(m, (cc & 1) ? X.v1 : X.v0, (cc & 2) ? Y.v1 : Y.v0,
+                                (cc & 4) ? Z.v1 : Z.v0, c, v);
+        val[j] = sdf_at(m, idx);
+        ok[j] = v ? 1 : 0;
+    }
+    float s[6];
+    bool valid[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        float sd[8];
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc) {
+            const int q = i * 8 + cc;
+            sd[cc] = gshfl(val[q / kRayGroup], q % kRayGroup);
+        }
+        valid[i] = gshfl(ok[(i * 8 + 7) / kRayGroup], (i * 8 + 7) % kRayGroup) != 0;
+        const AxisPair X = i == 0 ? xp : (i == 1 ? xm : x0);
+        const AxisPair Y = i == 2 ? yp : (i == 3 ? ym : y0);
+        const AxisPair Z = i == 4 ? zp : (i == 5 ? zm : z0);
+        const float fx = X.frac, fy = Y.frac, fz = Z.frac;
+        float r0, r1, r2, r3;
+        r0 = (1.0f - fx) * sd[0] + fx * sd[1];
+        r1 = (1.0f - fx) * sd[2] + fx * sd[3];
+        r2 = (1.0f - fy) * r0 + fy * r1;
+        r0 = (1.0f - fx) * sd[4] + fx * sd[5];
+        r1 = (1.0f - fx) * sd[6] + fx * sd[7];
+        r3 = (1.0f - fy) * r0 + fy * r1;
+        s[i] = (1.0f - fz) * r2 + fz * r3;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+        if (isnan_(s[i]) || s[i] == 1.0f || !valid[i]) return false;
+    n = normalised(mv(pose.Rinv, mk(s[0] - s[1], s[2] - s[3], s[4] - s[5])));
+    return true;
+}
+
+__global__ __launch_bounds__(256) void raycast_long_kernel(MapDev m, View<float4> vmap,
+                                                           View<float4> nmap, View<float> zmin,
+                                                           View<float> zmax, PoseArg pa,
+                                                           RayArgs g, const int* gate,
+                                                           const uint32_t* nonempty) {
+    if (gate && *gate == 0) return;
+    if (nonempty && *nonempty == 0) return;
+    const uint32_t n = min(*m.rq_count, m.rq_cap);
+    const int k = threadIdx.x & (kRayGroup - 1);
+    const int gbase = (threadIdx.x & 63) & ~(kRayGroup - 1);  // first lane of the group
+    const uint32_t groups = gridDim.x * (blockDim.x / kRayGroup);
+    const PoseF pose = load_pose(pa);
+    const float nan = qnanf();
+    for (uint32_t i = blockIdx.x * (blockDim.x / kRayGroup) + (threadIdx.x / kRayGroup); i < n;
+         i += groups) {
+        const RayState st = m.rq[i];
+        const int x = st.px % g.cols, y = st.px / g.cols;
+        Ray ray;
+        ray_setup(m, zmin, zmax, pose, g, x, y, ray);  // true: the first pass marched it
+        v3 result{st.rx, st.ry, st.rz};
+        float dist_s = st.dist_s, sdf = st.sdf;
+        bool back_face = false;
+        BlockCache cache{0, 0, 0, -1};
+        while (dist_s < ray.dist_e) {
+            v3 P = result;
+            float D = dist_s;
+#pragma unroll
+            for (int j = 0; j < kRayGroup - 1; ++j)
+                if (j < k) {
+                    P = add(P, scale(1.0f, ray.dir));
+                    D += 1.0f;
+                }
+            const bool inr = D < ray.dist_e;
+            float sv = nan;
+            int exists = 0, wok = 0;
+            if (inr) {
+                bool v;
+                const long idx = find_voxel_i(m, floor_i(P.x), floor_i(P.y), floor_i(P.z), cache, v);
+                if (idx >= 0) {
+                    const uint2 w = ((const uint2*)m.geom)[idx];
+                    sv = __uint_as_float(w.x);
+                    exists = 1;
+                    wok = (w.y & 0xffu) != 0u;
+                }
+            }
+            float sp = __shfl_up(sv, 1, kRayGroup);
+            if (k == 0) sp = sdf;
+            const bool band = wok && sv <= 0.1f && sv >= -0.5f;
+            const bool cross = wok && !band && ((sp < 0.f && sv >= 0.f) || (sp > 0.f && sv <= 0.f));
+            const bool event = !inr || !wok || band || cross;
+            const uint32_t gm = (uint32_t)(__ballot(event) >> gbase) & ((1u << kRayGroup) - 1u);
+            if (gm == 0u) {  // kRayGroup one-voxel steps
+                result = add(gshfl3(P, kRayGroup - 1), scale(1.0f, ray.dir));
+                dist_s = gshfl(D, kRayGroup - 1) + 1.0f;
+                sdf = gshfl(sv, kRayGroup - 1);
+                continue;
+            }
+            const int ks = __ffs(gm) - 1;
+            result = gshfl3(P, ks);
+            dist_s = gshfl(D, ks);
+            const float prev = gshfl(sp, ks);
+            if (!gshfl((int)inr, ks)) {  // the loop condition fails here
+                sdf = prev;
+                break;
+            }
+            const float sk = gshfl(sv, ks);
+            if (!gshfl(wok, ks)) {  // missing block (NaN) or empty voxel: block-width step
+                sdf = gshfl(exists, ks) ? sk : nan;
+                result = add(result, scale((float)kBlockSide, ray.dir));
+                dist_s += (float)kBlockSide;
+                continue;
+            }
+            float s1 = sk;
+            if (gshfl((int)band, ks)) {
+                bool v;
+                s1 = coop_interp(m, result, k, cache, v);
+            }
+            if (prev < 0.f && s1 >= 0.f) {
+                back_face = true;
+                break;
+            }
+            if (prev > 0.f && s1 <= 0.f) {
+                sdf = s1;
+                break;
+            }
+            sdf = s1;
+            result = add(result, scale(1.0f, ray.dir));
+            dist_s += 1.0f;
+        }
+        float4 vout = make_float4(nan, nan, nan, nan);
+        float4 nout = vout;
+        if (!back_face && sdf <= 0.0f) {
+            float step = sdf * m.step_scale;
+            result = add(result, scale(step, ray.dir));
+            bool v;
+            sdf = coop_interp(m, result, k, cache, v);
+            step = sdf * m.step_scale;
+            result = add(result, scale(step, ray.dir));
+            v3 normal;
+            if (coop_normal(m, result, k, cache, pose, normal)) {
+                v3 r = mv(pose.Rinv, sub(scale(result, m.voxel), pose.t));
+                vout = make_float4(r.x, r.y, r.z, 1.0f);
+                nout = make_float4(normal.x, normal.y, normal.z, 1.0f);
+            }
+        }
+        if (k == 0) {
+            vmap.at(y, x) = vout;
+            nmap.at(y, x) = nout;
+        }
+    }
+}
+
+__global__ void ray_queue_reset_kernel(uint32_t* count) { *count = 0u; }
 
 }  // namespace
 
@@ -275,24 +636,27 @@ int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible,
     return SICP_OK;
 }
 
-int launch_raycast(MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin,
-                   View<float> zmax, PoseArg pose, float invfx, float invfy, float cx, float cy,
-                   const int* gate, hipStream_t s) {
-    RayArgs g{invfx, invfy, cx, cy, vmap.cols, vmap.rows};
-    dim3 grid(div_up(vmap.cols, 8), div_up(vmap.rows, 8));
-    raycast_kernel<<<grid, 64, 0, s>>>(m, vmap, nmap, zmin, zmax, pose, g, gate, nullptr);
-    SICP_LAUNCH_CHECK("raycast_kernel");
-    return SICP_OK;
-}
-
+// Both passes; the queue count is cleared first.
 int launch_raycast_gated(MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin,
                          View<float> zmax, PoseArg pose, float invfx, float invfy, float cx,
                          float cy, const int* gate, const uint32_t* nonempty, hipStream_t s) {
+    if ((size_t)vmap.cols * vmap.rows > m.rq_cap) return invalid("raycast: image above the map's max_cols x max_rows");
     RayArgs g{invfx, invfy, cx, cy, vmap.cols, vmap.rows};
+    ray_queue_reset_kernel<<<1, 1, 0, s>>>(m.rq_count);
+    SICP_LAUNCH_CHECK("ray_queue_reset_kernel");
     dim3 grid(div_up(vmap.cols, 8), div_up(vmap.rows, 8));
     raycast_kernel<<<grid, 64, 0, s>>>(m, vmap, nmap, zmin, zmax, pose, g, gate, nonempty);
     SICP_LAUNCH_CHECK("raycast_kernel");
+    raycast_long_kernel<<<1024, 256, 0, s>>>(m, vmap, nmap, zmin, zmax, pose, g, gate, nonempty);
+    SICP_LAUNCH_CHECK("raycast_long_kernel");
     return SICP_OK;
+}
+
+int launch_raycast(MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin,
+                   View<float> zmax, PoseArg pose, float invfx, float invfy, float cx, float cy,
+                   const int* gate, hipStream_t s) {
+    return launch_raycast_gated(m, vmap, nmap, zmin, zmax, pose, invfx, invfy, cx, cy, gate,
+                                nullptr, s);
 }
 
 }  // namespace sicp

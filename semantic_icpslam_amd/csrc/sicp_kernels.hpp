@@ -99,6 +99,14 @@ void se3_update_host(const double* A, const double* b, double* delta, const doub
                      double* next_pose, double* x_out);
 
 // ---------------------------------------------------------------- map -----
+// A ray handed from the first raycast pass to the second: its pixel and the
+// march state at the hand-over (the rest is recomputed from the pixel).
+struct RayState {
+    int px;
+    float rx, ry, rz, dist_s, sdf;
+    int pad[2];
+};
+
 struct MapDev {
     sicp_hash_entry* table;   // 2 * nb
     sicp_hash_entry* excess;  // num_blocks, indexed by heap slot
@@ -109,6 +117,9 @@ struct MapDev {
     int num_blocks;
     float voxel, voxel_inv, trunc, step_scale, block_width, dmin, dmax;
     int max_weight, max_count;
+    RayState* rq;        // raycast hand-over queue, rq_cap entries
+    uint32_t* rq_count;  // entries queued by the first pass
+    uint32_t rq_cap;
 };
 
 struct MapCounters {
@@ -157,6 +168,10 @@ int launch_fuse(MapDev m, MapCounters* ctr, const sicp_block* visible, View<floa
                 View<uint8_t> rgb, View<float4> nmap, View<uint8_t> mask, int num_detection,
                 PoseArg pose, sicp_intrinsics K, uint32_t* wave_counts, const int* gate,
                 hipStream_t s);
+// The per-wavefront counts of launch_fuse into the map's counters (always run
+// after it).
+int launch_fuse_count(MapCounters* ctr, const uint32_t* wave_counts, const int* gate,
+                      hipStream_t s);
 int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible, View<float> zmin,
                           View<float> zmax, PoseArg pose, sicp_intrinsics K, float dmax,
                           float dmin, const int* gate, hipStream_t s);

@@ -86,7 +86,7 @@ struct sicp_system {
 namespace sicp {
 
 enum Stage { kStagePyramid = 0, kStageTrack, kStageAlloc, kStageVisible, kStageFuse,
-             kStagePrepass, kStageRaycast, kStageCount };
+             kStagePrepass, kStageRaycast, kStageFuseCount, kStageFrame, kStageCount };
 
 // Returns the index of the start event, or SIZE_MAX when timing is off.
 static size_t stage_begin(sicp_system* s, hipStream_t st) {
@@ -221,9 +221,20 @@ sicp_map* sicp_system_map(sicp_system* sys) { return sys ? sys->map : nullptr; }
 
 // System::GrabImage (Core/System.cc:131-172) with Tracker::GrabFrame /
 // Track / TrackFrame / ComputeSE3 (Tracking.cc:65-119,157-173,246-280,369-517).
+static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
+                      sicp_image labels_u8, int num_detection, void* stream);
+
 int sicp_system_grab_image(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
                            sicp_image labels_u8, int num_detection, void* stream) {
     if (!s) return invalid("system_grab_image: null system");
+    const size_t ev = stage_begin(s, (hipStream_t)stream);
+    const int rc = grab_frame(s, depth_u16, rgb_u8x3, labels_u8, num_detection, stream);
+    if (rc == SICP_OK) stage_end(s, kStageFrame, ev, (hipStream_t)stream);
+    return rc;
+}
+
+static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
+                      sicp_image labels_u8, int num_detection, void* stream) {
     const int c = s->cfg.cols, r = s->cfg.rows;
     if (!image_ok(depth_u16, 2, c, r) || !image_ok(rgb_u8x3, 3, c, r))
         return invalid("system_grab_image: depth / rgb shape mismatch");
@@ -289,10 +300,12 @@ int sicp_system_grab_image(sicp_system* s, sicp_image depth_u16, sicp_image rgb_
     ev = stage_begin(s, st);
     if ((rc = launch_fuse(m->dev, m->ctr, m->visible, f.range.v(), view<uint8_t>(rgb_u8x3),
                           f.nmap[0].v(), mask, labels ? num_detection : 0, pose, s->K[0],
-                          m->alloc.fuse_counts, gate,
-                          st)))
+                          m->alloc.fuse_counts, gate, st)))
         return rc;
     stage_end(s, kStageFuse, ev, st);
+    ev = stage_begin(s, st);
+    if ((rc = launch_fuse_count(m->ctr, m->alloc.fuse_counts, gate, st))) return rc;
+    stage_end(s, kStageFuseCount, ev, st);
     // Mapping::RayTrace (Mapping.cc:142-161): prepass, then the raycast into
     // the frame's level-0 model maps when any block produced a tile.
     ev = stage_begin(s, st);
@@ -309,10 +322,9 @@ int sicp_system_grab_image(sicp_system* s, sicp_image depth_u16, sicp_image rgb_
     return rc;
 }
 
-// Stage timing of the frames grabbed since it was enabled: per stage, the
-// summed event time in ms and the number of timed instances. Disables and
-// clears the recorded marks. Stages: 0 pyramid, 1 track, 2 allocate,
-// 3 visibility, 4 fuse, 5 render prepass, 6 raycast.
+// Stage timing of the frames grabbed since it was enabled. Stages: 0 pyramid,
+// 1 track, 2 allocate, 3 visibility, 4 fuse kernel, 5 render prepass,
+// 6 raycast kernel, 7 fusion count, 8 whole frame.
 int sicp_system_set_timing(sicp_system* s, int enable) {
     if (!s) return invalid("system_set_timing: null system");
     s->timing = enable != 0;
@@ -339,6 +351,24 @@ int sicp_system_read_timing(sicp_system* s, double* total_ms, int* counts, int n
     }
     s->marks.clear();
     s->pool_used = 0;
+    return SICP_OK;
+}
+
+int sicp_system_read_timing_detail(sicp_system* s, int stage, double* ms_out, int cap,
+                                   int* count, void* stream) {
+    if (!s || !count || (cap > 0 && !ms_out)) return invalid("system_read_timing_detail: null argument");
+    SICP_HIP(hipStreamSynchronize((hipStream_t)stream));
+    int n = 0;
+    for (auto& mk : s->marks) {
+        if (mk.first != stage) continue;
+        if (n < cap) {
+            float ms = 0.f;
+            SICP_HIP(hipEventElapsedTime(&ms, s->pool[mk.second], s->pool[mk.second + 1]));
+            ms_out[n] = ms;
+        }
+        ++n;
+    }
+    *count = n;
     return SICP_OK;
 }
 

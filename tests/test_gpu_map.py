@@ -163,3 +163,5 @@ def test_prepass_and_raycast_bit_exact(dev, c):
     assert bits_equal(host(n), on), mismatch_report(host(n), on)
     hit = ~np.isnan(ov[..., 0])
     assert hit.mean() > 0.3
+    if c is CFG_VGA:  # grazing rays reach the second (cooperative) raycast pass
+        assert gm.stats()["ray_handover"] > 0
