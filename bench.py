@@ -22,7 +22,7 @@ sys.path.insert(0, ROOT)
 WORKLOAD = dict(cols=640, rows=480, f=525.0, cx=319.5, cy=239.5, voxel=0.005, trunc=0.015,
                 buckets=1 << 20, blocks=400000, seed=0)
 STAGES = ["pyramid", "track", "allocate", "visibility", "fuse", "prepass", "raycast",
-          "fuse_count", "frame"]
+          "fuse_count", "frame", "icp_iteration", "track_finish"]
 HBM_PEAK_GBS = 8000.0
 # Bytes of one fused voxel: 8 B geometry + 12 B semantic record read and
 # written, plus its depth (4), RGB (3), normal (16) and label (1) gathers.
@@ -190,42 +190,56 @@ def main():
 
     cfg = B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"], w["voxel"],
                           w["trunc"], w["buckets"], w["blocks"])
-    sysg = B.System(cfg)
     stream = torch.cuda.current_stream()
-    log(f"[bench] warmup {args.warmup} frames")
-    for i in range(args.warmup):
-        sysg.grab(depth[i], rgb[i], lab[i], stream)
-    torch.cuda.synchronize()
-    map_h = sysg.map_handle()
-    st0 = B.map_stats(map_h, stream)
 
-    # Timed region: K frames, stage events recorded on the same stream.
-    B.check(B.lib().sicp_system_set_timing(sysg.h, 1))
-    log(f"[bench] timing {args.steps} frames")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.warmup, n):
-        sysg.grab(depth[i], rgb[i], lab[i], stream)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    tot = np.zeros(len(STAGES))
-    cnt = np.zeros(len(STAGES), np.int32)
-    frame_ms = np.zeros(args.steps)
-    nf = B.C.c_int(0)
-    B.check(B.lib().sicp_system_read_timing_detail(sysg.h, STAGES.index("frame"),
-                                                   frame_ms.ctypes.data, args.steps,
-                                                   B.C.byref(nf), B.stream_ptr(stream)))
-    B.check(B.lib().sicp_system_read_timing(sysg.h, tot.ctypes.data, cnt.ctypes.data, len(STAGES),
-                                            B.stream_ptr(stream)))
-    B.check(B.lib().sicp_system_set_timing(sysg.h, 0))
-    st1 = B.map_stats(map_h, stream)
-    res = sysg.result(stream)
+    def run_pass(timing):
+        """Warm-up frames, then the K timed frames on a fresh system; with
+        `timing`, HIP events bracket every stage and kernel launch."""
+        sysg = B.System(cfg)
+        for i in range(args.warmup):
+            sysg.grab(depth[i], rgb[i], lab[i], stream)
+        torch.cuda.synchronize()
+        map_h = sysg.map_handle()
+        st0 = B.map_stats(map_h, stream)
+        if timing:
+            B.check(B.lib().sicp_system_set_timing(sysg.h, 1))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.warmup, n):
+            sysg.grab(depth[i], rgb[i], lab[i], stream)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        out = {"elapsed": elapsed, "st0": st0, "st1": B.map_stats(map_h, stream),
+               "res": sysg.result(stream)}
+        if timing:
+            tot = np.zeros(len(STAGES))
+            cnt = np.zeros(len(STAGES), np.int32)
+            frame_ms = np.zeros(args.steps)
+            nf = B.C.c_int(0)
+            B.check(B.lib().sicp_system_read_timing_detail(sysg.h, STAGES.index("frame"),
+                                                           frame_ms.ctypes.data, args.steps,
+                                                           B.C.byref(nf), B.stream_ptr(stream)))
+            B.check(B.lib().sicp_system_read_timing(sysg.h, tot.ctypes.data, cnt.ctypes.data,
+                                                    len(STAGES), B.stream_ptr(stream)))
+            B.check(B.lib().sicp_system_set_timing(sysg.h, 0))
+            out.update(tot=tot, cnt=cnt, frame_ms=frame_ms[: nf.value])
+        sysg.close()
+        return out
+
+    # Timed region: K frames with no instrumentation (the reported value),
+    # then the same frames again on a fresh system with per-stage HIP events
+    # (stage times, per-frame percentiles, per-kernel rooflines).
+    log(f"[bench] warmup {args.warmup} + timing {args.steps} frames")
+    clean = run_pass(False)
+    log("[bench] instrumented pass (HIP events per stage and kernel)")
+    inst = run_pass(True)
+    st0, st1, res = inst["st0"], inst["st1"], clean["res"]
+    tot, cnt, fm = inst["tot"], inst["cnt"], inst["frame_ms"]
     gt = syn.pose(n - 1)
-    ms_per_frame = 1000.0 * elapsed / args.steps
+    ms_per_frame = 1000.0 * clean["elapsed"] / args.steps
     stage_ms = {s: round(float(tot[k] / max(cnt[k], 1)), 4) for k, s in enumerate(STAGES)}
+    stage_ms["icp_iteration"] = round(float(tot[STAGES.index("icp_iteration")] / args.steps), 4)
     log("[bench] per-stage ms/frame: " + json.dumps(stage_ms))
-    fm = frame_ms[: nf.value]
-    sysg.close()
 
     traffic = load_traffic()
     kernels = {}
@@ -253,16 +267,27 @@ def main():
                              RAY_BYTES_PER_STEP * work["march_steps"] +
                              RAY_BYTES_PER_INTERP * work["interp_reads"] +
                              RAY_BYTES_PER_LOOKUP * work["ray_lookups"])
-        kernels["raycast_kernel"] = roofline_entry(
-            "raycast_kernel", ray_bytes, tot[STAGES.index("raycast")] / 1000.0,
-            cnt[STAGES.index("raycast")], traffic.get("raycast_kernel"))
-        kernels["raycast_kernel"]["work_per_ray"] = {
+        ray_traffic = None
+        if "raycast_kernel" in traffic and "raycast_long_kernel" in traffic:
+            ray_traffic = traffic["raycast_kernel"] + traffic["raycast_long_kernel"]
+        kernels["raycast"] = roofline_entry(
+            "raycast_kernel + raycast_long_kernel", ray_bytes,
+            tot[STAGES.index("raycast")] / 1000.0, cnt[STAGES.index("raycast")], ray_traffic)
+        kernels["raycast"]["work_per_ray"] = {
             k: round(work[k] / max(work["rays"], 1), 3)
             for k in ("march_steps", "interp_reads", "ray_lookups")}
-        kernels["raycast_kernel"]["work_frames"] = work["frames"]
-    # The dominant kernel: the single-launch-per-frame kernel with the longest
-    # launch.
-    dom = max(kernels, key=lambda k: kernels[k]["avg_launch_us"])
+        kernels["raycast"]["work_frames"] = work["frames"]
+    # ICP: every Gauss-Newton launch reads 64 B per pixel of its level.
+    lv = [(w["cols"] >> l) * (w["rows"] >> l) for l in range(3)]
+    icp_launches = int(cnt[STAGES.index("icp_iteration")])
+    icp_bytes = ICP_BYTES_PER_PIXEL * args.steps * (10 * lv[0] + 5 * lv[1] + 3 * lv[2])
+    kernels["icp_iteration_kernel"] = roofline_entry(
+        "icp_iteration_kernel", icp_bytes, tot[STAGES.index("icp_iteration")] / 1000.0,
+        icp_launches, traffic.get("icp_iteration_kernel"))
+    kernels["icp_iteration_kernel"]["launches_per_frame"] = round(icp_launches / args.steps, 2)
+    # The dominant kernel: the most time per frame (summed over its launches).
+    dom = max(kernels, key=lambda k: kernels[k]["avg_launch_us"] *
+              kernels[k].get("launches_per_frame", 1.0))
     roofline = kernels[dom]
 
     pose_check = None

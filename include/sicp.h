@@ -294,8 +294,9 @@ sicp_map* sicp_system_map(sicp_system* sys);
 /* Stage timing with HIP events on the frame's stream (no counterpart in the
  * reference, whose only timing is std::clock around GrabImage,
  * Core/System.cc:142-146). Stages: 0 pyramid, 1 track, 2 allocate,
- * 3 visibility, 4 fuse kernel, 5 render prepass, 6 raycast kernel, 7 fusion
- * count, 8 whole frame. read_timing synchronises `stream`, returns per-stage
+ * 3 visibility, 4 fuse kernel, 5 render prepass, 6 raycast (both passes),
+ * 7 fusion count, 8 whole frame, 9 each ICP iteration launch (nested in 1),
+ * 10 tracking finish (nested in 1). read_timing synchronises `stream`, returns per-stage
  * summed milliseconds and instance counts of the frames grabbed since
  * set_timing(1), and clears them; read_timing_detail (call it first) gives
  * one stage's per-instance milliseconds in grab order without clearing. */

@@ -86,14 +86,18 @@ struct sicp_system {
 namespace sicp {
 
 enum Stage { kStagePyramid = 0, kStageTrack, kStageAlloc, kStageVisible, kStageFuse,
-             kStagePrepass, kStageRaycast, kStageFuseCount, kStageFrame, kStageCount };
+             kStagePrepass, kStageRaycast, kStageFuseCount, kStageFrame, kStageIcpIter,
+             kStageTrackFinish, kStageCount };
 
 // Returns the index of the start event, or SIZE_MAX when timing is off.
 static size_t stage_begin(sicp_system* s, hipStream_t st) {
     if (!s->timing) return (size_t)-1;
     while (s->pool.size() < s->pool_used + 2) {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return (size_t)-1;
+        // Timing-only events: no system-scope fence (cache writeback and
+        // invalidate) at each record; read_timing synchronises the stream.
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess)
+            return (size_t)-1;
         s->pool.push_back(e);
     }
     size_t i = s->pool_used;
@@ -265,19 +269,23 @@ static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
             const IcpLevel cur{nf.vmap[l].v(), lf.vmap[l].v(), nf.nmap[l].v(), lf.nmap[l].v(),
                                s->K[l], s->cfg.icp_min_count[l]};
             for (int j = 0; j < s->cfg.icp_iters[l]; ++j, ++k) {
+                const size_t ek = stage_begin(s, st);
                 rc = launch_icp_iteration(cur, k ? &prev : nullptr, lf.meta, s->cfg.icp_thresh,
                                           iter + ((k + 1) & 1), iter + (k & 1),
                                           s->partials + psz * ((k + 1) & 1),
                                           s->partials + psz * (k & 1), st);
                 if (rc) return rc;
+                stage_end(s, kStageIcpIter, ek, st);
                 prev = cur;
             }
         }
+        const size_t ef = stage_begin(s, st);
         if ((rc = launch_track_finish(k ? &prev : nullptr, lf.meta, s->cfg.icp_thresh,
                                       iter + ((k + 1) & 1), s->partials + psz * ((k + 1) & 1),
                                       s->st, nf.meta, lf.vmap[0].v(), lf.nmap[0].v(),
                                       nf.vmap[0].v(), nf.nmap[0].v(), st)))
             return rc;
+        stage_end(s, kStageTrackFinish, ef, st);
     }
     stage_end(s, kStageTrack, ev, st);
     std::swap(s->next, s->last);  // SwapFrame (Tracking.cc:318-320)
@@ -324,7 +332,8 @@ static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
 
 // Stage timing of the frames grabbed since it was enabled. Stages: 0 pyramid,
 // 1 track, 2 allocate, 3 visibility, 4 fuse kernel, 5 render prepass,
-// 6 raycast kernel, 7 fusion count, 8 whole frame.
+// 6 raycast (both passes), 7 fusion count, 8 whole frame, 9 each ICP
+// iteration launch, 10 tracking finish.
 int sicp_system_set_timing(sicp_system* s, int enable) {
     if (!s) return invalid("system_set_timing: null system");
     s->timing = enable != 0;
