@@ -102,6 +102,44 @@ def cpu_baseline(w, frames, first_timed, seconds=20.0):
     return out, work
 
 
+SMALL = dict(cols=320, rows=240, f=262.5, cx=159.5, cy=119.5, voxel=0.01, trunc=0.03,
+             buckets=1 << 18, blocks=60000, seed=0)
+
+
+def cpu_small():
+    """The reference method on the CPU at the small size: 30 frames at
+    320x240, 1 cm voxels (the oracle, all host cores and one thread), with
+    the pose difference to the FP64 transcription."""
+    from oracle import binding as O
+    from semantic_icpslam_amd import binding as B
+    w = SMALL
+    _, frames = make_frames(30, w)
+    cfg = B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"], w["voxel"],
+                          w["trunc"], w["buckets"], w["blocks"])
+    cores = O.set_threads(0)
+    out = {"workload": "320x240 / 1 cm voxels / T 3 cm / 2^18x2 buckets / 30 frames",
+           "cores": int(cores)}
+    poses = {}
+    for label, threads, fp64 in (("all_cores", cores, False), ("one_thread", 1, False),
+                                 ("fp64", cores, True)):
+        O.set_threads(threads)
+        sysc = O.OracleSystem(cfg, fp64=fp64)
+        ts, ps = [], []
+        for d, rgb, lab in frames:
+            t = time.perf_counter()
+            r = sysc.grab(d, rgb, lab)
+            ts.append(1000.0 * (time.perf_counter() - t))
+            ps.append(r["pose"])
+        poses[label] = ps
+        if not fp64:
+            out[label] = {"ms_p50": round(float(np.percentile(ts[1:], 50)), 3),
+                          "ms_p99": round(float(np.percentile(ts[1:], 99)), 3)}
+    O.set_threads(cores)
+    dt = [float(np.linalg.norm(a[:3, 3] - b[:3, 3])) for a, b in zip(poses["all_cores"], poses["fp64"])]
+    out["pose_vs_fp64"] = {"max_translation_m": max(dt), "last_translation_m": dt[-1]}
+    return out
+
+
 def pose_error_vs_fp64(w, frames, cfg, m=30):
     """Per-frame pose of the GPU system against the FP64 oracle on the first m
     frames (an untimed pass): max rotation angle (rad) of T_fp64^-1 T and max
@@ -192,12 +230,21 @@ def main():
                           w["trunc"], w["buckets"], w["blocks"])
     stream = torch.cuda.current_stream()
 
-    def run_pass(timing):
+    def run_pass(timing, graph=False):
         """Warm-up frames, then the K timed frames on a fresh system; with
-        `timing`, HIP events bracket every stage and kernel launch."""
+        `timing`, HIP events bracket every stage and kernel launch; with
+        `graph`, every frame is copied into the system's staged inputs and
+        runs as one hipGraph launch."""
         sysg = B.System(cfg)
+
+        def grab(i):
+            if graph:
+                sysg.grab_staged(depth[i], rgb[i], lab[i], True, stream)
+            else:
+                sysg.grab(depth[i], rgb[i], lab[i], stream)
+
         for i in range(args.warmup):
-            sysg.grab(depth[i], rgb[i], lab[i], stream)
+            grab(i)
         torch.cuda.synchronize()
         map_h = sysg.map_handle()
         st0 = B.map_stats(map_h, stream)
@@ -206,7 +253,7 @@ def main():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in range(args.warmup, n):
-            sysg.grab(depth[i], rgb[i], lab[i], stream)
+            grab(i)
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         out = {"elapsed": elapsed, "st0": st0, "st1": B.map_stats(map_h, stream),
@@ -231,6 +278,8 @@ def main():
     # (stage times, per-frame percentiles, per-kernel rooflines).
     log(f"[bench] warmup {args.warmup} + timing {args.steps} frames")
     clean = run_pass(False)
+    log("[bench] graph pass (staged inputs, one hipGraph launch per frame)")
+    graphed = run_pass(False, graph=True)
     log("[bench] instrumented pass (HIP events per stage and kernel)")
     inst = run_pass(True)
     st0, st1, res = inst["st0"], inst["st1"], clean["res"]
@@ -290,6 +339,10 @@ def main():
               kernels[k].get("launches_per_frame", 1.0))
     roofline = kernels[dom]
 
+    small = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU reference method at the small size")
+        small = cpu_small()
     pose_check = None
     if not args.no_pose_check:
         log("[bench] pose check against the FP64 oracle")
@@ -316,12 +369,18 @@ def main():
         "last_frame_tracked": res["tracked"],
         "last_frame_translation_vs_ground_truth_m": round(
             float(np.linalg.norm(res["pose"][:3, 3] - gt[:3, 3])), 5),
+        "graph_form": {
+            "ms_per_step": round(1000.0 * graphed["elapsed"] / args.steps, 4),
+            "what": "the same frames, each copied into the system's staged inputs and run as "
+                    "one hipGraph launch (plain FP32)",
+            "same_final_pose": bool(np.array_equal(graphed["res"]["pose"], res["pose"]))},
         "frame_ms_p50": round(float(np.percentile(fm, 50)), 4) if fm.size else None,
         "frame_ms_p99": round(float(np.percentile(fm, 99)), 4) if fm.size else None,
         "roofline": roofline,
         "roofline_kernels": kernels,
         "pose_vs_fp64_oracle": pose_check,
         "cpu_baseline": cpu,
+        "cpu_small": small,
     }
     print(json.dumps(out), flush=True)
 

@@ -305,6 +305,17 @@ int sicp_system_read_timing(sicp_system* sys, double* total_ms, int* counts, int
                             void* stream);
 int sicp_system_read_timing_detail(sicp_system* sys, int stage, double* ms_out, int cap,
                                    int* count, void* stream);
+/* Graph form of GrabImage (no counterpart in the reference). The system owns
+ * staged input images (uint16 depth, uchar3 RGB, uint8 labels); the caller
+ * fills them, e.g. with device-to-device copies on `stream`, and calls
+ * grab_staged. From the second frame on, with `use_graph`, the whole frame
+ * (pyramid, 18 ICP launches, allocation, fusion, prepass, raycast) is one
+ * hipGraph launch: two graphs, one per frame-buffer parity, recorded on first
+ * use and again when num_detection changes. Results equal grab_image's. */
+int sicp_system_staging(sicp_system* sys, sicp_image* depth_u16, sicp_image* rgb_u8x3,
+                        sicp_image* labels_u8);
+int sicp_system_grab_staged(sicp_system* sys, int num_detection, int use_graph, void* stream);
+
 /* Frame buffers of the system, for inspection (the public members of Frame,
  * Core/Frame.h): which = 0 vmap, 1 nmap, 2 raw depth (level 0 only),
  * 3 filtered depth, 4 intensity, 5 d/dx, 6 d/dy at pyramid `level`, of the

@@ -139,6 +139,8 @@ _SIGS = {
     "sicp_system_map": (_P, [_P]),
     "sicp_system_set_timing": (C.c_int, [_P, C.c_int]),
     "sicp_system_read_timing": (C.c_int, [_P, _P, _P, C.c_int, _P]),
+    "sicp_system_staging": (C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image)]),
+    "sicp_system_grab_staged": (C.c_int, [_P, C.c_int, C.c_int, _P]),
     "sicp_system_read_timing_detail": (C.c_int, [_P, C.c_int, _P, C.c_int, C.POINTER(C.c_int),
                                                  _P]),
     "sicp_system_frame_image": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Image)]),
@@ -149,6 +151,23 @@ _SIGS = {
 }
 
 EXPORTED = tuple(_SIGS.keys())
+
+
+_hip = None
+
+
+def copy2d(dst, src, stream=None):
+    """Device-to-device copy of a torch image tensor into a pitched image."""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpy2DAsync.argtypes = [_P, C.c_size_t, _P, C.c_size_t, C.c_size_t, C.c_size_t,
+                                          C.c_int, _P]
+    s = image(src)
+    width = src.shape[1] * src.element_size() * (src.shape[2] if src.dim() == 3 else 1)
+    if _hip.hipMemcpy2DAsync(dst.ptr, dst.pitch, s.ptr, s.pitch, width, s.rows, 3,
+                             stream_ptr(stream)) != 0:
+        raise SicpError(SICP_ERR_HIP, "hipMemcpy2DAsync failed")
 
 
 def lib():
@@ -360,6 +379,24 @@ class System:
         nd = 1 if labels is not None else 0
         check(lib().sicp_system_grab_image(self.h, image(depth), image(rgb), image(labels), nd,
                                            stream_ptr(stream)), "sicp_system_grab_image")
+
+    def staging(self):
+        """The system's staged input images (depth, rgb, labels) for grab_staged."""
+        ims = [Image(), Image(), Image()]
+        check(lib().sicp_system_staging(self.h, *[C.byref(i) for i in ims]), "system_staging")
+        return ims
+
+    def grab_staged(self, depth, rgb, labels=None, use_graph=True, stream=None):
+        """Copies the frame into the staged inputs on `stream` (device to device)
+        and runs GrabImage from them; from the second frame on as one graph."""
+        if not hasattr(self, "_staged"):
+            self._staged = self.staging()
+        for dst, src in zip(self._staged, (depth, rgb, labels)):
+            if src is not None:
+                copy2d(dst, src, stream)
+        nd = 1 if labels is not None else 0
+        check(lib().sicp_system_grab_staged(self.h, nd, 1 if use_graph else 0, stream_ptr(stream)),
+              "system_grab_staged")
 
     def result(self, stream=None):
         r = FrameResult()

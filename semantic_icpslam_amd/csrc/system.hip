@@ -81,6 +81,13 @@ struct sicp_system {
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
     std::vector<std::pair<int, size_t>> marks;  // (stage, index of the start event)
+    // Graph form: staged inputs owned by the system and, per frame parity
+    // (which buffer is "next"), one instantiated graph of the whole frame.
+    sicp::DevImage<uint16_t> in_depth;
+    sicp::DevImage<uint8_t> in_rgb, in_labels;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_nd = -1;  // num_detection the graphs were captured with
+    hipStream_t capture_stream = nullptr;
 };
 
 namespace sicp {
@@ -142,6 +149,12 @@ static int frame_pyramid(sicp_system* s, FrameBuf& f, FrameBuf& last,
 static void system_free(sicp_system* s) {
     if (!s) return;
     for (hipEvent_t e : s->pool) (void)hipEventDestroy(e);
+    for (hipGraphExec_t& g : s->graph)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (s->capture_stream) (void)hipStreamDestroy(s->capture_stream);
+    s->in_depth.release();
+    s->in_rgb.release();
+    s->in_labels.release();
     s->fa.release();
     s->fb.release();
     s->zmin.release();
@@ -400,6 +413,86 @@ int sicp_system_last_result(sicp_system* s, sicp_frame_result* out, void* stream
     out->visible_blocks = c.visible;
     out->allocated_blocks = c.heap_used;
     std::memcpy(out->pose, fm.pose, sizeof(out->pose));
+    return SICP_OK;
+}
+
+int sicp_system_staging(sicp_system* s, sicp_image* depth_u16, sicp_image* rgb_u8x3,
+                        sicp_image* labels_u8) {
+    if (!s) return invalid("system_staging: null system");
+    if (!s->in_depth.ptr) {
+        int rc;
+        const int c = s->cfg.cols, r = s->cfg.rows;
+        if ((rc = s->in_depth.create(c, r)) || (rc = s->in_rgb.create(c, r, 3)) ||
+            (rc = s->in_labels.create(c, r)))
+            return rc;
+    }
+    if (depth_u16) *depth_u16 = s->in_depth.img();
+    if (rgb_u8x3) *rgb_u8x3 = s->in_rgb.img();
+    if (labels_u8) *labels_u8 = s->in_labels.img();
+    return SICP_OK;
+}
+
+// Records the frame sequence of both frame parities into graphs: the host
+// state (frame pointers, frame index) is set for each parity, the launches
+// are captured on a private stream, and the host state is restored.
+static int capture_graphs(sicp_system* s, int nd) {
+    if (!s->capture_stream)
+        SICP_HIP(hipStreamCreateWithFlags(&s->capture_stream, hipStreamNonBlocking));
+    for (hipGraphExec_t& g : s->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    const bool timing = s->timing;
+    FrameBuf* const next0 = s->next;
+    FrameBuf* const last0 = s->last;
+    const int index0 = s->frame_index;
+    s->timing = false;
+    const sicp_image lab = nd > 0 ? s->in_labels.img() : sicp_image{nullptr, 0, 0, 0};
+    int rc = SICP_OK;
+    for (int p = 0; p < 2 && rc == SICP_OK; ++p) {
+        s->next = p == 0 ? &s->fa : &s->fb;
+        s->last = p == 0 ? &s->fb : &s->fa;
+        s->frame_index = index0 > 0 ? index0 : 1;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(s->capture_stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "hipStreamBeginCapture");
+            break;
+        }
+        rc = grab_frame(s, s->in_depth.img(), s->in_rgb.img(), lab, nd, s->capture_stream);
+        e = hipStreamEndCapture(s->capture_stream, &g);
+        if (rc == SICP_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+        if (rc == SICP_OK) {
+            e = hipGraphInstantiate(&s->graph[p], g, nullptr, nullptr, 0);
+            if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+        }
+        if (g) (void)hipGraphDestroy(g);
+    }
+    s->next = next0;
+    s->last = last0;
+    s->frame_index = index0;
+    s->timing = timing;
+    s->graph_nd = rc == SICP_OK ? nd : -1;
+    return rc;
+}
+
+int sicp_system_grab_staged(sicp_system* s, int num_detection, int use_graph, void* stream) {
+    if (!s) return invalid("system_grab_staged: null system");
+    if (!s->in_depth.ptr) return invalid("system_grab_staged: call sicp_system_staging first");
+    const int nd = num_detection > 0 ? num_detection : 0;
+    const sicp_image lab = nd > 0 ? s->in_labels.img() : sicp_image{nullptr, 0, 0, 0};
+    // The first frame (tracking initialisation) and timed frames run eagerly.
+    if (!use_graph || s->frame_index == 0 || s->timing)
+        return sicp_system_grab_image(s, s->in_depth.img(), s->in_rgb.img(), lab, nd, stream);
+    if (!s->graph[0] || s->graph_nd != nd) {
+        const int rc = capture_graphs(s, nd);
+        if (rc) return rc;
+    }
+    const int p = (s->next == &s->fa) ? 0 : 1;
+    SICP_HIP(hipGraphLaunch(s->graph[p], (hipStream_t)stream));
+    std::swap(s->next, s->last);  // what the recorded frame's host code does
+    s->frame_index++;
     return SICP_OK;
 }
 

@@ -138,3 +138,37 @@ def test_system_lost_frame_keeps_model(dev):
         if i is None:
             assert rg["tracked"] == 0
         assert np.abs(rg["pose"] - ro["pose"]).max() < 1e-4
+
+
+@pytest.mark.parametrize("c", [CFG_SMALL, CFG_VGA])
+def test_graph_form_matches_eager(dev, c):
+    """GrabImage as one hipGraph launch per frame (staged inputs, two graphs
+    by frame parity) gives the same result as the eager launches, frame by
+    frame, with eager work and synchronisations between the replays; the
+    final maps are identical."""
+    import torch
+    from helpers import bits_equal, download, mismatch_report
+    cfg = sys_cfg(c)
+    ge, gg = B.System(cfg), B.System(cfg)
+    s = synth(c)
+    frames = 9 if c is CFG_SMALL else 5
+    for i in range(frames):
+        d, rgb, lab = s.render(i)
+        t = [to_dev(a, dev) for a in (d, rgb, lab)]
+        ge.grab(*t)
+        gg.grab_staged(*t, use_graph=True)
+        x = torch.randn(1 << 16, device=dev)  # unrelated eager work between replays
+        (x * 2).sum().item()
+        re, rg = ge.result(), gg.result()
+        assert re["tracked"] == rg["tracked"], i
+        assert np.array_equal(re["pose"], rg["pose"]), (i, np.abs(re["pose"] - rg["pose"]).max())
+        assert re["allocated_blocks"] == rg["allocated_blocks"], i
+        assert re["visible_blocks"] == rg["visible_blocks"], i
+    rows, cols = c["rows"], c["cols"]
+    for which in (0, 1):
+        a = download(ge.frame_image(which, 0), rows, cols, np.float32, 4)
+        b = download(gg.frame_image(which, 0), rows, cols, np.float32, 4)
+        assert bits_equal(a, b), (which, mismatch_report(a, b))
+    ma, mb = B.map_stats(ge.map_handle()), B.map_stats(gg.map_handle())
+    for k in ("allocated_blocks", "updated_total", "visible_total", "fusions"):
+        assert ma[k] == mb[k], k
