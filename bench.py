@@ -46,11 +46,13 @@ def log(*a):
 
 
 def make_frames(n, w):
+    """Renders n frames of the seeded sequence, several at a time (the
+    renderer is reentrant and releases the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
     from semantic_icpslam_amd import binding as B
     syn = B.Synth(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"], seed=w["seed"])
-    frames = []
-    for i in range(n):
-        frames.append(syn.render(i))
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        frames = list(ex.map(syn.render, range(n)))
     return syn, frames
 
 
@@ -187,6 +189,99 @@ def load_traffic():
         return {}
 
 
+# Eight independent sequences of uneven resolution and voxel size.
+BATCHED = (
+    [dict(cols=640, rows=480, f=525.0, cx=319.5, cy=239.5, voxel=0.005, seed=k) for k in (0, 1, 2)] +
+    [dict(cols=320, rows=240, f=262.5, cx=159.5, cy=119.5, voxel=0.008, seed=k) for k in (3, 4, 5)] +
+    [dict(cols=1280, rows=720, f=1050.0, cx=639.5, cy=359.5, voxel=0.004, seed=k) for k in (6, 7)])
+
+
+def run_batched(args):
+    """Eight independent sequences, each with its own system, every frame one
+    hipGraph launch; one round advances every sequence by a frame. Three
+    passes over the same frames on fresh systems: one stream per sequence
+    (the value), all on one stream (for comparison), and one stream per
+    sequence synchronised after every round (round latency)."""
+    import torch
+    from semantic_icpslam_amd import binding as B
+    nfr = args.warmup + args.steps
+    log(f"[bench] rendering {len(BATCHED)} sequences x {nfr} frames")
+    seqs = []
+    for w in BATCHED:
+        w = dict(w, trunc=3 * w["voxel"], buckets=1 << 20, blocks=300000)
+        _, frames = make_frames(nfr, w)
+        dev = torch.device("cuda:0")
+        seqs.append(dict(w=w, frames=[tuple(torch.from_numpy(a).to(dev) for a in f)
+                                      for f in frames]))
+    torch.cuda.synchronize()
+
+    def run(concurrent, sync_rounds):
+        """Fresh systems; warm-up rounds, then the timed rounds. Concurrent:
+        one stream per sequence; otherwise all on one stream."""
+        systems, streams = [], []
+        shared = torch.cuda.Stream()
+        for q in seqs:
+            w = q["w"]
+            systems.append(B.System(B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"],
+                                                    w["cy"], w["voxel"], w["trunc"], w["buckets"],
+                                                    w["blocks"])))
+            streams.append(torch.cuda.Stream() if concurrent else shared)
+
+        def round_(i):
+            for q, g, st in zip(seqs, systems, streams):
+                d, c, l = q["frames"][i]
+                g.grab_staged(d, c, l, True, st)
+
+        for i in range(args.warmup):
+            round_(i)
+        torch.cuda.synchronize()
+        lat = []
+        t0 = time.perf_counter()
+        for i in range(args.warmup, nfr):
+            t = time.perf_counter()
+            round_(i)
+            if sync_rounds:
+                torch.cuda.synchronize()
+                lat.append(1000.0 * (time.perf_counter() - t))
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        tracked = [g.result(st)["tracked"] for g, st in zip(systems, streams)]
+        for g in systems:
+            g.close()
+        return elapsed, lat, tracked
+
+    log("[bench] batched: concurrent streams")
+    thr_s, _, tracked = run(True, False)
+    log("[bench] batched: one stream")
+    ser_s, _, _ = run(False, False)
+    log("[bench] batched: concurrent, synchronised per round")
+    _, lat, _ = run(True, True)
+    frames_done = args.steps * len(seqs)
+    out = {
+        "metric": "frames/s, 8 independent sequences (3x640x480 @5 mm, 3x320x240 @8 mm, "
+                  "2x1280x720 @4 mm), one hipGraph per frame",
+        "value": round(frames_done / thr_s, 1),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * thr_s / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural rooms, seeds 0-7)",
+        "config": {"workload": "batched8", "sequences": len(seqs), "frames_per_sequence": nfr},
+        "one_stream_frames_per_s": round(frames_done / ser_s, 1),
+        "round_ms_p50": round(float(np.percentile(lat, 50)), 4),
+        "round_ms_p99": round(float(np.percentile(lat, 99)), 4),
+        "round_what": "one round = every sequence advances one frame (step); p50/p99 from a "
+                      "pass synchronised after every round",
+        "last_frame_tracked": tracked,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,12 +292,21 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB first (counter calibration)")
     ap.add_argument("--no-pose-check", action="store_true")
+    ap.add_argument("--workload", default="vga", choices=("vga", "batched8"),
+                    help="vga: the 640x480 / 5 mm sequence (default); batched8: eight "
+                         "independent sequences at once")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(independent sequences would shard one per GPU); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batched8":
+        if args.steps == 290:  # the default frame count is that of the 640x480 sequence
+            args.steps = 190
+        run_batched(args)
+        return
 
     import torch
     from semantic_icpslam_amd import binding as B
