@@ -254,6 +254,15 @@ int sicp_map_download(sicp_map* map, sicp_hash_entry* table, sicp_hash_entry* ex
 int sicp_map_download_visible(sicp_map* map, sicp_block* visible, uint32_t cap,
                               uint32_t* count);
 
+/* Map snapshot (System::WriteMapToDisk / ReadMapFromDisk,
+ * Core/System.cc:327-403): a versioned file holding the hash table, the used
+ * heap blocks (overflow entries, block records, both voxel planes), the
+ * visible list and the counters. Loading resets `map` first and requires
+ * the same voxel size, truncation, depth range, caps and bucket count, and
+ * a heap at least as large as the snapshot's used part. Synchronous. */
+int sicp_map_save(sicp_map* map, const char* path);
+int sicp_map_load(sicp_map* map, const char* path);
+
 /* ---- per-frame system (Core/System.cc:131-172) --------------------------- */
 typedef struct sicp_system_config {
     int32_t cols, rows;
@@ -264,6 +273,10 @@ typedef struct sicp_system_config {
     int32_t icp_min_count[3];/* MIN_ICP_COUNT = {2000,1000,100} (Tracking.h:112) */
     float icp_thresh;        /* THRESH_ICP_SE3 = 1e-4 (Tracking.h:117)         */
     sicp_map_config map;
+    int32_t icp_only;        /* 1: ComputeSE3(true, ...), point-to-plane only;
+                                0: ComputeSE3(false, ...), the photometric term
+                                of RGBStep added (TrackFrame, Tracking.cc:170) */
+    float rgb_weight;        /* w of the photometric term, 1e-4 (Tracking.cc:392) */
 } sicp_system_config;
 
 typedef struct sicp_system sicp_system;

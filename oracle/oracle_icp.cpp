@@ -101,6 +101,116 @@ template void icp_sums<double>(const double*, const double*, const double*, cons
                                int, const Pose<double>&, const Pose<double>&, Intr<double>,
                                double*, float*);
 
+// ---- photometric term: RGBReduction / RGBStep (Reduction.cu:387-642) ------
+// One pixel's correspondence of RGBReduction::FindCorresp (:413-463): inside
+// a 5-pixel border, a 4x4 window of non-zero intensity, a non-zero gradient,
+// a valid current vertex projected into the last frame (float argument,
+// +0.5 added in double, rounded down as __float2int_rd), a valid last vertex
+// within 0.1 of the current one (float4 distance, compared unmoved) and a
+// non-zero last intensity. Returns false or the last pixel and the
+// intensity difference.
+template <class R>
+static inline bool rgb_corresp(const uint8_t* nimg, const uint8_t* limg, const R* nvmap,
+                               const R* lvmap, const int16_t* dx, const int16_t* dy, int cols,
+                               int rows, int x, int y, const Pose<R>& curr, const Pose<R>& last,
+                               Intr<R> K, int& u, int& v, int& diff) {
+    if (!(x >= 5 && x < cols - 5 && y >= 5 && y < rows - 5)) return false;
+    for (int a = std::max(y - 2, 0); a < std::min(y + 2, rows); ++a)
+        for (int b = std::max(x - 2, 0); b < std::min(x + 2, cols); ++b)
+            if (!(nimg[(size_t)a * cols + b] > 0)) return false;
+    const int gx = dx[(size_t)y * cols + x], gy = dy[(size_t)y * cols + x];
+    if (!(std::sqrt((float)(gx * gx + gy * gy)) > 0)) return false;
+    const R* vc = nvmap + ((size_t)y * cols + x) * 4;
+    if (isnan_(vc[0])) return false;
+    V3<R> vcurr_g = add(mv(curr.R_, V3<R>{vc[0], vc[1], vc[2]}), curr.t);
+    V3<R> p = mv(last.Rinv, sub(vcurr_g, last.t));
+    R tu = (R)((double)(K.fx * p.x / p.z + K.cx) + 0.5);
+    R tv = (R)((double)(K.fy * p.y / p.z + K.cy) + 0.5);
+    u = f2i(std::floor(tu));
+    v = f2i(std::floor(tv));
+    if (!(u >= 0 && v >= 0 && u < cols && v < rows)) return false;
+    const R* vl = lvmap + ((size_t)v * cols + u) * 4;
+    if (isnan_(vl[0])) return false;
+    R d0 = vl[0] - vc[0], d1 = vl[1] - vc[1], d2 = vl[2] - vc[2], d3 = vl[3] - vc[3];
+    R dist = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3);
+    if (!(dist < (R)0.1 && limg[(size_t)v * cols + u] != 0)) return false;
+    diff = (int)nimg[(size_t)y * cols + x] - (int)limg[(size_t)v * cols + u];
+    return true;
+}
+
+// RGBStep: the correspondence pass with its int sums (count, sum of squared
+// differences; 32-bit, wrapping as the reference's int accumulation), sigma
+// = sqrt(sum / max(count, 1)) in float, then RGBReduction::GetRow
+// (:482-521) with the robust weight 1 / (sigma + |diff|) and the 29 sums.
+template <class R>
+void rgb_sums(const uint8_t* nimg, const uint8_t* limg, const R* nvmap, const R* lvmap,
+              const int16_t* dx, const int16_t* dy, int cols, int rows, const Pose<R>& curr,
+              const Pose<R>& last, Intr<R> K, double* sums, int* res2, float* sigma_out) {
+    const int N = cols * rows;
+    uint32_t cnt = 0, sq = 0;
+#pragma omp parallel for schedule(static) reduction(+ : cnt, sq)
+    for (int i = 0; i < N; ++i) {
+        int y = i / cols, x = i - y * cols, u, v, diff;
+        if (rgb_corresp(nimg, limg, nvmap, lvmap, dx, dy, cols, rows, x, y, curr, last, K, u, v,
+                        diff)) {
+            cnt += 1u;
+            sq += (uint32_t)(diff * diff);
+        }
+    }
+    res2[0] = (int)cnt;
+    res2[1] = (int)sq;
+    const float sigma = std::sqrt((float)res2[1] / (float)(res2[0] == 0 ? 1 : res2[0]));
+    if (sigma_out) *sigma_out = sigma;
+    for (int j = 0; j < 29; ++j) sums[j] = 0.0;
+#pragma omp parallel
+    {
+        double loc[29] = {0};
+#pragma omp for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            int y = i / cols, x = i - y * cols, u, v, diff;
+            R row[7] = {0, 0, 0, 0, 0, 0, 0};
+            const bool ok = rgb_corresp(nimg, limg, nvmap, lvmap, dx, dy, cols, rows, x, y, curr,
+                                        last, K, u, v, diff);
+            if (ok) {
+                R w = (R)sigma + (R)std::abs(diff);
+                w = (double)w < 1e-3 ? (R)1 : (R)1 / w;
+                if ((double)sigma < 1e-6) w = (R)1;
+                const R* vl = lvmap + ((size_t)v * cols + u) * 4;
+                V3<R> q = add(mv(last.R_, V3<R>{vl[0], vl[1], vl[2]}), last.t);
+                V3<R> point = mv(curr.Rinv, sub(q, curr.t));
+                R gx = (R)((double)dx[(size_t)y * cols + x] / 9.0);
+                R gy = (R)((double)dy[(size_t)y * cols + x] / 9.0);
+                R invz = (R)1 / point.z;
+                V3<R> left;
+                left.x = gx * K.fx * invz;
+                left.y = gy * K.fy * invz;
+                left.z = -(left.x * point.x + left.y * point.y) * invz;
+                V3<R> c = cross(point, left);
+                row[0] = w * left.x;
+                row[1] = w * left.y;
+                row[2] = w * left.z;
+                row[3] = w * c.x;
+                row[4] = w * c.y;
+                row[5] = w * c.z;
+                row[6] = -w * (R)diff;
+            }
+            int k = 0;
+            for (int a = 0; a < 7; ++a)
+                for (int b = a; b < 7; ++b) loc[k++] += (double)(row[a] * row[b]);
+            loc[28] += ok ? 1.0 : 0.0;
+        }
+#pragma omp critical
+        for (int j = 0; j < 29; ++j) sums[j] += loc[j];
+    }
+}
+
+template void rgb_sums<float>(const uint8_t*, const uint8_t*, const float*, const float*,
+                              const int16_t*, const int16_t*, int, int, const Pose<float>&,
+                              const Pose<float>&, Intr<float>, double*, int*, float*);
+template void rgb_sums<double>(const uint8_t*, const uint8_t*, const double*, const double*,
+                               const int16_t*, const int16_t*, int, int, const Pose<double>&,
+                               const Pose<double>&, Intr<double>, double*, int*, float*);
+
 // CreateMatrix<6,7> (Reduction.cu:6-16): upper-triangle sums -> mirrored A, b.
 void create_matrix(const double* s, double* A, double* b) {
     int shift = 0;

@@ -44,6 +44,10 @@ template <class R> struct Intr { R fx, fy, cx, cy; };
 template <class R>
 void icp_sums(const R* vcurr, const R* vlast, const R* ncurr, const R* nlast, int cols, int rows,
               const Pose<R>& curr, const Pose<R>& last, Intr<R> K, double* sums, float* rows_out);
+template <class R>
+void rgb_sums(const uint8_t* nimg, const uint8_t* limg, const R* nvmap, const R* lvmap,
+              const int16_t* dx, const int16_t* dy, int cols, int rows, const Pose<R>& curr,
+              const Pose<R>& last, Intr<R> K, double* sums, int* res2, float* sigma_out);
 void create_matrix(const double* sums29, double* A, double* b);
 bool cholesky_solve6(const double* A, const double* b, double* x);
 void se3_exp(const double* xi, double* T);

@@ -1,7 +1,7 @@
 // oracle_system.cpp — CPU restatement of the per-frame sequence of
 // System::GrabImage (Core/System.cc:131-172): Frame::ResizeImages /
 // FillImages (Core/Frame.cc:68-98), Tracker::Track / TrackFrame /
-// ComputeSE3 with icpOnly = true (Tracking/Tracking.cc:65-119,157-173,
+// ComputeSE3 (Tracking/Tracking.cc:65-119,157-173,
 // 369-517), Mapping::FuseColor and Mapping::RayTrace (Mapping/Mapping.cc:
 // 76-97,142-161). TEST INFRASTRUCTURE ONLY (see oracle.h); also the CPU
 // baseline that bench.py times.
@@ -95,7 +95,9 @@ template <> void round_sums<float>(double* s) {
 }
 template <> void round_sums<double>(double*) {}
 
-// Tracker::ComputeSE3 (Tracking.cc:369-517), icpOnly branch (:488-490).
+// Tracker::ComputeSE3 (Tracking.cc:369-517): the point-to-plane term alone
+// (icpOnly, :488-490) or with the weighted photometric term (:400-433,
+// :484-487), as the configuration says.
 template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res) {
     OFrame<R>& nf = *s.next;
     OFrame<R>& lf = *s.last;
@@ -104,12 +106,33 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
     std::memcpy(pose0, nf.pose, sizeof(pose0));
     for (int i = 0; i < 16; ++i) delta[i] = (i % 5 == 0) ? 1.0 : 0.0;
     std::memcpy(candidate, nf.pose, sizeof(candidate));
-    R icp_error = 0, last_error = (R)100.f, curr_error = 0, candidate_icp = 0;
+    R icp_error = 0, last_error = (R)100.f, curr_error = 0, candidate_icp = 0, rgb_error = 0;
     int icp_count = 0, break_count = 0, applied = 0;
+    const bool icp_only = s.cfg.icp_only != 0;
+    const float w = s.cfg.rgb_weight;
     for (int i = 2; i >= 0; --i) {
         for (int j = 0; j < s.cfg.icp_iters[i]; ++j) {
             Pose<R> pc = pose_from_d<R>(nf.pose);
             Pose<R> pl = pose_from_d<R>(lf.pose);
+            double A_rgb[36], b_rgb[6];
+            if (!icp_only) {
+                // RGBStep and its failure test (Tracking.cc:402-433)
+                double rs[29];
+                int res2[2];
+                rgb_sums<R>(nf.image[i].data(), lf.image[i].data(), nf.vmap[i].data(),
+                            lf.vmap[i].data(), nf.dx[i].data(), nf.dy[i].data(), nf.cols[i],
+                            nf.rows[i], pc, pl, s.K[i], rs, res2, nullptr);
+                round_sums<R>(rs);
+                create_matrix(rs, A_rgb, b_rgb);
+                R q0 = (R)rs[27], q1 = (R)rs[28];
+                rgb_error = std::sqrt(q0) / q1;
+                const int rgb_count = (int)q1;
+                if (isnan_(rgb_error) || rgb_count < s.cfg.icp_min_count[i]) {
+                    std::memcpy(nf.pose, last_pose, sizeof(last_pose));
+                    res->iterations = applied;
+                    return false;
+                }
+            }
             double sums[29], A[36], b[6];
             icp_sums<R>(nf.vmap[i].data(), lf.vmap[i].data(), nf.nmap[i].data(),
                         lf.nmap[i].data(), nf.cols[i], nf.rows[i], pc, pl, s.K[i], sums, nullptr);
@@ -125,7 +148,7 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
                 res->iterations = applied;
                 return false;
             }
-            curr_error = icp_error;
+            curr_error = rgb_error * (R)w + icp_error;  // Tracking.cc:458
             if (last_error < curr_error) {
                 break_count++;
                 if (break_count == 1) std::memcpy(candidate, nf.pose, sizeof(candidate));
@@ -139,6 +162,11 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
                 break;
             }
             last_error = curr_error;
+            if (!icp_only)  // matA = matA_rgb * w + matA_icp (Tracking.cc:484-487)
+                for (int k = 0; k < 36; ++k) {
+                    A[k] = A_rgb[k] * (double)w + A[k];
+                    if (k < 6) b[k] = b_rgb[k] * (double)w + b[k];
+                }
             double x[6], dT[16], inv[16];
             cholesky_solve6(A, b, x);
             se3_exp(x, dT);

@@ -59,6 +59,7 @@ class SystemConfig(C.Structure):
         ("depth_scale", C.c_float), ("depth_cutoff", C.c_float),
         ("icp_iters", C.c_int32 * 3), ("icp_min_count", C.c_int32 * 3),
         ("icp_thresh", C.c_float), ("map", MapConfig),
+        ("icp_only", C.c_int32), ("rgb_weight", C.c_float),
     ]
 
 
@@ -127,6 +128,8 @@ _SIGS = {
                                    C.c_float, C.c_float, C.c_float, C.c_float, _P]),
     "sicp_map_visible_list": (C.c_int, [_P, C.POINTER(_P), C.POINTER(C.c_uint32)]),
     "sicp_map_get_stats": (C.c_int, [_P, C.POINTER(MapStats), _P]),
+    "sicp_map_save": (C.c_int, [_P, C.c_char_p]),
+    "sicp_map_load": (C.c_int, [_P, C.c_char_p]),
     "sicp_map_download": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_uint32]),
     "sicp_map_download_visible": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "sicp_system_config_default": (None, [C.POINTER(SystemConfig), C.c_int, C.c_int, C.c_float,

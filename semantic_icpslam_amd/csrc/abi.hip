@@ -1,10 +1,13 @@
 // abi.hip — the extern "C" entry points of include/sicp.h: argument
 // validation, map ownership and the reference-shaped operations.
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "se3_math.hpp"
+
 #include "sicp_internal.hpp"
 
 namespace sicp {
@@ -485,6 +488,111 @@ int sicp_map_download_visible(sicp_map* map, sicp_block* visible, uint32_t cap,
     uint32_t n = c.visible < cap ? c.visible : cap;
     if (visible && n)
         SICP_HIP(hipMemcpy(visible, map->visible, n * sizeof(sicp_block), hipMemcpyDeviceToHost));
+    return SICP_OK;
+}
+
+// ---- snapshot -------------------------------------------------------------
+// System::WriteMapToDisk / ReadMapFromDisk (Core/System.cc:327-403) dump
+// every map buffer raw. The snapshot here is versioned and holds only what
+// the map uses: the table, the first heap_used blocks' overflow entries,
+// records and voxels, the visible list and the counters.
+}  // extern "C"
+
+namespace {
+struct SnapshotHeader {
+    char magic[8];
+    uint32_t version, header_bytes;
+    sicp_map_config cfg;
+    MapCounters ctr;
+    uint64_t table_entries, blocks, visible;
+};
+constexpr char kSnapMagic[8] = {'S', 'I', 'C', 'P', 'M', 'A', 'P', '\0'};
+constexpr uint32_t kSnapVersion = 1;
+
+struct File {
+    FILE* f;
+    ~File() {
+        if (f) fclose(f);
+    }
+};
+
+template <class T> int put(FILE* f, const T* dev, size_t n, const char* what) {
+    if (!n) return SICP_OK;
+    std::vector<T> h(n);
+    SICP_HIP(hipMemcpy(h.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost));
+    if (fwrite(h.data(), sizeof(T), n, f) != n) return invalid(what);
+    return SICP_OK;
+}
+template <class T> int get(FILE* f, T* dev, size_t n, const char* what) {
+    if (!n) return SICP_OK;
+    std::vector<T> h(n);
+    if (fread(h.data(), sizeof(T), n, f) != n) return invalid(what);
+    SICP_HIP(hipMemcpy(dev, h.data(), n * sizeof(T), hipMemcpyHostToDevice));
+    return SICP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int sicp_map_save(sicp_map* map, const char* path) {
+    if (!map || !path) return invalid("map_save: null argument");
+    SICP_HIP(hipDeviceSynchronize());
+    SnapshotHeader h{};
+    std::memcpy(h.magic, kSnapMagic, sizeof(h.magic));
+    h.version = kSnapVersion;
+    h.header_bytes = sizeof(SnapshotHeader);
+    h.cfg = map->cfg;
+    SICP_HIP(hipMemcpy(&h.ctr, map->ctr, sizeof(h.ctr), hipMemcpyDeviceToHost));
+    h.table_entries = 2 * (uint64_t)map->cfg.num_buckets;
+    h.blocks = h.ctr.heap_used;
+    h.visible = h.ctr.visible;
+    File file{fopen(path, "wb")};
+    if (!file.f) return invalid("map_save: cannot open the file for writing");
+    if (fwrite(&h, sizeof(h), 1, file.f) != 1) return invalid("map_save: write failed");
+    int rc;
+    const size_t nb = h.blocks;
+    if ((rc = put(file.f, map->dev.table, h.table_entries, "map_save: write failed")) ||
+        (rc = put(file.f, map->dev.excess, nb, "map_save: write failed")) ||
+        (rc = put(file.f, map->dev.blocks, nb, "map_save: write failed")) ||
+        (rc = put(file.f, map->dev.geom, nb * kBlockVoxels, "map_save: write failed")) ||
+        (rc = put(file.f, map->dev.sem, nb * kBlockVoxels, "map_save: write failed")) ||
+        (rc = put(file.f, map->visible, (size_t)h.visible, "map_save: write failed")))
+        return rc;
+    return SICP_OK;
+}
+
+int sicp_map_load(sicp_map* map, const char* path) {
+    if (!map || !path) return invalid("map_load: null argument");
+    File file{fopen(path, "rb")};
+    if (!file.f) return invalid("map_load: cannot open the file");
+    SnapshotHeader h;
+    if (fread(&h, sizeof(h), 1, file.f) != 1) return invalid("map_load: truncated header");
+    if (std::memcmp(h.magic, kSnapMagic, sizeof(h.magic)) != 0)
+        return invalid("map_load: not a map snapshot");
+    if (h.version != kSnapVersion || h.header_bytes != sizeof(SnapshotHeader))
+        return invalid("map_load: unsupported snapshot version");
+    const sicp_map_config& a = map->cfg;
+    const sicp_map_config& b = h.cfg;
+    if (a.num_buckets != b.num_buckets || a.voxel_size != b.voxel_size ||
+        a.truncation != b.truncation || a.depth_min != b.depth_min ||
+        a.depth_max != b.depth_max || a.max_weight != b.max_weight ||
+        a.max_label_count != b.max_label_count)
+        return invalid("map_load: the snapshot was written by a map of another configuration");
+    if (h.blocks > (uint64_t)a.num_blocks || h.visible > h.blocks ||
+        h.table_entries != 2 * (uint64_t)a.num_buckets)
+        return invalid("map_load: the snapshot does not fit this map");
+    int rc = launch_map_reset(map->dev, map->ctr, map->alloc, nullptr);
+    if (rc) return rc;
+    SICP_HIP(hipDeviceSynchronize());
+    const size_t nb = h.blocks;
+    if ((rc = get(file.f, map->dev.table, h.table_entries, "map_load: truncated file")) ||
+        (rc = get(file.f, map->dev.excess, nb, "map_load: truncated file")) ||
+        (rc = get(file.f, map->dev.blocks, nb, "map_load: truncated file")) ||
+        (rc = get(file.f, map->dev.geom, nb * kBlockVoxels, "map_load: truncated file")) ||
+        (rc = get(file.f, map->dev.sem, nb * kBlockVoxels, "map_load: truncated file")) ||
+        (rc = get(file.f, map->visible, (size_t)h.visible, "map_load: truncated file")))
+        return rc;
+    SICP_HIP(hipMemcpy(map->ctr, &h.ctr, sizeof(h.ctr), hipMemcpyHostToDevice));
     return SICP_OK;
 }
 

@@ -183,6 +183,8 @@ void sicp_system_config_default(sicp_system_config* cfg, int cols, int rows, flo
     cfg->icp_iters[0] = 10;     // ITERATIONS_SE3 (Tracking.h:110)
     cfg->icp_iters[1] = 5;
     cfg->icp_iters[2] = 3;
+    cfg->icp_only = 1;             // the survey's tracking decision: ICP only
+    cfg->rgb_weight = 1e-4f;       // Tracking.cc:392
     cfg->icp_min_count[0] = 2000;  // MIN_ICP_COUNT (Tracking.h:112)
     cfg->icp_min_count[1] = 1000;
     cfg->icp_min_count[2] = 100;

@@ -165,3 +165,38 @@ def test_prepass_and_raycast_bit_exact(dev, c):
     assert hit.mean() > 0.3
     if c is CFG_VGA:  # grazing rays reach the second (cooperative) raycast pass
         assert gm.stats()["ray_handover"] > 0
+
+
+def test_map_snapshot_round_trip(dev, tmp_path):
+    """sicp_map_save / sicp_map_load (System::WriteMapToDisk / ReadMapFromDisk):
+    a map saved after frames 0-4 and loaded into a fresh map continues with
+    frames 6-8 exactly as the uninterrupted map does; a snapshot of another
+    configuration is refused."""
+    c = CFG_SMALL
+    cfg = map_cfg(c)
+    ga, om = B.Map(cfg), O.OracleMap(cfg)
+    s, _ = _fuse_both(dev, c, [0, 2, 4], ga, om)
+    path = str(tmp_path / "map.bin").encode()
+    B.check(B.lib().sicp_map_save(ga.h, path))
+    gb = B.Map(cfg)
+    B.check(B.lib().sicp_map_load(gb.h, path))
+    assert gb.stats() == ga.stats()
+    for m in (ga, gb):
+        for i in (6, 8):
+            raw, rgb, n, lab = _inputs(c, s, i)
+            pose = B.pose_f(s.pose(i))
+            tr = [to_dev(a, dev) for a in (raw, rgb, n, lab)]
+            B.check(B.lib().sicp_fuse_map_color(m.h, *[B.image(t) for t in tr], 1,
+                                                B.C.byref(pose), None, K(c), None,
+                                                B.stream_ptr()))
+            torch.cuda.synchronize()
+    a, b = ga.download(n_blocks=0), gb.download(n_blocks=0)
+    na = ga.stats()["allocated_blocks"]
+    assert gb.stats()["allocated_blocks"] == na
+    for k in ("table", "excess", "blocks"):
+        assert np.array_equal(a[k][: len(a[k]) if k == "table" else na],
+                              b[k][: len(b[k]) if k == "table" else na]), k
+    assert bits_equal(a["geom"]["sdf"][: na * 512], b["geom"]["sdf"][: na * 512])
+    assert np.array_equal(a["sem"][: na * 512], b["sem"][: na * 512])
+    other = B.Map(map_cfg(dict(c, voxel=0.02)))
+    assert B.lib().sicp_map_load(other.h, path) == B.SICP_ERR_INVALID_ARGUMENT
