@@ -185,6 +185,25 @@ int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_ima
                   double* A_host, double* b_host, float* residual_host, void* stream);
 size_t sicp_icp_partials_size(void);
 
+/* RGBStep (Tracking/Reduction.h:81-90, Reduction.cu:575-642): the
+ * photometric term of one Gauss-Newton iteration. A correspondence pass
+ * (FindCorresp) gives the count and the int sum of squared intensity
+ * differences (`corresp_host`, the reference's outRes) and sigma =
+ * sqrt(sum / max(count, 1)) (`sigma_host`); the row pass (GetRow) then gives
+ * the 29 robust-weighted sums into `sums_dev` (29 floats, device), as for
+ * sicp_icp_step. Images: intensities (u8) and derivatives (int16) of the next
+ * frame, the last frame's intensity, both frames' vertex maps; one shape.
+ * `scratch_dev` holds `scratch_bytes` >= sicp_rgb_scratch_size(cols, rows)
+ * bytes, 8-byte aligned. Host outputs may be NULL; any non-NULL one
+ * synchronises `stream`. */
+int sicp_rgb_step(sicp_image next_image, sicp_image last_image, sicp_image next_vmap,
+                  sicp_image last_vmap, sicp_image didx, sicp_image didy,
+                  const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
+                  void* scratch_dev, size_t scratch_bytes, float* sums_dev, float* host_sums,
+                  int32_t* corresp_host, float* sigma_host, double* A_host, double* b_host,
+                  float* residual_host, void* stream);
+size_t sicp_rgb_scratch_size(int32_t cols, int32_t rows);
+
 /* FP64 6x6 solve + SE(3) update of one Gauss-Newton iteration, on the host
  * arrays of Tracking/Tracking.cc:493-498: x = A^-1 b (Cholesky),
  * delta = exp(x) * delta, next = last * delta^-1. Poses are 4x4 row-major. */
@@ -290,6 +309,9 @@ typedef struct sicp_frame_result {
     uint32_t visible_blocks;
     uint32_t allocated_blocks;
     double pose[16];        /* camera -> world, row-major                      */
+    float rgb_error;        /* photometric term: sqrt(sum r^2) / count of the
+                               last iteration (0 when the term is off)         */
+    int32_t rgb_count;      /* its correspondences                             */
 } sicp_frame_result;
 
 void sicp_system_config_default(sicp_system_config* cfg, int cols, int rows, float fx,

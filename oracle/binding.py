@@ -38,6 +38,8 @@ _SIGS = {
     "oracle_icp_sums": (None, [_P, _P, _P, _P, C.c_int, C.c_int, C.POINTER(PoseF),
                                C.POINTER(PoseF), Intrinsics, _P, _P]),
     "oracle_icp_sums_fp64": (None, [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
+    "oracle_rgb_sums": (None, [_P, _P, _P, _P, _P, _P, C.c_int, C.c_int, C.POINTER(PoseF),
+                               C.POINTER(PoseF), Intrinsics, _P, _P, _P]),
     "oracle_create_matrix": (None, [_P, _P, _P]),
     "oracle_cholesky_solve6": (C.c_int, [_P, _P, _P]),
     "oracle_se3_exp": (None, [_P, _P]),
@@ -189,6 +191,24 @@ def icp_sums_fp64(vcurr, vlast, ncurr, nlast, T_curr, T_last, K4):
     return sums
 
 
+def rgb_sums(nimg, limg, nvmap, lvmap, dx, dy, pose_curr, pose_last, K):
+    """RGBStep: (29 FP64 sums, (count, sum of squared differences), sigma)."""
+    ni, li = _c(nimg, np.uint8), _c(limg, np.uint8)
+    nv, lv = _c(nvmap, np.float32), _c(lvmap, np.float32)
+    gx, gy = _c(dx, np.int16), _c(dy, np.int16)
+    rows, cols = ni.shape[:2]
+    for a in (li, gx, gy):
+        assert a.shape[:2] == (rows, cols)
+    assert nv.shape == lv.shape == (rows, cols, 4)
+    sums = np.zeros(29, np.float64)
+    corr = np.zeros(2, np.int32)
+    sig = np.zeros(1, np.float32)
+    lib().oracle_rgb_sums(_p(ni), _p(li), _p(nv), _p(lv), _p(gx), _p(gy), cols, rows,
+                          C.byref(pose_curr), C.byref(pose_last), K, _p(sums), _p(corr),
+                          _p(sig))
+    return sums, (int(corr[0]), int(corr[1])), float(sig[0])
+
+
 def create_matrix(sums29):
     s = _c(sums29, np.float32)
     A = np.zeros(36, np.float64)
@@ -331,7 +351,8 @@ class OracleSystem:
         return dict(frame_index=res.frame_index, tracked=res.tracked, icp_count=res.icp_count,
                     iterations=res.iterations, icp_error=res.icp_error,
                     visible_blocks=res.visible_blocks, allocated_blocks=res.allocated_blocks,
-                    pose=np.array(list(res.pose)).reshape(4, 4))
+                    pose=np.array(list(res.pose)).reshape(4, 4),
+                    rgb_error=res.rgb_error, rgb_count=res.rgb_count)
 
     def work(self):
         v = np.zeros(8, np.uint64)

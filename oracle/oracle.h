@@ -53,6 +53,14 @@ void oracle_icp_sums(const float* vcurr, const float* vlast, const float* ncurr,
 void oracle_icp_sums_fp64(const double* vcurr, const double* vlast, const double* ncurr,
                           const double* nlast, int cols, int rows, const double* curr_pose,
                           const double* last_pose, const double* K4, double* sums);
+/* RGBStep (Tracking/Reduction.cu:387-642): images u8, vertex maps float4,
+ * derivatives int16, one shape. The 29 photometric sums (FP32 rows, FP64
+ * accumulation) into sums_fp64; corresp = {count, int sum of squared
+ * differences}; sigma as the reference forms it. */
+void oracle_rgb_sums(const uint8_t* nimg, const uint8_t* limg, const float* nvmap,
+                     const float* lvmap, const int16_t* dx, const int16_t* dy, int cols, int rows,
+                     const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
+                     double* sums_fp64, int32_t* corresp, float* sigma);
 void oracle_create_matrix(const float* sums29, double* A, double* b);
 int oracle_cholesky_solve6(const double* A, const double* b, double* x);
 void oracle_se3_exp(const double* xi, double* T);

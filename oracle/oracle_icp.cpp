@@ -377,6 +377,21 @@ void oracle_icp_sums_fp64(const double* vcurr, const double* vlast, const double
                      Intr<double>{K4[0], K4[1], K4[2], K4[3]}, sums, nullptr);
 }
 
+void oracle_rgb_sums(const uint8_t* nimg, const uint8_t* limg, const float* nvmap,
+                     const float* lvmap, const int16_t* dx, const int16_t* dy, int cols, int rows,
+                     const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
+                     double* sums_fp64, int32_t* corresp, float* sigma) {
+    Pose<float> pc = pose_from_f<float>(curr->R, curr->Rinv, curr->t);
+    Pose<float> pl = pose_from_f<float>(last->R, last->Rinv, last->t);
+    int res2[2];
+    rgb_sums<float>(nimg, limg, nvmap, lvmap, dx, dy, cols, rows, pc, pl,
+                    Intr<float>{K.fx, K.fy, K.cx, K.cy}, sums_fp64, res2, sigma);
+    if (corresp) {
+        corresp[0] = res2[0];
+        corresp[1] = res2[1];
+    }
+}
+
 void oracle_create_matrix(const float* sums29, double* A, double* b) {
     double s[29];
     for (int i = 0; i < 29; ++i) s[i] = (double)sums29[i];

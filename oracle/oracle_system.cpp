@@ -127,8 +127,11 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
                 R q0 = (R)rs[27], q1 = (R)rs[28];
                 rgb_error = std::sqrt(q0) / q1;
                 const int rgb_count = (int)q1;
+                res->rgb_error = (float)rgb_error;
+                res->rgb_count = rgb_count;
                 if (isnan_(rgb_error) || rgb_count < s.cfg.icp_min_count[i]) {
                     std::memcpy(nf.pose, last_pose, sizeof(last_pose));
+                    res->icp_error = (float)icp_error;  // the previous iteration's
                     res->iterations = applied;
                     return false;
                 }

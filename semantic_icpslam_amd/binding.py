@@ -68,6 +68,7 @@ class FrameResult(C.Structure):
         ("frame_index", C.c_int32), ("tracked", C.c_int32), ("icp_count", C.c_int32),
         ("iterations", C.c_int32), ("icp_error", C.c_float), ("visible_blocks", C.c_uint32),
         ("allocated_blocks", C.c_uint32), ("pose", C.c_double * 16),
+        ("rgb_error", C.c_float), ("rgb_count", C.c_int32),
     ]
 
 
@@ -106,6 +107,10 @@ _SIGS = {
     "sicp_resize_map": (C.c_int, [Image, Image, Image, Image, _P]),
     "sicp_icp_step": (C.c_int, [Image, Image, Image, Image, C.POINTER(PoseF), C.POINTER(PoseF),
                                 Intrinsics, _P, _P, _P, _P, _P, _P, _P]),
+    "sicp_rgb_step": (C.c_int, [Image, Image, Image, Image, Image, Image, C.POINTER(PoseF),
+                                C.POINTER(PoseF), Intrinsics, _P, C.c_size_t, _P, _P, _P, _P,
+                                _P, _P, _P, _P]),
+    "sicp_rgb_scratch_size": (C.c_size_t, [C.c_int32, C.c_int32]),
     "sicp_icp_partials_size": (C.c_size_t, []),
     "sicp_se3_update_host": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "sicp_map_config_default": (None, [C.POINTER(MapConfig), C.c_float, C.c_float, C.c_int32,
@@ -408,7 +413,8 @@ class System:
         return dict(frame_index=r.frame_index, tracked=r.tracked, icp_count=r.icp_count,
                     iterations=r.iterations, icp_error=r.icp_error,
                     visible_blocks=r.visible_blocks, allocated_blocks=r.allocated_blocks,
-                    pose=np.array(list(r.pose)).reshape(4, 4))
+                    pose=np.array(list(r.pose)).reshape(4, 4),
+                    rgb_error=r.rgb_error, rgb_count=r.rgb_count)
 
     def map_handle(self):
         return lib().sicp_system_map(self.h)

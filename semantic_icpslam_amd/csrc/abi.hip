@@ -262,6 +262,74 @@ int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_ima
     return SICP_OK;
 }
 
+// The photometric scratch: rows, records, counts, totals, keys.
+static size_t rgb_scratch_bytes(size_t n) {
+    const size_t pr = (size_t)kIcpSums * kIcpMaxBlocks * sizeof(double);
+    const size_t ip = 2 * kIcpMaxBlocks * sizeof(uint32_t);
+    return pr + n * sizeof(int2) + ip + 16 + n * sizeof(int16_t);
+}
+
+size_t sicp_rgb_scratch_size(int32_t cols, int32_t rows) {
+    if (cols <= 0 || rows <= 0) return 0;
+    return rgb_scratch_bytes((size_t)cols * rows);
+}
+
+int sicp_rgb_step(sicp_image next_image, sicp_image last_image, sicp_image next_vmap,
+                  sicp_image last_vmap, sicp_image didx, sicp_image didy,
+                  const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
+                  void* scratch_dev, size_t scratch_bytes, float* sums_dev, float* host_sums,
+                  int32_t* corresp_host, float* sigma_host, double* A_host, double* b_host,
+                  float* residual_host, void* stream) {
+    const int c = next_image.cols, r = next_image.rows;
+    if (!image_ok(next_image, 1, c, r) || !image_ok(last_image, 1, c, r) ||
+        !image_ok(next_vmap, 16, c, r) || !image_ok(last_vmap, 16, c, r) ||
+        !image_ok(didx, 2, c, r) || !image_ok(didy, 2, c, r))
+        return invalid("rgb_step: the six images must share one shape");
+    if (!curr || !last || !scratch_dev || !sums_dev) return invalid("rgb_step: null argument");
+    const size_t n = (size_t)c * r;
+    if (scratch_bytes < rgb_scratch_bytes(n))
+        return invalid("rgb_step: scratch smaller than sicp_rgb_scratch_size(cols, rows)");
+    if (((uintptr_t)scratch_dev & 7) != 0) return invalid("rgb_step: scratch not 8-byte aligned");
+    char* b = (char*)scratch_dev;
+    const size_t pr = (size_t)kIcpSums * kIcpMaxBlocks * sizeof(double);
+    const size_t ip = 2 * kIcpMaxBlocks * sizeof(uint32_t);
+    RgbScratch rs;
+    rs.partials = (double*)b;
+    rs.rec = (int2*)(b + pr);
+    rs.ipart = (uint32_t*)(b + pr + n * sizeof(int2));
+    rs.totals = (int32_t*)(b + pr + n * sizeof(int2) + ip);
+    int16_t* key = (int16_t*)(b + pr + n * sizeof(int2) + ip + 16);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = launch_rgb_sums(view<uint8_t>(next_image), view<uint8_t>(last_image),
+                             view<float4>(next_vmap), view<float4>(last_vmap),
+                             view<int16_t>(didx), view<int16_t>(didy),
+                             make_pose_arg(curr, nullptr), make_pose_arg(last, nullptr), K, key,
+                             rs, sums_dev, s);
+    if (rc) return rc;
+    if (host_sums || corresp_host || sigma_host || A_host || b_host || residual_host) {
+        float h[kIcpSums];
+        int32_t t[3];
+        SICP_HIP(hipMemcpyAsync(h, sums_dev, sizeof(h), hipMemcpyDeviceToHost, s));
+        SICP_HIP(hipMemcpyAsync(t, rs.totals, sizeof(t), hipMemcpyDeviceToHost, s));
+        SICP_HIP(hipStreamSynchronize(s));
+        if (host_sums) std::memcpy(host_sums, h, sizeof(h));
+        if (corresp_host) {
+            corresp_host[0] = t[0];
+            corresp_host[1] = t[1];
+        }
+        if (sigma_host) std::memcpy(sigma_host, &t[2], sizeof(float));
+        double A[36], bv[6];
+        create_matrix(h, A, bv);
+        if (A_host) std::memcpy(A_host, A, sizeof(A));
+        if (b_host) std::memcpy(b_host, bv, sizeof(bv));
+        if (residual_host) {
+            residual_host[0] = h[27];
+            residual_host[1] = h[28];
+        }
+    }
+    return SICP_OK;
+}
+
 int sicp_se3_update_host(const double* A, const double* b, double* delta,
                          const double* last_pose, double* next_pose, double* x_out) {
     if (!A || !b || !delta || !last_pose || !next_pose) return invalid("se3_update_host: null argument");

@@ -31,7 +31,7 @@ struct IcpGeom {
 // reference's FP64 promotions (invz = 1.0 / z in double stored to float,
 // +0.5 added in double before the truncating cast; Q9).
 struct IcpProj {
-    v3 vcurr_g;
+    v3 vcurr_g, vcurr_p;  // current vertex in world and in the last camera
     int u, v;
     bool ok;
 };
@@ -40,7 +40,8 @@ SICP_DEV IcpProj icp_project(float4 vc4, const PoseF& cur, const PoseF& last, co
     v3 vcurr_c = xyz(vc4);
     p.ok = !(isnan_(vcurr_c.x) || (double)vcurr_c.z < 1e-3);
     p.vcurr_g = add(mv(cur.R, vcurr_c), cur.t);
-    v3 vcurr_p = mv(last.Rinv, sub(p.vcurr_g, last.t));
+    const v3 vcurr_p = mv(last.Rinv, sub(p.vcurr_g, last.t));
+    p.vcurr_p = vcurr_p;
     float invz = (float)(1.0 / (double)vcurr_p.z);
     p.u = (int)((double)(vcurr_p.x * invz * g.fx + g.cx) + 0.5);
     p.v = (int)((double)(vcurr_p.y * invz * g.fy + g.cy) + 0.5);
@@ -76,13 +77,72 @@ SICP_DEV bool icp_finish(const IcpProj& p, float4 nc4, float4 vl4, float4 nl4, c
     return true;
 }
 
-// The current frame's vertices and normals of one thread's pixels: they do
-// not depend on the pose, so their loads are issued before the pose is known.
+// RGBReduction::FindCorresp's projection into the last frame
+// (Reduction.cu:436-439): __float2int_rd(fx * x / z + cx + 0.5), the sum
+// taken in double and narrowed to float by the intrinsic's argument. `p` is
+// the current vertex in the last camera (the same value as ICP's).
+SICP_DEV bool rgb_project(v3 p, const IcpGeom& g, int& u, int& v) {
+    const float tu = (float)((double)(g.fx * p.x / p.z + g.cx) + 0.5);
+    const float tv = (float)((double)(g.fy * p.y / p.z + g.cy) + 0.5);
+    u = f2i(floorf(tu));
+    v = f2i(floorf(tv));
+    return u >= 0 && v >= 0 && u < g.cols && v < g.rows;
+}
+
+// The rest of FindCorresp (:441-452): the last vertex valid and within 0.1 of
+// the current one (float4 distance, w included), the last intensity
+// non-zero. The record is {last pixel index, intensity difference}, or
+// {-1, 0} without a correspondence.
+SICP_DEV int2 rgb_record(int key, float4 vc, float4 vl, int li, int u, int v, int cols) {
+    const float d0 = vl.x - vc.x, d1 = vl.y - vc.y, d2 = vl.z - vc.z, d3 = vl.w - vc.w;
+    const float dist = sqrtf(d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3);
+    if (isnan_(vl.x) || !(dist < 0.1f) || li == 0) return make_int2(-1, 0);
+    return make_int2(v * cols + u, key - li);
+}
+
+// RGBReduction::GetRow (:482-521) of one correspondence: the robust weight
+// 1 / (sigma + |diff|), J = w [left, point x left], r = -w diff, with the
+// last vertex in the current camera.
+SICP_DEV void rgb_row(int diff, float4 vl, int gx, int gy, float sigma, const PoseF& cur,
+                      const PoseF& last, const IcpGeom& g, float row[7]) {
+    float w = sigma + (float)abs(diff);
+    w = (double)w < 1e-3 ? 1.0f : 1.0f / w;
+    if ((double)sigma < 1e-6) w = 1.0f;
+    const v3 point = mv(cur.Rinv, sub(add(mv(last.R, xyz(vl)), last.t), cur.t));
+    const float gxf = (float)((double)gx / 9.0), gyf = (float)((double)gy / 9.0);
+    const float invz = 1.0f / point.z;
+    v3 left;
+    left.x = gxf * g.fx * invz;
+    left.y = gyf * g.fy * invz;
+    left.z = -(left.x * point.x + left.y * point.y) * invz;
+    const v3 c = cross(point, left);
+    row[0] = w * left.x;
+    row[1] = w * left.y;
+    row[2] = w * left.z;
+    row[3] = w * c.x;
+    row[4] = w * c.y;
+    row[5] = w * c.z;
+    row[6] = -w * (float)diff;
+}
+
+// The photometric views and outputs of the correspondence pass.
+struct RgbPass {
+    View<int16_t> key;
+    View<uint8_t> ilast;
+    int2* rec;
+    uint32_t* ipart;
+};
+
+// The current frame's vertices, normals and photometric keys of one thread's
+// pixels: they do not depend on the pose, so their loads are issued before
+// the pose is known.
 struct IcpBatch {
     float4 vc[kPixPerThread], nc[kPixPerThread];
+    int key[kPixPerThread];
 };
+template <bool kRgb>
 SICP_DEV void icp_load_batch(IcpBatch& bt, int base, int stride, const View<float4>& vcurr,
-                             const View<float4>& ncurr, const IcpGeom& g) {
+                             const View<float4>& ncurr, const RgbPass& rp, const IcpGeom& g) {
 #pragma unroll
     for (int k = 0; k < kPixPerThread; ++k) {
         const int i = base + k * stride;
@@ -90,56 +150,44 @@ SICP_DEV void icp_load_batch(IcpBatch& bt, int base, int stride, const View<floa
         const int y = ii / g.cols, x = ii - y * g.cols;
         bt.vc[k] = vcurr.at(y, x);
         bt.nc[k] = ncurr.at(y, x);
-        if (i >= g.n) bt.vc[k].x = qnanf();
+        if (kRgb) bt.key[k] = rp.key.at(y, x);
+        if (i >= g.n) {
+            bt.vc[k].x = qnanf();
+            bt.key[k] = -1;
+        }
     }
 }
 
-// Per-thread accumulation + workgroup reduction into this workgroup's
-// partial row (column-major: partials[j * kIcpMaxBlocks + workgroup]).
-// `get_pose(cur, last)` is called by every thread once the first pixels'
-// loads are in flight; it returns false when there is nothing to do.
-template <class PoseFn>
-SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlast,
-                             const View<float4>& ncurr, const View<float4>& nlast,
-                             const IcpGeom& g, double* partials, PoseFn get_pose) {
-    const int stride = gridDim.x * blockDim.x;
-    int base = blockIdx.x * blockDim.x + threadIdx.x;
-    IcpBatch bt;
-    icp_load_batch(bt, base, stride, vcurr, ncurr, g);
-    PoseF cur, last;
-    if (!get_pose(cur, last)) return;
-    double acc[kIcpSums];
+// Sum of two u32 values over the workgroup (wrapping, so any order gives the
+// same result, as the reference's int sums). Every thread gets the totals.
+SICP_DEV uint2 block_sum_u32x2(uint32_t a, uint32_t b) {
+    __shared__ uint32_t red[2][kIcpThreads / 64];
 #pragma unroll
-    for (int j = 0; j < kIcpSums; ++j) acc[j] = 0.0;
-    while (base < g.n) {
-        float4 vl[kPixPerThread], nl[kPixPerThread];
-        IcpProj pr[kPixPerThread];
-#pragma unroll
-        for (int k = 0; k < kPixPerThread; ++k) {
-            pr[k] = icp_project(bt.vc[k], cur, last, g);
-            const int u = pr[k].ok ? pr[k].u : 0, v = pr[k].ok ? pr[k].v : 0;
-            vl[k] = vlast.at(v, u);
-            nl[k] = nlast.at(v, u);
-        }
-#pragma unroll
-        for (int k = 0; k < kPixPerThread; ++k) {
-            float row[7];
-            if (pr[k].ok && icp_finish(pr[k], bt.nc[k], vl[k], nl[k], cur, last, row)) {
-                int q = 0;
-#pragma unroll
-                for (int a = 0; a < 7; ++a)
-#pragma unroll
-                    for (int b = a; b < 7; ++b) acc[q++] += (double)(row[a] * row[b]);
-                acc[28] += 1.0;
-            }
-        }
-        base += stride * kPixPerThread;
-        if (base < g.n) icp_load_batch(bt, base, stride, vcurr, ncurr, g);
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
     }
-    // Workgroup reduction by transposition through LDS: every thread stores its
-    // 29 sums as one row; thread (j, k) then adds rows k, k+8, ... of column j
-    // and eight lanes fold with three butterflies. Fixed order; much cheaper
-    // than folding 29 doubles across 64 lanes with cross-lane permutes.
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    uint2 r = make_uint2(0u, 0u);
+#pragma unroll
+    for (int w = 0; w < kIcpThreads / 64; ++w) {
+        r.x += red[0][w];
+        r.y += red[1][w];
+    }
+    return r;
+}
+
+// Workgroup reduction of every thread's 29 FP64 sums into this workgroup's
+// partial row (column-major: partials[j * kIcpMaxBlocks + workgroup]), by
+// transposition through LDS: every thread stores its sums as one row; thread
+// (j, k) then adds rows k, k+8, ... of column j and eight lanes fold with
+// three butterflies. Fixed order; much cheaper than folding 29 doubles
+// across 64 lanes with cross-lane permutes.
+SICP_DEV void block_rows_to_partials(const double acc[kIcpSums], double* partials) {
     __shared__ double tile[kIcpThreads][kIcpSums + 1];
 #pragma unroll
     for (int j = 0; j < kIcpSums; ++j) tile[threadIdx.x][j] = acc[j];
@@ -156,6 +204,151 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
         // column-major: sum j of every workgroup is contiguous
         if (k == 0) partials[j * kIcpMaxBlocks + blockIdx.x] = s;
     }
+}
+
+SICP_DEV void add_row(double acc[kIcpSums], const float row[7]) {
+    int q = 0;
+#pragma unroll
+    for (int a = 0; a < 7; ++a)
+#pragma unroll
+        for (int b = a; b < 7; ++b) acc[q++] += (double)(row[a] * row[b]);
+    acc[28] += 1.0;
+}
+
+// Per-thread accumulation + workgroup reduction into this workgroup's
+// partial row. `get_pose(cur, last)` is called by every thread once the first
+// pixels' loads are in flight; it returns false when there is nothing to do.
+// kRgb: the same pass also runs RGBStep's correspondence test
+// (FindCorresp, Reduction.cu:409-461) on every pixel, writing its record and
+// the workgroup's int count and sum of squared differences.
+template <bool kRgb, class PoseFn>
+SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlast,
+                             const View<float4>& ncurr, const View<float4>& nlast,
+                             const IcpGeom& g, double* partials, const RgbPass& rp,
+                             PoseFn get_pose) {
+    const int stride = gridDim.x * blockDim.x;
+    int base = blockIdx.x * blockDim.x + threadIdx.x;
+    IcpBatch bt;
+    icp_load_batch<kRgb>(bt, base, stride, vcurr, ncurr, rp, g);
+    PoseF cur, last;
+    if (!get_pose(cur, last)) return;
+    double acc[kIcpSums];
+#pragma unroll
+    for (int j = 0; j < kIcpSums; ++j) acc[j] = 0.0;
+    uint32_t cnt = 0, sq = 0;
+    while (base < g.n) {
+        float4 vl[kPixPerThread], nl[kPixPerThread], rvl[kPixPerThread];
+        IcpProj pr[kPixPerThread];
+        int ru[kPixPerThread], rv[kPixPerThread], rli[kPixPerThread];
+        bool rok[kPixPerThread];
+#pragma unroll
+        for (int k = 0; k < kPixPerThread; ++k) {
+            pr[k] = icp_project(bt.vc[k], cur, last, g);
+            const int u = pr[k].ok ? pr[k].u : 0, v = pr[k].ok ? pr[k].v : 0;
+            vl[k] = vlast.at(v, u);
+            nl[k] = nlast.at(v, u);
+            if (kRgb) {
+                rok[k] = rgb_project(pr[k].vcurr_p, g, ru[k], rv[k]) && bt.key[k] >= 0;
+                const int uu = rok[k] ? ru[k] : 0, vv = rok[k] ? rv[k] : 0;
+                rvl[k] = vlast.at(vv, uu);
+                rli[k] = rp.ilast.at(vv, uu);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPixPerThread; ++k) {
+            float row[7];
+            if (pr[k].ok && icp_finish(pr[k], bt.nc[k], vl[k], nl[k], cur, last, row))
+                add_row(acc, row);
+            if (kRgb) {
+                const int i = base + k * stride;
+                if (i < g.n) {
+                    const int2 rc = rok[k] ? rgb_record(bt.key[k], bt.vc[k], rvl[k], rli[k],
+                                                        ru[k], rv[k], g.cols)
+                                           : make_int2(-1, 0);
+                    rp.rec[i] = rc;
+                    if (rc.x >= 0) {
+                        cnt += 1u;
+                        sq += (uint32_t)(rc.y * rc.y);
+                    }
+                }
+            }
+        }
+        base += stride * kPixPerThread;
+        if (base < g.n) icp_load_batch<kRgb>(bt, base, stride, vcurr, ncurr, rp, g);
+    }
+    if (kRgb) {
+        const uint2 r = block_sum_u32x2(cnt, sq);
+        if (threadIdx.x == 0) {
+            rp.ipart[blockIdx.x] = r.x;
+            rp.ipart[kIcpMaxBlocks + blockIdx.x] = r.y;
+        }
+    }
+    block_rows_to_partials(acc, partials);
+}
+
+// RGBStep's row pass over the records of the correspondence pass of
+// `nb_match` workgroups: sigma = sqrt(sum diff^2 / max(count, 1)) from the
+// int sums (RGBStep, Reduction.cu:618-620; every workgroup forms the same
+// value), then GetRow of every record into this workgroup's partial row.
+// Workgroup 0 also stores the count, the sum and sigma in `rs.totals`.
+template <class PoseFn>
+SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
+                       const View<int16_t>& dy, const IcpGeom& g, const RgbScratch& rs,
+                       int nb_match, PoseFn get_pose) {
+    const int stride = gridDim.x * blockDim.x;
+    int base = blockIdx.x * blockDim.x + threadIdx.x;
+    int2 rc[kPixPerThread];
+    auto load_records = [&]() {
+#pragma unroll
+        for (int k = 0; k < kPixPerThread; ++k) {
+            const int i = base + k * stride;
+            rc[k] = i < g.n ? rs.rec[i] : make_int2(-1, 0);
+        }
+    };
+    load_records();
+    uint32_t c = 0, q = 0;
+    for (int b = threadIdx.x; b < nb_match; b += blockDim.x) {
+        c += rs.ipart[b];
+        q += rs.ipart[kIcpMaxBlocks + b];
+    }
+    PoseF cur, last;
+    if (!get_pose(cur, last)) return;
+    const uint2 r = block_sum_u32x2(c, q);
+    const int count = (int)r.x, sum_sq = (int)r.y;
+    const float sigma = sqrtf((float)sum_sq / (float)(count == 0 ? 1 : count));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        rs.totals[0] = count;
+        rs.totals[1] = sum_sq;
+        rs.totals[2] = __float_as_int(sigma);
+    }
+    double acc[kIcpSums];
+#pragma unroll
+    for (int j = 0; j < kIcpSums; ++j) acc[j] = 0.0;
+    while (base < g.n) {
+        float4 vl[kPixPerThread];
+        int gx[kPixPerThread], gy[kPixPerThread];
+#pragma unroll
+        for (int k = 0; k < kPixPerThread; ++k) {
+            const int li = rc[k].x >= 0 ? rc[k].x : 0;
+            const int ly = li / g.cols, lx = li - ly * g.cols;
+            vl[k] = vlast.at(ly, lx);
+            const int i = rc[k].x >= 0 ? base + k * stride : 0;
+            const int y = i / g.cols, x = i - y * g.cols;
+            gx[k] = dx.at(y, x);
+            gy[k] = dy.at(y, x);
+        }
+#pragma unroll
+        for (int k = 0; k < kPixPerThread; ++k) {
+            if (rc[k].x >= 0) {
+                float row[7];
+                rgb_row(rc[k].y, vl[k], gx[k], gy[k], sigma, cur, last, g, row);
+                add_row(acc, row);
+            }
+        }
+        base += stride * kPixPerThread;
+        if (base < g.n) load_records();
+    }
+    block_rows_to_partials(acc, rs.partials);
 }
 
 // Sum of the partial rows of `nb` workgroups into totals (LDS), in a fixed
@@ -194,11 +387,12 @@ SICP_DEV void icp_totals(const double* partials, int nb, double* totals) {
 __global__ __launch_bounds__(kIcpThreads) void icp_partials_kernel(
     View<float4> vcurr, View<float4> vlast, View<float4> ncurr, View<float4> nlast, PoseArg pc,
     PoseArg pl, IcpGeom g, double* partials) {
-    icp_accumulate(vcurr, vlast, ncurr, nlast, g, partials, [&](PoseF& cur, PoseF& last) {
-        cur = load_pose(pc);
-        last = load_pose(pl);
-        return true;
-    });
+    icp_accumulate<false>(vcurr, vlast, ncurr, nlast, g, partials, RgbPass{},
+                          [&](PoseF& cur, PoseF& last) {
+                              cur = load_pose(pc);
+                              last = load_pose(pl);
+                              return true;
+                          });
 }
 
 __global__ __launch_bounds__(kIcpThreads) void icp_sums_finalize_kernel(const double* partials,
@@ -215,10 +409,12 @@ SICP_DEV void copy16(double* d, const double* s) {
 }
 
 // The update of one iteration of the inner loop of ComputeSE3
-// (Tracking.cc:433-514, icpOnly branch :488-490), on one thread, from the
-// iteration's 29 sums. The state lives in LDS; the poses are read into
-// registers once and written back once.
-SICP_DEV void gauss_newton_step(TrackState* st, const double* totals, int min_count) {
+// (Tracking.cc:400-514), on one thread, from the iteration's 29 ICP sums and,
+// with the photometric term, its 29 RGB sums (`rgb_totals`, else null). The
+// state lives in LDS; the poses are read into registers once and written
+// back once.
+SICP_DEV void gauss_newton_step(TrackState* st, const double* totals, const double* rgb_totals,
+                                int min_count) {
     float sums[kIcpSums];
 #pragma unroll
     for (int j = 0; j < kIcpSums; ++j) sums[j] = (float)totals[j];  // the float download
@@ -226,6 +422,22 @@ SICP_DEV void gauss_newton_step(TrackState* st, const double* totals, int min_co
     copy16(last_pose, st->last_pose);
     copy16(next_pose, st->next_pose);
     copy16(delta, st->delta);
+    float rgb_error = 0.f;
+    if (rgb_totals) {
+        // rgbError and "track rgb failed" (Tracking.cc:425-432), before ICPStep's
+        const float q0 = (float)rgb_totals[27], q1 = (float)rgb_totals[28];
+        rgb_error = sqrtf(q0) / q1;
+        const int rgb_count = (int)q1;
+        st->rgb_error = rgb_error;
+        st->rgb_count = rgb_count;
+        if (isnan_(rgb_error) || rgb_count < min_count) {
+            copy16(st->next_pose, last_pose);
+            st->failed = 1;
+            st->done = 1;
+            pose_to_float(last_pose, &st->curr_f);
+            return;
+        }
+    }
     int break_count = st->break_count;
     const float last_error = st->last_error;
     const float r0 = sums[27], r1 = sums[28];
@@ -241,7 +453,7 @@ SICP_DEV void gauss_newton_step(TrackState* st, const double* totals, int min_co
         pose_to_float(last_pose, &st->curr_f);
         return;
     }
-    const float curr_error = icp_error;
+    const float curr_error = rgb_error * st->rgb_weight + icp_error;  // Tracking.cc:458
     if (last_error < curr_error) {
         break_count++;
         if (break_count == 1) copy16(st->candidate, next_pose);
@@ -263,6 +475,19 @@ SICP_DEV void gauss_newton_step(TrackState* st, const double* totals, int min_co
     st->icp_error = icp_error;
     double A[36], b[6], x[6], dT[16], inv[16];
     create_matrix(sums, A, b);
+    if (rgb_totals) {
+        // matA = matA_rgb * w + matA_icp, vecb likewise (Tracking.cc:484-487)
+        float rs[kIcpSums];
+#pragma unroll
+        for (int j = 0; j < kIcpSums; ++j) rs[j] = (float)rgb_totals[j];
+        double Ar[36], br[6];
+        create_matrix(rs, Ar, br);
+        const double w = (double)st->rgb_weight;
+#pragma unroll
+        for (int k = 0; k < 36; ++k) A[k] = Ar[k] * w + A[k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) b[k] = br[k] * w + b[k];
+    }
     ldlt_solve6(A, b, x);
     se3_exp(x, dT);
     mat4_mul(dT, delta, delta);
@@ -276,7 +501,7 @@ SICP_DEV void gauss_newton_step(TrackState* st, const double* totals, int min_co
 
 // ComputeSE3's set-up (Tracking.cc:369-431 with TrackFrame :157-173): the
 // iteration state of a new frame, from the last frame's pose.
-SICP_DEV void track_begin_state(TrackState* st, const FrameMeta* last, float thresh) {
+SICP_DEV void track_begin_state(TrackState* st, const FrameMeta* last, const TrackParams& tp) {
     double lp[16], I[16];
     copy16(lp, last->pose);
     set_identity(I);
@@ -288,7 +513,11 @@ SICP_DEV void track_begin_state(TrackState* st, const FrameMeta* last, float thr
     st->icp_error = 0.f;
     st->last_error = 100.f;
     st->candidate_icp = 0.f;
-    st->thresh = thresh;
+    st->thresh = tp.thresh;
+    st->rgb_error = 0.f;
+    st->rgb_weight = tp.rgb_weight;
+    st->rgb_count = 0;
+    st->rgb_on = tp.rgb;
     st->icp_count = 0;
     st->break_count = 0;
     st->applied = 0;
@@ -324,22 +553,28 @@ static_assert(kStateWords <= kIcpThreads, "TrackState copied one word per thread
 // iteration's Gauss-Newton update from its partial rows. Every workgroup of
 // a launch computes the same state (same inputs, same order), so no
 // workgroup waits for another; workgroup 0 publishes it to `st_out`.
+// With `rgb_prev` (the previous iteration's photometric partial rows) the
+// update folds those too; `totals` then holds 2 * kIcpSums values.
 template <bool kFirst>
 SICP_DEV void icp_state(TrackState* S, double* totals, const TrackState* st_in,
-                        const FrameMeta* last, float thresh, const double* partials_prev,
-                        int nb_prev, int min_count_prev, TrackState* st_out) {
+                        const FrameMeta* last, const TrackParams& tp,
+                        const double* partials_prev, const double* rgb_prev, int nb_prev,
+                        int min_count_prev, TrackState* st_out) {
     if (kFirst) {
-        if (threadIdx.x == 0) track_begin_state(S, last, thresh);
+        if (threadIdx.x == 0) track_begin_state(S, last, tp);
         __syncthreads();
     } else {
         const unsigned long long* src = reinterpret_cast<const unsigned long long*>(st_in);
         unsigned long long w = threadIdx.x < kStateWords ? src[threadIdx.x] : 0ull;
-        double s;
+        double s, sr = 0.0;
         icp_totals_issue(partials_prev, nb_prev, s);
+        if (rgb_prev) icp_totals_issue(rgb_prev, nb_prev, sr);
         if (threadIdx.x < kStateWords) reinterpret_cast<unsigned long long*>(S)[threadIdx.x] = w;
         icp_totals_fold(s, totals);
+        if (rgb_prev) icp_totals_fold(sr, totals + kIcpSums);
         __syncthreads();
-        if (threadIdx.x == 0 && !S->done) gauss_newton_step(S, totals, min_count_prev);
+        if (threadIdx.x == 0 && !S->done)
+            gauss_newton_step(S, totals, rgb_prev ? totals + kIcpSums : nullptr, min_count_prev);
         __syncthreads();
     }
     if (st_out && blockIdx.x == 0 && threadIdx.x < kStateWords)
@@ -349,21 +584,74 @@ SICP_DEV void icp_state(TrackState* S, double* totals, const TrackState* st_in,
 
 // One launch per Gauss-Newton iteration: the update of the previous
 // iteration, then this iteration's pixel pass (ICPReduce at the state's pose).
-template <bool kFirst>
+// kRgb: the photometric term is on; its correspondence pass runs here and
+// the previous iteration's photometric rows (`rgb_prev`) join the update.
+template <bool kFirst, bool kRgb>
 __global__ __launch_bounds__(kIcpThreads) void icp_iteration_kernel(
     View<float4> vcurr, View<float4> vlast, View<float4> ncurr, View<float4> nlast, IcpGeom g,
-    const TrackState* st_in, const FrameMeta* last, float thresh, const double* partials_prev,
-    int nb_prev, int min_count_prev, TrackState* st_out, double* partials) {
+    const TrackState* st_in, const FrameMeta* last, TrackParams tp, const double* partials_prev,
+    const double* rgb_prev, int nb_prev, int min_count_prev, TrackState* st_out,
+    double* partials, RgbPass rp) {
     __shared__ TrackState S;
-    __shared__ double totals[kIcpSums];
-    icp_accumulate(vcurr, vlast, ncurr, nlast, g, partials, [&](PoseF& cur, PoseF& lst) {
-        icp_state<kFirst>(&S, totals, st_in, last, thresh, partials_prev, nb_prev,
-                          min_count_prev, st_out);
-        if (S.done) return false;
-        cur = to_pose(S.curr_f);
-        lst = to_pose(S.last_f);
+    __shared__ double totals[2 * kIcpSums];
+    icp_accumulate<kRgb>(vcurr, vlast, ncurr, nlast, g, partials, rp,
+                         [&](PoseF& cur, PoseF& lst) {
+                             icp_state<kFirst>(&S, totals, st_in, last, tp, partials_prev,
+                                               kRgb ? rgb_prev : nullptr, nb_prev,
+                                               min_count_prev, st_out);
+                             if (S.done) return false;
+                             cur = to_pose(S.curr_f);
+                             lst = to_pose(S.last_f);
+                             return true;
+                         });
+}
+
+// RGBStep's row pass. The pose is the iteration state's (`st`, published by
+// the iteration's pixel pass) or, without a state, `pc` / `pl`.
+__global__ __launch_bounds__(kIcpThreads) void rgb_rows_kernel(
+    View<float4> vlast, View<int16_t> dx, View<int16_t> dy, IcpGeom g, RgbScratch rs,
+    int nb_match, const TrackState* st, PoseArg pc, PoseArg pl) {
+    rgb_rows(vlast, dx, dy, g, rs, nb_match, [&](PoseF& cur, PoseF& lst) {
+        if (st) {
+            if (st->done) return false;
+            cur = to_pose(st->curr_f);
+            lst = to_pose(st->last_f);
+        } else {
+            cur = load_pose(pc);
+            lst = load_pose(pl);
+        }
         return true;
     });
+}
+
+// RGBStep's correspondence pass on its own (the pixel pass of the tracking
+// loop runs the same test fused with ICP's).
+__global__ __launch_bounds__(kIcpThreads) void rgb_match_kernel(View<float4> vcurr,
+                                                                View<float4> vlast, IcpGeom g,
+                                                                PoseArg pc, PoseArg pl,
+                                                                RgbPass rp) {
+    const PoseF cur = load_pose(pc), last = load_pose(pl);
+    uint32_t cnt = 0, sq = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x) {
+        const int y = i / g.cols, x = i - y * g.cols;
+        const int key = rp.key.at(y, x);
+        const float4 vc = vcurr.at(y, x);
+        int2 rc = make_int2(-1, 0);
+        int u, v;
+        const v3 p = mv(last.Rinv, sub(add(mv(cur.R, xyz(vc)), cur.t), last.t));
+        if (rgb_project(p, g, u, v) && key >= 0)
+            rc = rgb_record(key, vc, vlast.at(v, u), rp.ilast.at(v, u), u, v, g.cols);
+        rp.rec[i] = rc;
+        if (rc.x >= 0) {
+            cnt += 1u;
+            sq += (uint32_t)(rc.y * rc.y);
+        }
+    }
+    const uint2 r = block_sum_u32x2(cnt, sq);
+    if (threadIdx.x == 0) {
+        rp.ipart[blockIdx.x] = r.x;
+        rp.ipart[kIcpMaxBlocks + blockIdx.x] = r.y;
+    }
 }
 
 // The end of ComputeSE3 and TrackFrame: the last iteration's update, the
@@ -372,13 +660,14 @@ __global__ __launch_bounds__(kIcpThreads) void icp_iteration_kernel(
 // swap keeps the last model (Tracking.cc:98-117 keeps lastFrame on failure).
 template <bool kFirst>
 __global__ __launch_bounds__(kIcpThreads) void track_finish_kernel(
-    const TrackState* st_in, const FrameMeta* last, float thresh, const double* partials_prev,
-    int nb_prev, int min_count_prev, TrackState* st_final, FrameMeta* next, View<float4> vlast,
-    View<float4> nlast, View<float4> vnext, View<float4> nnext) {
+    const TrackState* st_in, const FrameMeta* last, TrackParams tp, const double* partials_prev,
+    const double* rgb_prev, int nb_prev, int min_count_prev, TrackState* st_final,
+    FrameMeta* next, View<float4> vlast, View<float4> nlast, View<float4> vnext,
+    View<float4> nnext, FrameImages images) {
     __shared__ TrackState S;
-    __shared__ double totals[kIcpSums];
-    icp_state<kFirst>(&S, totals, st_in, last, thresh, partials_prev, nb_prev, min_count_prev,
-                      nullptr);
+    __shared__ double totals[2 * kIcpSums];
+    icp_state<kFirst>(&S, totals, st_in, last, tp, partials_prev, rgb_prev, nb_prev,
+                      min_count_prev, nullptr);
     if (threadIdx.x == 0) track_end_state(&S);
     __syncthreads();
     if (blockIdx.x == 0) {
@@ -395,6 +684,18 @@ __global__ __launch_bounds__(kIcpThreads) void track_finish_kernel(
         vnext.at(y, x) = vlast.at(y, x);
         nnext.at(y, x) = nlast.at(y, x);
     }
+    // The photometric term reads the last frame's intensity pyramid.
+    for (int l = 0; l < 3; ++l) {
+        const View<uint8_t>& src = images.last[l];
+        const View<uint8_t>& dst = images.next[l];
+        if (!dst.ptr) continue;
+        const int m = dst.cols * dst.rows;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m;
+             i += gridDim.x * blockDim.x) {
+            int y = i / dst.cols, x = i - y * dst.cols;
+            dst.at(y, x) = src.at(y, x);
+        }
+    }
 }
 
 __global__ void track_init_kernel(FrameMeta* next, FrameMeta* last, TrackState* st) {
@@ -410,6 +711,8 @@ __global__ void track_init_kernel(FrameMeta* next, FrameMeta* last, TrackState* 
     st->applied = 0;
     st->icp_count = 0;
     st->icp_error = 0.f;
+    st->rgb_count = 0;
+    st->rgb_error = 0.f;
 }
 
 // Four pixels per thread, at most kIcpMaxBlocks workgroups.
@@ -433,44 +736,102 @@ int launch_icp_sums(View<float4> vcurr, View<float4> vlast, View<float4> ncurr,
     return SICP_OK;
 }
 
+namespace {
+IcpGeom geom_of(const IcpLevel& l) {
+    return IcpGeom{l.vcurr.cols, l.vcurr.rows, l.vcurr.cols * l.vcurr.rows,
+                   l.K.fx,       l.K.fy,       l.K.cx,
+                   l.K.cy};
+}
+}  // namespace
+
 int launch_icp_iteration(const IcpLevel& cur, const IcpLevel* prev, const FrameMeta* last,
-                         float thresh, const TrackState* st_in, TrackState* st_out,
-                         const double* partials_prev, double* partials, hipStream_t s) {
-    IcpGeom g{cur.vcurr.cols, cur.vcurr.rows, cur.vcurr.cols * cur.vcurr.rows,
-              cur.K.fx,       cur.K.fy,       cur.K.cx,
-              cur.K.cy};
+                         TrackParams tp, const TrackState* st_in, TrackState* st_out,
+                         const double* partials_prev, double* partials, const RgbScratch* rgb,
+                         hipStream_t s) {
+    const IcpGeom g = geom_of(cur);
     const int nb = icp_blocks(g.n);
-    if (prev) {
-        const int nb_prev = icp_blocks(prev->vcurr.cols * prev->vcurr.rows);
-        icp_iteration_kernel<false><<<nb, kIcpThreads, 0, s>>>(
-            cur.vcurr, cur.vlast, cur.ncurr, cur.nlast, g, st_in, last, thresh, partials_prev,
-            nb_prev, prev->min_count, st_out, partials);
-    } else {
-        icp_iteration_kernel<true><<<nb, kIcpThreads, 0, s>>>(
-            cur.vcurr, cur.vlast, cur.ncurr, cur.nlast, g, nullptr, last, thresh, nullptr, 0, 0,
-            st_out, partials);
+    const int nb_prev = prev ? icp_blocks(prev->vcurr.cols * prev->vcurr.rows) : 0;
+    const int min_prev = prev ? prev->min_count : 0;
+    RgbPass rp{};
+    const double* rgb_prev = nullptr;
+    if (rgb) {
+        rp = RgbPass{cur.rgb.key, cur.rgb.ilast, rgb->rec, rgb->ipart};
+        rgb_prev = rgb->partials;
     }
+    if (prev && rgb)
+        icp_iteration_kernel<false, true><<<nb, kIcpThreads, 0, s>>>(
+            cur.vcurr, cur.vlast, cur.ncurr, cur.nlast, g, st_in, last, tp, partials_prev,
+            rgb_prev, nb_prev, min_prev, st_out, partials, rp);
+    else if (prev)
+        icp_iteration_kernel<false, false><<<nb, kIcpThreads, 0, s>>>(
+            cur.vcurr, cur.vlast, cur.ncurr, cur.nlast, g, st_in, last, tp, partials_prev,
+            nullptr, nb_prev, min_prev, st_out, partials, rp);
+    else if (rgb)
+        icp_iteration_kernel<true, true><<<nb, kIcpThreads, 0, s>>>(
+            cur.vcurr, cur.vlast, cur.ncurr, cur.nlast, g, nullptr, last, tp, nullptr, nullptr,
+            0, 0, st_out, partials, rp);
+    else
+        icp_iteration_kernel<true, false><<<nb, kIcpThreads, 0, s>>>(
+            cur.vcurr, cur.vlast, cur.ncurr, cur.nlast, g, nullptr, last, tp, nullptr, nullptr,
+            0, 0, st_out, partials, rp);
     SICP_LAUNCH_CHECK("icp_iteration_kernel");
     return SICP_OK;
 }
 
-int launch_track_finish(const IcpLevel* prev, const FrameMeta* last, float thresh,
+int launch_rgb_rows(const IcpLevel& cur, const TrackState* st, RgbScratch rgb, hipStream_t s) {
+    const IcpGeom g = geom_of(cur);
+    const int nb = icp_blocks(g.n);
+    rgb_rows_kernel<<<nb, kIcpThreads, 0, s>>>(cur.vlast, cur.rgb.dx, cur.rgb.dy, g, rgb, nb, st,
+                                                PoseArg{}, PoseArg{});
+    SICP_LAUNCH_CHECK("rgb_rows_kernel");
+    return SICP_OK;
+}
+
+int launch_track_finish(const IcpLevel* prev, const FrameMeta* last, TrackParams tp,
                         const TrackState* st_in, const double* partials_prev,
-                        TrackState* st_final, FrameMeta* next, View<float4> vlast,
-                        View<float4> nlast, View<float4> vnext, View<float4> nnext,
-                        hipStream_t s) {
+                        const RgbScratch* rgb, TrackState* st_final, FrameMeta* next,
+                        View<float4> vlast, View<float4> nlast, View<float4> vnext,
+                        View<float4> nnext, const FrameImages* images, hipStream_t s) {
     const int n = vnext.cols * vnext.rows;
     const int blocks = (int)div_up(n, kIcpThreads * 4);
+    FrameImages im{};
+    if (images) im = *images;
     if (prev) {
         const int nb_prev = icp_blocks(prev->vcurr.cols * prev->vcurr.rows);
         track_finish_kernel<false><<<blocks, kIcpThreads, 0, s>>>(
-            st_in, last, thresh, partials_prev, nb_prev, prev->min_count, st_final, next, vlast,
-            nlast, vnext, nnext);
+            st_in, last, tp, partials_prev, rgb ? rgb->partials : nullptr, nb_prev,
+            prev->min_count, st_final, next, vlast, nlast, vnext, nnext, im);
     } else {
         track_finish_kernel<true><<<blocks, kIcpThreads, 0, s>>>(
-            nullptr, last, thresh, nullptr, 0, 0, st_final, next, vlast, nlast, vnext, nnext);
+            nullptr, last, tp, nullptr, nullptr, 0, 0, st_final, next, vlast, nlast, vnext,
+            nnext, im);
     }
     SICP_LAUNCH_CHECK("track_finish_kernel");
+    return SICP_OK;
+}
+
+int launch_rgb_sums(View<uint8_t> inext, View<uint8_t> ilast, View<float4> vnext,
+                    View<float4> vlast, View<int16_t> dx, View<int16_t> dy, PoseArg curr,
+                    PoseArg last, sicp_intrinsics K, int16_t* key, RgbScratch rgb,
+                    float* sums_out, hipStream_t s) {
+    const int cols = inext.cols, rows = inext.rows;
+    View<int16_t> kv{(char*)key, (size_t)cols * sizeof(int16_t), cols, rows};
+    int rc = launch_rgb_keys(inext, dx, dy, vnext, kv, s);
+    if (rc) return rc;
+    IcpLevel lv{};
+    lv.vcurr = vnext;
+    lv.vlast = vlast;
+    lv.K = K;
+    lv.rgb = RgbLevel{kv, dx, dy, ilast};
+    const IcpGeom g = geom_of(lv);
+    const int nb = icp_blocks(g.n);
+    rgb_match_kernel<<<nb, kIcpThreads, 0, s>>>(vnext, vlast, g, curr, last,
+                                                 RgbPass{kv, ilast, rgb.rec, rgb.ipart});
+    SICP_LAUNCH_CHECK("rgb_match_kernel");
+    rgb_rows_kernel<<<nb, kIcpThreads, 0, s>>>(vlast, dx, dy, g, rgb, nb, nullptr, curr, last);
+    SICP_LAUNCH_CHECK("rgb_rows_kernel");
+    icp_sums_finalize_kernel<<<1, kIcpThreads, 0, s>>>(rgb.partials, nb, sums_out);
+    SICP_LAUNCH_CHECK("icp_sums_finalize_kernel");
     return SICP_OK;
 }
 

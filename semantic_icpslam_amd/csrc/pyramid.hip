@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void intensity_kernel(View<uint8_t> rgb, View<
 // Sobel pair of the reference, {1,0,-1, 2,0,-2, 1,0,-1} and its transpose,
 // written as closed forms of the tap index below.
 
-SICP_DEV void derivative_px(const View<uint8_t>& img, const View<int16_t>& dx,
+SICP_DEV int2 derivative_px(const View<uint8_t>& img, const View<int16_t>& dx,
                             const View<int16_t>& dy, int x, int y) {
     uint8_t tap[9];
 #pragma unroll
@@ -216,8 +216,38 @@ SICP_DEV void derivative_px(const View<uint8_t>& img, const View<int16_t>& dx,
             }
         }
     }
-    dx.at(y, x) = (int16_t)(int)dxv;
-    dy.at(y, x) = (int16_t)(int)dyv;
+    const int16_t gx = (int16_t)(int)dxv, gy = (int16_t)(int)dyv;
+    dx.at(y, x) = gx;
+    dy.at(y, x) = gy;
+    return make_int2(gx, gy);
+}
+
+// The tests of RGBReduction::FindCorresp (Tracking/Reduction.cu:420-433) that
+// do not depend on the pose: a 5-pixel border, a positive intensity over the
+// window rows y-2..y+1, columns x-2..x+1 (the loops' bounds), a non-zero
+// gradient (sqrtf(gx^2 + gy^2) > 0; the int sum cannot overflow for int16
+// inputs), a valid vertex. Returns the pixel's intensity when they pass,
+// else -1.
+SICP_DEV int16_t rgb_key_px(const View<uint8_t>& img, int gx, int gy, float vx, int x, int y) {
+    if (!(x >= 5 && x < img.cols - 5 && y >= 5 && y < img.rows - 5)) return -1;
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const uint8_t* row = &img.at(y - 2 + a, x - 2);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) ok = ok && row[b] > 0;
+    }
+    if (!ok || gx * gx + gy * gy == 0 || isnan_(vx)) return -1;
+    return (int16_t)img.at(y, x);
+}
+
+__global__ __launch_bounds__(256) void rgb_key_kernel(View<uint8_t> img, View<int16_t> dx,
+                                                      View<int16_t> dy, View<float4> vmap,
+                                                      View<int16_t> key) {
+    const int x = blockIdx.x * kTX + threadIdx.x;
+    const int y = blockIdx.y * kTY + threadIdx.y;
+    if (x >= img.cols || y >= img.rows) return;
+    key.at(y, x) = rgb_key_px(img, dx.at(y, x), dy.at(y, x), vmap.at(y, x).x, x, y);
 }
 
 __global__ __launch_bounds__(256) void derivative_kernel(View<uint8_t> img, View<int16_t> dx,
@@ -392,7 +422,8 @@ __global__ __launch_bounds__(256) void frame_maps_kernel(FramePyramid P, float c
         P.nmap[l].at(y, x) = nmap_of(c, vmap_px(d, x + 1, y, fx, fy, cx, cy, cutoff),
                                      vmap_px(d, x, y + 1, fx, fy, cx, cy, cutoff));
     }
-    derivative_px(P.image[l], P.dx[l], P.dy[l], x, y);
+    const int2 g = derivative_px(P.image[l], P.dx[l], P.dy[l], x, y);
+    if (P.key[l].ptr) P.key[l].at(y, x) = rgb_key_px(P.image[l], g.x, g.y, c.x, x, y);
 }
 
 inline int tiles(int cols, int rows) { return (int)(div_up(cols, kTX) * div_up(rows, kTY)); }
@@ -442,6 +473,13 @@ int launch_intensity(View<uint8_t> rgb, View<uint8_t> gray, hipStream_t s) {
 int launch_derivative(View<uint8_t> img, View<int16_t> dx, View<int16_t> dy, hipStream_t s) {
     derivative_kernel<<<grid2d(img.cols, img.rows), dim3(kTX, kTY), 0, s>>>(img, dx, dy);
     SICP_LAUNCH_CHECK("derivative_kernel");
+    return SICP_OK;
+}
+
+int launch_rgb_keys(View<uint8_t> img, View<int16_t> dx, View<int16_t> dy, View<float4> vmap,
+                    View<int16_t> key, hipStream_t s) {
+    rgb_key_kernel<<<grid2d(img.cols, img.rows), dim3(kTX, kTY), 0, s>>>(img, dx, dy, vmap, key);
+    SICP_LAUNCH_CHECK("rgb_key_kernel");
     return SICP_OK;
 }
 

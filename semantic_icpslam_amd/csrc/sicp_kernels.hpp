@@ -18,6 +18,9 @@ int launch_intensity(View<uint8_t> rgb, View<uint8_t> gray, hipStream_t s);
 int launch_derivative(View<uint8_t> img, View<int16_t> dx, View<int16_t> dy, hipStream_t s);
 int launch_resize_map(View<float4> vsrc, View<float4> nsrc, View<float4> vdst, View<float4> ndst,
                       hipStream_t s);
+// The photometric candidate key of every pixel (RgbLevel::key).
+int launch_rgb_keys(View<uint8_t> img, View<int16_t> dx, View<int16_t> dy, View<float4> vmap,
+                    View<int16_t> key, hipStream_t s);
 
 // The images of one frame at three pyramid levels (Frame, Core/Frame.h).
 struct FramePyramid {
@@ -25,6 +28,7 @@ struct FramePyramid {
     View<float4> vmap[3], nmap[3];
     View<uint8_t> image[3];
     View<int16_t> dx[3], dy[3];
+    View<int16_t> key[3];  // photometric candidate keys (RgbLevel::key); ptr null: not made
     float invfx[3], invfy[3], cx[3], cy[3];
 };
 // A frame's model maps (raycast at level 0, 2x2 means below).
@@ -59,6 +63,15 @@ struct TrackState {
     float icp_error, last_error, candidate_icp, thresh;
     int icp_count, break_count, applied, done;
     int failed, tracked, frame_index, pad;
+    float rgb_error, rgb_weight;  // photometric error of the last iteration; its weight w
+    int rgb_count, rgb_on;        // its correspondences; 1 when the term is used
+};
+
+// Arguments of ComputeSE3 (Tracking.cc:369) that are not images.
+struct TrackParams {
+    float thresh;      // thresh_icp
+    float rgb_weight;  // w (Tracking.cc:392)
+    int rgb;           // !icpOnly: RGBStep runs before ICPStep in every iteration
 };
 
 struct IcpScratch {
@@ -70,28 +83,66 @@ int launch_icp_sums(View<float4> vcurr, View<float4> vlast, View<float4> ncurr,
                     View<float4> nlast, PoseArg curr, PoseArg last, sicp_intrinsics K,
                     IcpScratch scratch, float* sums_out, hipStream_t s);
 
+// The photometric inputs of one level (RGBStep, Reduction.h:81-90): the
+// current frame's candidate key (its intensity where the pose-independent
+// tests of FindCorresp pass, else -1) and derivatives, the last frame's
+// intensity.
+struct RgbLevel {
+    View<int16_t> key, dx, dy;
+    View<uint8_t> ilast;
+};
+// Device memory of the photometric pass.
+struct RgbScratch {
+    int2* rec;         // per pixel: last-frame pixel index (-1: none), intensity difference
+    uint32_t* ipart;   // per workgroup: count, then sum of squared differences (2 * kIcpMaxBlocks)
+    double* partials;  // kIcpSums * kIcpMaxBlocks, column-major
+    int32_t* totals;   // count, sum of squared differences, sigma (float bits)
+};
+
 // The maps of one pyramid level and its correspondence minimum.
 struct IcpLevel {
     View<float4> vcurr, vlast, ncurr, nlast;
     sicp_intrinsics K;
     int min_count;
+    RgbLevel rgb;  // used when the photometric term is on
 };
 // One Gauss-Newton iteration of ComputeSE3: the update of the previous
 // iteration (`prev` = its level; null for the first iteration of a frame,
 // whose state is built from `last`'s pose), then this iteration's pixel pass
 // into `partials`. The state moves from `st_in` to `st_out` (two buffers,
 // alternating).
+// With `rgb` set (the photometric term on), the pixel pass also runs the
+// correspondence pass of RGBStep into `rgb->rec` / `rgb->ipart`, and the
+// update folds the previous iteration's photometric rows from
+// `rgb->partials`.
 int launch_icp_iteration(const IcpLevel& cur, const IcpLevel* prev, const FrameMeta* last,
-                         float thresh, const TrackState* st_in, TrackState* st_out,
-                         const double* partials_prev, double* partials, hipStream_t s);
+                         TrackParams tp, const TrackState* st_in, TrackState* st_out,
+                         const double* partials_prev, double* partials, const RgbScratch* rgb,
+                         hipStream_t s);
+// RGBStep's second pass for the iteration whose state is `st`: sigma from the
+// correspondence counts, then the 29 weighted photometric sums into
+// `rgb.partials`.
+int launch_rgb_rows(const IcpLevel& cur, const TrackState* st, RgbScratch rgb, hipStream_t s);
 // The last iteration's update, the final gate, the new frame's pose and,
 // when the frame is not tracked, the copy of the last frame's level-0 model
-// maps into the new frame. The frame's final state goes to `st_final`.
-int launch_track_finish(const IcpLevel* prev, const FrameMeta* last, float thresh,
+// maps (and, with the photometric term, its intensity pyramid) into the new
+// frame. The frame's final state goes to `st_final`.
+struct FrameImages {
+    View<uint8_t> last[3], next[3];
+};
+int launch_track_finish(const IcpLevel* prev, const FrameMeta* last, TrackParams tp,
                         const TrackState* st_in, const double* partials_prev,
-                        TrackState* st_final, FrameMeta* next, View<float4> vlast,
-                        View<float4> nlast, View<float4> vnext, View<float4> nnext,
-                        hipStream_t s);
+                        const RgbScratch* rgb, TrackState* st_final, FrameMeta* next,
+                        View<float4> vlast, View<float4> nlast, View<float4> vnext,
+                        View<float4> nnext, const FrameImages* images, hipStream_t s);
+
+// RGBStep on its own (the C ABI): key pass, correspondence pass, row pass,
+// then the 29 sums into `sums_out` (device) and the correspondence count,
+// sum of squared differences and sigma into `rgb.totals`.
+int launch_rgb_sums(View<uint8_t> inext, View<uint8_t> ilast, View<float4> vnext,
+                    View<float4> vlast, View<int16_t> dx, View<int16_t> dy, PoseArg curr,
+                    PoseArg last, sicp_intrinsics K, int16_t* key, RgbScratch rgb,
+                    float* sums_out, hipStream_t s);
 int launch_track_init(FrameMeta* next, FrameMeta* last, TrackState* st, hipStream_t s);
 
 // Host FP64 solve + SE(3) update (same code path as the device).

@@ -11,6 +11,7 @@
 // puts the last model maps and pose into the frame that becomes "last", so
 // the next frame tracks against the same model as in the reference, where
 // the swap is skipped.
+#include <cmath>
 #include <cstring>
 
 #include "se3_math.hpp"
@@ -25,9 +26,10 @@ struct FrameBuf {
     DevImage<float4> vmap[3], nmap[3];
     DevImage<uint8_t> image[3];
     DevImage<int16_t> dx[3], dy[3];
+    DevImage<int16_t> key[3];   // photometric candidate keys (with the photometric term)
     FrameMeta* meta = nullptr;  // device
 
-    int create(int c, int r) {
+    int create(int c, int r, bool rgb) {
         int rc;
         if ((rc = range.create(c, r))) return rc;
         for (int i = 0; i < 3; ++i) {
@@ -37,6 +39,7 @@ struct FrameBuf {
                 (rc = nmap[i].create(cols[i], rows[i])) || (rc = image[i].create(cols[i], rows[i])) ||
                 (rc = dx[i].create(cols[i], rows[i])) || (rc = dy[i].create(cols[i], rows[i])))
                 return rc;
+            if (rgb && (rc = key[i].create(cols[i], rows[i]))) return rc;
         }
         hipError_t e = hipMalloc((void**)&meta, sizeof(FrameMeta));
         if (e != hipSuccess) return hip_fail(e, "hipMalloc(FrameMeta)");
@@ -56,6 +59,7 @@ struct FrameBuf {
             image[i].release();
             dx[i].release();
             dy[i].release();
+            key[i].release();
         }
         if (meta) (void)hipFree(meta);
         meta = nullptr;
@@ -72,6 +76,8 @@ struct sicp_system {
     sicp::FrameBuf* last = &fb;
     sicp::TrackState* st = nullptr;    // [0] the frame's final state, [1..2] iterations
     double* partials = nullptr;        // two sets of ICP partial rows, alternating
+    void* rgb_mem = nullptr;           // photometric scratch (records, counts, rows)
+    sicp::RgbScratch rgb{};
     sicp::DevImage<float> zmin, zmax;
     sicp_intrinsics K[3];
     int frame_index = 0;
@@ -134,6 +140,7 @@ static int frame_pyramid(sicp_system* s, FrameBuf& f, FrameBuf& last,
         P.image[i] = f.image[i].v();
         P.dx[i] = f.dx[i].v();
         P.dy[i] = f.dy[i].v();
+        P.key[i] = f.key[i].ptr ? f.key[i].v() : View<int16_t>{};
         P.invfx[i] = (float)(1.0 / (double)k.fx);
         P.invfy[i] = (float)(1.0 / (double)k.fy);
         P.cx[i] = k.cx;
@@ -161,6 +168,7 @@ static void system_free(sicp_system* s) {
     s->zmax.release();
     if (s->st) (void)hipFree(s->st);
     if (s->partials) (void)hipFree(s->partials);
+    if (s->rgb_mem) (void)hipFree(s->rgb_mem);
     map_free(s->map);
     delete s;
 }
@@ -199,6 +207,11 @@ int sicp_system_create(const sicp_system_config* cfg, sicp_system** out) {
     if (cfg->cols < 8 || cfg->rows < 8) return invalid("system_create: image too small");
     if ((size_t)cfg->cols * cfg->rows > (size_t)cfg->map.max_cols * cfg->map.max_rows)
         return invalid("system_create: map max_cols x max_rows smaller than the image");
+    if (cfg->icp_only != 0 && cfg->icp_only != 1)
+        return invalid("system_create: icp_only must be 0 or 1");
+    if (!cfg->icp_only && !std::isfinite(cfg->rgb_weight))
+        return invalid("system_create: rgb_weight must be finite");
+    const bool rgb = !cfg->icp_only;
     sicp_system* s = new sicp_system();
     s->cfg = *cfg;
     int rc = map_alloc(cfg->map, &s->map);
@@ -206,7 +219,8 @@ int sicp_system_create(const sicp_system_config* cfg, sicp_system** out) {
         delete s;
         return rc;
     }
-    if ((rc = s->fa.create(cfg->cols, cfg->rows)) || (rc = s->fb.create(cfg->cols, cfg->rows)) ||
+    if ((rc = s->fa.create(cfg->cols, cfg->rows, rgb)) ||
+        (rc = s->fb.create(cfg->cols, cfg->rows, rgb)) ||
         (rc = s->zmin.create((cfg->cols + 7) / 8, (cfg->rows + 7) / 8)) ||
         (rc = s->zmax.create((cfg->cols + 7) / 8, (cfg->rows + 7) / 8))) {
         system_free(s);
@@ -216,6 +230,20 @@ int sicp_system_create(const sicp_system_config* cfg, sicp_system** out) {
     if (e == hipSuccess) e = hipMemset(s->st, 0, 3 * sizeof(TrackState));
     if (e == hipSuccess)
         e = hipMalloc((void**)&s->partials, 2 * sizeof(double) * kIcpMaxBlocks * kIcpSums);
+    if (e == hipSuccess && rgb) {
+        // records (int2 per level-0 pixel), counts, rows, totals: one allocation
+        const size_t n = (size_t)cfg->cols * cfg->rows;
+        const size_t rec = n * sizeof(int2), ip = 2 * kIcpMaxBlocks * sizeof(uint32_t);
+        const size_t pr = (size_t)kIcpSums * kIcpMaxBlocks * sizeof(double);
+        e = hipMalloc(&s->rgb_mem, rec + ip + pr + 16);
+        if (e == hipSuccess) {
+            char* b = (char*)s->rgb_mem;  // rows first: 8-byte aligned
+            s->rgb.partials = (double*)b;
+            s->rgb.rec = (int2*)(b + pr);
+            s->rgb.ipart = (uint32_t*)(b + pr + rec);
+            s->rgb.totals = (int32_t*)(b + pr + rec + ip);
+        }
+    }
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         rc = hip_fail(e, "system_create");
@@ -276,29 +304,42 @@ static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
     } else {
         // Every launch runs the previous iteration's update and its own pixel
         // pass; states and partial rows alternate between two buffers.
+        // With the photometric term, RGBStep's row pass follows each pixel
+        // pass (its sigma needs the whole correspondence pass).
         TrackState* iter = s->st + 1;
         const size_t psz = (size_t)kIcpMaxBlocks * kIcpSums;
+        const bool rgb = !s->cfg.icp_only;
+        const TrackParams tp{s->cfg.icp_thresh, s->cfg.rgb_weight, rgb ? 1 : 0};
         IcpLevel prev{};
         int k = 0;
         for (int l = 2; l >= 0; --l) {
-            const IcpLevel cur{nf.vmap[l].v(), lf.vmap[l].v(), nf.nmap[l].v(), lf.nmap[l].v(),
-                               s->K[l], s->cfg.icp_min_count[l]};
+            IcpLevel cur{nf.vmap[l].v(), lf.vmap[l].v(), nf.nmap[l].v(), lf.nmap[l].v(),
+                         s->K[l], s->cfg.icp_min_count[l], RgbLevel{}};
+            if (rgb) cur.rgb = RgbLevel{nf.key[l].v(), nf.dx[l].v(), nf.dy[l].v(), lf.image[l].v()};
             for (int j = 0; j < s->cfg.icp_iters[l]; ++j, ++k) {
                 const size_t ek = stage_begin(s, st);
-                rc = launch_icp_iteration(cur, k ? &prev : nullptr, lf.meta, s->cfg.icp_thresh,
+                rc = launch_icp_iteration(cur, k ? &prev : nullptr, lf.meta, tp,
                                           iter + ((k + 1) & 1), iter + (k & 1),
                                           s->partials + psz * ((k + 1) & 1),
-                                          s->partials + psz * (k & 1), st);
+                                          s->partials + psz * (k & 1), rgb ? &s->rgb : nullptr,
+                                          st);
                 if (rc) return rc;
+                if (rgb && (rc = launch_rgb_rows(cur, iter + (k & 1), s->rgb, st))) return rc;
                 stage_end(s, kStageIcpIter, ek, st);
                 prev = cur;
             }
         }
+        FrameImages im{};
+        if (rgb)
+            for (int l = 0; l < 3; ++l) {
+                im.last[l] = lf.image[l].v();
+                im.next[l] = nf.image[l].v();
+            }
         const size_t ef = stage_begin(s, st);
-        if ((rc = launch_track_finish(k ? &prev : nullptr, lf.meta, s->cfg.icp_thresh,
-                                      iter + ((k + 1) & 1), s->partials + psz * ((k + 1) & 1),
+        if ((rc = launch_track_finish(k ? &prev : nullptr, lf.meta, tp, iter + ((k + 1) & 1),
+                                      s->partials + psz * ((k + 1) & 1), rgb ? &s->rgb : nullptr,
                                       s->st, nf.meta, lf.vmap[0].v(), lf.nmap[0].v(),
-                                      nf.vmap[0].v(), nf.nmap[0].v(), st)))
+                                      nf.vmap[0].v(), nf.nmap[0].v(), rgb ? &im : nullptr, st)))
             return rc;
         stage_end(s, kStageTrackFinish, ef, st);
     }
@@ -412,6 +453,8 @@ int sicp_system_last_result(sicp_system* s, sicp_frame_result* out, void* stream
     out->icp_count = t.icp_count;
     out->iterations = t.applied;
     out->icp_error = t.icp_error;
+    out->rgb_error = t.rgb_error;
+    out->rgb_count = t.rgb_count;
     out->visible_blocks = c.visible;
     out->allocated_blocks = c.heap_used;
     std::memcpy(out->pose, fm.pose, sizeof(out->pose));

@@ -110,3 +110,37 @@ inline void ICPStep(DeviceArray2D<float4>& nextVMap, DeviceArray2D<float4>& last
                            matrixA_host, vectorB_host, residual, nullptr),
              "ICPStep");
 }
+
+// RGBStep (Reduction.h:81-90; Reduction.cu:575-642). `sum` is the scratch
+// (grown to sicp_rgb_scratch_size(cols, rows) bytes when smaller), `out` the
+// 29 device sums, `outRes` the correspondence count and int sum of squared
+// differences (device, 2 ints); `sumRes` is not needed and left untouched.
+// On the host: matrixA_host (6x6 row-major, mirrored), vectorB_host[6],
+// residual = {sum r^2, count}.
+inline void RGBStep(const DeviceArray2D<unsigned char>& nextImage,
+                    const DeviceArray2D<unsigned char>& lastImage,
+                    const DeviceArray2D<float4>& nextVMap, const DeviceArray2D<float4>& lastVMap,
+                    const DeviceArray2D<short>& dIdx, const DeviceArray2D<short>& dIdy,
+                    Matrix3f Rcurr, Matrix3f RcurrInv, Matrix3f Rlast, Matrix3f RlastInv,
+                    float3 tcurr, float3 tlast, Intrinsics K, DeviceArray2D<float>& sum,
+                    DeviceArray<float>& out, DeviceArray2D<int>& sumRes, DeviceArray<int>& outRes,
+                    float* residual, double* matrixA_host, double* vectorB_host) {
+    (void)sumRes;
+    const size_t need = sicp_rgb_scratch_size(nextImage.cols, nextImage.rows);
+    if (!sum.data || sum.step * (size_t)sum.rows < need) {
+        const size_t row = 29 * sizeof(float);
+        sum.create(29, (int)((need + row - 1) / row));
+    }
+    if (!out.data || out.size < 29) out.create(29);
+    if (!outRes.data || outRes.size < 2) outRes.create(2);
+    const sicp_pose_f curr = sicp_pose_of(Rcurr, RcurrInv, tcurr);
+    const sicp_pose_f last = sicp_pose_of(Rlast, RlastInv, tlast);
+    float host_sums[29];
+    int32_t corresp[2];
+    SicpCall(sicp_rgb_step(nextImage.image(), lastImage.image(), nextVMap.image(),
+                           lastVMap.image(), dIdx.image(), dIdy.image(), &curr, &last, K.abi(),
+                           sum.data, sum.step * (size_t)sum.rows, (float*)out.data, host_sums,
+                           corresp, nullptr, matrixA_host, vectorB_host, residual, nullptr),
+             "RGBStep");
+    outRes.upload(corresp, 2);
+}

@@ -101,6 +101,36 @@ int mirror_icp(const float* vcurr, const float* vlast, const float* ncurr, const
     });
 }
 
+// Tracker::ComputeSE3's RGBStep call (Tracking.cc:402-423) through the
+// mirror: the photometric sums of one iteration, and the correspondence
+// count / sum of squared differences read back from outRes.
+int mirror_rgb(const uint8_t* nimg, const uint8_t* limg, const float* nvmap, const float* lvmap,
+               const int16_t* dx, const int16_t* dy, int cols, int rows, float fx, float fy,
+               float cx, float cy, const float* pose_curr, const float* pose_last, double* A,
+               double* b, float* residual, int* corresp) {
+    return guarded([&] {
+        DeviceArray2D<unsigned char> ni(cols, rows), li(cols, rows);
+        DeviceArray2D<float4> nv(cols, rows), lv(cols, rows);
+        DeviceArray2D<short> gx(cols, rows), gy(cols, rows);
+        ni.upload(nimg, cols);
+        li.upload(limg, cols);
+        nv.upload(nvmap, 16 * cols);
+        lv.upload(lvmap, 16 * cols);
+        gx.upload(dx, 2 * cols);
+        gy.upload(dy, 2 * cols);
+        DeviceArray2D<float> sum;
+        DeviceArray<float> out;
+        DeviceArray2D<int> sumRes;
+        DeviceArray<int> outRes;
+        Matrix3f Rc = mat(pose_curr), Rl = mat(pose_last);
+        RGBStep(ni, li, nv, lv, gx, gy, Rc, sicp_transpose(Rc), Rl, sicp_transpose(Rl),
+                make_float3(pose_curr[9], pose_curr[10], pose_curr[11]),
+                make_float3(pose_last[9], pose_last[10], pose_last[11]),
+                Intrinsics(fx, fy, cx, cy), sum, out, sumRes, outRes, residual, A, b);
+        outRes.download(corresp, 2);
+    });
+}
+
 // Mapping::FuseColor then Mapping::RayTrace (Mapping.cc:76-97,142-161)
 // through the mirror, for `n` frames of (raw depth, RGB, normals, labels,
 // pose {R[9], t[3]}) fused into a fresh map; then the rendering prepass and

@@ -180,7 +180,11 @@ def test_map_snapshot_round_trip(dev, tmp_path):
     B.check(B.lib().sicp_map_save(ga.h, path))
     gb = B.Map(cfg)
     B.check(B.lib().sicp_map_load(gb.h, path))
-    assert gb.stats() == ga.stats()
+    # every counter but the last raycast's hand-over count (not map state)
+    sa, sb = ga.stats(), gb.stats()
+    sa.pop("ray_handover")
+    sb.pop("ray_handover")
+    assert sa == sb
     for m in (ga, gb):
         for i in (6, 8):
             raw, rgb, n, lab = _inputs(c, s, i)

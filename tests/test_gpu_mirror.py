@@ -110,6 +110,36 @@ def test_mirror_icp_step(dev):
     assert abs(res[0] - os_[27]) <= 1e-6 * os_[27]
 
 
+def test_mirror_rgb_step(dev):
+    c = CFG_SMALL
+    s = synth(c)
+    fr = []
+    for i in (0, 3):
+        d, rgb, _ = s.render(i)
+        _, filt = O.filter_depth(d, 1000.0, 3.0)
+        v = O.compute_vmap(filt, c["f"], c["f"], c["cx"], c["cy"], 3.0)
+        g = O.image_to_intensity(rgb)
+        fr.append((v, g) + tuple(O.derivative_image(g)))
+    (v0, g0, _, _), (v1, g1, dx1, dy1) = fr
+    Tc, Tl = s.pose(3), np.eye(4)
+    A = np.zeros(36)
+    b = np.zeros(6)
+    res = np.zeros(2, np.float32)
+    corr = np.zeros(2, np.int32)
+    f = _lib().mirror_rgb
+    f.argtypes = [C.c_void_p] * 6 + [C.c_int, C.c_int] + [C.c_float] * 4 + [C.c_void_p] * 6
+    rc = f(_p(g1), _p(g0), _p(v1), _p(v0), _p(dx1), _p(dy1), c["cols"], c["rows"], c["f"],
+           c["f"], c["cx"], c["cy"], _p(_pose12(Tc)), _p(_pose12(Tl)), _p(A), _p(b), _p(res),
+           _p(corr))
+    assert rc == 0
+    os_, ocorr, _ = O.rgb_sums(g1, g0, v1, v0, dx1, dy1, B.pose_f(Tc), B.pose_f(Tl), K(c))
+    Ao, bo = O.create_matrix(os_)
+    assert tuple(corr) == ocorr and res[1] == os_[28] and os_[28] > 500
+    assert np.abs(A.reshape(6, 6) - Ao).max() <= 1e-6 * np.abs(Ao).max()
+    assert np.abs(b - bo).max() <= 1e-6 * np.abs(bo).max()
+    assert abs(res[0] - os_[27]) <= 1e-6 * os_[27]
+
+
 def test_mirror_fuse_and_raycast(dev):
     c = CFG_SMALL
     cfg = map_cfg(c)

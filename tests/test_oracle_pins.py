@@ -222,6 +222,148 @@ def test_icp_recovers_known_motion():
     assert np.abs(nxt[:3, :3] - gt[:3, :3]).max() < 3e-4
 
 
+# ------------------------------------------------------------ photometric --
+PLANE = dict(cols=96, rows=72, f=90.0, cx=47.5, cy=35.5)
+
+
+def _plane(rng, zeros=0.0):
+    """A textured fronto-parallel plane at 1 m: vertex map, intensity with
+    optional zero pixels, and its derivatives."""
+    rows, cols = PLANE["rows"], PLANE["cols"]
+    v = O.compute_vmap(np.ones((rows, cols), np.float32), PLANE["f"], PLANE["f"], PLANE["cx"],
+                       PLANE["cy"], 3.0)
+    img = rng.integers(20, 230, size=(rows, cols)).astype(np.uint8)
+    if zeros:
+        img[rng.random((rows, cols)) < zeros] = 0
+    dx, dy = O.derivative_image(img)
+    return v, img, dx, dy
+
+
+def _plane_K():
+    return B.intrinsics(PLANE["f"], PLANE["f"], PLANE["cx"], PLANE["cy"])
+
+
+def _candidates(img, dx, dy, v, shift=0):
+    """FindCorresp's tests written out per pixel: border 5, the window rows
+    y-2..y+1 and columns x-2..x+1 positive, a non-zero gradient, a valid
+    vertex; with a shift of the last frame, the matched pixel x + shift in
+    the image and non-zero there."""
+    rows, cols = img.shape
+    ok = np.zeros((rows, cols), bool)
+    for y in range(5, rows - 5):
+        for x in range(5, cols - 5):
+            u = x + shift
+            ok[y, x] = (np.all(img[y - 2:y + 2, x - 2:x + 2] > 0) and
+                        (int(dx[y, x]) ** 2 + int(dy[y, x]) ** 2) > 0 and
+                        not np.isnan(v[y, x, 0]) and 0 <= u < cols)
+    return ok
+
+
+def _photometric_A(img, dx, dy, v, ok):
+    """sum J^T J of the textbook photometric Jacobian
+    J = grad(I) . d(pi)/dp . [I | -[p]x] (twist order translation, rotation)."""
+    f = PLANE["f"]
+    A = np.zeros((6, 6))
+    Jsum = np.zeros(6)
+    for y, x in zip(*np.nonzero(ok)):
+        p = v[y, x, :3].astype(np.float64)
+        g = np.array([dx[y, x], dy[y, x]], np.float64) / 9.0
+        Jpi = np.array([[f / p[2], 0, -f * p[0] / p[2] ** 2], [0, f / p[2], -f * p[1] / p[2] ** 2]])
+        P = np.array([[0, -p[2], p[1]], [p[2], 0, -p[0]], [-p[1], p[0], 0]])
+        J = g @ Jpi @ np.hstack([np.eye(3), -P])
+        A += np.outer(J, J)
+        Jsum += J
+    return A, Jsum
+
+
+def test_rgb_identical_frames_textbook_jacobian():
+    """Zero motion, identical frames with scattered zero intensities: the
+    count is the number of pixels passing the candidate tests, every
+    difference is 0 (sigma 0, weight 1, b 0), and A is the textbook sum."""
+    rng = np.random.default_rng(11)
+    v, img, dx, dy = _plane(rng, zeros=0.01)
+    p = B.pose_f(np.eye(4))
+    s, (cnt, sq), sigma = O.rgb_sums(img, img, v, v, dx, dy, p, p, _plane_K())
+    ok = _candidates(img, dx, dy, v)
+    assert cnt == ok.sum() == s[28] and sq == 0 and sigma == 0.0
+    A, b = O.create_matrix(s.astype(np.float32))
+    assert s[27] == 0.0 and np.all(b == 0.0)
+    At, _ = _photometric_A(img, dx, dy, v, ok)
+    assert np.abs(A - At).max() < 1e-5 * np.abs(At).max()
+
+
+def test_rgb_constant_offset_weight_and_residual():
+    """The last image brighter by 10 everywhere: every difference is -10, so
+    sum d^2 = 100 n, sigma = 10, the weight 1 / (sigma + |d|) = 1/20, the
+    residual -w d = 0.5: A = A0 / 400, b = sum(J) / 40, sum r^2 = n / 4."""
+    rng = np.random.default_rng(12)
+    v, img, dx, dy = _plane(rng)
+    last = (img.astype(np.int32) + 10).astype(np.uint8)
+    p = B.pose_f(np.eye(4))
+    s, (cnt, sq), sigma = O.rgb_sums(img, last, v, v, dx, dy, p, p, _plane_K())
+    ok = _candidates(img, dx, dy, v)
+    n = ok.sum()
+    assert cnt == n == s[28] and sq == 100 * n and sigma == 10.0
+    A, b = O.create_matrix(s.astype(np.float32))
+    At, Jsum = _photometric_A(img, dx, dy, v, ok)
+    assert np.abs(A - At / 400).max() < 1e-5 * np.abs(At / 400).max()
+    assert np.abs(b - Jsum / 40).max() < 1e-5 * np.abs(Jsum / 40).max()
+    assert abs(s[27] - n / 4) < 1e-6 * n
+
+
+@pytest.mark.parametrize("shift", [3, -2])
+def test_rgb_projection_follows_translation(shift):
+    """The current camera moved by t along x over a plane at 1 m: pixel x
+    sees the last frame's pixel x + f t. With the last image shifted to
+    match, every correspondence has difference 0, and the count is the
+    candidates whose matched pixel is inside the image."""
+    rng = np.random.default_rng(13)
+    v, img, dx, dy = _plane(rng)
+    cols = PLANE["cols"]
+    last = np.zeros_like(img)
+    if shift > 0:
+        last[:, shift:] = img[:, :cols - shift]
+    else:
+        last[:, :cols + shift] = img[:, -shift:]
+    T = np.eye(4)
+    T[0, 3] = shift / PLANE["f"]
+    s, (cnt, sq), _ = O.rgb_sums(img, last, v, v, dx, dy, B.pose_f(T), B.pose_f(np.eye(4)),
+                                 _plane_K())
+    ok = _candidates(img, dx, dy, v, shift)
+    assert cnt == ok.sum() and sq == 0 and s[27] == 0.0
+
+
+def test_rgb_gauss_newton_moves_towards_known_motion():
+    """Photometric-only Gauss-Newton on two rendered frames of a known motion
+    reduces the pose error and the intensity residual: pins the Jacobian
+    signs against the update composition (its nearest-pixel lookup stops
+    it at a few millimetres)."""
+    c = CFG_SMALL
+
+    def frame(i):
+        d, rgb, _ = synth(c, noise=False, dropout=0.0).render(i)
+        raw, _ = O.filter_depth(d, 1000.0, 3.0)
+        v = O.compute_vmap(raw, c["f"], c["f"], c["cx"], c["cy"], 3.0)
+        g = O.image_to_intensity(rgb)
+        return (v, g) + tuple(O.derivative_image(g))
+
+    v0, g0, _, _ = frame(0)
+    v1, g1, dx1, dy1 = frame(2)
+    gt = synth(c).pose(2)
+    last, nxt, delta = np.eye(4), np.eye(4), np.eye(4)
+    err = lambda T: (np.abs(T[:3, 3] - gt[:3, 3]).max(), np.abs(T[:3, :3] - gt[:3, :3]).max())
+    t0, r0 = err(nxt)
+    sq = []
+    for _ in range(5):
+        s, (cnt, q), _ = O.rgb_sums(g1, g0, v1, v0, dx1, dy1, B.pose_f(nxt), B.pose_f(last), K(c))
+        sq.append(q / cnt)
+        A, b = O.create_matrix(s.astype(np.float32))
+        delta, nxt, _ = O.se3_update(A, b, delta, last)
+    t1, r1 = err(nxt)
+    assert t1 < 0.3 * t0 and r1 < 0.7 * r0
+    assert sq[-1] < 0.2 * sq[0]
+
+
 def test_solve_matches_numpy():
     rng = np.random.default_rng(4)
     for _ in range(20):
