@@ -334,12 +334,12 @@ def main():
                           w["trunc"], w["buckets"], w["blocks"])
     stream = torch.cuda.current_stream()
 
-    def run_pass(timing, graph=False):
+    def run_pass(timing, graph=False, system_cfg=None):
         """Warm-up frames, then the K timed frames on a fresh system; with
         `timing`, HIP events bracket every stage and kernel launch; with
         `graph`, every frame is copied into the system's staged inputs and
         runs as one hipGraph launch."""
-        sysg = B.System(cfg)
+        sysg = B.System(system_cfg or cfg)
 
         def grab(i):
             if graph:
@@ -384,6 +384,11 @@ def main():
     clean = run_pass(False)
     log("[bench] graph pass (staged inputs, one hipGraph launch per frame)")
     graphed = run_pass(False, graph=True)
+    log("[bench] photometric pass (ComputeSE3(false): RGBStep + ICPStep per iteration)")
+    cfg_rgb = B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"],
+                              w["voxel"], w["trunc"], w["buckets"], w["blocks"])
+    cfg_rgb.icp_only = 0
+    photometric = run_pass(False, system_cfg=cfg_rgb)
     log("[bench] instrumented pass (HIP events per stage and kernel)")
     inst = run_pass(True)
     st0, st1, res = inst["st0"], inst["st1"], clean["res"]
@@ -478,6 +483,14 @@ def main():
             "what": "the same frames, each copied into the system's staged inputs and run as "
                     "one hipGraph launch (plain FP32)",
             "same_final_pose": bool(np.array_equal(graphed["res"]["pose"], res["pose"]))},
+        "photometric_tracking": {
+            "ms_per_step": round(1000.0 * photometric["elapsed"] / args.steps, 4),
+            "what": "the same frames with the reference's default tracking, ComputeSE3(false): "
+                    "RGBStep (correspondence pass fused into the ICP pass, then a row pass) "
+                    "and ICPStep in every iteration, photometric weight 1e-4",
+            "last_frame_tracked": photometric["res"]["tracked"],
+            "last_frame_translation_vs_ground_truth_m": round(
+                float(np.linalg.norm(photometric["res"]["pose"][:3, 3] - gt[:3, 3])), 5)},
         "frame_ms_p50": round(float(np.percentile(fm, 50)), 4) if fm.size else None,
         "frame_ms_p99": round(float(np.percentile(fm, 99)), 4) if fm.size else None,
         "roofline": roofline,
