@@ -292,6 +292,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB first (counter calibration)")
     ap.add_argument("--no-pose-check", action="store_true")
+    ap.add_argument("--no-photometric", action="store_true",
+                    help="skip the pass with the photometric tracking term")
     ap.add_argument("--workload", default="vga", choices=("vga", "batched8"),
                     help="vga: the 640x480 / 5 mm sequence (default); batched8: eight "
                          "independent sequences at once")
@@ -384,11 +386,13 @@ def main():
     clean = run_pass(False)
     log("[bench] graph pass (staged inputs, one hipGraph launch per frame)")
     graphed = run_pass(False, graph=True)
-    log("[bench] photometric pass (ComputeSE3(false): RGBStep + ICPStep per iteration)")
-    cfg_rgb = B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"],
-                              w["voxel"], w["trunc"], w["buckets"], w["blocks"])
-    cfg_rgb.icp_only = 0
-    photometric = run_pass(False, system_cfg=cfg_rgb)
+    photometric = None
+    if not args.no_photometric:
+        log("[bench] photometric pass (ComputeSE3(false): RGBStep + ICPStep per iteration)")
+        cfg_rgb = B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"],
+                                  w["voxel"], w["trunc"], w["buckets"], w["blocks"])
+        cfg_rgb.icp_only = 0
+        photometric = run_pass(False, system_cfg=cfg_rgb)
     log("[bench] instrumented pass (HIP events per stage and kernel)")
     inst = run_pass(True)
     st0, st1, res = inst["st0"], inst["st1"], clean["res"]
@@ -483,7 +487,7 @@ def main():
             "what": "the same frames, each copied into the system's staged inputs and run as "
                     "one hipGraph launch (plain FP32)",
             "same_final_pose": bool(np.array_equal(graphed["res"]["pose"], res["pose"]))},
-        "photometric_tracking": {
+        "photometric_tracking": None if photometric is None else {
             "ms_per_step": round(1000.0 * photometric["elapsed"] / args.steps, 4),
             "what": "the same frames with the reference's default tracking, ComputeSE3(false): "
                     "RGBStep (correspondence pass fused into the ICP pass, then a row pass) "

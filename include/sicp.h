@@ -204,6 +204,19 @@ int sicp_rgb_step(sicp_image next_image, sicp_image last_image, sicp_image next_
                   float* residual_host, void* stream);
 size_t sicp_rgb_scratch_size(int32_t cols, int32_t rows);
 
+/* SO3Step (Tracking/Reduction.h:67-72, Reduction.cu:67-211): the
+ * rotation-only photometric alignment of Tracker::ComputeSO3
+ * (Tracking.cc:322-366). Every pixel's bearing in the last frame is rotated
+ * into the current camera (curr->Rinv * last->R) and matched by rounded
+ * projection; the 11 sums (A 3x3 upper and b interleaved, sum r^2, count) go
+ * to `sums_dev` (11 floats, device). `partials_dev` is device scratch of
+ * sicp_icp_partials_size() bytes. Host outputs as for sicp_icp_step, with
+ * CreateMatrix<3,4>: A row-major 3x3 (mirrored), b[3]. */
+int sicp_so3_step(sicp_image next_image, sicp_image last_image, sicp_image didx,
+                  sicp_image didy, const sicp_pose_f* curr, const sicp_pose_f* last,
+                  sicp_intrinsics K, void* partials_dev, float* sums_dev, float* host_sums,
+                  double* A_host, double* b_host, float* residual_host, void* stream);
+
 /* FP64 6x6 solve + SE(3) update of one Gauss-Newton iteration, on the host
  * arrays of Tracking/Tracking.cc:493-498: x = A^-1 b (Cholesky),
  * delta = exp(x) * delta, next = last * delta^-1. Poses are 4x4 row-major. */

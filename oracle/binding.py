@@ -40,6 +40,8 @@ _SIGS = {
     "oracle_icp_sums_fp64": (None, [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
     "oracle_rgb_sums": (None, [_P, _P, _P, _P, _P, _P, C.c_int, C.c_int, C.POINTER(PoseF),
                                C.POINTER(PoseF), Intrinsics, _P, _P, _P]),
+    "oracle_so3_sums": (None, [_P, _P, _P, _P, C.c_int, C.c_int, C.POINTER(PoseF),
+                               C.POINTER(PoseF), Intrinsics, _P]),
     "oracle_create_matrix": (None, [_P, _P, _P]),
     "oracle_cholesky_solve6": (C.c_int, [_P, _P, _P]),
     "oracle_se3_exp": (None, [_P, _P]),
@@ -207,6 +209,35 @@ def rgb_sums(nimg, limg, nvmap, lvmap, dx, dy, pose_curr, pose_last, K):
                           C.byref(pose_curr), C.byref(pose_last), K, _p(sums), _p(corr),
                           _p(sig))
     return sums, (int(corr[0]), int(corr[1])), float(sig[0])
+
+
+def so3_sums(nimg, limg, dx, dy, pose_curr, pose_last, K):
+    """SO3Step: the 11 FP64 sums."""
+    ni, li = _c(nimg, np.uint8), _c(limg, np.uint8)
+    gx, gy = _c(dx, np.int16), _c(dy, np.int16)
+    rows, cols = ni.shape[:2]
+    for a in (li, gx, gy):
+        assert a.shape[:2] == (rows, cols)
+    sums = np.zeros(11, np.float64)
+    lib().oracle_so3_sums(_p(ni), _p(li), _p(gx), _p(gy), cols, rows, C.byref(pose_curr),
+                          C.byref(pose_last), K, _p(sums))
+    return sums
+
+
+def create_matrix3(sums11):
+    """CreateMatrix<3,4> (Reduction.cu:6-16) of the float sums."""
+    s = np.asarray(sums11, np.float32).astype(np.float64)
+    A = np.zeros((3, 3))
+    b = np.zeros(3)
+    k = 0
+    for i in range(3):
+        for j in range(i, 4):
+            if j == 3:
+                b[i] = s[k]
+            else:
+                A[i, j] = A[j, i] = s[k]
+            k += 1
+    return A, b
 
 
 def create_matrix(sums29):

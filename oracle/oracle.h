@@ -61,6 +61,11 @@ void oracle_rgb_sums(const uint8_t* nimg, const uint8_t* limg, const float* nvma
                      const float* lvmap, const int16_t* dx, const int16_t* dy, int cols, int rows,
                      const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
                      double* sums_fp64, int32_t* corresp, float* sigma);
+/* SO3Step (Tracking/Reduction.cu:67-211): 11 sums (FP32 rows, FP64
+ * accumulation); rotations curr->Rinv and last->R. */
+void oracle_so3_sums(const uint8_t* nimg, const uint8_t* limg, const int16_t* dx,
+                     const int16_t* dy, int cols, int rows, const sicp_pose_f* curr,
+                     const sicp_pose_f* last, sicp_intrinsics K, double* sums);
 void oracle_create_matrix(const float* sums29, double* A, double* b);
 int oracle_cholesky_solve6(const double* A, const double* b, double* x);
 void oracle_se3_exp(const double* xi, double* T);

@@ -211,6 +211,51 @@ template void rgb_sums<double>(const uint8_t*, const uint8_t*, const double*, co
                                const int16_t*, const int16_t*, int, int, const Pose<double>&,
                                const Pose<double>&, Intr<double>, double*, int*, float*);
 
+// SO3Reduce (Tracking/Reduction.cu:67-160): the last frame's pixel bearing
+// ((x - cx) / fx, (y - cy) / fy, 1) rotated by RcurrInv * Rlast, matched by
+// the rounded projection (__float2int_rn: nearest, ties to even) with a
+// 5-pixel border on both pixels; J = -(left x p) from the next frame's
+// gradient at the match (float / 9.0f), r = -(I_next(u, v) - I_last(x, y)).
+// 10 FP32 products accumulated in FP64, then the count.
+void so3_sums(const uint8_t* nimg, const uint8_t* limg, const int16_t* dx, const int16_t* dy,
+              int cols, int rows, const M3<float>& rcurr_inv, const M3<float>& rlast,
+              Intr<float> K, double* sums) {
+    const int N = cols * rows;
+    for (int j = 0; j < 11; ++j) sums[j] = 0.0;
+#pragma omp parallel
+    {
+        double loc[11] = {0};
+#pragma omp for schedule(static)
+        for (int k = 0; k < N; ++k) {
+            const int y = k / cols, x = k - y * cols;
+            const V3<float> vl{((float)x - K.cx) / K.fx, ((float)y - K.cy) / K.fy, 1.0f};
+            const V3<float> p = mv(rcurr_inv, mv(rlast, vl));
+            const int u = f2i(std::nearbyint(K.fx * p.x / p.z + K.cx));
+            const int v = f2i(std::nearbyint(K.fy * p.y / p.z + K.cy));
+            if (!(u >= 5 && v >= 5 && u < cols - 5 && v < rows - 5 && x >= 5 && y >= 5 &&
+                  x < cols - 5 && y < rows - 5))
+                continue;
+            const float gx = (float)dx[(size_t)v * cols + u] / 9.0f;
+            const float gy = (float)dy[(size_t)v * cols + u] / 9.0f;
+            const float invz = 1.0f / p.z;
+            V3<float> left;
+            left.x = gx * K.fx * invz;
+            left.y = gy * K.fy * invz;
+            left.z = -(p.x * left.x + p.y * left.y) * invz;
+            const V3<float> c = cross(left, p);
+            const float row[4] = {-c.x, -c.y, -c.z,
+                                  -((float)nimg[(size_t)v * cols + u] -
+                                    (float)limg[(size_t)y * cols + x])};
+            int q = 0;
+            for (int a = 0; a < 4; ++a)
+                for (int b = a; b < 4; ++b) loc[q++] += (double)(row[a] * row[b]);
+            loc[10] += 1.0;
+        }
+#pragma omp critical
+        for (int j = 0; j < 11; ++j) sums[j] += loc[j];
+    }
+}
+
 // CreateMatrix<6,7> (Reduction.cu:6-16): upper-triangle sums -> mirrored A, b.
 void create_matrix(const double* s, double* A, double* b) {
     int shift = 0;
@@ -390,6 +435,15 @@ void oracle_rgb_sums(const uint8_t* nimg, const uint8_t* limg, const float* nvma
         corresp[0] = res2[0];
         corresp[1] = res2[1];
     }
+}
+
+void oracle_so3_sums(const uint8_t* nimg, const uint8_t* limg, const int16_t* dx,
+                     const int16_t* dy, int cols, int rows, const sicp_pose_f* curr,
+                     const sicp_pose_f* last, sicp_intrinsics K, double* sums) {
+    Pose<float> pc = pose_from_f<float>(curr->R, curr->Rinv, curr->t);
+    Pose<float> pl = pose_from_f<float>(last->R, last->Rinv, last->t);
+    so3_sums(nimg, limg, dx, dy, cols, rows, pc.Rinv, pl.R_, Intr<float>{K.fx, K.fy, K.cx, K.cy},
+             sums);
 }
 
 void oracle_create_matrix(const float* sums29, double* A, double* b) {

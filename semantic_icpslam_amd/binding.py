@@ -111,6 +111,8 @@ _SIGS = {
                                 C.POINTER(PoseF), Intrinsics, _P, C.c_size_t, _P, _P, _P, _P,
                                 _P, _P, _P, _P]),
     "sicp_rgb_scratch_size": (C.c_size_t, [C.c_int32, C.c_int32]),
+    "sicp_so3_step": (C.c_int, [Image, Image, Image, Image, C.POINTER(PoseF), C.POINTER(PoseF),
+                                Intrinsics, _P, _P, _P, _P, _P, _P, _P]),
     "sicp_icp_partials_size": (C.c_size_t, []),
     "sicp_se3_update_host": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "sicp_map_config_default": (None, [C.POINTER(MapConfig), C.c_float, C.c_float, C.c_int32,

@@ -262,6 +262,47 @@ int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_ima
     return SICP_OK;
 }
 
+int sicp_so3_step(sicp_image next_image, sicp_image last_image, sicp_image didx,
+                  sicp_image didy, const sicp_pose_f* curr, const sicp_pose_f* last,
+                  sicp_intrinsics K, void* partials_dev, float* sums_dev, float* host_sums,
+                  double* A_host, double* b_host, float* residual_host, void* stream) {
+    const int c = next_image.cols, r = next_image.rows;
+    if (!image_ok(next_image, 1, c, r) || !image_ok(last_image, 1, c, r) ||
+        !image_ok(didx, 2, c, r) || !image_ok(didy, 2, c, r))
+        return invalid("so3_step: the four images must share one shape");
+    if (!curr || !last || !partials_dev || !sums_dev) return invalid("so3_step: null argument");
+    IcpScratch sc{(double*)partials_dev};
+    hipStream_t s = (hipStream_t)stream;
+    int rc = launch_so3_sums(view<uint8_t>(next_image), view<uint8_t>(last_image),
+                             view<int16_t>(didx), view<int16_t>(didy), make_pose_arg(curr, nullptr),
+                             make_pose_arg(last, nullptr), K, sc, sums_dev, s);
+    if (rc) return rc;
+    if (host_sums || A_host || b_host || residual_host) {
+        float h[11];
+        SICP_HIP(hipMemcpyAsync(h, sums_dev, sizeof(h), hipMemcpyDeviceToHost, s));
+        SICP_HIP(hipStreamSynchronize(s));
+        if (host_sums) std::memcpy(host_sums, h, sizeof(h));
+        // CreateMatrix<3,4> (Reduction.cu:6-16)
+        double A[9], b[3];
+        int shift = 0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = i; j < 4; ++j) {
+                const double value = (double)h[shift++];
+                if (j == 3)
+                    b[i] = value;
+                else
+                    A[j * 3 + i] = A[i * 3 + j] = value;
+            }
+        if (A_host) std::memcpy(A_host, A, sizeof(A));
+        if (b_host) std::memcpy(b_host, b, sizeof(b));
+        if (residual_host) {
+            residual_host[0] = h[9];
+            residual_host[1] = h[10];
+        }
+    }
+    return SICP_OK;
+}
+
 // The photometric scratch: rows, records, counts, totals, keys.
 static size_t rgb_scratch_bytes(size_t n) {
     const size_t pr = (size_t)kIcpSums * kIcpMaxBlocks * sizeof(double);

@@ -187,13 +187,14 @@ SICP_DEV uint2 block_sum_u32x2(uint32_t a, uint32_t b) {
 // (j, k) then adds rows k, k+8, ... of column j and eight lanes fold with
 // three butterflies. Fixed order; much cheaper than folding 29 doubles
 // across 64 lanes with cross-lane permutes.
-SICP_DEV void block_rows_to_partials(const double acc[kIcpSums], double* partials) {
-    __shared__ double tile[kIcpThreads][kIcpSums + 1];
+template <int NS>
+SICP_DEV void block_rows_to_partials(const double (&acc)[NS], double* partials) {
+    __shared__ double tile[kIcpThreads][NS + 1];
 #pragma unroll
-    for (int j = 0; j < kIcpSums; ++j) tile[threadIdx.x][j] = acc[j];
+    for (int j = 0; j < NS; ++j) tile[threadIdx.x][j] = acc[j];
     __syncthreads();
     const int t = threadIdx.x;
-    if (t < kIcpSums * 8) {
+    if (t < NS * 8) {
         const int j = t >> 3, k = t & 7;
         double s = 0.0;
 #pragma unroll 8
@@ -353,11 +354,12 @@ SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
 
 // Sum of the partial rows of `nb` workgroups into totals (LDS), in a fixed
 // order: thread (j, k) adds rows k, k+8, ..., k+504 of column j (every load
-// issued before the first add), then eight lanes fold.
+// issued before the first add), then eight lanes fold. NS sums per row.
+template <int NS = kIcpSums>
 SICP_DEV void icp_totals_issue(const double* partials, int nb, double& s) {
     const int t = threadIdx.x;
     s = 0.0;
-    if (t < kIcpSums * 8) {
+    if (t < NS * 8) {
         const int j = t >> 3, k = t & 7;
         const double* col = partials + j * kIcpMaxBlocks;
         double v[kIcpMaxBlocks / 8];
@@ -370,17 +372,19 @@ SICP_DEV void icp_totals_issue(const double* partials, int nb, double& s) {
         for (int r = 0; r < kIcpMaxBlocks / 8; ++r) s += v[r];
     }
 }
+template <int NS = kIcpSums>
 SICP_DEV void icp_totals_fold(double s, double* totals) {
     const int t = threadIdx.x;
     s += __shfl_xor(s, 1, 64);
     s += __shfl_xor(s, 2, 64);
     s += __shfl_xor(s, 4, 64);
-    if (t < kIcpSums * 8 && (t & 7) == 0) totals[t >> 3] = s;
+    if (t < NS * 8 && (t & 7) == 0) totals[t >> 3] = s;
 }
+template <int NS = kIcpSums>
 SICP_DEV void icp_totals(const double* partials, int nb, double* totals) {
     double s;
-    icp_totals_issue(partials, nb, s);
-    icp_totals_fold(s, totals);
+    icp_totals_issue<NS>(partials, nb, s);
+    icp_totals_fold<NS>(s, totals);
     __syncthreads();
 }
 
@@ -395,12 +399,54 @@ __global__ __launch_bounds__(kIcpThreads) void icp_partials_kernel(
                           });
 }
 
+template <int NS = kIcpSums>
 __global__ __launch_bounds__(kIcpThreads) void icp_sums_finalize_kernel(const double* partials,
                                                                         int nb,
                                                                         float* sums_out) {
-    __shared__ double totals[kIcpSums];
-    icp_totals(partials, nb, totals);
-    if (threadIdx.x < kIcpSums) sums_out[threadIdx.x] = (float)totals[threadIdx.x];
+    __shared__ double totals[NS];
+    icp_totals<NS>(partials, nb, totals);
+    if (threadIdx.x < NS) sums_out[threadIdx.x] = (float)totals[threadIdx.x];
+}
+
+// SO3Reduce::GetRow (Tracking/Reduction.cu:96-131): the last frame's pixel
+// bearing rotated into the current camera, its rounded projection
+// (__float2int_rn), and the rotation-only photometric row
+// J = -(left x p), r = -(I_next(u, v) - I_last(x, y)) with the next frame's
+// gradient at (u, v); 10 products and the match flag.
+constexpr int kSo3Sums = 11;
+__global__ __launch_bounds__(kIcpThreads) void so3_partials_kernel(
+    View<uint8_t> inext, View<uint8_t> ilast, View<int16_t> dx, View<int16_t> dy, PoseArg pc,
+    PoseArg pl, IcpGeom g, double* partials) {
+    const PoseF cur = load_pose(pc), last = load_pose(pl);
+    double acc[kSo3Sums];
+#pragma unroll
+    for (int j = 0; j < kSo3Sums; ++j) acc[j] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x) {
+        const int y = i / g.cols, x = i - y * g.cols;
+        const v3 vl = v3{((float)x - g.cx) / g.fx, ((float)y - g.cy) / g.fy, 1.0f};
+        const v3 p = mv(cur.Rinv, mv(last.R, vl));
+        const int u = f2i(rintf(g.fx * p.x / p.z + g.cx));
+        const int v = f2i(rintf(g.fy * p.y / p.z + g.cy));
+        const bool found = u >= 5 && v >= 5 && u < g.cols - 5 && v < g.rows - 5 && x >= 5 &&
+                           y >= 5 && x < g.cols - 5 && y < g.rows - 5;
+        if (!found) continue;
+        const float gx = (float)dx.at(v, u) / 9.0f, gy = (float)dy.at(v, u) / 9.0f;
+        const float invz = 1.0f / p.z;
+        v3 left;
+        left.x = gx * g.fx * invz;
+        left.y = gy * g.fy * invz;
+        left.z = -(p.x * left.x + p.y * left.y) * invz;
+        const v3 c = cross(left, p);
+        const float row[4] = {-c.x, -c.y, -c.z,
+                              -((float)inext.at(v, u) - (float)ilast.at(y, x))};
+        int q = 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = a; b < 4; ++b) acc[q++] += (double)(row[a] * row[b]);
+        acc[10] += 1.0;
+    }
+    block_rows_to_partials(acc, partials);
 }
 
 SICP_DEV void copy16(double* d, const double* s) {
@@ -731,7 +777,7 @@ int launch_icp_sums(View<float4> vcurr, View<float4> vlast, View<float4> ncurr,
     icp_partials_kernel<<<nb, kIcpThreads, 0, s>>>(vcurr, vlast, ncurr, nlast, curr, last, g,
                                                      scratch.partials);
     SICP_LAUNCH_CHECK("icp_partials_kernel");
-    icp_sums_finalize_kernel<<<1, kIcpThreads, 0, s>>>(scratch.partials, nb, sums_out);
+    icp_sums_finalize_kernel<><<<1, kIcpThreads, 0, s>>>(scratch.partials, nb, sums_out);
     SICP_LAUNCH_CHECK("icp_sums_finalize_kernel");
     return SICP_OK;
 }
@@ -830,7 +876,20 @@ int launch_rgb_sums(View<uint8_t> inext, View<uint8_t> ilast, View<float4> vnext
     SICP_LAUNCH_CHECK("rgb_match_kernel");
     rgb_rows_kernel<<<nb, kIcpThreads, 0, s>>>(vlast, dx, dy, g, rgb, nb, nullptr, curr, last);
     SICP_LAUNCH_CHECK("rgb_rows_kernel");
-    icp_sums_finalize_kernel<<<1, kIcpThreads, 0, s>>>(rgb.partials, nb, sums_out);
+    icp_sums_finalize_kernel<><<<1, kIcpThreads, 0, s>>>(rgb.partials, nb, sums_out);
+    SICP_LAUNCH_CHECK("icp_sums_finalize_kernel");
+    return SICP_OK;
+}
+
+int launch_so3_sums(View<uint8_t> inext, View<uint8_t> ilast, View<int16_t> dx,
+                    View<int16_t> dy, PoseArg curr, PoseArg last, sicp_intrinsics K,
+                    IcpScratch scratch, float* sums_out, hipStream_t s) {
+    const IcpGeom g{inext.cols, inext.rows, inext.cols * inext.rows, K.fx, K.fy, K.cx, K.cy};
+    const int nb = icp_blocks(g.n);
+    so3_partials_kernel<<<nb, kIcpThreads, 0, s>>>(inext, ilast, dx, dy, curr, last, g,
+                                                     scratch.partials);
+    SICP_LAUNCH_CHECK("so3_partials_kernel");
+    icp_sums_finalize_kernel<kSo3Sums><<<1, kIcpThreads, 0, s>>>(scratch.partials, nb, sums_out);
     SICP_LAUNCH_CHECK("icp_sums_finalize_kernel");
     return SICP_OK;
 }

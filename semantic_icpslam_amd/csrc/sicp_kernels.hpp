@@ -145,6 +145,13 @@ int launch_rgb_sums(View<uint8_t> inext, View<uint8_t> ilast, View<float4> vnext
                     float* sums_out, hipStream_t s);
 int launch_track_init(FrameMeta* next, FrameMeta* last, TrackState* st, hipStream_t s);
 
+// SO3Step: the 11 sums of the rotation-only photometric alignment (the 6
+// upper entries of A and b interleaved row by row, sum r^2, count) into
+// `sums_out`.
+int launch_so3_sums(View<uint8_t> inext, View<uint8_t> ilast, View<int16_t> dx,
+                    View<int16_t> dy, PoseArg curr, PoseArg last, sicp_intrinsics K,
+                    IcpScratch scratch, float* sums_out, hipStream_t s);
+
 // Host FP64 solve + SE(3) update (same code path as the device).
 void se3_update_host(const double* A, const double* b, double* delta, const double* last_pose,
                      double* next_pose, double* x_out);

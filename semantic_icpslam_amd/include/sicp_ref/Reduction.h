@@ -111,6 +111,32 @@ inline void ICPStep(DeviceArray2D<float4>& nextVMap, DeviceArray2D<float4>& last
              "ICPStep");
 }
 
+// SO3Step (Reduction.h:67-72; Reduction.cu:162-211). `sum` is the
+// partial-row scratch (grown to sicp_icp_partials_size() bytes when
+// smaller), `out` the 11 device sums; on the host: matrixA_host (3x3
+// row-major, mirrored), vectorB_host[3], residual = {sum r^2, count}.
+inline void SO3Step(const DeviceArray2D<unsigned char>& nextImage,
+                    const DeviceArray2D<unsigned char>& lastImage,
+                    const DeviceArray2D<short>& dIdx, const DeviceArray2D<short>& dIdy,
+                    Matrix3f RcurrInv, Matrix3f Rlast, Intrinsics K, DeviceArray2D<float>& sum,
+                    DeviceArray<float>& out, float* residual, double* matrixA_host,
+                    double* vectorB_host) {
+    const size_t need = sicp_icp_partials_size();
+    if (!sum.data || sum.step * (size_t)sum.rows < need) {
+        const size_t row = 11 * sizeof(float);
+        sum.create(11, (int)((need + row - 1) / row));
+    }
+    if (!out.data || out.size < 11) out.create(11);
+    const float3 zero = make_float3(0, 0, 0);
+    const sicp_pose_f curr = sicp_pose_of(sicp_transpose(RcurrInv), RcurrInv, zero);
+    const sicp_pose_f last = sicp_pose_of(Rlast, sicp_transpose(Rlast), zero);
+    float host_sums[11];
+    SicpCall(sicp_so3_step(nextImage.image(), lastImage.image(), dIdx.image(), dIdy.image(), &curr,
+                           &last, K.abi(), sum.data, (float*)out.data, host_sums, matrixA_host,
+                           vectorB_host, residual, nullptr),
+             "SO3Step");
+}
+
 // RGBStep (Reduction.h:81-90; Reduction.cu:575-642). `sum` is the scratch
 // (grown to sicp_rgb_scratch_size(cols, rows) bytes when smaller), `out` the
 // 29 device sums, `outRes` the correspondence count and int sum of squared

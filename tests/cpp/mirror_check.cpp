@@ -131,6 +131,27 @@ int mirror_rgb(const uint8_t* nimg, const uint8_t* limg, const float* nvmap, con
     });
 }
 
+// Tracker::ComputeSO3's SO3Step call (Tracking.cc:339-350) through the
+// mirror.
+int mirror_so3(const uint8_t* nimg, const uint8_t* limg, const int16_t* dx, const int16_t* dy,
+               int cols, int rows, float fx, float fy, float cx, float cy,
+               const float* pose_curr, const float* pose_last, double* A, double* b,
+               float* residual) {
+    return guarded([&] {
+        DeviceArray2D<unsigned char> ni(cols, rows), li(cols, rows);
+        DeviceArray2D<short> gx(cols, rows), gy(cols, rows);
+        ni.upload(nimg, cols);
+        li.upload(limg, cols);
+        gx.upload(dx, 2 * cols);
+        gy.upload(dy, 2 * cols);
+        DeviceArray2D<float> sum;
+        DeviceArray<float> out;
+        Matrix3f Rc = mat(pose_curr), Rl = mat(pose_last);
+        SO3Step(ni, li, gx, gy, sicp_transpose(Rc), Rl, Intrinsics(fx, fy, cx, cy), sum, out,
+                residual, A, b);
+    });
+}
+
 // Mapping::FuseColor then Mapping::RayTrace (Mapping.cc:76-97,142-161)
 // through the mirror, for `n` frames of (raw depth, RGB, normals, labels,
 // pose {R[9], t[3]}) fused into a fresh map; then the rendering prepass and

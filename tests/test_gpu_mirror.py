@@ -140,6 +140,31 @@ def test_mirror_rgb_step(dev):
     assert abs(res[0] - os_[27]) <= 1e-6 * os_[27]
 
 
+def test_mirror_so3_step(dev):
+    c = CFG_SMALL
+    s = synth(c)
+    gs = []
+    for i in (0, 3):
+        _, rgb, _ = s.render(i)
+        gs.append(O.image_to_intensity(rgb))
+    g0, g1 = gs
+    dx1, dy1 = O.derivative_image(g1)
+    Tc, Tl = s.pose(3), np.eye(4)
+    A = np.zeros(9)
+    b = np.zeros(3)
+    res = np.zeros(2, np.float32)
+    f = _lib().mirror_so3
+    f.argtypes = [C.c_void_p] * 4 + [C.c_int, C.c_int] + [C.c_float] * 4 + [C.c_void_p] * 5
+    rc = f(_p(g1), _p(g0), _p(dx1), _p(dy1), c["cols"], c["rows"], c["f"], c["f"], c["cx"],
+           c["cy"], _p(_pose12(Tc)), _p(_pose12(Tl)), _p(A), _p(b), _p(res))
+    assert rc == 0
+    os_ = O.so3_sums(g1, g0, dx1, dy1, B.pose_f(Tc), B.pose_f(Tl), K(c))
+    Ao, bo = O.create_matrix3(os_)
+    assert res[1] == os_[10] and os_[10] > 500
+    assert np.abs(A.reshape(3, 3) - Ao).max() <= 1e-6 * np.abs(Ao).max()
+    assert np.abs(b - bo).max() <= 1e-6 * np.abs(bo).max()
+
+
 def test_mirror_fuse_and_raycast(dev):
     c = CFG_SMALL
     cfg = map_cfg(c)

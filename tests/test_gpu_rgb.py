@@ -1,5 +1,6 @@
-"""Parity of RGBStep (Tracking/Reduction.cu:387-642) and of tracking with the
-photometric term (ComputeSE3(false), Tracking.cc:369-517) against the oracle.
+"""Parity of RGBStep (Tracking/Reduction.cu:387-642), SO3Step (:67-211) and
+of tracking with the photometric term (ComputeSE3(false),
+Tracking.cc:369-517) against the oracle.
 
 The correspondence pass is integer work: its count and its int sum of
 squared differences must match exactly, and so must sigma (formed from them
@@ -123,6 +124,45 @@ def test_rgb_step_rejects_bad_arguments(dev):
                                small.data_ptr(), 1024, sums.data_ptr(), None, None, None, None,
                                None, None, B.stream_ptr())
     assert rc != 0
+
+
+def _gpu_so3(dev, ni, li, dx, dy, pc, pl, k, pad=0):
+    t = [to_dev(a, dev, pad_cols=pad) for a in (ni, li, dx, dy)]
+    partials = torch.zeros(B.lib().sicp_icp_partials_size(), dtype=torch.uint8, device=dev)
+    sums = torch.zeros(11, dtype=torch.float32, device=dev)
+    hs = np.zeros(11, np.float32)
+    A = np.zeros(9)
+    b = np.zeros(3)
+    res = np.zeros(2, np.float32)
+    p = lambda a: a.ctypes.data_as(B.C.c_void_p)
+    B.check(B.lib().sicp_so3_step(*[B.image(x) for x in t], B.C.byref(pc), B.C.byref(pl), k,
+                                  partials.data_ptr(), sums.data_ptr(), p(hs), p(A), p(b),
+                                  p(res), B.stream_ptr()))
+    return hs, A.reshape(3, 3), b, res
+
+
+@pytest.mark.parametrize("c", [CFG_SMALL, CFG_VGA])
+@pytest.mark.parametrize("level", [0, 2])
+def test_so3_step_vs_oracle(dev, c, level):
+    """ComputeSO3's call (level 2) and level 0: count exact, the other sums
+    within 1e-6 of the largest of their kind; pitched inputs give the same
+    sums."""
+    _, g0, _, _ = _frame(c, 0, level)
+    _, g1, dx1, dy1 = _frame(c, 3, level)
+    pc = B.pose_f(synth(c).pose(3))
+    pl = B.pose_f(np.eye(4))
+    hs, A, b, res = _gpu_so3(dev, g1, g0, dx1, dy1, pc, pl, K(c, level))
+    os_ = O.so3_sums(g1, g0, dx1, dy1, pc, pl, K(c, level))
+    assert hs[10] == os_[10] and os_[10] > 500
+    idxA, idxb = [0, 1, 2, 4, 5, 7], [3, 6, 8]
+    for idx in (idxA, idxb, [9]):
+        scale = np.abs(os_[idx]).max() + 1e-30
+        assert np.abs(hs[idx].astype(np.float64) - os_[idx]).max() / scale < REL, idx
+    Ao, bo = O.create_matrix3(hs)
+    assert np.array_equal(A, Ao) and np.array_equal(b, bo)
+    assert res[0] == hs[9] and res[1] == hs[10]
+    hp = _gpu_so3(dev, g1, g0, dx1, dy1, pc, pl, K(c, level), pad=40)[0]
+    assert np.array_equal(hs, hp)
 
 
 def _rgb_cfg(c):

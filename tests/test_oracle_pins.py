@@ -364,6 +364,106 @@ def test_rgb_gauss_newton_moves_towards_known_motion():
     assert sq[-1] < 0.2 * sq[0]
 
 
+def _skew(w):
+    return np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+
+
+def _rot_pose(R):
+    T = np.eye(4)
+    T[:3, :3] = R
+    return B.pose_f(T)
+
+
+def _so3_A_textbook(img_dx, img_dy, rows, cols, f, cx, cy, offset_r=None):
+    """sum J^T J (and sum J r) of the rotation-only photometric Jacobian
+    J = grad(I) . d(pi)/dp . (-[p]x) at every interior pixel (identity
+    rotations: the match is the pixel itself, p its bearing)."""
+    A = np.zeros((3, 3))
+    Jsum = np.zeros(3)
+    for y in range(5, rows - 5):
+        for x in range(5, cols - 5):
+            p = np.array([(x - cx) / f, (y - cy) / f, 1.0])
+            g = np.array([img_dx[y, x], img_dy[y, x]], np.float64) / 9.0
+            Jpi = np.array([[f / p[2], 0, -f * p[0] / p[2] ** 2],
+                            [0, f / p[2], -f * p[1] / p[2] ** 2]])
+            J = g @ Jpi @ (-_skew(p))
+            A += np.outer(J, J)
+            Jsum += J
+    return A, Jsum
+
+
+def test_so3_identical_frames_textbook_jacobian():
+    """Identity rotations on identical images: every interior pixel (5-pixel
+    border) matches itself by the rounded projection, r = 0, and A is the
+    textbook rotation-only Jacobian summed in FP64."""
+    rng = np.random.default_rng(21)
+    _, img, dx, dy = _plane(rng)
+    rows, cols = img.shape
+    s = O.so3_sums(img, img, dx, dy, _rot_pose(np.eye(3)), _rot_pose(np.eye(3)), _plane_K())
+    assert s[10] == (rows - 10) * (cols - 10)
+    A, b = O.create_matrix3(s)
+    assert s[9] == 0.0 and np.all(b == 0.0)
+    At, _ = _so3_A_textbook(dx, dy, rows, cols, PLANE["f"], PLANE["cx"], PLANE["cy"])
+    assert np.abs(A - At).max() < 1e-5 * np.abs(At).max()
+
+
+def test_so3_constant_offset_residual():
+    """The last image brighter by 10: r = -(I_next - I_last) = 10 at every
+    match, so sum r^2 = 100 n and b = 10 sum(J)."""
+    rng = np.random.default_rng(22)
+    _, img, dx, dy = _plane(rng)
+    rows, cols = img.shape
+    last = (img.astype(np.int32) + 10).astype(np.uint8)
+    s = O.so3_sums(img, last, dx, dy, _rot_pose(np.eye(3)), _rot_pose(np.eye(3)), _plane_K())
+    n = (rows - 10) * (cols - 10)
+    assert s[10] == n and abs(s[9] - 100 * n) < 1e-9 * n
+    _, b = O.create_matrix3(s)
+    _, Jsum = _so3_A_textbook(dx, dy, rows, cols, PLANE["f"], PLANE["cx"], PLANE["cy"])
+    assert np.abs(b - 10 * Jsum).max() < 1e-5 * np.abs(10 * Jsum).max()
+
+
+def test_so3_rounded_projection_and_pixel_roles():
+    """A yaw of 0.6 pixel (at the centre; up to 0.8 at the edges) moves every
+    match one column right under round-to-nearest (truncation would keep it):
+    with the last image shifted one column to match, every residual
+    -(I_next(u, v) - I_last(x, y)) is 0, and the count loses the one column
+    whose match leaves the border."""
+    rng = np.random.default_rng(23)
+    _, img, dx, dy = _plane(rng)
+    rows, cols = img.shape
+    Rc = scipy.linalg.expm(_skew([0, -0.6 / PLANE["f"], 0]))
+    last = np.zeros_like(img)
+    last[:, :-1] = img[:, 1:]
+    s = O.so3_sums(img, last, dx, dy, _rot_pose(Rc), _rot_pose(np.eye(3)), _plane_K())
+    assert s[10] == (rows - 10) * (cols - 11) and s[9] == 0.0
+
+
+def test_so3_step_recovers_known_rotation():
+    """Images that depend only on the viewing direction (a pure rotation is
+    an exact warp): one Gauss-Newton step of ComputeSO3's update
+    (delta = exp(x) delta, next = last delta^-1) removes most of a known
+    rotation, which pins the Jacobian's sign against the update."""
+    rows, cols, f, cx, cy = 120, 160, 131.25, 79.5, 59.5
+
+    def render(R):
+        ys, xs = np.mgrid[0:rows, 0:cols].astype(np.float64)
+        d = np.stack([(xs - cx) / f, (ys - cy) / f, np.ones_like(xs)], -1) @ R.T
+        d /= np.linalg.norm(d, axis=-1, keepdims=True)
+        i = 128 + 50 * np.sin(7 * d[..., 0]) * np.cos(5 * d[..., 1]) + \
+            30 * np.sin(4 * d[..., 1] + 2 * d[..., 0])
+        return np.clip(np.round(i), 1, 255).astype(np.uint8)
+
+    Rc = scipy.linalg.expm(_skew([0.01, -0.015, 0.008]))
+    last, nxt = render(np.eye(3)), render(Rc)
+    dx, dy = O.derivative_image(nxt)
+    s = O.so3_sums(nxt, last, dx, dy, _rot_pose(np.eye(3)), _rot_pose(np.eye(3)),
+                   B.intrinsics(f, f, cx, cy))
+    A, b = O.create_matrix3(s)
+    delta = scipy.linalg.expm(_skew(np.linalg.solve(A, b)))
+    nxt_rot = np.linalg.inv(delta)
+    assert np.abs(nxt_rot - Rc).max() < 0.25 * np.abs(np.eye(3) - Rc).max()
+
+
 def test_solve_matches_numpy():
     rng = np.random.default_rng(4)
     for _ in range(20):
