@@ -294,6 +294,8 @@ def main():
     ap.add_argument("--no-pose-check", action="store_true")
     ap.add_argument("--no-photometric", action="store_true",
                     help="skip the pass with the photometric tracking term")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the graph-form pass (counter collection runs)")
     ap.add_argument("--workload", default="vga", choices=("vga", "batched8"),
                     help="vga: the 640x480 / 5 mm sequence (default); batched8: eight "
                          "independent sequences at once")
@@ -384,8 +386,10 @@ def main():
     # (stage times, per-frame percentiles, per-kernel rooflines).
     log(f"[bench] warmup {args.warmup} + timing {args.steps} frames")
     clean = run_pass(False)
-    log("[bench] graph pass (staged inputs, one hipGraph launch per frame)")
-    graphed = run_pass(False, graph=True)
+    graphed = None
+    if not args.no_graph:
+        log("[bench] graph pass (staged inputs, one hipGraph launch per frame)")
+        graphed = run_pass(False, graph=True)
     photometric = None
     if not args.no_photometric:
         log("[bench] photometric pass (ComputeSE3(false): RGBStep + ICPStep per iteration)")
@@ -482,7 +486,7 @@ def main():
         "last_frame_tracked": res["tracked"],
         "last_frame_translation_vs_ground_truth_m": round(
             float(np.linalg.norm(res["pose"][:3, 3] - gt[:3, 3])), 5),
-        "graph_form": {
+        "graph_form": None if graphed is None else {
             "ms_per_step": round(1000.0 * graphed["elapsed"] / args.steps, 4),
             "what": "the same frames, each copied into the system's staged inputs and run as "
                     "one hipGraph launch (plain FP32)",
