@@ -168,6 +168,33 @@ int sicp_compute_derivative_image(sicp_image gray_u8, sicp_image dx_i16, sicp_im
 int sicp_resize_map(sicp_image vsrc, sicp_image nsrc, sicp_image vdst, sicp_image ndst,
                     void* stream);
 
+/* ---- Mesh extraction (Mapping/MeshScene.cu:4-356) ------------------------ */
+
+/* MeshScene + Mapping::CreateModel (MeshScene.cu:308-356, Mapping.cc:183-192):
+ * marching cubes over every allocated block. A voxel cell makes triangles
+ * when its eight corners are neither empty (NaN) nor exactly 1 and the
+ * voxels either side of every corner along each axis exist. Per triangle,
+ * three vertices of `vertices_dev` / `normals_dev` (float x, y, z each, in
+ * metres / unit length) and `colors_dev` (uchar r, g, b each). The order is
+ * canonical: heap slot, voxel z*64 + y*8 + x, triangle table order.
+ * Triangles past `max_triangles` are dropped (MaxTriangles,
+ * DeviceMap.h:181); `n_triangles` gets the number written and `n_total`
+ * (may be NULL) the number found. With max_triangles = 0 the call only
+ * counts. Synchronises `stream`. */
+int sicp_mesh_scene(sicp_map* map, float* vertices_dev, float* normals_dev, uint8_t* colors_dev,
+                    uint32_t max_triangles, uint32_t* n_triangles, uint32_t* n_total,
+                    void* stream);
+/* The three tables MeshScene takes (edge, triangle and vertex-count tables,
+ * Mapping/Constant.h; Mapping.cc builds them): edge_table[256] (bit j: edge j
+ * cut), triangle_table[256][16] (edge triples, -1 terminated),
+ * vertex_table[256] (3 x triangles). Host copies; any may be NULL. */
+int sicp_mesh_tables(int32_t* edge_table, int8_t* triangle_table, int32_t* vertex_table);
+/* System::WriteMeshToDisk (Core/System.cc:272-325): ASCII PLY with one
+ * vertex per triangle corner (x y z nx ny nz r g b) and one face per
+ * triangle, from host arrays as sicp_mesh_scene writes them. */
+int sicp_mesh_write_ply(const char* path, const float* vertices, const float* normals,
+                        const uint8_t* colors, uint32_t n_triangles);
+
 /* ---- ICP reduction (Tracking/Reduction.h:74-79) -------------------------- */
 
 /* ICPStep (Tracking/Reduction.cu:328-385). Computes the 29 sums (27 upper

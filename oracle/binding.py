@@ -64,6 +64,7 @@ _SIGS = {
     "oracle_map_get_stats": (None, [_P, C.POINTER(MapStats)]),
     "oracle_map_export": (None, [_P, _P, _P, _P, _P, _P]),
     "oracle_map_visible": (C.c_uint32, [_P, _P, C.c_uint32]),
+    "oracle_mesh_scene": (C.c_uint32, [_P, _P, _P, _P, C.c_uint32, _P, _P, _P, _P]),
     "oracle_map_counters": (None, [_P, _P]),
     "oracle_system_create": (_P, [C.POINTER(SystemConfig), C.c_int]),
     "oracle_system_destroy": (None, [_P]),
@@ -310,6 +311,20 @@ class OracleMap:
         m = _c(mask, np.uint8) if mask is not None else None
         return lib().oracle_fuse_map_color(self.h, _p(d), _p(r), _p(n), _p(m), cols, rows,
                                            num_detection, C.byref(pose), K)
+
+    def mesh(self, tables, max_tris):
+        """MeshScene with the given (edge, tri, nvert) tables: (vertices,
+        normals, colours) of the triangles written, and the number found."""
+        edge, tri, nvert = (np.ascontiguousarray(tables[0], np.int32),
+                            np.ascontiguousarray(tables[1], np.int8),
+                            np.ascontiguousarray(tables[2], np.int32))
+        v = np.zeros((max_tris, 3, 3), np.float32)
+        n = np.zeros((max_tris, 3, 3), np.float32)
+        c = np.zeros((max_tris, 3, 3), np.uint8)
+        tot = np.zeros(1, np.uint32)
+        k = lib().oracle_mesh_scene(self.h, _p(edge), _p(tri), _p(nvert), max_tris, _p(v), _p(n),
+                                    _p(c), _p(tot))
+        return v[:k], n[:k], c[:k], int(tot[0])
 
     def visibility(self, pose, cols, rows, K, dmax, dmin):
         return lib().oracle_check_visibility(self.h, C.byref(pose), cols, rows, K, dmax, dmin)

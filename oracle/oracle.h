@@ -100,6 +100,12 @@ void oracle_map_get_stats(void* map, sicp_map_stats* out);
 void oracle_map_export(void* map, sicp_hash_entry* table, sicp_hash_entry* excess,
                        sicp_block* blocks, sicp_voxel_geom* geom, sicp_voxel_sem* sem);
 uint32_t oracle_map_visible(void* map, sicp_block* out, uint32_t cap);
+/* MeshScene (Mapping/MeshScene.cu:4-356) on an FP32 map, with the given
+ * tables (edge[256], tri[256][16], nvert[256]); returns the triangles
+ * written (at most max_tris); `total` gets the number found. */
+uint32_t oracle_mesh_scene(void* map, const int32_t* edge, const int8_t* tri,
+                           const int32_t* nvert, uint32_t max_tris, float* vertices,
+                           float* normals, uint8_t* colors, uint32_t* total);
 /* Counters of the last passes: table probes of the allocation, block
  * lookups and march steps of the raycast, and the raycast outcomes (no depth
  * range, back-face exit, no zero crossing, normal rejected, surface found). */

@@ -786,6 +786,123 @@ struct OracleMapHandle {
     OMap<double> d;
 };
 
+// ------------------------------------------------------------ meshing -----
+// MeshEngine (Mapping/MeshScene.cu:4-305) run serially: every allocated
+// block in heap-slot order, its voxels in index order z*64 + y*8 + x, each
+// voxel's triangles in table order, written until max_tris. The tables are
+// inputs, as they are arguments of the reference's MeshScene.
+namespace {
+struct MeshVox {
+    float sdf;
+    uint8_t c[3];
+};
+// FindVoxel(int3) (DeviceMap.cu:97-103): the default voxel (NaN sdf, black)
+// when the block is missing.
+MeshVox mesh_voxel(const OMap<float>& m, int x, int y, int z) {
+    I3 v{x, y, z};
+    I3 bp = voxel_to_block(v);
+    const int slot = map_lookup(m, bp.x, bp.y, bp.z);
+    if (slot < 0) return MeshVox{std::nanf(""), {0, 0, 0}};
+    const size_t i = (size_t)slot * kBS3 + voxel_local_idx(v);
+    return MeshVox{m.sdf[i], {m.geom[i].r, m.geom[i].g, m.geom[i].b}};
+}
+// readVertexAndColor's corner order (MeshScene.cu:120-152).
+const int kMC[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                       {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+// buildVertexList's edges as (from, to) corners (MeshScene.cu:200-291).
+const int kME[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                        {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+int mesh_corner(int ox, int oy, int oz) {
+    for (int k = 0; k < 8; ++k)
+        if (kMC[k][0] == ox && kMC[k][1] == oy && kMC[k][2] == oz) return k;
+    return -1;
+}
+// MeshEngine::interp (MeshScene.cu:156-164).
+float mesh_interp(float v1, float v2) {
+    if (std::fabs(0 - v1) < 1e-6) return 0;
+    if (std::fabs(0 - v2) < 1e-6) return 1;
+    if (std::fabs(v1 - v2) < 1e-6) return 0;
+    return (0 - v1) / (v2 - v1);
+}
+}  // namespace
+
+uint32_t mesh_scene(const OMap<float>& m, const int32_t* edge, const int8_t* tri,
+                    const int32_t* nvert, uint32_t max_tris, float* V, float* N, uint8_t* Cout,
+                    uint32_t* total) {
+    uint32_t nt = 0;
+    for (uint32_t b = 0; b < m.heap_used; ++b) {
+        const sicp_block& blk = m.blocks[b];
+        for (int li = 0; li < kBS3; ++li) {
+            const int lx = li & 7, ly = (li >> 3) & 7, lz = li >> 6;
+            const int px = blk.x * kBS + lx, py = blk.y * kBS + ly, pz = blk.z * kBS + lz;
+            // readVertexAndColor (:117-153)
+            float sdf[8];
+            uint8_t col[8][3];
+            bool ok = true;
+            for (int k = 0; k < 8 && ok; ++k) {
+                MeshVox v = mesh_voxel(m, px + kMC[k][0], py + kMC[k][1], pz + kMC[k][2]);
+                sdf[k] = v.sdf;
+                std::memcpy(col[k], v.c, 3);
+                ok = !(sdf[k] == 1.0 || std::isnan(sdf[k]));
+            }
+            if (!ok) continue;
+            // readNormal (:39-115): central differences at every corner
+            V3<float> nrm[8];
+            for (int k = 0; k < 8 && ok; ++k) {
+                float g[3];
+                for (int a = 0; a < 3 && ok; ++a) {
+                    int o[3] = {kMC[k][0], kMC[k][1], kMC[k][2]};
+                    int q[3] = {o[0], o[1], o[2]};
+                    q[a] = o[a] ? 2 : -1;
+                    const float far = mesh_voxel(m, px + q[0], py + q[1], pz + q[2]).sdf;
+                    if (std::isnan(far)) ok = false;
+                    int r[3] = {o[0], o[1], o[2]};
+                    r[a] = 1 - o[a];
+                    const float near = sdf[mesh_corner(r[0], r[1], r[2])];
+                    g[a] = o[a] ? far - near : near - far;
+                }
+                nrm[k] = V3<float>{g[0], g[1], g[2]};
+            }
+            if (!ok) continue;
+            int cube = 0;
+            for (int k = 0; k < 8; ++k)
+                if (sdf[k] < 0) cube |= 1 << k;
+            if (edge[cube] == 0 || cube == 0) continue;
+            const int ntri = nvert[cube] / 3;
+            for (int j = 0; j < ntri; ++j, ++nt) {
+                if (nt >= max_tris) continue;
+                for (int v = 0; v < 3; ++v) {
+                    const int e = tri[cube * 16 + 3 * j + v];
+                    const int a = kME[e][0], bb = kME[e][1];
+                    const float val = mesh_interp(sdf[a], sdf[bb]);
+                    const int P[3] = {px, py, pz};
+                    const size_t o = ((size_t)nt * 3 + v) * 3;
+                    for (int ax = 0; ax < 3; ++ax) {
+                        const int ca = kMC[a][ax], cb = kMC[bb][ax];
+                        const float comp = ca == cb ? (float)ca : (ca == 0 ? val : 1 - val);
+                        V[o + ax] = ((float)P[ax] + comp) * m.cfg.voxel_size;
+                    }
+                    const V3<float> na = nrm[a], nb = nrm[bb];
+                    V3<float> n{na.x + val * (nb.x - na.x), na.y + val * (nb.y - na.y),
+                                na.z + val * (nb.z - na.z)};
+                    const float len = std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+                    N[o + 0] = n.x / len;
+                    N[o + 1] = n.y / len;
+                    N[o + 2] = n.z / len;
+                    // uchar3 + (int)val * (uchar3 - uchar3), narrowing mod 256
+                    const int iv = (int)val;
+                    for (int ch = 0; ch < 3; ++ch) {
+                        const uint8_t d = (uint8_t)(col[bb][ch] - col[a][ch]);
+                        Cout[o + ch] = (uint8_t)(col[a][ch] + (uint8_t)(d * iv));
+                    }
+                }
+            }
+        }
+    }
+    if (total) *total = nt;
+    return nt < max_tris ? nt : max_tris;
+}
+
 extern "C" {
 
 void* oracle_map_create(const sicp_map_config* cfg, int fp64) {
@@ -880,6 +997,13 @@ void oracle_map_export(void* map, sicp_hash_entry* table, sicp_hash_entry* exces
         go(h->d);
     else
         go(h->f);
+}
+
+uint32_t oracle_mesh_scene(void* map, const int32_t* edge, const int8_t* tri,
+                           const int32_t* nvert, uint32_t max_tris, float* vertices,
+                           float* normals, uint8_t* colors, uint32_t* total) {
+    OracleMapHandle* h = (OracleMapHandle*)map;
+    return mesh_scene(h->f, edge, tri, nvert, max_tris, vertices, normals, colors, total);
 }
 
 uint32_t oracle_map_visible(void* map, sicp_block* out, uint32_t cap) {

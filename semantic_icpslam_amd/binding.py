@@ -113,6 +113,9 @@ _SIGS = {
     "sicp_rgb_scratch_size": (C.c_size_t, [C.c_int32, C.c_int32]),
     "sicp_so3_step": (C.c_int, [Image, Image, Image, Image, C.POINTER(PoseF), C.POINTER(PoseF),
                                 Intrinsics, _P, _P, _P, _P, _P, _P, _P]),
+    "sicp_mesh_scene": (C.c_int, [_P, _P, _P, _P, C.c_uint32, _P, _P, _P]),
+    "sicp_mesh_tables": (C.c_int, [_P, _P, _P]),
+    "sicp_mesh_write_ply": (C.c_int, [C.c_char_p, _P, _P, _P, C.c_uint32]),
     "sicp_icp_partials_size": (C.c_size_t, []),
     "sicp_se3_update_host": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "sicp_map_config_default": (None, [C.POINTER(MapConfig), C.c_float, C.c_float, C.c_int32,
@@ -164,6 +167,17 @@ EXPORTED = tuple(_SIGS.keys())
 
 
 _hip = None
+
+
+def mesh_tables():
+    """The marching-cubes tables sicp_mesh_scene uses: edge (256,),
+    triangle (256, 16), vertex count (256,). Host only."""
+    edge = np.zeros(256, np.int32)
+    tri = np.zeros((256, 16), np.int8)
+    nvert = np.zeros(256, np.int32)
+    check(lib().sicp_mesh_tables(edge.ctypes.data, tri.ctypes.data, nvert.ctypes.data),
+          "sicp_mesh_tables")
+    return edge, tri, nvert
 
 
 def copy2d(dst, src, stream=None):
@@ -296,6 +310,20 @@ class Map:
 
     def stats(self, stream=None):
         return map_stats(self.h, stream)
+
+    def mesh(self, max_triangles, stream=None):
+        """MeshScene into device tensors: (vertices, normals, colours) of
+        shape (n, 3, 3) on the current device, and the number found."""
+        import torch
+        dev = torch.device("cuda", torch.cuda.current_device())
+        v = torch.empty((max(max_triangles, 1), 3, 3), dtype=torch.float32, device=dev)
+        n = torch.empty_like(v)
+        c = torch.empty((max(max_triangles, 1), 3, 3), dtype=torch.uint8, device=dev)
+        k, tot = C.c_uint32(0), C.c_uint32(0)
+        check(lib().sicp_mesh_scene(self.h, v.data_ptr(), n.data_ptr(), c.data_ptr(),
+                                    max_triangles, C.byref(k), C.byref(tot), stream_ptr(stream)),
+              "sicp_mesh_scene")
+        return v[: k.value], n[: k.value], c[: k.value], tot.value
 
     def download(self, n_blocks=0, voxels=True):
         cfg = self.cfg
