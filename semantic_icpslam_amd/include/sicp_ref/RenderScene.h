@@ -99,6 +99,38 @@ inline void CheckBlockVisibility(DeviceMap map, DeviceArray<uint>& noVisibleBloc
     sicp_ref_detail::visible_status(rc, "CheckBlockVisibility");
 }
 
+// MeshScene (RenderScene.h:29-38; MeshScene.cu:308-356). The three table
+// arguments are accepted and not read: the library meshes with its own
+// tables (sicp_mesh_tables), which give the same vertices, normals and
+// colours and their own triangle connectivity. `vertex` / `normal` /
+// `color` hold three entries per triangle; the smallest of them bounds the
+// triangles written (the reference's MaxTriangles). `noOccupiedBlocks` gets
+// the allocated blocks, `noTotalTriangles` the triangles written; no block
+// list is needed (`blockPoses`). Returns the triangles written.
+inline uint MeshScene(DeviceArray<uint>& noOccupiedBlocks, DeviceArray<uint>& noTotalTriangles,
+                      DeviceMap map, const DeviceArray<int>& edgeTable,
+                      const DeviceArray<int>& vertexTable, const DeviceArray2D<int>& triangleTable,
+                      DeviceArray<float3>& normal, DeviceArray<float3>& vertex,
+                      DeviceArray<uchar3>& color, DeviceArray<int3>& blockPoses) {
+    (void)edgeTable;
+    (void)vertexTable;
+    (void)triangleTable;
+    (void)blockPoses;
+    size_t cap = vertex.size;
+    if (normal.size < cap) cap = normal.size;
+    if (color.size < cap) cap = color.size;
+    const uint32_t max_tris = (uint32_t)(cap / 3);
+    uint32_t n = 0;
+    SicpCall(sicp_mesh_scene(map.get(), (float*)vertex.data, (float*)normal.data,
+                             (uint8_t*)color.data, max_tris, &n, nullptr, nullptr),
+             "MeshScene");
+    sicp_map_stats st;
+    SicpCall(sicp_map_get_stats(map.get(), &st, nullptr), "MeshScene");
+    sicp_ref_detail::store_count(noOccupiedBlocks, st.allocated_blocks);
+    sicp_ref_detail::store_count(noTotalTriangles, n);
+    return n;
+}
+
 // FuseMapColor (RenderScene.h:45-55; FuseMap.cu:1039-1162).
 inline void FuseMapColor(const DeviceArray2D<float>& depth, const DeviceArray2D<uchar3>& color,
                          const DeviceArray2D<float4>& nmap, const DeviceArray2D<unsigned char>& mask,

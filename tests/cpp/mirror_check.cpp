@@ -161,7 +161,8 @@ int mirror_map(const sicp_map_config* cfg, int n, const float* depth, const uint
                const float* nmap_in, const uint8_t* mask, const float* poses, int cols, int rows,
                float fx, float fy, float cx, float cy, uint32_t* visible_out,
                uint32_t* check_visible, int* rendered, float* zmin, float* zmax, float* vmap,
-               float* nmap) {
+               float* nmap, uint32_t max_tris, float* mesh_v, uint8_t* mesh_c,
+               uint32_t* n_tris) {
     return guarded([&] {
         DeviceMap map(*cfg);
         ResetMap(map);
@@ -207,6 +208,17 @@ int mirror_map(const sicp_map_config* cfg, int n, const float* depth, const uint
         Raycast(map, vm, nv, zx, zy, R, Rinv, t, (float)(1.0 / fx), (float)(1.0 / fy), cx, cy);
         vm.download(vmap, 16 * cols);
         nv.download(nmap, 16 * cols);
+        // System::WriteMeshToDisk's CreateModel (Mapping.cc:183-192)
+        DeviceArray<uint> nOcc(1), nTri(1);
+        DeviceArray<int> edgeTable, vertexTable;
+        DeviceArray2D<int> triangleTable;
+        DeviceArray<float3> mv(3 * max_tris), mn(3 * max_tris);
+        DeviceArray<uchar3> mc(3 * max_tris);
+        DeviceArray<int3> blockPoses;
+        *n_tris = MeshScene(nOcc, nTri, map, edgeTable, vertexTable, triangleTable, mn, mv, mc,
+                            blockPoses);
+        mv.download(mesh_v, 3 * (size_t)*n_tris);
+        mc.download(mesh_c, 3 * (size_t)*n_tris);
     });
 }
 

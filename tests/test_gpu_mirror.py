@@ -190,13 +190,22 @@ def test_mirror_fuse_and_raycast(dev):
     zmax = np.zeros((zr, zc), np.float32)
     vm = np.zeros((rows, cols, 4), np.float32)
     nm = np.zeros((rows, cols, 4), np.float32)
+    max_tris = 400000
+    mesh_v = np.zeros((max_tris, 3, 3), np.float32)
+    mesh_c = np.zeros((max_tris, 3, 3), np.uint8)
+    ntris = C.c_uint32(0)
     f = _lib().mirror_map
     f.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 5 + [C.c_int, C.c_int] + \
-        [C.c_float] * 4 + [C.c_void_p] * 7
+        [C.c_float] * 4 + [C.c_void_p] * 7 + [C.c_uint32] + [C.c_void_p] * 3
     rc = f(C.byref(cfg), len(frames), _p(stack(0)), _p(stack(1)), _p(stack(2)), _p(stack(3)),
            _p(stack(4)), cols, rows, c["f"], c["f"], c["cx"], c["cy"], _p(vis), C.byref(chk),
-           C.byref(rendered), _p(zmin), _p(zmax), _p(vm), _p(nm))
+           C.byref(rendered), _p(zmin), _p(zmax), _p(vm), _p(nm), max_tris, _p(mesh_v),
+           _p(mesh_c), C.byref(ntris))
     assert rc == 0
+    ov_, _, oc_, ototal = om.mesh(B.mesh_tables(), max_tris)
+    assert ntris.value == ototal == len(ov_) and ototal > 1000
+    assert bits_equal(mesh_v[: ntris.value], ov_)
+    assert np.array_equal(mesh_c[: ntris.value], oc_)
     assert list(vis) == ovis
     pose = B.pose_f(s.pose(frames[-1]))
     assert chk.value == om.visibility(pose, cols, rows, K(c), 3.0, 0.1)
