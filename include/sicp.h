@@ -111,6 +111,10 @@ typedef struct sicp_map_config {
     int32_t request_capacity; /* per-frame new-block request set, pow2     */
     int32_t max_cols;       /* largest image width the map is used with     */
     int32_t max_rows;       /* largest image height                         */
+    int32_t unify_labels;   /* 1: every fusion with labels is followed by the
+                               neighbour label pass (UnifyNeighbor, as the
+                               reference's FuseMapColor runs it,
+                               FuseMap.cu:1140-1160); 0 (default): not run */
 } sicp_map_config;
 
 /* Fills `cfg` with the reference's constants scaled to the given voxel size
@@ -167,6 +171,16 @@ int sicp_compute_derivative_image(sicp_image gray_u8, sicp_image dx_i16, sicp_im
 /* ResizeMap (Pyrdown.cu:318-358): 2x2 mean of vertex / normal maps. */
 int sicp_resize_map(sicp_image vsrc, sicp_image nsrc, sicp_image vdst, sicp_image ndst,
                     void* stream);
+
+/* UnifyNeighbor (Fusion::UnifyNeighbor / UpdateNeighborLabel,
+ * Mapping/FuseMap.cu:757-948): every labelled voxel of the map's visible
+ * blocks takes the most frequent label among its 26 neighbours (those with
+ * a non-zero count), keeping its previous label and count as the secondary
+ * pair when the label changes. Deterministic: every voxel sees the labels of
+ * the state before the pass. Runs on the map's last visible list; `stream`
+ * ordered. Device scratch of num_blocks x 2 KiB is allocated on the first
+ * call unless the map was created with unify_labels = 1. */
+int sicp_unify_neighbor_labels(sicp_map* map, void* stream);
 
 /* ---- Mesh extraction (Mapping/MeshScene.cu:4-356) ------------------------ */
 

@@ -65,6 +65,8 @@ _SIGS = {
     "oracle_map_export": (None, [_P, _P, _P, _P, _P, _P]),
     "oracle_map_visible": (C.c_uint32, [_P, _P, C.c_uint32]),
     "oracle_mesh_scene": (C.c_uint32, [_P, _P, _P, _P, C.c_uint32, _P, _P, _P, _P]),
+    "oracle_unify_neighbor": (None, [_P]),
+    "oracle_map_set_label": (None, [_P] + [C.c_int] * 7),
     "oracle_map_counters": (None, [_P, _P]),
     "oracle_system_create": (_P, [C.POINTER(SystemConfig), C.c_int]),
     "oracle_system_destroy": (None, [_P]),
@@ -311,6 +313,12 @@ class OracleMap:
         m = _c(mask, np.uint8) if mask is not None else None
         return lib().oracle_fuse_map_color(self.h, _p(d), _p(r), _p(n), _p(m), cols, rows,
                                            num_detection, C.byref(pose), K)
+
+    def unify_neighbor(self):
+        lib().oracle_unify_neighbor(self.h)
+
+    def set_label(self, slot, idx, weight, label, count, label2=0, count2=0):
+        lib().oracle_map_set_label(self.h, slot, idx, weight, label, count, label2, count2)
 
     def mesh(self, tables, max_tris):
         """MeshScene with the given (edge, tri, nvert) tables: (vertices,

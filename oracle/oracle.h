@@ -100,6 +100,12 @@ void oracle_map_get_stats(void* map, sicp_map_stats* out);
 void oracle_map_export(void* map, sicp_hash_entry* table, sicp_hash_entry* excess,
                        sicp_block* blocks, sicp_voxel_geom* geom, sicp_voxel_sem* sem);
 uint32_t oracle_map_visible(void* map, sicp_block* out, uint32_t cap);
+/* UnifyNeighbor (Mapping/FuseMap.cu:757-948) over the last visible list,
+ * every voxel seeing the labels before the pass. */
+void oracle_unify_neighbor(void* map);
+/* Test hook: one voxel's weight and label state on an FP32 map. */
+void oracle_map_set_label(void* map, int slot, int idx, int weight, int label, int count,
+                          int label2, int count2);
 /* MeshScene (Mapping/MeshScene.cu:4-356) on an FP32 map, with the given
  * tables (edge[256], tri[256][16], nvert[256]); returns the triangles
  * written (at most max_tris); `total` gets the number found. */

@@ -95,6 +95,7 @@ uint32_t hash_bucket(int x, int y, int z, uint32_t nb);
 template <class R> void map_init(OMap<R>& m, const sicp_map_config& cfg);
 template <class R> void map_reset(OMap<R>& m);
 template <class R> int map_lookup(const OMap<R>& m, int x, int y, int z);
+template <class R> void unify_neighbor(OMap<R>& m);
 template <class R>
 uint32_t fuse_map_color(OMap<R>& m, const R* depth, const uint8_t* rgb, const R* nmap,
                         const uint8_t* mask, int cols, int rows, int num_detection,

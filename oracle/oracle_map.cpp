@@ -453,6 +453,85 @@ static uint32_t fuse_voxels(OMap<R>& m, const R* depth, const uint8_t* rgb, cons
     return (uint32_t)updated;
 }
 
+// Fusion::UnifyNeighbor + UpdateNeighborLabel (FuseMap.cu:757-948) over the
+// visible blocks, serially. Every voxel reads its neighbours' labels as they
+// were before the pass (the reference reads them while other threads write
+// them). Per column (x, y) of a block, the neighbour tallies carry over from
+// one z to the next, as the reference's per-thread arrays do.
+template <class R> void unify_neighbor(OMap<R>& m) {
+    // the 26 neighbour steps in the order of FuseMap.cu:838-892
+    static const int kN[26][3] = {
+        {1, 0, 0},   {1, 1, 0},   {1, -1, 0},  {1, 0, 1},   {1, 0, -1},   {1, 1, 1},  {1, 1, -1},
+        {1, -1, 1},  {1, -1, -1}, {-1, 0, 0},  {-1, 1, 0},  {-1, -1, 0},  {-1, 0, 1}, {-1, 0, -1},
+        {-1, 1, 1},  {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}, {0, 1, 0},   {0, 1, 1},  {0, 1, -1},
+        {0, -1, 0},  {0, -1, 1},  {0, -1, -1}, {0, 0, 1},   {0, 0, -1}};
+    const std::vector<sicp_voxel_sem> sem0 = m.sem;
+    const R vs = (R)m.cfg.voxel_size;
+    const int mask = m.cfg.max_label_count <= 255 ? 0xff : 0xffff;
+    for (const sicp_block& blk : m.visible) {
+        for (int ty = 0; ty < kBS; ++ty)
+            for (int tx = 0; tx < kBS; ++tx) {
+                int n_label[4] = {-1, -1, -1, -1}, n_count[4] = {0, 0, 0, 0}, n_lc[4] = {0, 0, 0, 0};
+                for (int z = 0; z < kBS; ++z) {
+                    const size_t own = (size_t)blk.slot * kBS3 + z * 64 + ty * 8 + tx;
+                    if (m.geom[own].weight == 0) continue;  // cur_label == -1
+                    const int cur_label = sem0[own].label;
+                    const int vx = blk.x * kBS + tx, vy = blk.y * kBS + ty, vz = blk.z * kBS + z;
+                    // voxelPosToWorldPos (DeviceMap.cu:165-167) and the +-VoxelSize steps
+                    const R px = (R)vx * vs, py = (R)vy * vs, pz = (R)vz * vs;
+                    const R step = vs * (R)1;
+                    for (int k = 0; k < 26; ++k) {
+                        V3<R> q{kN[k][0] > 0 ? px + step : (kN[k][0] < 0 ? px - step : px),
+                                kN[k][1] > 0 ? py + step : (kN[k][1] < 0 ? py - step : py),
+                                kN[k][2] > 0 ? pz + step : (kN[k][2] < 0 ? pz - step : pz)};
+                        // FindVoxel(float3, Voxel&): floor(pos / VoxelSize)
+                        const I3 v = floor_i3(V3<R>{q.x / vs, q.y / vs, q.z / vs});
+                        const I3 bp = voxel_to_block(v);
+                        const int slot = map_lookup(m, bp.x, bp.y, bp.z);
+                        if (slot < 0) continue;
+                        const size_t vi = (size_t)slot * kBS3 + voxel_local_idx(v);
+                        if (m.geom[vi].weight == 0) continue;
+                        const int tl = sem0[vi].label, tc = sem0[vi].count;
+                        for (int l = 0; l < 4; ++l) {
+                            if (n_label[l] == tl) {
+                                n_count[l]++;
+                                n_lc[l] = n_lc[l] > tc ? n_lc[l] : tc;
+                                break;
+                            } else if (n_label[l] == -1) {
+                                n_label[l] = tl;
+                                n_count[l] = 1;
+                                n_lc[l] = tc;
+                                break;
+                            }
+                        }
+                    }
+                    int nl = -1, nc = 0, nlc = 0;
+                    for (int j = 0; j < 4; ++j)
+                        if (n_label[j] != -1 && n_lc[j] != 0 && n_count[j] > nc) {
+                            nl = n_label[j];
+                            nc = n_count[j];
+                            nlc = n_lc[j];
+                        }
+                    sicp_voxel_sem& cur = m.sem[own];
+                    if (cur_label != nl) {
+                        const int tmp = cur_label == (int)sem0[own].label2
+                                            ? (nlc > (int)sem0[own].count2 ? nlc
+                                                                           : (int)sem0[own].count2 + 5)
+                                            : nlc;
+                        cur.label2 = (uint8_t)cur_label;
+                        cur.count2 = sem0[own].count;
+                        cur.label = (uint8_t)nl;
+                        cur.count = (uint16_t)(tmp & mask);
+                    } else {
+                        cur.count = (uint16_t)(nlc & mask);
+                    }
+                }
+            }
+    }
+}
+template void unify_neighbor<float>(OMap<float>&);
+template void unify_neighbor<double>(OMap<double>&);
+
 template <class R>
 uint32_t fuse_map_color(OMap<R>& m, const R* depth, const uint8_t* rgb, const R* nmap,
                         const uint8_t* mask, int cols, int rows, int num_detection,
@@ -466,6 +545,8 @@ uint32_t fuse_map_color(OMap<R>& m, const R* depth, const uint8_t* rgb, const R*
     if (nv == 0) return 0;  // "no visible blocks found" (FuseMap.cu:1117-1120)
     m.stats.updated_voxels =
         fuse_voxels(m, depth, rgb, nmap, mask, cols, rows, num_detection, pose, K);
+    // FuseMapColor runs UnifyNeighbor after the label update (FuseMap.cu:1140-1160)
+    if (m.cfg.unify_labels && num_detection > 0 && mask) unify_neighbor(m);
     return nv;
 }
 
@@ -997,6 +1078,26 @@ void oracle_map_export(void* map, sicp_hash_entry* table, sicp_hash_entry* exces
         go(h->d);
     else
         go(h->f);
+}
+
+// Test hook: sets one voxel's weight and label state (FP32 map).
+void oracle_map_set_label(void* map, int slot, int idx, int weight, int label, int count,
+                          int label2, int count2) {
+    OracleMapHandle* h = (OracleMapHandle*)map;
+    const size_t i = (size_t)slot * kBS3 + idx;
+    h->f.geom[i].weight = (uint8_t)weight;
+    h->f.sem[i].label = (uint8_t)label;
+    h->f.sem[i].count = (uint16_t)count;
+    h->f.sem[i].label2 = (uint8_t)label2;
+    h->f.sem[i].count2 = (uint16_t)count2;
+}
+
+void oracle_unify_neighbor(void* map) {
+    OracleMapHandle* h = (OracleMapHandle*)map;
+    if (h->fp64)
+        unify_neighbor(h->d);
+    else
+        unify_neighbor(h->f);
 }
 
 uint32_t oracle_mesh_scene(void* map, const int32_t* edge, const int8_t* tri,

@@ -43,6 +43,7 @@ class MapConfig(C.Structure):
         ("num_buckets", C.c_int32), ("num_blocks", C.c_int32),
         ("max_weight", C.c_int32), ("max_label_count", C.c_int32),
         ("request_capacity", C.c_int32), ("max_cols", C.c_int32), ("max_rows", C.c_int32),
+        ("unify_labels", C.c_int32),
     ]
 
 
@@ -114,6 +115,7 @@ _SIGS = {
     "sicp_so3_step": (C.c_int, [Image, Image, Image, Image, C.POINTER(PoseF), C.POINTER(PoseF),
                                 Intrinsics, _P, _P, _P, _P, _P, _P, _P]),
     "sicp_mesh_scene": (C.c_int, [_P, _P, _P, _P, C.c_uint32, _P, _P, _P]),
+    "sicp_unify_neighbor_labels": (C.c_int, [_P, _P]),
     "sicp_mesh_tables": (C.c_int, [_P, _P, _P]),
     "sicp_mesh_write_ply": (C.c_int, [C.c_char_p, _P, _P, _P, C.c_uint32]),
     "sicp_icp_partials_size": (C.c_size_t, []),

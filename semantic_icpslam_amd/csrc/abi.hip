@@ -52,6 +52,7 @@ int map_alloc(const sicp_map_config& cfg, sicp_map** out) {
     if ((size_t)cfg.max_cols * cfg.max_rows > (size_t)4096 * 1024) return invalid("image area above 4M pixels");
     if (cfg.max_weight < 1 || cfg.max_weight > 255) return invalid("max_weight outside 1..255");
     if (cfg.max_label_count < 1 || cfg.max_label_count > 65535) return invalid("max_label_count outside 1..65535");
+    if (cfg.unify_labels != 0 && cfg.unify_labels != 1) return invalid("unify_labels must be 0 or 1");
     sicp_map* m = new sicp_map();
     m->cfg = cfg;
     int rc = SICP_OK;
@@ -77,6 +78,19 @@ int map_alloc(const sicp_map_config& cfg, sicp_map** out) {
         (rc = dmalloc(m, &d.rq, px)) || (rc = dmalloc(m, &d.rq_count, 1))) {
         map_free(m);
         return rc;
+    }
+    if (cfg.unify_labels) {
+        // scratch of the neighbour label pass, here so that no later call allocates
+        if ((rc = dmalloc(m, &m->unify_snap, nblk * kBlockVoxels)) ||
+            (rc = dmalloc(m, &m->unify_mark, nblk))) {
+            map_free(m);
+            return rc;
+        }
+        hipError_t e = hipMemset(m->unify_mark, 0, nblk * sizeof(uint32_t));
+        if (e != hipSuccess) {
+            map_free(m);
+            return hip_fail(e, "hipMemset(unify marks)");
+        }
     }
     a.req_mask = (uint32_t)(q - 1);
     a.cap_pixels = (uint32_t)px;
@@ -148,6 +162,7 @@ void sicp_map_config_default(sicp_map_config* cfg, float voxel_size, float trunc
     cfg->num_blocks = num_blocks;
     cfg->max_weight = 255;   // min(255, w + 1) (FuseMap.cu:505)
     cfg->max_label_count = 65535;
+    cfg->unify_labels = 0;
     cfg->request_capacity = 1 << 20;
     cfg->max_cols = 640;
     cfg->max_rows = 480;
@@ -406,6 +421,11 @@ static int read_u32(const uint32_t* dev, uint32_t* host, hipStream_t s) {
     return SICP_OK;
 }
 
+int sicp_unify_neighbor_labels(sicp_map* map, void* stream) {
+    if (!map) return invalid("unify_neighbor_labels: null map");
+    return launch_unify(map, nullptr, (hipStream_t)stream);
+}
+
 int sicp_fuse_map_color(sicp_map* map, sicp_image depth_f32, sicp_image rgb_u8x3,
                         sicp_image nmap_f4, sicp_image mask_u8, int num_detection,
                         const sicp_pose_f* pose_host, const sicp_pose_f* pose_dev,
@@ -440,6 +460,10 @@ int sicp_fuse_map_color(sicp_map* map, sicp_image depth_f32, sicp_image rgb_u8x3
     if (rc) return rc;
     rc = launch_fuse_count(map->ctr, map->alloc.fuse_counts, nullptr, s);
     if (rc) return rc;
+    if (map->cfg.unify_labels && num_detection > 0 && mask_u8.ptr) {
+        rc = launch_unify(map, nullptr, s);
+        if (rc) return rc;
+    }
     if (visible_host) {
         rc = read_u32(&map->ctr->visible, visible_host, s);
         if (rc) return rc;

@@ -11,6 +11,9 @@ struct sicp_map {
     sicp::MapCounters* ctr = nullptr;  // device
     sicp::AllocScratch alloc{};
     sicp_block* visible = nullptr;     // device, num_blocks
+    uint32_t* unify_snap = nullptr;    // neighbour label pass: label snapshot, num_blocks * 512
+    uint32_t* unify_mark = nullptr;    // per heap slot: epoch of the last snapshot
+    uint32_t unify_epoch = 0;
     std::vector<void*> owned;          // every device allocation of the map
 };
 
@@ -18,6 +21,8 @@ namespace sicp {
 
 int map_alloc(const sicp_map_config& cfg, sicp_map** out);
 void map_free(sicp_map* m);
+// The neighbour label pass over the map's visible list (unify.hip).
+int launch_unify(sicp_map* map, const int* gate, hipStream_t s);
 int launch_raycast_gated(MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin,
                          View<float> zmax, PoseArg pose, float invfx, float invfy, float cx,
                          float cy, const int* gate, const uint32_t* nonempty, hipStream_t s);

@@ -369,6 +369,7 @@ static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
     stage_end(s, kStageFuse, ev, st);
     ev = stage_begin(s, st);
     if ((rc = launch_fuse_count(m->ctr, m->alloc.fuse_counts, gate, st))) return rc;
+    if (m->cfg.unify_labels && labels && (rc = launch_unify(m, gate, st))) return rc;
     stage_end(s, kStageFuseCount, ev, st);
     // Mapping::RayTrace (Mapping.cc:142-161): prepass, then the raycast into
     // the frame's level-0 model maps when any block produced a tile.

@@ -4,6 +4,8 @@ numpy's linear solve, a plain 5x5 convolution), hand-traced sequences of the
 label state machine, and invariants of the hash table. The reference holds
 no golden vectors or tests for this path, so these are the oracle's anchors.
 """
+import itertools
+
 import numpy as np
 import pytest
 import scipy.linalg
@@ -695,3 +697,128 @@ def test_raycast_empty_map_is_nan():
     assert not any_ and np.all(zmin == 100.0) and np.all(zmax == 0.0)
     v, n = m.raycast(64, 48, zmin, zmax, B.pose_f(np.eye(4)), k)
     assert np.all(np.isnan(v)) and np.all(np.isnan(n))
+
+
+# -------------------------------------------------- neighbour label pass ---
+def _unify_map():
+    """A fused wall (allocated, visible blocks), every voxel of every
+    allocated block relabelled by the caller."""
+    cfg = map_cfg(WALL, num_buckets=WALL["buckets"], num_blocks=WALL["blocks"])
+    m = O.OracleMap(cfg)
+    _wall_fuse(m, 1.0, 3)
+    blocks = m.export()["blocks"][: m.stats()["allocated_blocks"]]
+    vis = m.visible()
+    assert len(vis) == len(blocks) > 4
+    return m, blocks
+
+
+def _set_all(m, blocks, weight, label, count):
+    for b in blocks:
+        for i in range(512):
+            m.set_label(int(b["slot"]), i, weight, label, count)
+
+
+def _sem(m, slot, idx):
+    s = m.export()["sem"][slot * 512 + idx]
+    return int(s["label"]), int(s["count"]), int(s["label2"]), int(s["count2"])
+
+
+def test_unify_uniform_labels_are_a_fixed_point():
+    m, blocks = _unify_map()
+    _set_all(m, blocks, 1, 3, 4)
+    m.unify_neighbor()
+    sem = m.export()["sem"]
+    slots = blocks["slot"].astype(np.int64)
+    idx = (slots[:, None] * 512 + np.arange(512)[None, :]).ravel()
+    assert np.all(sem["label"][idx] == 3) and np.all(sem["count"][idx] == 4)
+    assert np.all(sem["label2"][idx] == 0) and np.all(sem["count2"][idx] == 0)
+
+
+def test_unify_outlier_takes_the_surrounding_label():
+    """An interior voxel labelled 7 among voxels labelled 3: it takes 3 with
+    the neighbours' largest count and keeps (7, its count) as the secondary
+    pair; its neighbours, outvoted 25 to 1, keep 3."""
+    m, blocks = _unify_map()
+    _set_all(m, blocks, 1, 3, 4)
+    slot, idx = int(blocks[0]["slot"]), 4 * 64 + 4 * 8 + 4
+    m.set_label(slot, idx, 1, 7, 2)
+    m.unify_neighbor()
+    assert _sem(m, slot, idx) == (3, 4, 7, 2)
+    assert _sem(m, slot, idx + 1) == (3, 4, 0, 0)
+    assert _sem(m, slot, idx - 64) == (3, 4, 0, 0)
+
+
+def _neighbour_hits_self(v, vs):
+    """Whether one of the 26 neighbour probes of voxel v lands back on v:
+    the probes step in world coordinates, (v * VoxelSize +- VoxelSize) /
+    VoxelSize in float, then floor (FuseMap.cu:830-892, DeviceMap.cu:154-163),
+    which can round back to v."""
+    vs = np.float32(vs)
+    p = np.asarray(v, np.float32) * vs
+    for d in itertools.product((-1, 0, 1), repeat=3):
+        if d == (0, 0, 0):
+            continue
+        q = p + np.asarray(d, np.float32) * vs
+        if np.array_equal(np.floor(q / vs).astype(int), np.asarray(v)):
+            return True
+    return False
+
+
+def _isolated_case(m, blocks, want_self):
+    b = blocks[0]
+    for idx in range(512):
+        lx, ly, lz = idx & 7, (idx >> 3) & 7, idx >> 6
+        if not (2 <= lx <= 5 and 2 <= ly <= 5 and 2 <= lz <= 5):
+            continue
+        v = (int(b["x"]) * 8 + lx, int(b["y"]) * 8 + ly, int(b["z"]) * 8 + lz)
+        if _neighbour_hits_self(v, WALL["voxel"]) == want_self:
+            return int(b["slot"]), idx
+    pytest.skip("no such voxel in the block")
+
+
+def test_unify_isolated_voxel_narrows_minus_one():
+    """A labelled voxel with no labelled neighbour: the chosen label is -1,
+    stored as 255; its label equals its secondary label, so the count becomes
+    count2 + 5 (FuseMap.cu:926-940)."""
+    m, blocks = _unify_map()
+    _set_all(m, blocks, 0, 0, 0)
+    slot, idx = _isolated_case(m, blocks, want_self=False)
+    m.set_label(slot, idx, 1, 5, 3, 5, 1)
+    m.unify_neighbor()
+    assert _sem(m, slot, idx) == (255, 6, 5, 3)
+
+
+def test_unify_float_probe_can_count_the_voxel_itself():
+    """Where a neighbour probe rounds back onto the voxel, the isolated voxel
+    counts itself: the chosen label is its own, its count stays."""
+    m, blocks = _unify_map()
+    _set_all(m, blocks, 0, 0, 0)
+    slot, idx = _isolated_case(m, blocks, want_self=True)
+    m.set_label(slot, idx, 1, 5, 3, 5, 1)
+    m.unify_neighbor()
+    assert _sem(m, slot, idx) == (5, 3, 5, 1)
+
+
+def test_unify_reads_the_state_before_the_pass():
+    """Two x-adjacent labelled voxels in different columns, nothing else
+    labelled, neither probe landing on itself: each takes the other's label
+    as it was before the pass, so they swap (an in-place pass would leave the
+    second one unchanged)."""
+    m, blocks = _unify_map()
+    _set_all(m, blocks, 0, 0, 0)
+    b = blocks[0]
+    slot = int(b["slot"])
+    base = lambda lx, ly, lz: (int(b["x"]) * 8 + lx, int(b["y"]) * 8 + ly, int(b["z"]) * 8 + lz)
+    pair = None
+    for lz, ly, lx in itertools.product(range(2, 6), range(2, 6), range(2, 5)):
+        if not (_neighbour_hits_self(base(lx, ly, lz), WALL["voxel"]) or
+                _neighbour_hits_self(base(lx + 1, ly, lz), WALL["voxel"])):
+            pair = (lz * 64 + ly * 8 + lx, lz * 64 + ly * 8 + lx + 1)
+            break
+    assert pair is not None
+    a, c = pair
+    m.set_label(slot, a, 1, 1, 6)
+    m.set_label(slot, c, 1, 2, 9)
+    m.unify_neighbor()
+    assert _sem(m, slot, a) == (2, 9, 1, 6)
+    assert _sem(m, slot, c) == (1, 6, 2, 9)
