@@ -182,6 +182,35 @@ int sicp_resize_map(sicp_image vsrc, sicp_image nsrc, sicp_image vdst, sicp_imag
  * call unless the map was created with unify_labels = 1. */
 int sicp_unify_neighbor_labels(sicp_map* map, void* stream);
 
+/* ---- Raycast adjoint ------------------------------------------------------ */
+
+/* The derivative of Raycast's output (RenderScene.cu:380-488) for sampled
+ * pixels. No counterpart in the reference: it serves callers that optimise
+ * a pose or the TSDF through the renderer. The vertex and normal of each
+ * pixel are functions of the camera pose and of the TSDF values, with the
+ * zRange images and the march's discrete decisions held fixed; the march is
+ * recomputed per pixel (nothing is stored by the forward pass). Given
+ * d loss / d vertex (`grad_vertex_dev`, n x 3, camera frame, metres) and
+ * d loss / d normal (`grad_normal_dev`, n x 3, may be NULL) of the pixels
+ * `pixels_dev` (n indices y * cols + x; others contribute nothing):
+ *  - `grad_pose_dev` (6 doubles, device) gets d loss / d xi for the camera
+ *    perturbation T' = T exp(xi), xi = (translation, rotation), camera frame;
+ *  - `grad_pose_px_dev` (n x 6 floats, may be NULL) the same per pixel;
+ *  - `grad_sdf_dev` (num_blocks x 512 floats, heap-slot order, may be NULL)
+ *    accumulates d loss / d sdf (normalised units); the caller zeroes it.
+ * Pixels whose forward output is NaN contribute 0. Arguments as
+ * sicp_raycast_inv; `scratch_dev` holds sicp_raycast_backward_scratch_size()
+ * bytes, 8-byte aligned. TSDF gradients are summed with atomics (their
+ * rounding depends on the order); the pose sums are in a fixed order. */
+int sicp_raycast_backward(sicp_map* map, sicp_image zmin, sicp_image zmax,
+                          const sicp_pose_f* pose, float invfx, float invfy, float cx, float cy,
+                          int32_t cols, int32_t rows, const int32_t* pixels_dev,
+                          int32_t n_pixels, const float* grad_vertex_dev,
+                          const float* grad_normal_dev, float* grad_sdf_dev,
+                          float* grad_pose_px_dev, double* grad_pose_dev, void* scratch_dev,
+                          size_t scratch_bytes, void* stream);
+size_t sicp_raycast_backward_scratch_size(void);
+
 /* ---- Mesh extraction (Mapping/MeshScene.cu:4-356) ------------------------ */
 
 /* MeshScene + Mapping::CreateModel (MeshScene.cu:308-356, Mapping.cc:183-192):

@@ -66,6 +66,11 @@ _SIGS = {
     "oracle_map_visible": (C.c_uint32, [_P, _P, C.c_uint32]),
     "oracle_mesh_scene": (C.c_uint32, [_P, _P, _P, _P, C.c_uint32, _P, _P, _P, _P]),
     "oracle_unify_neighbor": (None, [_P]),
+    "oracle_map_clone_fp64": (_P, [_P]),
+    "oracle_raycast_fp64": (None, [_P, _P, _P, C.c_int, C.c_int, _P, _P, C.c_int, _P, C.c_double,
+                                   C.c_double, C.c_double, C.c_double]),
+    "oracle_map_get_sdf": (C.c_double, [_P, C.c_int, C.c_int]),
+    "oracle_map_set_sdf": (None, [_P, C.c_int, C.c_int, C.c_double]),
     "oracle_map_set_label": (None, [_P] + [C.c_int] * 7),
     "oracle_map_counters": (None, [_P, _P]),
     "oracle_system_create": (_P, [C.POINTER(SystemConfig), C.c_int]),
@@ -313,6 +318,31 @@ class OracleMap:
         m = _c(mask, np.uint8) if mask is not None else None
         return lib().oracle_fuse_map_color(self.h, _p(d), _p(r), _p(n), _p(m), cols, rows,
                                            num_detection, C.byref(pose), K)
+
+    def clone_fp64(self):
+        """An FP64 map with exactly this FP32 map's state."""
+        o = OracleMap.__new__(OracleMap)
+        o.cfg = self.cfg
+        o.fp64 = True
+        o.h = lib().oracle_map_clone_fp64(self.h)
+        return o
+
+    def raycast_fp64(self, cols, rows, zmin, zmax, T, invfx, invfy, cx, cy):
+        """FP64 map only: the raycast at the 4x4 double pose T."""
+        v = np.empty((rows, cols, 4), np.float64)
+        n = np.empty((rows, cols, 4), np.float64)
+        zmin = _c(zmin, np.float32)
+        zmax = _c(zmax, np.float32)
+        T = _c(T, np.float64)
+        lib().oracle_raycast_fp64(self.h, _p(v), _p(n), cols, rows, _p(zmin), _p(zmax),
+                                  zmin.shape[1], _p(T), invfx, invfy, cx, cy)
+        return v, n
+
+    def get_sdf(self, slot, idx):
+        return lib().oracle_map_get_sdf(self.h, slot, idx)
+
+    def set_sdf(self, slot, idx, v):
+        lib().oracle_map_set_sdf(self.h, slot, idx, v)
 
     def unify_neighbor(self):
         lib().oracle_unify_neighbor(self.h)

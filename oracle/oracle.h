@@ -103,6 +103,15 @@ uint32_t oracle_map_visible(void* map, sicp_block* out, uint32_t cap);
 /* UnifyNeighbor (Mapping/FuseMap.cu:757-948) over the last visible list,
  * every voxel seeing the labels before the pass. */
 void oracle_unify_neighbor(void* map);
+/* A new FP64 map with exactly the state of the FP32 map `src`. */
+void* oracle_map_clone_fp64(void* src);
+/* Raycast of an FP64 map at a double pose (4x4 camera -> world). */
+void oracle_raycast_fp64(void* map, double* vmap, double* nmap, int cols, int rows,
+                         const float* zmin, const float* zmax, int zcols, const double* pose16,
+                         double invfx, double invfy, double cx, double cy);
+/* Test hooks: one voxel's sdf. */
+double oracle_map_get_sdf(void* map, int slot, int idx);
+void oracle_map_set_sdf(void* map, int slot, int idx, double v);
 /* Test hook: one voxel's weight and label state on an FP32 map. */
 void oracle_map_set_label(void* map, int slot, int idx, int weight, int label, int count,
                           int label2, int count2);

@@ -1055,6 +1055,56 @@ void oracle_raycast(void* map, float* vmap, float* nmap, int cols, int rows, con
             (float)(1.0 / (double)K.fy), K.cx, K.cy);
 }
 
+// A new FP64 map holding exactly the state of an FP32 one (table, blocks,
+// sdf widened to double, weights, colours, labels, visible list).
+void* oracle_map_clone_fp64(void* src) {
+    OracleMapHandle* s = (OracleMapHandle*)src;
+    OracleMapHandle* h = new OracleMapHandle();
+    h->fp64 = 1;
+    map_init(h->d, s->f.cfg);
+    h->d.table = s->f.table;
+    h->d.excess = s->f.excess;
+    h->d.blocks = s->f.blocks;
+    for (size_t i = 0; i < s->f.sdf.size(); ++i) h->d.sdf[i] = (double)s->f.sdf[i];
+    h->d.geom = s->f.geom;
+    h->d.sem = s->f.sem;
+    h->d.heap_used = s->f.heap_used;
+    h->d.visible = s->f.visible;
+    h->d.stats = s->f.stats;
+    return h;
+}
+
+// The raycast of an FP64 map at a double pose (4x4 camera -> world), the
+// zRange images given as floats; intrinsics as Raycast receives them.
+void oracle_raycast_fp64(void* map, double* vmap, double* nmap, int cols, int rows,
+                         const float* zmin, const float* zmax, int zcols, const double* pose16,
+                         double invfx, double invfy, double cx, double cy) {
+    OracleMapHandle* h = (OracleMapHandle*)map;
+    const int zrows = (rows + 7) / 8;
+    std::vector<double> z0((size_t)zcols * zrows), z1((size_t)zcols * zrows);
+    for (size_t i = 0; i < z0.size(); ++i) {
+        z0[i] = zmin[i];
+        z1[i] = zmax[i];
+    }
+    Pose<double> p = pose_from_d<double>(pose16);
+    raycast(h->d, vmap, nmap, cols, rows, z0.data(), z1.data(), zcols, p, invfx, invfy, cx, cy);
+}
+
+// Test hooks: one voxel's sdf (FP32 or FP64 map).
+double oracle_map_get_sdf(void* map, int slot, int idx) {
+    OracleMapHandle* h = (OracleMapHandle*)map;
+    const size_t i = (size_t)slot * kBS3 + idx;
+    return h->fp64 ? h->d.sdf[i] : (double)h->f.sdf[i];
+}
+void oracle_map_set_sdf(void* map, int slot, int idx, double v) {
+    OracleMapHandle* h = (OracleMapHandle*)map;
+    const size_t i = (size_t)slot * kBS3 + idx;
+    if (h->fp64)
+        h->d.sdf[i] = v;
+    else
+        h->f.sdf[i] = (float)v;
+}
+
 void oracle_map_get_stats(void* map, sicp_map_stats* out) {
     OracleMapHandle* h = (OracleMapHandle*)map;
     *out = h->fp64 ? h->d.stats : h->f.stats;

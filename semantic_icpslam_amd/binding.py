@@ -116,6 +116,10 @@ _SIGS = {
                                 Intrinsics, _P, _P, _P, _P, _P, _P, _P]),
     "sicp_mesh_scene": (C.c_int, [_P, _P, _P, _P, C.c_uint32, _P, _P, _P]),
     "sicp_unify_neighbor_labels": (C.c_int, [_P, _P]),
+    "sicp_raycast_backward": (C.c_int, [_P, Image, Image, C.POINTER(PoseF), C.c_float, C.c_float,
+                                        C.c_float, C.c_float, C.c_int32, C.c_int32, _P,
+                                        C.c_int32, _P, _P, _P, _P, _P, _P, C.c_size_t, _P]),
+    "sicp_raycast_backward_scratch_size": (C.c_size_t, []),
     "sicp_mesh_tables": (C.c_int, [_P, _P, _P]),
     "sicp_mesh_write_ply": (C.c_int, [C.c_char_p, _P, _P, _P, C.c_uint32]),
     "sicp_icp_partials_size": (C.c_size_t, []),

@@ -421,6 +421,33 @@ static int read_u32(const uint32_t* dev, uint32_t* host, hipStream_t s) {
     return SICP_OK;
 }
 
+size_t sicp_raycast_backward_scratch_size(void) { return raycast_adjoint_scratch_bytes(); }
+
+int sicp_raycast_backward(sicp_map* map, sicp_image zmin, sicp_image zmax,
+                          const sicp_pose_f* pose, float invfx, float invfy, float cx, float cy,
+                          int32_t cols, int32_t rows, const int32_t* pixels_dev,
+                          int32_t n_pixels, const float* grad_vertex_dev,
+                          const float* grad_normal_dev, float* grad_sdf_dev,
+                          float* grad_pose_px_dev, double* grad_pose_dev, void* scratch_dev,
+                          size_t scratch_bytes, void* stream) {
+    if (!map || !pose || !grad_vertex_dev || !grad_pose_dev || !scratch_dev ||
+        (n_pixels > 0 && !pixels_dev))
+        return invalid("raycast_backward: null argument");
+    if (n_pixels < 0) return invalid("raycast_backward: negative pixel count");
+    if (cols <= 0 || rows <= 0) return invalid("raycast_backward: empty image");
+    const int zc = (cols + 7) / 8, zr = (rows + 7) / 8;
+    if (!image_ok(zmin, 4, zc, zr) || !image_ok(zmax, 4, zc, zr))
+        return invalid("raycast_backward: zRange images must be ceil(cols/8) x ceil(rows/8)");
+    if (scratch_bytes < raycast_adjoint_scratch_bytes())
+        return invalid("raycast_backward: scratch smaller than sicp_raycast_backward_scratch_size()");
+    if (((uintptr_t)scratch_dev & 7) != 0) return invalid("raycast_backward: scratch not 8-byte aligned");
+    return launch_raycast_adjoint(map->dev, view<float>(zmin), view<float>(zmax),
+                                  make_pose_arg(pose, nullptr), invfx, invfy, cx, cy, cols, rows,
+                                  pixels_dev, n_pixels, grad_vertex_dev, grad_normal_dev,
+                                  grad_sdf_dev, grad_pose_px_dev, (double*)scratch_dev,
+                                  grad_pose_dev, (hipStream_t)stream);
+}
+
 int sicp_unify_neighbor_labels(sicp_map* map, void* stream) {
     if (!map) return invalid("unify_neighbor_labels: null map");
     return launch_unify(map, nullptr, (hipStream_t)stream);

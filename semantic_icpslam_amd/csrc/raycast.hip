@@ -622,7 +622,280 @@ __global__ __launch_bounds__(256) void raycast_long_kernel(MapDev m, View<float4
 
 __global__ void ray_queue_reset_kernel(uint32_t* count) { *count = 0u; }
 
+// ------------------------------------------------------------------ adjoint --
+// The raycast's vertex and normal as functions of the camera pose and of the
+// TSDF values, with the zRange images and every discrete decision of the
+// march held fixed. Per sampled pixel the march is recomputed (nothing is
+// stored by the forward pass), then the chain
+//   r0 = block_s + M dir                      (M: the march's summed steps)
+//   r1 = r0 + k s0 dir, s0 = F(r0) or raw voxel (k: stepScale)
+//   r2 = r1 + k s1 dir, s1 = F(r1)
+//   v  = R^T (voxel r2 - t)
+//   n  = normalised(R^T g), g_a = F(r2 + e_a) - F(r2 - e_a)
+// is run backwards (F: the trilinear read). The pose perturbation is
+// T' = T exp(xi), xi = (translation, rotation) in the camera frame:
+// R' = R (I + [w]x), t' = t + R v.
+
+// A trilinear read (readSdfInterped, RenderScene.cu:310-343) with its eight
+// corner voxels, their weights and its gradient in voxel units. The value is
+// formed exactly as the forward read forms it.
+struct TriRead {
+    long k[8];
+    float w[8];
+    v3 grad;
+    float val;
+    bool ok;  // every corner exists, the value is not NaN
+};
+SICP_DEV void tri_read(const MapDev& m, v3 p, TriRead& t) {
+    const AxisPair X = axis_pair(p.x), Y = axis_pair(p.y), Z = axis_pair(p.z);
+    float sd[8];
+    t.ok = true;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int vx = (i & 1) ? X.v1 : X.v0, vy = (i & 2) ? Y.v1 : Y.v0,
+                  vz = (i & 4) ? Z.v1 : Z.v0;
+        const int slot = map_lookup(m, vx >> 3, vy >> 3, vz >> 3);
+        if (slot < 0) {
+            t.k[i] = -1;
+            sd[i] = qnanf();
+            t.ok = false;
+        } else {
+            t.k[i] = (long)slot * kBlockVoxels + local_of(vx, vy, vz);
+            sd[i] = m.geom[t.k[i]].sdf;
+        }
+    }
+    const float fx = X.frac, fy = Y.frac, fz = Z.frac;
+    float r0, r1, r2, r3;
+    r0 = (1.0f - fx) * sd[0] + fx * sd[1];
+    r1 = (1.0f - fx) * sd[2] + fx * sd[3];
+    r2 = (1.0f - fy) * r0 + fy * r1;
+    const float a0 = r0, a1 = r1;
+    r0 = (1.0f - fx) * sd[4] + fx * sd[5];
+    r1 = (1.0f - fx) * sd[6] + fx * sd[7];
+    r3 = (1.0f - fy) * r0 + fy * r1;
+    t.val = (1.0f - fz) * r2 + fz * r3;
+    t.ok = t.ok && !isnan_(t.val);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        t.w[i] = ((i & 1) ? fx : 1.0f - fx) * ((i & 2) ? fy : 1.0f - fy) * ((i & 4) ? fz : 1.0f - fz);
+    t.grad.x = (1.0f - fz) * ((1.0f - fy) * (sd[1] - sd[0]) + fy * (sd[3] - sd[2])) +
+               fz * ((1.0f - fy) * (sd[5] - sd[4]) + fy * (sd[7] - sd[6]));
+    t.grad.y = (1.0f - fz) * (a1 - a0) + fz * (r1 - r0);
+    t.grad.z = r3 - r2;
+}
+
+// dL/d(values) += g * weights.
+SICP_DEV void tri_scatter(const TriRead& t, float g, float* grad_sdf) {
+    if (!grad_sdf || g == 0.0f) return;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        if (t.k[i] >= 0 && t.w[i] != 0.0f) atomicAdd(grad_sdf + t.k[i], g * t.w[i]);
+}
+
+struct AdjointArgs {
+    RayArgs g;
+    const int32_t* pixels;
+    int n;
+    const float* grad_v;   // n x 3, d loss / d vertex (camera frame)
+    const float* grad_n;   // n x 3, d loss / d normal; may be null
+    float* grad_sdf;       // num_blocks x 512, accumulated; may be null
+    float* grad_px;        // n x 6 per-pixel pose gradients; may be null
+    double* partials;      // 6 x kAdjMaxBlocks
+};
+constexpr int kAdjThreads = 256;
+constexpr int kAdjMaxBlocks = 256;
+
+// One pixel: false (and no gradient) where the forward pass writes NaN.
+SICP_DEV bool adjoint_pixel(const MapDev& m, const View<float>& zmin, const View<float>& zmax,
+                            const PoseF& pose, const RayArgs& g, int x, int y, v3 gv, v3 gn,
+                            float* grad_sdf, float gxi[6]) {
+    Ray ray;
+    if (!ray_setup(m, zmin, zmax, pose, g, x, y, ray)) return false;
+    // the march of raycast_kernel, uncapped, with its summed step
+    v3 result = ray.start;
+    float dist_s = ray.dist_s, sdf = 1.0f, msum = 0.0f;
+    bool valid_sdf = false, interp0 = false;
+    BlockCache cache{0, 0, 0, -1};
+    long k0 = -1;
+    while (dist_s < ray.dist_e) {
+        const float prev = sdf;
+        const long k = find_voxel_i(m, floor_i(result.x), floor_i(result.y), floor_i(result.z),
+                                    cache, valid_sdf);
+        if (k < 0) {
+            sdf = qnanf();
+        } else {
+            sdf = m.geom[k].sdf;
+            if (m.geom[k].weight == 0) valid_sdf = false;
+        }
+        k0 = k;
+        interp0 = false;
+        float step;
+        if (!valid_sdf) {
+            step = (float)kBlockSide;
+        } else {
+            if (sdf <= 0.1f && sdf >= -0.5f) {
+                sdf = read_sdf_interped(m, result, cache, valid_sdf);
+                interp0 = true;
+            }
+            if (prev < 0.f && sdf >= 0.f) return false;  // back face
+            if (prev > 0.f && sdf <= 0.f) break;
+            step = 1.0f;
+        }
+        result = add(result, scale(step, ray.dir));
+        dist_s += step;
+        msum += step;
+    }
+    if (!(sdf <= 0.0f)) return false;
+    const float ks = m.step_scale;
+    const v3 r0 = result;
+    TriRead t0;
+    if (interp0) tri_read(m, r0, t0);
+    const float s0 = sdf;
+    const v3 r1 = add(r0, scale(s0 * ks, ray.dir));
+    TriRead t1;
+    tri_read(m, r1, t1);
+    const float s1 = t1.val;
+    const v3 r2 = add(r1, scale(s1 * ks, ray.dir));
+    // readNormal's six reads: pt.a + 1.0f / pt.a + -1.0f on one axis, pt on
+    // the others, in order; the first failing one ends the pixel (the reads
+    // are repeated on the way back rather than kept)
+    auto normal_point = [&](int a) {
+        const float d = (a & 1) ? -1.0f : 1.0f;
+        const int ax = a >> 1;
+        return v3{ax == 0 ? r2.x + d : r2.x, ax == 1 ? r2.y + d : r2.y, ax == 2 ? r2.z + d : r2.z};
+    };
+    float sv[6];
+#pragma unroll 1
+    for (int a = 0; a < 6; ++a) {
+        TriRead tn;
+        tri_read(m, normal_point(a), tn);
+        sv[a] = tn.val;
+        if (!tn.ok || sv[a] == 1.0f) return false;
+    }
+    const v3 gvec = mk(sv[0] - sv[1], sv[2] - sv[3], sv[4] - sv[5]);
+    const v3 mvec = mv(pose.Rinv, gvec);
+    const float mlen = norm(mvec);
+    const v3 nrm = divs(mvec, mlen);
+    const v3 vout = mv(pose.Rinv, sub(scale(r2, m.voxel), pose.t));
+    // ---- backwards ----
+    v3 gw{0.f, 0.f, 0.f}, gt{0.f, 0.f, 0.f}, gr2{0.f, 0.f, 0.f}, gdir{0.f, 0.f, 0.f};
+    // normal: n = m / |m|, m = R'^T g
+    const float ndg = dot(nrm, gn);
+    const v3 gm = divs(sub(gn, scale(ndg, nrm)), mlen);
+    gw = add(gw, cross(gm, mvec));
+    const v3 gg = mv(pose.R, gm);
+#pragma unroll 1
+    for (int a = 0; a < 6; ++a) {
+        const float ga = (a >> 1) == 0 ? gg.x : ((a >> 1) == 1 ? gg.y : gg.z);
+        const float gsa = (a & 1) ? -ga : ga;
+        TriRead tn;
+        tri_read(m, normal_point(a), tn);
+        tri_scatter(tn, gsa, grad_sdf);
+        gr2 = add(gr2, scale(gsa, tn.grad));
+    }
+    // vertex: v = R'^T (voxel r2 - t')
+    gr2 = add(gr2, scale(m.voxel, mv(pose.R, gv)));
+    gt = sub(gt, gv);
+    gw = add(gw, cross(gv, vout));
+    // r2 = r1 + k s1 dir
+    v3 gr1 = gr2;
+    const float gs1 = ks * dot(ray.dir, gr2);
+    gdir = add(gdir, scale(ks * s1, gr2));
+    tri_scatter(t1, gs1, grad_sdf);
+    gr1 = add(gr1, scale(gs1, t1.grad));
+    // r1 = r0 + k s0 dir
+    v3 gr0 = gr1;
+    const float gs0 = ks * dot(ray.dir, gr1);
+    gdir = add(gdir, scale(ks * s0, gr1));
+    if (interp0) {
+        tri_scatter(t0, gs0, grad_sdf);
+        gr0 = add(gr0, scale(gs0, t0.grad));
+    } else if (grad_sdf && k0 >= 0 && gs0 != 0.0f) {
+        atomicAdd(grad_sdf + k0, gs0);
+    }
+    // r0 = block_s + M dir
+    gdir = add(gdir, scale(msum, gr0));
+    // dir = R' u, u the ray in the camera frame
+    const v3 u = mv(pose.Rinv, ray.dir);
+    gw = add(gw, cross(u, mv(pose.Rinv, gdir)));
+    // block_s = (R' p_s + t') / voxel
+    const float z0 = zmin.at(y >> 3, x >> 3);
+    const v3 ps{z0 * ((float)x - g.cx) * g.invfx, z0 * ((float)y - g.cy) * g.invfy, z0};
+    const v3 gb = scale(m.voxel_inv, mv(pose.Rinv, gr0));
+    gt = add(gt, gb);
+    gw = add(gw, cross(ps, gb));
+    gxi[0] = gt.x;
+    gxi[1] = gt.y;
+    gxi[2] = gt.z;
+    gxi[3] = gw.x;
+    gxi[4] = gw.y;
+    gxi[5] = gw.z;
+    return true;
+}
+
+__global__ __launch_bounds__(kAdjThreads) void raycast_adjoint_kernel(MapDev m, View<float> zmin,
+                                                                      View<float> zmax,
+                                                                      PoseArg pa, AdjointArgs a) {
+    const PoseF pose = load_pose(pa);
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        const int px = a.pixels[i];
+        if (px < 0 || px >= a.g.cols * a.g.rows) {  // outside the image: no gradient
+            if (a.grad_px)
+                for (int j = 0; j < 6; ++j) a.grad_px[6 * i + j] = 0.f;
+            continue;
+        }
+        const int y = px / a.g.cols, x = px - y * a.g.cols;
+        const v3 gv{a.grad_v[3 * i], a.grad_v[3 * i + 1], a.grad_v[3 * i + 2]};
+        const v3 gn = a.grad_n ? v3{a.grad_n[3 * i], a.grad_n[3 * i + 1], a.grad_n[3 * i + 2]}
+                               : v3{0.f, 0.f, 0.f};
+        float gxi[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (!adjoint_pixel(m, zmin, zmax, pose, a.g, x, y, gv, gn, a.grad_sdf, gxi))
+            for (int j = 0; j < 6; ++j) gxi[j] = 0.f;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc[j] += (double)gxi[j];
+        if (a.grad_px)
+            for (int j = 0; j < 6; ++j) a.grad_px[6 * i + j] = gxi[j];
+    }
+    // fixed-order workgroup sum: rows through LDS, thread j sums column j
+    __shared__ double tile[kAdjThreads][7];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) tile[threadIdx.x][j] = acc[j];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double s = 0.0;
+        for (int r = 0; r < kAdjThreads; ++r) s += tile[r][threadIdx.x];
+        a.partials[threadIdx.x * kAdjMaxBlocks + blockIdx.x] = s;
+    }
+}
+
+__global__ void adjoint_finish_kernel(const double* partials, int nb, double* grad_pose) {
+    if (threadIdx.x < 6) {
+        double s = 0.0;
+        for (int b = 0; b < nb; ++b) s += partials[threadIdx.x * kAdjMaxBlocks + b];
+        grad_pose[threadIdx.x] = s;
+    }
+}
+
 }  // namespace
+
+int launch_raycast_adjoint(MapDev m, View<float> zmin, View<float> zmax, PoseArg pose,
+                           float invfx, float invfy, float cx, float cy, int cols, int rows,
+                           const int32_t* pixels, int n, const float* grad_v,
+                           const float* grad_n, float* grad_sdf, float* grad_px,
+                           double* partials, double* grad_pose, hipStream_t s) {
+    AdjointArgs a{RayArgs{invfx, invfy, cx, cy, cols, rows}, pixels, n, grad_v, grad_n,
+                  grad_sdf, grad_px, partials};
+    int nb = (int)div_up(n > 0 ? n : 1, kAdjThreads);
+    if (nb > kAdjMaxBlocks) nb = kAdjMaxBlocks;
+    raycast_adjoint_kernel<<<nb, kAdjThreads, 0, s>>>(m, zmin, zmax, pose, a);
+    SICP_LAUNCH_CHECK("raycast_adjoint_kernel");
+    adjoint_finish_kernel<<<1, 64, 0, s>>>(partials, nb, grad_pose);
+    SICP_LAUNCH_CHECK("adjoint_finish_kernel");
+    return SICP_OK;
+}
+
+size_t raycast_adjoint_scratch_bytes() { return sizeof(double) * (6 * kAdjMaxBlocks + 8); }
 
 int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible, View<float> zmin,
                           View<float> zmax, PoseArg pose, sicp_intrinsics K, float dmax,

@@ -237,6 +237,18 @@ int launch_raycast(MapDev m, View<float4> vmap, View<float4> nmap, View<float> z
                    View<float> zmax, PoseArg pose, float invfx, float invfy, float cx, float cy,
                    const int* gate, hipStream_t s);
 
+// The raycast's adjoint for `n` sampled pixels (pixel index y * cols + x):
+// per-pixel pose gradients (n x 6, may be null), their sum into
+// `grad_pose` (6 doubles, device), TSDF gradients accumulated into
+// `grad_sdf` (num_blocks x 512, may be null). `partials`: scratch of
+// raycast_adjoint_scratch_bytes().
+int launch_raycast_adjoint(MapDev m, View<float> zmin, View<float> zmax, PoseArg pose,
+                           float invfx, float invfy, float cx, float cy, int cols, int rows,
+                           const int32_t* pixels, int n, const float* grad_v,
+                           const float* grad_n, float* grad_sdf, float* grad_px,
+                           double* partials, double* grad_pose, hipStream_t s);
+size_t raycast_adjoint_scratch_bytes();
+
 // Device exclusive scan of `n_dev` (device count, at most `cap`) u32 values.
 int launch_scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t cap,
                     uint32_t* partials, uint32_t* total_out, hipStream_t s);

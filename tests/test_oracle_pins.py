@@ -822,3 +822,60 @@ def test_unify_reads_the_state_before_the_pass():
     m.unify_neighbor()
     assert _sem(m, slot, a) == (2, 9, 1, 6)
     assert _sem(m, slot, c) == (1, 6, 2, 9)
+
+
+def test_raycast_pose_derivative_matches_plane_model():
+    """Central differences (FP64, h = 1e-6) of the raycast of a fused wall at
+    0.9637 m against the closed form for a plane: an exact ray-plane
+    intersection moves as n . dv/dxi = [-n, n x v]; the two half-steps
+    r += k F(r) dir (RenderScene.cu:465-474) leave the distance to the plane
+    times rho = (1 - c/2)^2 (c the ray-normal cosine), so
+    n . dv/dxi = (1 - rho) [-n, n x v_hit] - (n . (v_hit - v) / rho) drho/dxi.
+    Pins the march's final refinement and its pose dependence."""
+    c = dict(cols=64, rows=48, f=60.0, cx=31.5, cy=23.5, voxel=0.01, trunc=0.03,
+             buckets=1 << 12, blocks=4000)
+    Kc = B.intrinsics(c["f"], c["f"], c["cx"], c["cy"])
+    m = O.OracleMap(map_cfg(c))
+    rows, cols, z_w = c["rows"], c["cols"], 0.9637
+    depth = np.full((rows, cols), z_w, np.float32)
+    v = O.compute_vmap(depth, c["f"], c["f"], c["cx"], c["cy"], 3.0)
+    n = O.compute_nmap(v)
+    n[-1, :] = n[-2, :]
+    n[:, -1] = n[:, -2]
+    m.fuse(depth, np.full((rows, cols, 3), 100, np.uint8), n, np.ones((rows, cols), np.uint8), 1,
+           B.pose_f(np.eye(4)), Kc)
+    T = O.se3_exp(np.array([0.003, -0.002, 0.011, 0.0, 0.0, 0.002]))
+    zmin, zmax, _ = m.rendering_blocks(8, 6, 3.0, 0.1, B.pose_f(T), Kc)
+    v32, _ = m.raycast(cols, rows, zmin, zmax, B.pose_f(T), Kc)
+    m64 = m.clone_fp64()
+    invf = float(np.float32(1 / c["f"]))
+    R, t = T[:3, :3], T[:3, 3]
+    nrm = R.T @ np.array([0.0, 0.0, -1.0])
+    ez = R.T @ np.array([0.0, 0.0, 1.0])
+    h = 1e-6
+    Jfd = []
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = h
+        vp, _ = m64.raycast_fp64(cols, rows, zmin, zmax, T @ O.se3_exp(e), invf, invf, c["cx"],
+                                 c["cy"])
+        vm, _ = m64.raycast_fp64(cols, rows, zmin, zmax, T @ O.se3_exp(-e), invf, invf, c["cx"],
+                                 c["cy"])
+        Jfd.append((vp[..., :3] - vm[..., :3]) @ nrm / (2 * h))
+    Jfd = np.stack(Jfd, -1)
+    checked = 0
+    for y in range(16, 32, 3):
+        for x in range(20, 44, 3):
+            if np.isnan(v32[y, x, 0]):
+                continue
+            ray = np.array([(x - c["cx"]) * invf, (y - c["cy"]) * invf, 1.0])
+            u = ray / np.linalg.norm(ray)
+            cth = -nrm @ u
+            rho = (1 - 0.5 * cth) ** 2
+            v_hit = u * ((z_w - t[2]) / (ez @ u))
+            J = np.concatenate([-nrm, np.cross(nrm, v_hit)])
+            drho = np.concatenate([np.zeros(3), -(1 - 0.5 * cth) * np.cross(nrm, u)])
+            want = (1 - rho) * J - (nrm @ (v_hit - v32[y, x, :3]) / rho) * drho
+            assert np.abs(Jfd[y, x] - want).max() < 1e-4 * np.abs(want).max(), (y, x)
+            checked += 1
+    assert checked > 30
