@@ -282,6 +282,123 @@ def run_batched(args):
     print(json.dumps(out), flush=True)
 
 
+# The differentiable-renderer workload: 1280x720 / 2 mm voxels.
+ADJOINT = dict(cols=1280, rows=720, f=1050.0, cx=639.5, cy=359.5, voxel=0.002, trunc=0.006,
+               buckets=1 << 21, blocks=1_000_000, seed=0)
+
+
+def run_adjoint(args):
+    """The raycast adjoint at 1280x720 / 2 mm: 100 frames tracked and fused
+    by the system, the model rendered at the last pose, then the backward pass
+    of 10k seeded pixels with a hit, timed over `steps` calls after `warmup`.
+    Also the agreement of each pixel's point-to-plane pose derivative
+    n . dv/dxi with the ICP Jacobian [-n, n x v] (cosine; the march's two
+    half-steps scale it by about 3/4 on surfaces seen head-on)."""
+    import torch
+    from semantic_icpslam_amd import binding as B
+    w = ADJOINT
+    nfr = 100
+    log(f"[bench] rendering {nfr} frames of {w['cols']}x{w['rows']}")
+    _, frames = make_frames(nfr, w)
+    dev = torch.device("cuda:0")
+    cfg = B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"], w["voxel"],
+                          w["trunc"], w["buckets"], w["blocks"])
+    sysg = B.System(cfg)
+    log("[bench] tracking and fusing the sequence")
+    for d, c, l in frames:
+        sysg.grab(torch.from_numpy(d).to(dev), torch.from_numpy(c).to(dev),
+                  torch.from_numpy(l).to(dev))
+    res = sysg.result()
+    T = res["pose"]
+    pose = B.pose_f(T)
+    K = B.intrinsics(w["f"], w["f"], w["cx"], w["cy"])
+    invf = float(np.float32(1.0 / w["f"]))
+    zc, zr = (w["cols"] + 7) // 8, (w["rows"] + 7) // 8
+    zmin = torch.zeros((zr, zc), dtype=torch.float32, device=dev)
+    zmax = torch.zeros_like(zmin)
+    nonempty = B.C.c_int(0)
+    m = sysg.map_handle()
+    B.check(B.lib().sicp_create_rendering_blocks(m, B.image(zmin), B.image(zmax), 3.0, 0.1,
+                                                 B.C.byref(pose), None, K, B.C.byref(nonempty),
+                                                 B.stream_ptr()))
+    vmap = torch.empty((w["rows"], w["cols"], 4), dtype=torch.float32, device=dev)
+    nmap = torch.empty_like(vmap)
+    B.check(B.lib().sicp_raycast_inv(m, B.image(vmap), B.image(nmap), B.image(zmin),
+                                     B.image(zmax), B.C.byref(pose), None, invf, invf, w["cx"],
+                                     w["cy"], B.stream_ptr()))
+    torch.cuda.synchronize()
+    vh, nh = vmap.cpu().numpy(), nmap.cpu().numpy()
+    ys, xs = np.nonzero(~np.isnan(vh[..., 0]))
+    rng = np.random.default_rng(12345)
+    sel = rng.choice(len(ys), 10000, replace=False)
+    pix_h = (ys[sel] * w["cols"] + xs[sel]).astype(np.int32)
+    pix = torch.from_numpy(pix_h).to(dev)
+    n = len(pix_h)
+    gv = torch.from_numpy(rng.normal(size=(n, 3)).astype(np.float32)).to(dev)
+    gn = torch.from_numpy(rng.normal(size=(n, 3)).astype(np.float32)).to(dev)
+    nblk = cfg.map.num_blocks
+    gsdf = torch.zeros(nblk * 512, dtype=torch.float32, device=dev)
+    gpx = torch.zeros((n, 6), dtype=torch.float32, device=dev)
+    gpose = torch.zeros(6, dtype=torch.float64, device=dev)
+    scratch = torch.zeros(B.lib().sicp_raycast_backward_scratch_size(), dtype=torch.uint8,
+                          device=dev)
+
+    def backward(gv_t, gn_t, sdf_t):
+        B.check(B.lib().sicp_raycast_backward(
+            m, B.image(zmin), B.image(zmax), B.C.byref(pose), invf, invf, w["cx"], w["cy"],
+            w["cols"], w["rows"], pix.data_ptr(), n, gv_t.data_ptr(),
+            gn_t.data_ptr() if gn_t is not None else None,
+            sdf_t.data_ptr() if sdf_t is not None else None, gpx.data_ptr(), gpose.data_ptr(),
+            scratch.data_ptr(), scratch.numel(), B.stream_ptr()))
+
+    for _ in range(args.warmup):
+        backward(gv, gn, gsdf)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        backward(gv, gn, gsdf)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    touched = int(torch.count_nonzero(gsdf).item())
+    # point-to-plane pose derivative against the ICP Jacobian
+    nn = torch.from_numpy(nh[ys[sel], xs[sel], :3].copy()).to(dev)
+    backward(nn, None, None)
+    torch.cuda.synchronize()
+    a = gpx.cpu().numpy().astype(np.float64)
+    nv = nh[ys[sel], xs[sel], :3].astype(np.float64)
+    vv = vh[ys[sel], xs[sel], :3].astype(np.float64)
+    J = np.concatenate([-nv, np.cross(nv, vv)], 1)
+    cos = (a * J).sum(1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(J, axis=1) + 1e-30)
+    ratio = (a * J).sum(1) / ((J * J).sum(1) + 1e-30)
+    out = {
+        "metric": "ms per raycast-adjoint call, 10k pixels, 1280x720 / 2 mm voxels",
+        "value": round(ms, 4),
+        "unit": "ms/call",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural room, seed 0, 100 frames fused)",
+        "config": {"workload": "adjoint720", "pixels": n, "frames_fused": nfr,
+                   "allocated_blocks": B.map_stats(m)["allocated_blocks"]},
+        "pixels_per_s": round(n / (ms * 1e-3), 1),
+        "touched_voxels": touched,
+        "last_frame_tracked": res["tracked"],
+        "icp_jacobian_agreement": {
+            "what": "per pixel, n . dv/dxi from the adjoint against [-n, n x v]",
+            "cosine_median": round(float(np.median(cos)), 6),
+            "cosine_above_0_99": round(float(np.mean(cos > 0.99)), 4),
+            "scale_median": round(float(np.median(ratio)), 4)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,9 +413,10 @@ def main():
                     help="skip the pass with the photometric tracking term")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the graph-form pass (counter collection runs)")
-    ap.add_argument("--workload", default="vga", choices=("vga", "batched8"),
+    ap.add_argument("--workload", default="vga", choices=("vga", "batched8", "adjoint720"),
                     help="vga: the 640x480 / 5 mm sequence (default); batched8: eight "
-                         "independent sequences at once")
+                         "independent sequences at once; adjoint720: the raycast adjoint at "
+                         "1280x720 / 2 mm")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -310,6 +428,11 @@ def main():
         if args.steps == 290:  # the default frame count is that of the 640x480 sequence
             args.steps = 190
         run_batched(args)
+        return
+    if args.workload == "adjoint720":
+        if args.steps == 290:
+            args.steps = 50
+        run_adjoint(args)
         return
 
     import torch
