@@ -115,6 +115,12 @@ typedef struct sicp_map_config {
                                neighbour label pass (UnifyNeighbor, as the
                                reference's FuseMapColor runs it,
                                FuseMap.cu:1140-1160); 0 (default): not run */
+    int32_t tsdf_half;      /* 1: the precision form: each voxel's sdf is
+                               stored as IEEE half (round to nearest even)
+                               beside its weight in one 4-byte word, the
+                               colour in the semantic record; every update and
+                               interpolation is FP32. 0 (default): FP32 sdf.
+                               Downloads return the FP32 record either way. */
 } sicp_map_config;
 
 /* Fills `cfg` with the reference's constants scaled to the given voxel size

@@ -67,6 +67,8 @@ _SIGS = {
     "oracle_mesh_scene": (C.c_uint32, [_P, _P, _P, _P, C.c_uint32, _P, _P, _P, _P]),
     "oracle_unify_neighbor": (None, [_P]),
     "oracle_map_clone_fp64": (_P, [_P]),
+    "oracle_f32_to_f16": (C.c_uint16, [C.c_float]),
+    "oracle_f16_to_f32": (C.c_float, [C.c_uint16]),
     "oracle_raycast_fp64": (None, [_P, _P, _P, C.c_int, C.c_int, _P, _P, C.c_int, _P, C.c_double,
                                    C.c_double, C.c_double, C.c_double]),
     "oracle_map_get_sdf": (C.c_double, [_P, C.c_int, C.c_int]),

@@ -103,6 +103,9 @@ uint32_t oracle_map_visible(void* map, sicp_block* out, uint32_t cap);
 /* UnifyNeighbor (Mapping/FuseMap.cu:757-948) over the last visible list,
  * every voxel seeing the labels before the pass. */
 void oracle_unify_neighbor(void* map);
+/* IEEE binary16 rounding (nearest even) of the precision form, and back. */
+uint16_t oracle_f32_to_f16(float f);
+float oracle_f16_to_f32(uint16_t h);
 /* A new FP64 map with exactly the state of the FP32 map `src`. */
 void* oracle_map_clone_fp64(void* src);
 /* Raycast of an FP64 map at a double pose (4x4 camera -> world). */

@@ -25,6 +25,59 @@
 
 namespace oracle {
 
+// IEEE binary16 of a float, rounded to nearest even (overflow to infinity,
+// subnormal results rounded at their own precision, NaN kept quiet), and
+// back. The precision form stores the sdf this way.
+uint16_t f32_to_f16(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, sizeof(x));
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t e = (x >> 23) & 0xffu;
+    uint32_t m = x & 0x7fffffu;
+    if (e == 0xffu) return (uint16_t)(sign | 0x7c00u | (m ? 0x200u | (m >> 13) : 0u));
+    const int E = (int)e - 127 + 15;
+    if (E >= 31) return (uint16_t)(sign | 0x7c00u);
+    if (E <= 0) {
+        if (E < -10) return (uint16_t)sign;
+        m |= 0x800000u;
+        const int shift = 14 - E;
+        uint32_t hm = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u), halfway = 1u << (shift - 1);
+        if (rem > halfway || (rem == halfway && (hm & 1u))) ++hm;
+        return (uint16_t)(sign | hm);
+    }
+    uint32_t h = sign | ((uint32_t)E << 10) | (m >> 13);
+    const uint32_t rem = m & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;
+    return (uint16_t)h;
+}
+float f16_to_f32(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, mnt = h & 0x3ffu;
+    uint32_t bits;
+    if (e == 0x1fu)
+        bits = sign | 0x7f800000u | (mnt << 13);
+    else if (e != 0)
+        bits = sign | ((e + 112u) << 23) | (mnt << 13);
+    else if (mnt == 0)
+        bits = sign;
+    else {
+        int sh = 0;
+        uint32_t mm = mnt;
+        while (!(mm & 0x400u)) {
+            mm <<= 1;
+            ++sh;
+        }
+        bits = sign | ((uint32_t)(113 - sh) << 23) | ((mm & 0x3ffu) << 13);
+    }
+    float f;
+    std::memcpy(&f, &bits, sizeof(f));
+    return f;
+}
+template <class R> static inline R store_sdf(const OMap<R>& m, R v) {
+    if (std::is_same<R, float>::value && m.cfg.tsdf_half) return (R)f16_to_f32(f32_to_f16((float)v));
+    return v;
+}
+
 static const int kBS = 8;     // DeviceMap::BlockSize (DeviceMap.h:166)
 static const int kBS3 = 512;  // DeviceMap::BlockSize3
 
@@ -371,6 +424,7 @@ static uint32_t fuse_voxels(OMap<R>& m, const R* depth, const uint8_t* rgb, cons
                         vs = (vs * (R)w + sdf) / (R)(w + 1);
                         g.weight = (uint8_t)std::min(wmax, w + 1);
                     }
+                    vs = store_sdf(m, vs);  // the precision form keeps a half
                     g.sdf = (float)vs;
                     if (num_detection <= 0) continue;
                     // ---- analyzeSemantics ----
@@ -1129,6 +1183,9 @@ void oracle_map_export(void* map, sicp_hash_entry* table, sicp_hash_entry* exces
     else
         go(h->f);
 }
+
+uint16_t oracle_f32_to_f16(float f) { return f32_to_f16(f); }
+float oracle_f16_to_f32(uint16_t h) { return f16_to_f32(h); }
 
 // Test hook: sets one voxel's weight and label state (FP32 map).
 void oracle_map_set_label(void* map, int slot, int idx, int weight, int label, int count,

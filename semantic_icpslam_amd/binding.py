@@ -43,7 +43,7 @@ class MapConfig(C.Structure):
         ("num_buckets", C.c_int32), ("num_blocks", C.c_int32),
         ("max_weight", C.c_int32), ("max_label_count", C.c_int32),
         ("request_capacity", C.c_int32), ("max_cols", C.c_int32), ("max_rows", C.c_int32),
-        ("unify_labels", C.c_int32),
+        ("unify_labels", C.c_int32), ("tsdf_half", C.c_int32),
     ]
 
 

@@ -43,6 +43,30 @@ template <class T> static int dmalloc(sicp_map* m, T** p, size_t count) {
     return SICP_OK;
 }
 
+// IEEE half to float (exact), for downloads of the half form.
+static float half_to_float(uint32_t h) {
+    const uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, mnt = h & 0x3ffu;
+    uint32_t bits;
+    if (e == 0x1fu) {
+        bits = sign | 0x7f800000u | (mnt << 13);
+    } else if (e != 0) {
+        bits = sign | ((e + 112u) << 23) | (mnt << 13);
+    } else if (mnt == 0) {
+        bits = sign;
+    } else {  // subnormal: normalise
+        int sh = 0;
+        uint32_t m = mnt;
+        while (!(m & 0x400u)) {
+            m <<= 1;
+            ++sh;
+        }
+        bits = sign | ((uint32_t)(113 - sh) << 23) | ((m & 0x3ffu) << 13);
+    }
+    float f;
+    std::memcpy(&f, &bits, sizeof(f));
+    return f;
+}
+
 int map_alloc(const sicp_map_config& cfg, sicp_map** out) {
     if (cfg.voxel_size <= 0.f || cfg.truncation <= 0.f) return invalid("voxel_size / truncation must be > 0");
     if (cfg.num_buckets <= 0 || cfg.num_blocks <= 0) return invalid("num_buckets / num_blocks must be > 0");
@@ -53,6 +77,7 @@ int map_alloc(const sicp_map_config& cfg, sicp_map** out) {
     if (cfg.max_weight < 1 || cfg.max_weight > 255) return invalid("max_weight outside 1..255");
     if (cfg.max_label_count < 1 || cfg.max_label_count > 65535) return invalid("max_label_count outside 1..65535");
     if (cfg.unify_labels != 0 && cfg.unify_labels != 1) return invalid("unify_labels must be 0 or 1");
+    if (cfg.tsdf_half != 0 && cfg.tsdf_half != 1) return invalid("tsdf_half must be 0 or 1");
     sicp_map* m = new sicp_map();
     m->cfg = cfg;
     int rc = SICP_OK;
@@ -61,7 +86,9 @@ int map_alloc(const sicp_map_config& cfg, sicp_map** out) {
     const size_t px = (size_t)cfg.max_cols * cfg.max_rows;
     MapDev& d = m->dev;
     if ((rc = dmalloc(m, &d.table, 2 * nb)) || (rc = dmalloc(m, &d.excess, nblk)) ||
-        (rc = dmalloc(m, &d.blocks, nblk)) || (rc = dmalloc(m, &d.geom, nblk * kBlockVoxels)) ||
+        (rc = dmalloc(m, &d.blocks, nblk)) ||
+        (cfg.tsdf_half ? (rc = dmalloc(m, &d.geom_h, nblk * kBlockVoxels))
+                       : (rc = dmalloc(m, &d.geom, nblk * kBlockVoxels))) ||
         (rc = dmalloc(m, &d.sem, nblk * kBlockVoxels)) || (rc = dmalloc(m, &m->visible, nblk)) ||
         (rc = dmalloc(m, &m->ctr, 1))) {
         map_free(m);
@@ -107,6 +134,7 @@ int map_alloc(const sicp_map_config& cfg, sicp_map** out) {
     d.dmax = cfg.depth_max;
     d.max_weight = cfg.max_weight;
     d.max_count = cfg.max_label_count;
+    d.half = cfg.tsdf_half;
     rc = launch_map_reset(m->dev, m->ctr, m->alloc, nullptr);
     if (rc == SICP_OK) {
         hipError_t e = hipDeviceSynchronize();
@@ -629,12 +657,31 @@ int sicp_map_download(sicp_map* map, sicp_hash_entry* table, sicp_hash_entry* ex
     if (blocks)
         SICP_HIP(hipMemcpy(blocks, map->dev.blocks, nblk * sizeof(sicp_block),
                            hipMemcpyDeviceToHost));
-    if (geom)
+    if (geom && !map->dev.half)
         SICP_HIP(hipMemcpy(geom, map->dev.geom, nblk * kBlockVoxels * sizeof(sicp_voxel_geom),
                            hipMemcpyDeviceToHost));
     if (sem)
         SICP_HIP(hipMemcpy(sem, map->dev.sem, nblk * kBlockVoxels * sizeof(sicp_voxel_sem),
                            hipMemcpyDeviceToHost));
+    if (geom && map->dev.half) {
+        // the half form widened to the FP32 record; the colour from the
+        // semantic record's spare bytes (cleared in the returned records)
+        const size_t n = nblk * kBlockVoxels;
+        std::vector<uint32_t> w(n);
+        std::vector<sicp_voxel_sem> s(n);
+        SICP_HIP(hipMemcpy(w.data(), map->dev.geom_h, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        SICP_HIP(hipMemcpy(s.data(), map->dev.sem, n * sizeof(sicp_voxel_sem),
+                           hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            geom[i].sdf = half_to_float(w[i] & 0xffffu);
+            geom[i].weight = (uint8_t)((w[i] >> 16) & 0xffu);
+            geom[i].r = s[i].pad[0];
+            geom[i].g = s[i].pad[1];
+            geom[i].b = s[i].pad[2];
+        }
+        if (sem)
+            for (size_t i = 0; i < n; ++i) sem[i].pad[0] = sem[i].pad[1] = sem[i].pad[2] = 0;
+    }
     return SICP_OK;
 }
 
@@ -714,7 +761,9 @@ int sicp_map_save(sicp_map* map, const char* path) {
     if ((rc = put(file.f, map->dev.table, h.table_entries, "map_save: write failed")) ||
         (rc = put(file.f, map->dev.excess, nb, "map_save: write failed")) ||
         (rc = put(file.f, map->dev.blocks, nb, "map_save: write failed")) ||
-        (rc = put(file.f, map->dev.geom, nb * kBlockVoxels, "map_save: write failed")) ||
+        (map->dev.half
+             ? (rc = put(file.f, map->dev.geom_h, nb * kBlockVoxels, "map_save: write failed"))
+             : (rc = put(file.f, map->dev.geom, nb * kBlockVoxels, "map_save: write failed"))) ||
         (rc = put(file.f, map->dev.sem, nb * kBlockVoxels, "map_save: write failed")) ||
         (rc = put(file.f, map->visible, (size_t)h.visible, "map_save: write failed")))
         return rc;
@@ -736,7 +785,7 @@ int sicp_map_load(sicp_map* map, const char* path) {
     if (a.num_buckets != b.num_buckets || a.voxel_size != b.voxel_size ||
         a.truncation != b.truncation || a.depth_min != b.depth_min ||
         a.depth_max != b.depth_max || a.max_weight != b.max_weight ||
-        a.max_label_count != b.max_label_count)
+        a.max_label_count != b.max_label_count || a.tsdf_half != b.tsdf_half)
         return invalid("map_load: the snapshot was written by a map of another configuration");
     if (h.blocks > (uint64_t)a.num_blocks || h.visible > h.blocks ||
         h.table_entries != 2 * (uint64_t)a.num_buckets)
@@ -748,7 +797,9 @@ int sicp_map_load(sicp_map* map, const char* path) {
     if ((rc = get(file.f, map->dev.table, h.table_entries, "map_load: truncated file")) ||
         (rc = get(file.f, map->dev.excess, nb, "map_load: truncated file")) ||
         (rc = get(file.f, map->dev.blocks, nb, "map_load: truncated file")) ||
-        (rc = get(file.f, map->dev.geom, nb * kBlockVoxels, "map_load: truncated file")) ||
+        (map->dev.half
+             ? (rc = get(file.f, map->dev.geom_h, nb * kBlockVoxels, "map_load: truncated file"))
+             : (rc = get(file.f, map->dev.geom, nb * kBlockVoxels, "map_load: truncated file"))) ||
         (rc = get(file.f, map->dev.sem, nb * kBlockVoxels, "map_load: truncated file")) ||
         (rc = get(file.f, map->visible, (size_t)h.visible, "map_load: truncated file")))
         return rc;

@@ -329,7 +329,10 @@ __global__ __launch_bounds__(256) void alloc_clear_voxels_kernel(MapDev m,
     for (uint32_t r = blockIdx.x; r < created; r += gridDim.x) {
         size_t off = (size_t)(base + r) * kBlockVoxels;
         for (int v = threadIdx.x; v < kBlockVoxels; v += blockDim.x) {
-            ((uint2*)m.geom)[off + v] = g;
+            if (m.half)
+                m.geom_h[off + v] = kEmptyHalfVoxel;
+            else
+                ((uint2*)m.geom)[off + v] = g;
             ((uint3*)m.sem)[off + v] = z;
         }
     }
@@ -584,8 +587,21 @@ __global__ __launch_bounds__(256) void fuse_kernel(MapDev m, const MapCounters* 
             sdf = fminf(1.0f, sdf / T);
             const uint8_t* c = &rgb.at(v, 3 * u);
             const uint8_t c0 = c[0], c1 = c[1], c2 = c[2];
-            sicp_voxel_geom* gp = m.geom + vbase + (size_t)lz * 64;
-            sicp_voxel_geom gv = *gp;
+            const size_t kv = vbase + (size_t)lz * 64;
+            sicp_voxel_sem* sp = m.sem + kv;
+            sicp_voxel_geom gv;
+            sicp_voxel_sem sh;  // half form: the whole semantic record (it holds the colour)
+            if (m.half) {
+                const uint32_t w = m.geom_h[kv];
+                sh = *sp;
+                gv.sdf = h2f(w);
+                gv.weight = (uint8_t)((w >> 16) & 0xffu);
+                gv.r = sh.pad[0];
+                gv.g = sh.pad[1];
+                gv.b = sh.pad[2];
+            } else {
+                gv = m.geom[kv];
+            }
             ++updated;
             bool new_voxel = gv.weight == 0;
             if (new_voxel) {
@@ -595,11 +611,12 @@ __global__ __launch_bounds__(256) void fuse_kernel(MapDev m, const MapCounters* 
                 gv.g = c1;
                 gv.b = c2;
             }
-            sicp_voxel_sem* sp = m.sem + vbase + (size_t)lz * 64;
             sicp_voxel_sem sv;
             const bool semantics = g.num_detection > 0;
             if (new_voxel) {
                 sv = sicp_voxel_sem{};
+            } else if (m.half) {
+                sv = sh;
             } else if (semantics) {
                 sv = *sp;
             } else {
@@ -695,11 +712,18 @@ __global__ __launch_bounds__(256) void fuse_kernel(MapDev m, const MapCounters* 
                         }
                     }
                 }
-                *sp = sv;
-            } else if (new_voxel) {
-                *sp = sv;
             }
-            *gp = gv;
+            if (m.half) {
+                // the sdf rounded to half on the store; the colour in the spare bytes
+                sv.pad[0] = gv.r;
+                sv.pad[1] = gv.g;
+                sv.pad[2] = gv.b;
+                *sp = sv;
+                m.geom_h[kv] = f2h(gv.sdf) | ((uint32_t)gv.weight << 16);
+            } else {
+                if (semantics || new_voxel) *sp = sv;
+                m.geom[kv] = gv;
+            }
         }
     }
     // One count per wavefront into its own slot (a single device-wide counter

@@ -46,6 +46,50 @@ SICP_DEV int voxel_local_idx(i3 v) {
     return z * kBlockSide * kBlockSide + y * kBlockSide + x;
 }
 
+// ---------------------------------------------------------- voxel storage --
+// The FP32 form keeps sicp_voxel_geom records; the half form one 32-bit word
+// per voxel (sdf as IEEE half in bits 0-15, weight in bits 16-23) and the
+// colour in the semantic record's three spare bytes. Conversions round to
+// nearest even (v_cvt_f16_f32) and widen exactly.
+SICP_DEV float h2f(uint32_t bits) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(bits & 0xffffu));
+}
+SICP_DEV uint32_t f2h(float v) { return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v); }
+constexpr uint32_t kEmptyHalfVoxel = 0x00007fffu;  // NaN sdf, weight 0
+
+// sdf and weight of voxel k with one load.
+SICP_DEV void vox_load(const MapDev& m, size_t k, float& sdf, uint32_t& weight) {
+    if (m.half) {
+        const uint32_t w = m.geom_h[k];
+        sdf = h2f(w);
+        weight = (w >> 16) & 0xffu;
+    } else {
+        const uint2 w = ((const uint2*)m.geom)[k];
+        sdf = __uint_as_float(w.x);
+        weight = w.y & 0xffu;
+    }
+}
+SICP_DEV float vox_sdf(const MapDev& m, size_t k) {
+    return m.half ? h2f(m.geom_h[k]) : m.geom[k].sdf;
+}
+SICP_DEV uint32_t vox_weight(const MapDev& m, size_t k) {
+    return m.half ? (m.geom_h[k] >> 16) & 0xffu : (uint32_t)m.geom[k].weight;
+}
+// Colour of voxel k.
+SICP_DEV void vox_rgb(const MapDev& m, size_t k, uint8_t& r, uint8_t& g, uint8_t& b) {
+    if (m.half) {
+        const sicp_voxel_sem& s = m.sem[k];
+        r = s.pad[0];
+        g = s.pad[1];
+        b = s.pad[2];
+    } else {
+        const sicp_voxel_geom& q = m.geom[k];
+        r = q.r;
+        g = q.g;
+        b = q.b;
+    }
+}
+
 struct EntryView {
     uint64_t key;
     int slot, next;

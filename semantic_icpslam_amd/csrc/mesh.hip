@@ -206,14 +206,14 @@ SICP_DEV void neighbour_slots(const MapDev& m, const sicp_block& b, int* nslot) 
 
 // A voxel at block-relative voxel coordinates in [-8, 15] (FindVoxel,
 // DeviceMap.cu:97-103): the default voxel (NaN) when its block is missing.
-SICP_DEV const sicp_voxel_geom* voxel_at(const MapDev& m, const int* nslot, int x, int y, int z) {
+SICP_DEV long voxel_at(const int* nslot, int x, int y, int z) {
     const int s = nslot[((z >> 3) + 1) * 9 + ((y >> 3) + 1) * 3 + ((x >> 3) + 1)];
-    if (s < 0) return nullptr;
-    return m.geom + (size_t)s * 512 + ((z & 7) << 6) + ((y & 7) << 3) + (x & 7);
+    if (s < 0) return -1;
+    return (long)s * 512 + ((z & 7) << 6) + ((y & 7) << 3) + (x & 7);
 }
 SICP_DEV float sdf_at(const MapDev& m, const int* nslot, int x, int y, int z) {
-    const sicp_voxel_geom* p = voxel_at(m, nslot, x, y, z);
-    return p ? p->sdf : qnanf();
+    const long k = voxel_at(nslot, x, y, z);
+    return k >= 0 ? vox_sdf(m, (size_t)k) : qnanf();
 }
 
 // MeshEngine::interp (MeshScene.cu:156-164).
@@ -261,11 +261,13 @@ SICP_DEV void build_cell(const MapDev& m, const int* nslot, const MeshTables& tb
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int cx = d_corner(k, 0), cy = d_corner(k, 1), cz = d_corner(k, 2);
-        const sicp_voxel_geom* p = voxel_at(m, nslot, x + cx, y + cy, z + cz);
-        cl.s[k] = p ? p->sdf : qnanf();
-        cl.c[k][0] = p ? p->r : 0;
-        cl.c[k][1] = p ? p->g : 0;
-        cl.c[k][2] = p ? p->b : 0;
+        const long q = voxel_at(nslot, x + cx, y + cy, z + cz);
+        cl.s[k] = q >= 0 ? vox_sdf(m, (size_t)q) : qnanf();
+        uint8_t r = 0, g = 0, b = 0;
+        if (q >= 0) vox_rgb(m, (size_t)q, r, g, b);
+        cl.c[k][0] = r;
+        cl.c[k][1] = g;
+        cl.c[k][2] = b;
         ok = ok && !(cl.s[k] == 1.0f || isnan_(cl.s[k]));
     }
     if (!ok) return;

@@ -158,7 +158,7 @@ SICP_DEV long find_voxel_i(const MapDev& m, int vx, int vy, int vz, BlockCache& 
     return (long)slot * kBlockVoxels + local_of(vx, vy, vz);
 }
 
-SICP_DEV float sdf_at(const MapDev& m, long k) { return k < 0 ? qnanf() : m.geom[k].sdf; }
+SICP_DEV float sdf_at(const MapDev& m, long k) { return k < 0 ? qnanf() : vox_sdf(m, (size_t)k); }
 
 // One axis of an interpolated read centred at coordinate q: the voxel
 // coordinates of q and q + 1 (the reference adds 1.0f to the float point)
@@ -228,11 +228,11 @@ SICP_DEV float interp_axes(const MapDev& m, AxisPair X, AxisPair Y, AxisPair Z, 
     float sd[8];
     if (cache_has(c, X.v0 >> 3, Y.v0 >> 3, Z.v0 >> 3) && (X.v1 >> 3) == c.x &&
         (Y.v1 >> 3) == c.y && (Z.v1 >> 3) == c.z) {
-        const sicp_voxel_geom* g = m.geom + (long)c.slot * kBlockVoxels;
+        const size_t base = (size_t)c.slot * kBlockVoxels;
 #pragma unroll
         for (int i = 0; i < 8; ++i)
-            sd[i] = g[local_of((i & 1) ? X.v1 : X.v0, (i & 2) ? Y.v1 : Y.v0,
-                               (i & 4) ? Z.v1 : Z.v0)].sdf;
+            sd[i] = vox_sdf(m, base + local_of((i & 1) ? X.v1 : X.v0, (i & 2) ? Y.v1 : Y.v0,
+                                               (i & 4) ? Z.v1 : Z.v0));
         valid = true;
     } else {
         CornerBlocks cb;
@@ -241,10 +241,9 @@ SICP_DEV float interp_axes(const MapDev& m, AxisPair X, AxisPair Y, AxisPair Z, 
         for (int i = 0; i < 8; ++i) {
             const int sl = corner_slot(cb, X, Y, Z, i & 1, (i >> 1) & 1, i >> 2);
             sd[i] = sl < 0 ? qnanf()
-                           : m.geom[(long)sl * kBlockVoxels +
-                                    local_of((i & 1) ? X.v1 : X.v0, (i & 2) ? Y.v1 : Y.v0,
-                                             (i & 4) ? Z.v1 : Z.v0)]
-                                 .sdf;
+                           : vox_sdf(m, (size_t)sl * kBlockVoxels +
+                                            local_of((i & 1) ? X.v1 : X.v0, (i & 2) ? Y.v1 : Y.v0,
+                                                     (i & 4) ? Z.v1 : Z.v0));
         }
         valid = corner_slot(cb, X, Y, Z, 1, 1, 1) >= 0;
     }
@@ -364,9 +363,9 @@ __global__ __launch_bounds__(64) void raycast_kernel(MapDev m, View<float4> vmap
             if (k < 0) {
                 sdf = nan;
             } else {
-                uint2 w = ((const uint2*)m.geom)[k];
-                sdf = __uint_as_float(w.x);
-                if ((w.y & 0xffu) == 0u) valid_sdf = false;
+                uint32_t wt;
+                vox_load(m, (size_t)k, sdf, wt);
+                if (wt == 0u) valid_sdf = false;
             }
             if (!valid_sdf) {
                 step = (float)kBlockSide;
@@ -547,10 +546,10 @@ __global__ __launch_bounds__(256) void raycast_long_kernel(MapDev m, View<float4
                 bool v;
                 const long idx = find_voxel_i(m, floor_i(P.x), floor_i(P.y), floor_i(P.z), cache, v);
                 if (idx >= 0) {
-                    const uint2 w = ((const uint2*)m.geom)[idx];
-                    sv = __uint_as_float(w.x);
+                    uint32_t wt;
+                    vox_load(m, (size_t)idx, sv, wt);
                     exists = 1;
-                    wok = (w.y & 0xffu) != 0u;
+                    wok = wt != 0u;
                 }
             }
             float sp = __shfl_up(sv, 1, kRayGroup);
@@ -661,7 +660,7 @@ SICP_DEV void tri_read(const MapDev& m, v3 p, TriRead& t) {
             t.ok = false;
         } else {
             t.k[i] = (long)slot * kBlockVoxels + local_of(vx, vy, vz);
-            sd[i] = m.geom[t.k[i]].sdf;
+            sd[i] = vox_sdf(m, (size_t)t.k[i]);
         }
     }
     const float fx = X.frac, fy = Y.frac, fz = Z.frac;
@@ -724,8 +723,9 @@ SICP_DEV bool adjoint_pixel(const MapDev& m, const View<float>& zmin, const View
         if (k < 0) {
             sdf = qnanf();
         } else {
-            sdf = m.geom[k].sdf;
-            if (m.geom[k].weight == 0) valid_sdf = false;
+            uint32_t wt;
+            vox_load(m, (size_t)k, sdf, wt);
+            if (wt == 0u) valid_sdf = false;
         }
         k0 = k;
         interp0 = false;

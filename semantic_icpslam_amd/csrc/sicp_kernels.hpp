@@ -169,7 +169,9 @@ struct MapDev {
     sicp_hash_entry* table;   // 2 * nb
     sicp_hash_entry* excess;  // num_blocks, indexed by heap slot
     sicp_block* blocks;       // num_blocks
-    sicp_voxel_geom* geom;    // num_blocks * 512
+    sicp_voxel_geom* geom;    // num_blocks * 512 (FP32 form)
+    uint32_t* geom_h;         // num_blocks * 512 (half form): sdf half | weight << 16
+    int half;                 // 1: the half form (geom_h; colour in sem.pad)
     sicp_voxel_sem* sem;      // num_blocks * 512
     uint32_t nb, nb_mask;     // nb_mask = nb-1 when nb is a power of two, else 0
     int num_blocks;

@@ -24,8 +24,9 @@ namespace sicp {
 namespace {
 
 // Snapshot word: bits 0-7 label, bit 8 weight != 0, bits 16-31 count.
-SICP_DEV uint32_t pack_label(const sicp_voxel_geom& g, const sicp_voxel_sem& s) {
-    return (uint32_t)s.label | (g.weight != 0 ? 0x100u : 0u) | ((uint32_t)s.count << 16);
+SICP_DEV uint32_t pack_label(const MapDev& m, size_t k) {
+    const sicp_voxel_sem& s = m.sem[k];
+    return (uint32_t)s.label | (vox_weight(m, k) != 0 ? 0x100u : 0u) | ((uint32_t)s.count << 16);
 }
 
 __global__ __launch_bounds__(256) void unify_snapshot_kernel(MapDev m, const MapCounters* ctr,
@@ -39,7 +40,7 @@ __global__ __launch_bounds__(256) void unify_snapshot_kernel(MapDev m, const Map
         if (threadIdx.x == 0) mark[slot] = epoch;
         const size_t base = (size_t)slot * 512;
         for (int i = threadIdx.x; i < 512; i += blockDim.x)
-            snap[base + i] = pack_label(m.geom[base + i], m.sem[base + i]);
+            snap[base + i] = pack_label(m, base + i);
     }
 }
 
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(64) void unify_kernel(MapDev m, const MapCounters* 
                 if (from_snap)
                     w = snap[vi];
                 else
-                    w = pack_label(m.geom[vi], m.sem[vi]);
+                    w = pack_label(m, vi);
                 // UpdateNeighborLabel (FuseMap.cu:757-797)
                 if (!(w & 0x100u)) continue;
                 const int tl = (int)(w & 0xffu), tc = (int)(w >> 16);

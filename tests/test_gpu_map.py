@@ -66,19 +66,25 @@ def _compare_maps(gm, om):
     return n
 
 
+# The precision form: sdf stored as IEEE half, weights capped at 128.
+HALF = dict(tsdf_half=1, max_weight=128)
+
+
+@pytest.mark.parametrize("half", [False, True])
 @pytest.mark.parametrize("labels", [True, False])
-def test_fusion_small_bit_exact(dev, labels):
+def test_fusion_small_bit_exact(dev, labels, half):
     c = CFG_SMALL
-    cfg = map_cfg(c)
+    cfg = map_cfg(c, **(HALF if half else {}))
     gm, om = B.Map(cfg), O.OracleMap(cfg)
     _fuse_both(dev, c, range(0, 12, 2), gm, om, labels=labels)
     n = _compare_maps(gm, om)
     assert n > 500
 
 
-def test_fusion_vga_bit_exact(dev):
+@pytest.mark.parametrize("half", [False, True])
+def test_fusion_vga_bit_exact(dev, half):
     c = CFG_VGA
-    cfg = map_cfg(c)
+    cfg = map_cfg(c, **(HALF if half else {}))
     gm, om = B.Map(cfg), O.OracleMap(cfg)
     _fuse_both(dev, c, [0, 3], gm, om)
     n = _compare_maps(gm, om)
@@ -137,9 +143,10 @@ def test_empty_frame_no_visible(dev):
     assert rc == B.SICP_ERR_NO_VISIBLE and vh.value == 0
 
 
+@pytest.mark.parametrize("half", [False, True])
 @pytest.mark.parametrize("c", [CFG_SMALL, CFG_VGA])
-def test_prepass_and_raycast_bit_exact(dev, c):
-    cfg = map_cfg(c)
+def test_prepass_and_raycast_bit_exact(dev, c, half):
+    cfg = map_cfg(c, **(HALF if half else {}))
     gm, om = B.Map(cfg), O.OracleMap(cfg)
     s, _ = _fuse_both(dev, c, [0, 2, 4], gm, om)
     pose = B.pose_f(s.pose(4))
@@ -167,13 +174,14 @@ def test_prepass_and_raycast_bit_exact(dev, c):
         assert gm.stats()["ray_handover"] > 0
 
 
-def test_map_snapshot_round_trip(dev, tmp_path):
+@pytest.mark.parametrize("half", [False, True])
+def test_map_snapshot_round_trip(dev, tmp_path, half):
     """sicp_map_save / sicp_map_load (System::WriteMapToDisk / ReadMapFromDisk):
     a map saved after frames 0-4 and loaded into a fresh map continues with
     frames 6-8 exactly as the uninterrupted map does; a snapshot of another
     configuration is refused."""
     c = CFG_SMALL
-    cfg = map_cfg(c)
+    cfg = map_cfg(c, **(HALF if half else {}))
     ga, om = B.Map(cfg), O.OracleMap(cfg)
     s, _ = _fuse_both(dev, c, [0, 2, 4], ga, om)
     path = str(tmp_path / "map.bin").encode()

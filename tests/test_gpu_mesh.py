@@ -13,9 +13,10 @@ from test_gpu_map import _fuse_both
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("c,frames", [(CFG_SMALL, [0, 2, 4]), (CFG_VGA, [0, 3])])
-def test_mesh_vs_oracle(dev, c, frames):
-    cfg = map_cfg(c)
+@pytest.mark.parametrize("c,frames,half", [(CFG_SMALL, [0, 2, 4], 0), (CFG_SMALL, [0, 2, 4], 1),
+                                          (CFG_VGA, [0, 3], 0)])
+def test_mesh_vs_oracle(dev, c, frames, half):
+    cfg = map_cfg(c, tsdf_half=half)
     gm, om = B.Map(cfg), O.OracleMap(cfg)
     _fuse_both(dev, c, frames, gm, om)
     tables = B.mesh_tables()

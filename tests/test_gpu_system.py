@@ -15,9 +15,9 @@ from oracle import binding as O
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("c,frames", [(CFG_SMALL, 12), (CFG_VGA, 4)])
-def test_system_vs_oracle(dev, c, frames):
-    cfg = sys_cfg(c)
+@pytest.mark.parametrize("c,frames,half", [(CFG_SMALL, 12, 0), (CFG_VGA, 4, 0), (CFG_SMALL, 8, 1)])
+def test_system_vs_oracle(dev, c, frames, half):
+    cfg = sys_cfg(c, **(dict(tsdf_half=1, max_weight=128) if half else {}))
     g = B.System(cfg)
     o = O.OracleSystem(cfg)
     s = synth(c)

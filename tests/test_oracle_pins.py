@@ -879,3 +879,23 @@ def test_raycast_pose_derivative_matches_plane_model():
             assert np.abs(Jfd[y, x] - want).max() < 1e-4 * np.abs(want).max(), (y, x)
             checked += 1
     assert checked > 30
+
+
+def test_half_rounding_matches_numpy():
+    """The precision form's float -> binary16 rounding (nearest even,
+    subnormal results at their own precision, overflow to infinity) against
+    numpy's conversion, over normal, subnormal, halfway and special values."""
+    rng = np.random.default_rng(17)
+    xs = np.concatenate([
+        rng.uniform(-1.0, 1.0, 20000),
+        rng.uniform(-1e-4, 1e-4, 20000),          # half subnormals
+        np.ldexp(rng.integers(1, 1 << 11, 4000) + 0.5, -24 - 10),  # halfway points
+        np.float32([0.0, -0.0, 1.0, -1.0, 65504.0, 65520.0, 1e6, -1e6, 6.1e-5, 5.96e-8, 2.98e-8,
+                    np.inf, -np.inf]),
+    ]).astype(np.float32)
+    got = np.array([O.lib().oracle_f32_to_f16(float(x)) for x in xs], np.uint16)
+    want = xs.astype(np.float16).view(np.uint16)
+    assert np.array_equal(got, want), xs[got != want][:10]
+    back = np.array([O.lib().oracle_f16_to_f32(int(h)) for h in want], np.float32)
+    assert np.array_equal(back, want.view(np.float16).astype(np.float32))
+    assert O.lib().oracle_f32_to_f16(float("nan")) & 0x7c00 == 0x7c00
