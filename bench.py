@@ -142,28 +142,119 @@ def cpu_small():
     return out
 
 
-def pose_error_vs_fp64(w, frames, cfg, m=30):
-    """Per-frame pose of the GPU system against the FP64 oracle on the first m
-    frames (an untimed pass): max rotation angle (rad) of T_fp64^-1 T and max
-    translation difference (m)."""
+def pose_error_vs_fp64(w, frames, cfgs, m=30):
+    """Per-frame pose of the GPU system, for each configuration in `cfgs`,
+    against the FP64 oracle on the first m frames (an untimed pass): max
+    rotation angle (rad) of T_fp64^-1 T and max translation difference (m)."""
     import torch
     from oracle import binding as O
     from semantic_icpslam_amd import binding as B
     dev = torch.device("cuda:0")
-    g = B.System(cfg)
-    o = O.OracleSystem(cfg, fp64=True)
-    rot, tr = [], []
+    o = O.OracleSystem(cfgs[0], fp64=True)
+    ref = [o.grab(d, rgb, lab)["pose"] for d, rgb, lab in frames[:m]]
+    out = []
+    for cfg in cfgs:
+        g = B.System(cfg)
+        rot, tr = [], []
+        for i, (d, rgb, lab) in enumerate(frames[:m]):
+            g.grab(torch.from_numpy(d).to(dev), torch.from_numpy(rgb).to(dev),
+                   torch.from_numpy(lab).to(dev))
+            a, b = g.result()["pose"], ref[i]
+            dR = b[:3, :3].T @ a[:3, :3]
+            rot.append(float(np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1))))
+            tr.append(float(np.linalg.norm(a[:3, 3] - b[:3, 3])))
+        g.close()
+        out.append({"frames": m, "max_rotation_rad": max(rot), "max_translation_m": max(tr),
+                    "last_translation_m": tr[-1]})
+    return out
+
+
+def precision_report(w, frames, syn, cfg32, cfg16, m=30):
+    """The precision form against FP64 on the same frames: the first m frames
+    fused at their ground-truth poses into an FP32-form map and a half-form
+    map (GPU) and into the FP64 oracle map (CPU); TSDF RMS over the voxels
+    all three hold (blocks matched by coordinate), and the raycast depth RMS
+    at the last pose (one zRange for all three)."""
+    import torch
+    from oracle import binding as O
+    from semantic_icpslam_amd import binding as B
+    import copy
+    dev = torch.device("cuda:0")
+    K = B.intrinsics(w["f"], w["f"], w["cx"], w["cy"])
+    mc = []
+    for c in (cfg32.map, cfg16.map):  # a heap sized for m frames keeps the downloads small
+        c = copy.copy(c)
+        c.num_blocks = 40000
+        mc.append(c)
+    maps = [B.Map(mc[0]), B.Map(mc[1])]
+    o64 = O.OracleMap(mc[0], fp64=True)
     for i, (d, rgb, lab) in enumerate(frames[:m]):
-        g.grab(torch.from_numpy(d).to(dev), torch.from_numpy(rgb).to(dev),
-               torch.from_numpy(lab).to(dev))
-        a = g.result()["pose"]
-        b = o.grab(d, rgb, lab)["pose"]
-        dR = b[:3, :3].T @ a[:3, :3]
-        rot.append(float(np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1))))
-        tr.append(float(np.linalg.norm(a[:3, 3] - b[:3, 3])))
-    g.close()
-    return {"frames": m, "max_rotation_rad": max(rot), "max_translation_m": max(tr),
-            "last_translation_m": tr[-1]}
+        raw, filt = O.filter_depth(d, w.get("scale", 1000.0), 3.0)
+        nmap = O.compute_nmap(O.compute_vmap(filt, w["f"], w["f"], w["cx"], w["cy"], 3.0))
+        T = syn.pose(i)
+        pose = B.pose_f(T)
+        t = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (raw, rgb, nmap, lab)]
+        for g in maps:
+            B.check(B.lib().sicp_fuse_map_color(g.h, *[B.image(x) for x in t], 1, B.C.byref(pose),
+                                                None, K, None, B.stream_ptr()))
+        o64.fuse_fp64(raw, rgb, nmap, lab, 1, T, [w["f"], w["f"], w["cx"], w["cy"]])
+    torch.cuda.synchronize()
+
+    def voxels(exported, n):
+        b = exported["blocks"][:n]
+        return {(int(x), int(y), int(z)): int(sl) for x, y, z, sl in
+                zip(b["x"], b["y"], b["z"], b["slot"])}, exported["geom"]
+
+    d64 = o64.export()
+    k64, g64 = voxels(d64, o64.stats()["allocated_blocks"])
+    out = {"frames": m, "what": "fused at ground-truth poses; RMS over voxels held by all "
+                                 "three maps with weight > 0"}
+    dl = []
+    for name, g in (("fp32", maps[0]), ("half", maps[1])):
+        na = g.stats()["allocated_blocks"]
+        e = g.download(n_blocks=na)
+        kk, gg = voxels(e, na)
+        common = [(kk[key], k64[key]) for key in kk.keys() & k64.keys()]
+        a = np.concatenate([gg["sdf"][s * 512:(s + 1) * 512] for s, _ in common])
+        wa = np.concatenate([gg["weight"][s * 512:(s + 1) * 512] for s, _ in common])
+        b = np.concatenate([g64["sdf"][s * 512:(s + 1) * 512] for _, s in common])
+        wb = np.concatenate([g64["weight"][s * 512:(s + 1) * 512] for _, s in common])
+        ok = (wa > 0) & (wb > 0) & ~np.isnan(a) & ~np.isnan(b)
+        diff = a[ok].astype(np.float64) - b[ok]
+        rms = float(np.sqrt(np.mean(diff ** 2)))
+        out[f"tsdf_rms_{name}"] = rms
+        out[f"tsdf_rms_{name}_mm"] = rms * w["trunc"] * 1000.0
+        out[f"tsdf_max_{name}"] = float(np.abs(diff).max())
+        out["voxels_compared"] = int(ok.sum())
+    # depth at the last pose, one zRange (from the FP32-form map) for all
+    T = syn.pose(m - 1)
+    pose = B.pose_f(T)
+    zc, zr = (w["cols"] + 7) // 8, (w["rows"] + 7) // 8
+    zmin = torch.zeros((zr, zc), dtype=torch.float32, device=dev)
+    zmax = torch.zeros_like(zmin)
+    ne = B.C.c_int(0)
+    B.check(B.lib().sicp_create_rendering_blocks(maps[0].h, B.image(zmin), B.image(zmax), 3.0,
+                                                 0.1, B.C.byref(pose), None, K, B.C.byref(ne),
+                                                 B.stream_ptr()))
+    invf = float(np.float32(1.0 / w["f"]))
+    depths = []
+    for g in maps:
+        v = torch.empty((w["rows"], w["cols"], 4), dtype=torch.float32, device=dev)
+        n = torch.empty_like(v)
+        B.check(B.lib().sicp_raycast_inv(g.h, B.image(v), B.image(n), B.image(zmin),
+                                         B.image(zmax), B.C.byref(pose), None, invf, invf,
+                                         w["cx"], w["cy"], B.stream_ptr()))
+        torch.cuda.synchronize()
+        depths.append(v[..., 2].cpu().numpy().astype(np.float64))
+    v64, _ = o64.raycast_fp64(w["cols"], w["rows"], zmin.cpu().numpy(), zmax.cpu().numpy(), T,
+                              invf, invf, w["cx"], w["cy"])
+    for name, dpt in zip(("fp32", "half"), depths):
+        ok = ~np.isnan(dpt) & ~np.isnan(v64[..., 2])
+        out[f"depth_rms_{name}_mm"] = float(np.sqrt(np.mean((dpt[ok] - v64[..., 2][ok]) ** 2)) * 1e3)
+        out[f"depth_pixels_{name}"] = int(ok.sum())
+    for g in maps:
+        g.close()
+    return out
 
 
 def roofline_entry(kernel, nbytes, seconds, launches, traffic_kib=None):
@@ -513,6 +604,13 @@ def main():
     if not args.no_graph:
         log("[bench] graph pass (staged inputs, one hipGraph launch per frame)")
         graphed = run_pass(False, graph=True)
+    # The precision form (half TSDF, weights capped at 128) in the graph form.
+    cfg16 = B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"], w["voxel"],
+                            w["trunc"], w["buckets"], w["blocks"], tsdf_half=1, max_weight=128)
+    precise = None
+    if not args.no_graph:
+        log("[bench] precision form pass (half TSDF, one hipGraph launch per frame)")
+        precise = run_pass(False, graph=True, system_cfg=cfg16)
     photometric = None
     if not args.no_photometric:
         log("[bench] photometric pass (ComputeSE3(false): RGBStep + ICPStep per iteration)")
@@ -583,10 +681,12 @@ def main():
     if not args.no_cpu_baseline:
         log("[bench] CPU reference method at the small size")
         small = cpu_small()
-    pose_check = None
+    pose_check = pose_half = prec = None
     if not args.no_pose_check:
-        log("[bench] pose check against the FP64 oracle")
-        pose_check = pose_error_vs_fp64(w, frames, cfg)
+        log("[bench] pose check against the FP64 oracle (both forms)")
+        pose_check, pose_half = pose_error_vs_fp64(w, frames, [cfg, cfg16])
+        log("[bench] TSDF and depth of both forms against FP64")
+        prec = precision_report(w, frames, syn, cfg, cfg16)
 
     out = {
         "metric": "ms/frame track+fuse+raycast 640x480 (plain FP32)",
@@ -614,6 +714,17 @@ def main():
             "what": "the same frames, each copied into the system's staged inputs and run as "
                     "one hipGraph launch (plain FP32)",
             "same_final_pose": bool(np.array_equal(graphed["res"]["pose"], res["pose"]))},
+        "precision_form": None if precise is None else {
+            "ms_per_step": round(1000.0 * precise["elapsed"] / args.steps, 4),
+            "what": "the same frames with the sdf stored as IEEE half (one 4-byte word with the "
+                    "weight; colour in the semantic record), weights capped at 128, FP32 "
+                    "updates and interpolation, one hipGraph launch per frame",
+            "voxel_bytes": {"fp32_form": 8 + 12, "half_form": 4 + 12},
+            "last_frame_tracked": precise["res"]["tracked"],
+            "last_frame_translation_vs_ground_truth_m": round(
+                float(np.linalg.norm(precise["res"]["pose"][:3, 3] - gt[:3, 3])), 5),
+            "pose_vs_fp64_oracle": pose_half,
+            "vs_fp64": prec},
         "photometric_tracking": None if photometric is None else {
             "ms_per_step": round(1000.0 * photometric["elapsed"] / args.steps, 4),
             "what": "the same frames with the reference's default tracking, ComputeSE3(false): "

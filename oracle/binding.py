@@ -321,6 +321,18 @@ class OracleMap:
         return lib().oracle_fuse_map_color(self.h, _p(d), _p(r), _p(n), _p(m), cols, rows,
                                            num_detection, C.byref(pose), K)
 
+    def fuse_fp64(self, depth, rgb, nmap, mask, num_detection, T, K4):
+        """FP64 map: FuseMapColor at the 4x4 double pose T, K4 = (fx, fy, cx, cy)."""
+        d = _c(depth, np.float64)
+        rows, cols = d.shape
+        r = _c(rgb, np.uint8)
+        n = _c(nmap, np.float64) if nmap is not None else None
+        m = _c(mask, np.uint8) if mask is not None else None
+        t = _c(T, np.float64)
+        k = _c(K4, np.float64)
+        return lib().oracle_fuse_map_color_fp64(self.h, _p(d), _p(r), _p(n), _p(m), cols, rows,
+                                                num_detection, _p(t), _p(k))
+
     def clone_fp64(self):
         """An FP64 map with exactly this FP32 map's state."""
         o = OracleMap.__new__(OracleMap)
