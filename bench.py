@@ -604,12 +604,15 @@ def main():
     if not args.no_graph:
         log("[bench] graph pass (staged inputs, one hipGraph launch per frame)")
         graphed = run_pass(False, graph=True)
-    # The precision form (half TSDF, weights capped at 128) in the graph form.
+    # The precision form (half TSDF, weights capped at 128, FP32 compensated
+    # Gauss-Newton sums) in the graph form.
     cfg16 = B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"], w["voxel"],
                             w["trunc"], w["buckets"], w["blocks"], tsdf_half=1, max_weight=128)
+    cfg16.icp_sums_fp32 = 1
     precise = None
     if not args.no_graph:
-        log("[bench] precision form pass (half TSDF, one hipGraph launch per frame)")
+        log("[bench] precision form pass (half TSDF, FP32 compensated sums, one hipGraph "
+            "launch per frame)")
         precise = run_pass(False, graph=True, system_cfg=cfg16)
     photometric = None
     if not args.no_photometric:
@@ -718,7 +721,9 @@ def main():
             "ms_per_step": round(1000.0 * precise["elapsed"] / args.steps, 4),
             "what": "the same frames with the sdf stored as IEEE half (one 4-byte word with the "
                     "weight; colour in the semantic record), weights capped at 128, FP32 "
-                    "updates and interpolation, one hipGraph launch per frame",
+                    "updates and interpolation, the ICP sums in FP32 with compensated "
+                    "summation (TwoSum at every addition; FP64 only in the 6x6 solve and the "
+                    "pose composition), one hipGraph launch per frame",
             "voxel_bytes": {"fp32_form": 8 + 12, "half_form": 4 + 12},
             "last_frame_tracked": precise["res"]["tracked"],
             "last_frame_translation_vs_ground_truth_m": round(

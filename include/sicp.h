@@ -260,6 +260,16 @@ int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_ima
                   void* partials_dev, float* sums_dev, float* host_sums,
                   double* A_host, double* b_host, float* residual_host, void* stream);
 size_t sicp_icp_partials_size(void);
+/* ICPStep in the precision form: the same 29 sums, arguments and outputs as
+ * sicp_icp_step, accumulated in FP32 with compensated summation (Kahan's
+ * update per product, TwoSum where partial sums meet) instead of FP64 — the
+ * reference's own sums are plain FP32 (BlockReduce / Reduce<float, 29>,
+ * Reduction.cu:29-60, 315, 374). Bitwise reproducible for a given image
+ * size. */
+int sicp_icp_step_fp32(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nlast,
+                       const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
+                       void* partials_dev, float* sums_dev, float* host_sums,
+                       double* A_host, double* b_host, float* residual_host, void* stream);
 
 /* RGBStep (Tracking/Reduction.h:81-90, Reduction.cu:575-642): the
  * photometric term of one Gauss-Newton iteration. A correspondence pass
@@ -385,6 +395,12 @@ typedef struct sicp_system_config {
                                 0: ComputeSE3(false, ...), the photometric term
                                 of RGBStep added (TrackFrame, Tracking.cc:170) */
     float rgb_weight;        /* w of the photometric term, 1e-4 (Tracking.cc:392) */
+    int32_t icp_sums_fp32;   /* 0: the Gauss-Newton sums (ICP and photometric)
+                                accumulated in FP64; 1: the precision form, FP32
+                                with compensated summation (Kahan per thread,
+                                TwoSum where partial sums meet). The reference
+                                sums in plain FP32 (BlockReduce / Reduce<float,
+                                29>, Reduction.cu:29-60, 315, 374). */
 } sicp_system_config;
 
 typedef struct sicp_system sicp_system;

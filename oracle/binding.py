@@ -38,6 +38,7 @@ _SIGS = {
     "oracle_icp_sums": (None, [_P, _P, _P, _P, C.c_int, C.c_int, C.POINTER(PoseF),
                                C.POINTER(PoseF), Intrinsics, _P, _P]),
     "oracle_icp_sums_fp64": (None, [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
+    "oracle_sums_k32": (None, [_P, C.c_int, C.c_int, _P]),
     "oracle_rgb_sums": (None, [_P, _P, _P, _P, _P, _P, C.c_int, C.c_int, C.POINTER(PoseF),
                                C.POINTER(PoseF), Intrinsics, _P, _P, _P]),
     "oracle_so3_sums": (None, [_P, _P, _P, _P, C.c_int, C.c_int, C.POINTER(PoseF),
@@ -189,6 +190,17 @@ def icp_sums(vcurr, vlast, ncurr, nlast, pose_curr, pose_last, K, rows_out=False
     lib().oracle_icp_sums(_p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), cols, rows,
                           C.byref(pose_curr), C.byref(pose_last), K, _p(sums), _p(ro))
     return (sums, ro) if rows_out else sums
+
+
+def sums_k32(rows29, compensate=True):
+    """The precision form's FP32 compensated sums of per-pixel rows (N x 29,
+    slot 28 = 1 where the pixel contributes), in the GPU reduction's order;
+    compensate=False: plain FP32 in the same order."""
+    r = _c(rows29, np.float32)
+    assert r.ndim == 2 and r.shape[1] == 29
+    out = np.zeros(29, np.float32)
+    lib().oracle_sums_k32(_p(r), r.shape[0], 1 if compensate else 0, _p(out))
+    return out
 
 
 def icp_sums_fp64(vcurr, vlast, ncurr, nlast, T_curr, T_last, K4):

@@ -49,6 +49,11 @@ void oracle_icp_sums(const float* vcurr, const float* vlast, const float* ncurr,
                      const float* nlast, int cols, int rows, const sicp_pose_f* curr,
                      const sicp_pose_f* last, sicp_intrinsics K, double* sums_fp64,
                      float* rows_out);
+/* The precision form's sums of per-pixel rows (29 floats a pixel as
+ * oracle_icp_sums writes them, slot 28 = 1 where the pixel contributes): FP32
+ * with compensation, in the GPU reduction's order; compensate = 0: plain
+ * FP32 in the same order. */
+void oracle_sums_k32(const float* rows29, int n, int compensate, float* sums);
 /* The same computed entirely in double from double poses (4x4 row-major). */
 void oracle_icp_sums_fp64(const double* vcurr, const double* vlast, const double* ncurr,
                           const double* nlast, int cols, int rows, const double* curr_pose,

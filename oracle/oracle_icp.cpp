@@ -145,7 +145,8 @@ static inline bool rgb_corresp(const uint8_t* nimg, const uint8_t* limg, const R
 template <class R>
 void rgb_sums(const uint8_t* nimg, const uint8_t* limg, const R* nvmap, const R* lvmap,
               const int16_t* dx, const int16_t* dy, int cols, int rows, const Pose<R>& curr,
-              const Pose<R>& last, Intr<R> K, double* sums, int* res2, float* sigma_out) {
+              const Pose<R>& last, Intr<R> K, double* sums, int* res2, float* sigma_out,
+              float* rows_out) {
     const int N = cols * rows;
     uint32_t cnt = 0, sq = 0;
 #pragma omp parallel for schedule(static) reduction(+ : cnt, sq)
@@ -196,8 +197,14 @@ void rgb_sums(const uint8_t* nimg, const uint8_t* limg, const R* nvmap, const R*
             }
             int k = 0;
             for (int a = 0; a < 7; ++a)
-                for (int b = a; b < 7; ++b) loc[k++] += (double)(row[a] * row[b]);
+                for (int b = a; b < 7; ++b) {
+                    const R p = row[a] * row[b];
+                    loc[k] += (double)p;
+                    if (rows_out) rows_out[(size_t)i * 29 + k] = (float)p;
+                    ++k;
+                }
             loc[28] += ok ? 1.0 : 0.0;
+            if (rows_out) rows_out[(size_t)i * 29 + 28] = ok ? 1.f : 0.f;
         }
 #pragma omp critical
         for (int j = 0; j < 29; ++j) sums[j] += loc[j];
@@ -206,10 +213,110 @@ void rgb_sums(const uint8_t* nimg, const uint8_t* limg, const R* nvmap, const R*
 
 template void rgb_sums<float>(const uint8_t*, const uint8_t*, const float*, const float*,
                               const int16_t*, const int16_t*, int, int, const Pose<float>&,
-                              const Pose<float>&, Intr<float>, double*, int*, float*);
+                              const Pose<float>&, Intr<float>, double*, int*, float*, float*);
 template void rgb_sums<double>(const uint8_t*, const uint8_t*, const double*, const double*,
                                const int16_t*, const int16_t*, int, int, const Pose<double>&,
-                               const Pose<double>&, Intr<double>, double*, int*, float*);
+                               const Pose<double>&, Intr<double>, double*, int*, float*, float*);
+
+// ---- the precision form's sums: FP32 with compensation ---------------------
+// The reference sums the per-pixel rows in plain FP32 (BlockReduce and
+// Reduce<float, 29>, Reduction.cu:29-60, 315, 374). The precision form keeps
+// FP32 but compensates: each partial sum is a pair (s, e) whose value is
+// s + e. Every addition — a thread adding a product p, two pairs meeting —
+// is Knuth's TwoSum of the sums (t = a + b, z = t - a,
+// r = (a - (t - z)) + (b - z): r is the exact rounding error of t), the
+// error terms added in FP32 (cascaded summation, Ogita-Rump-Oishi). The
+// order is that of the GPU reduction this build runs (a decision of this
+// build, not the reference's), so the result is bit-exact:
+// - 256 threads per workgroup, nb = clamp(ceil(n / 1024), 1, 512)
+//   workgroups, stride = 256 nb; thread g adds the contributing pixels
+//   g, g + stride, g + 2 stride, ... in that order (28 products, then 1 to
+//   the count);
+// - in each workgroup, for each sum, eight partial sums k = 0..7 each meet
+//   the threads k, k + 8, ..., k + 248 in turn (from (0, 0)), then a
+//   butterfly over the eight (distance 1, 2, 4);
+// - over the workgroups likewise: partial k meets the workgroup rows
+//   k, k + 8, ..., k + 504 (rows past nb count as (0, 0)), then the same
+//   butterfly; the sum is s + e rounded to FP32.
+// `compensate` = false keeps every step but drops the error terms: plain
+// FP32 in the same order (used by the tests to show what compensation buys).
+struct K32 {
+    float s, e;
+};
+static inline K32 k32_merge(K32 a, const K32& b, bool comp) {
+    if (!comp) return K32{a.s + b.s, 0.f};
+    const float t = a.s + b.s;
+    const float z = t - a.s;
+    const float r = (a.s - (t - z)) + (b.s - z);
+    return K32{t, (a.e + b.e) + r};
+}
+static inline void k32_add(K32& a, float p, bool comp) {
+    if (!comp) {
+        a.s = a.s + p;
+        return;
+    }
+    const float t = a.s + p;
+    const float z = t - a.s;
+    const float r = (a.s - (t - z)) + (p - z);
+    a.e = a.e + r;
+    a.s = t;
+}
+// Butterfly over eight lanes (distances 1, 2, 4); every lane ends with the
+// same pair (TwoSum's error is exact, so a meeting is symmetric); lane 0's.
+static inline K32 k32_fold8(K32 lane[8], bool comp) {
+    for (int m = 1; m < 8; m <<= 1) {
+        K32 nx[8];
+        for (int k = 0; k < 8; ++k) nx[k] = k32_merge(lane[k], lane[k ^ m], comp);
+        for (int k = 0; k < 8; ++k) lane[k] = nx[k];
+    }
+    return lane[0];
+}
+
+void sums_k32(const float* rows29, int n, bool compensate, float* sums) {
+    constexpr int kThreads = 256, kMaxWg = 512, kPix = 4;
+    int nb = (n + kThreads * kPix - 1) / (kThreads * kPix);
+    nb = nb < 1 ? 1 : (nb > kMaxWg ? kMaxWg : nb);
+    const long stride = (long)nb * kThreads;
+    std::vector<K32> part((size_t)nb * 29);
+#pragma omp parallel for schedule(static)
+    for (int w = 0; w < nb; ++w) {
+        std::vector<K32> acc((size_t)kThreads * 29, K32{0.f, 0.f});
+        for (int t = 0; t < kThreads; ++t) {
+            K32* a = &acc[(size_t)t * 29];
+            for (long i = (long)w * kThreads + t; i < n; i += stride) {
+                const float* r = rows29 + (size_t)i * 29;
+                if (r[28] == 0.f) continue;
+                for (int j = 0; j < 28; ++j) k32_add(a[j], r[j], compensate);
+                k32_add(a[28], 1.0f, compensate);
+            }
+        }
+        for (int j = 0; j < 29; ++j) {
+            K32 lane[8];
+            for (int k = 0; k < 8; ++k) {
+                K32 s{0.f, 0.f};
+                for (int r = k; r < kThreads; r += 8)
+                    s = k32_merge(s, acc[(size_t)r * 29 + j], compensate);
+                lane[k] = s;
+            }
+            part[(size_t)w * 29 + j] = k32_fold8(lane, compensate);
+        }
+    }
+    for (int j = 0; j < 29; ++j) {
+        K32 lane[8];
+        for (int k = 0; k < 8; ++k) {
+            K32 s{0.f, 0.f};
+            for (int r = 0; r < kMaxWg / 8; ++r) {
+                const int bi = k + 8 * r;
+                s = k32_merge(s, bi < nb ? part[(size_t)bi * 29 + j] : K32{0.f, 0.f},
+                              compensate);
+            }
+            lane[k] = s;
+        }
+        const K32 v = k32_fold8(lane, compensate);
+        sums[j] = v.s + v.e;
+    }
+}
+
 
 // SO3Reduce (Tracking/Reduction.cu:67-160): the last frame's pixel bearing
 // ((x - cx) / fx, (y - cy) / fy, 1) rotated by RcurrInv * Rlast, matched by
@@ -413,6 +520,10 @@ void oracle_icp_sums(const float* vcurr, const float* vlast, const float* ncurr,
                     Intr<float>{K.fx, K.fy, K.cx, K.cy}, sums_fp64, rows_out);
 }
 
+void oracle_sums_k32(const float* rows29, int n, int compensate, float* sums) {
+    sums_k32(rows29, n, compensate != 0, sums);
+}
+
 void oracle_icp_sums_fp64(const double* vcurr, const double* vlast, const double* ncurr,
                           const double* nlast, int cols, int rows, const double* curr_pose,
                           const double* last_pose, const double* K4, double* sums) {
@@ -430,7 +541,7 @@ void oracle_rgb_sums(const uint8_t* nimg, const uint8_t* limg, const float* nvma
     Pose<float> pl = pose_from_f<float>(last->R, last->Rinv, last->t);
     int res2[2];
     rgb_sums<float>(nimg, limg, nvmap, lvmap, dx, dy, cols, rows, pc, pl,
-                    Intr<float>{K.fx, K.fy, K.cx, K.cy}, sums_fp64, res2, sigma);
+                    Intr<float>{K.fx, K.fy, K.cx, K.cy}, sums_fp64, res2, sigma, nullptr);
     if (corresp) {
         corresp[0] = res2[0];
         corresp[1] = res2[1];

@@ -47,7 +47,12 @@ void icp_sums(const R* vcurr, const R* vlast, const R* ncurr, const R* nlast, in
 template <class R>
 void rgb_sums(const uint8_t* nimg, const uint8_t* limg, const R* nvmap, const R* lvmap,
               const int16_t* dx, const int16_t* dy, int cols, int rows, const Pose<R>& curr,
-              const Pose<R>& last, Intr<R> K, double* sums, int* res2, float* sigma_out);
+              const Pose<R>& last, Intr<R> K, double* sums, int* res2, float* sigma_out,
+              float* rows_out = nullptr);
+// The precision form's FP32 compensated sums of per-pixel rows (29 floats a
+// pixel, slot 28 = 1 where the pixel contributes) in the GPU reduction's
+// order; `compensate` = false: plain FP32 in the same order.
+void sums_k32(const float* rows29, int n, bool compensate, float* sums);
 void create_matrix(const double* sums29, double* A, double* b);
 bool cholesky_solve6(const double* A, const double* b, double* x);
 void se3_exp(const double* xi, double* T);

@@ -7,6 +7,8 @@
 // baseline that bench.py times.
 #include <omp.h>
 
+#include <type_traits>
+
 #include "oracle_impl.hpp"
 
 namespace oracle {
@@ -95,6 +97,12 @@ template <> void round_sums<float>(double* s) {
 }
 template <> void round_sums<double>(double*) {}
 
+static void compensated_sums(const std::vector<float>& rows, int n, double* sums) {
+    float f[29];
+    sums_k32(rows.data(), n, true, f);
+    for (int j = 0; j < 29; ++j) sums[j] = (double)f[j];
+}
+
 // Tracker::ComputeSE3 (Tracking.cc:369-517): the point-to-plane term alone
 // (icpOnly, :488-490) or with the weighted photometric term (:400-433,
 // :484-487), as the configuration says.
@@ -110,6 +118,10 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
     int icp_count = 0, break_count = 0, applied = 0;
     const bool icp_only = s.cfg.icp_only != 0;
     const float w = s.cfg.rgb_weight;
+    // The precision form's FP32 compensated sums (the FP64 restatement keeps
+    // its own arithmetic).
+    const bool k32 = std::is_same<R, float>::value && s.cfg.icp_sums_fp32 != 0;
+    std::vector<float> rows(k32 ? (size_t)nf.cols[0] * nf.rows[0] * 29 : 0);
     for (int i = 2; i >= 0; --i) {
         for (int j = 0; j < s.cfg.icp_iters[i]; ++j) {
             Pose<R> pc = pose_from_d<R>(nf.pose);
@@ -121,8 +133,12 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
                 int res2[2];
                 rgb_sums<R>(nf.image[i].data(), lf.image[i].data(), nf.vmap[i].data(),
                             lf.vmap[i].data(), nf.dx[i].data(), nf.dy[i].data(), nf.cols[i],
-                            nf.rows[i], pc, pl, s.K[i], rs, res2, nullptr);
-                round_sums<R>(rs);
+                            nf.rows[i], pc, pl, s.K[i], rs, res2, nullptr,
+                            k32 ? rows.data() : nullptr);
+                if (k32)
+                    compensated_sums(rows, nf.cols[i] * nf.rows[i], rs);
+                else
+                    round_sums<R>(rs);
                 create_matrix(rs, A_rgb, b_rgb);
                 R q0 = (R)rs[27], q1 = (R)rs[28];
                 rgb_error = std::sqrt(q0) / q1;
@@ -138,8 +154,12 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
             }
             double sums[29], A[36], b[6];
             icp_sums<R>(nf.vmap[i].data(), lf.vmap[i].data(), nf.nmap[i].data(),
-                        lf.nmap[i].data(), nf.cols[i], nf.rows[i], pc, pl, s.K[i], sums, nullptr);
-            round_sums<R>(sums);
+                        lf.nmap[i].data(), nf.cols[i], nf.rows[i], pc, pl, s.K[i], sums,
+                        k32 ? rows.data() : nullptr);
+            if (k32)
+                compensated_sums(rows, nf.cols[i] * nf.rows[i], sums);
+            else
+                round_sums<R>(sums);
             create_matrix(sums, A, b);
             R r0 = (R)sums[27], r1 = (R)sums[28];
             icp_error = std::sqrt(r0) / r1;

@@ -60,7 +60,7 @@ class SystemConfig(C.Structure):
         ("depth_scale", C.c_float), ("depth_cutoff", C.c_float),
         ("icp_iters", C.c_int32 * 3), ("icp_min_count", C.c_int32 * 3),
         ("icp_thresh", C.c_float), ("map", MapConfig),
-        ("icp_only", C.c_int32), ("rgb_weight", C.c_float),
+        ("icp_only", C.c_int32), ("rgb_weight", C.c_float), ("icp_sums_fp32", C.c_int32),
     ]
 
 
@@ -108,6 +108,8 @@ _SIGS = {
     "sicp_resize_map": (C.c_int, [Image, Image, Image, Image, _P]),
     "sicp_icp_step": (C.c_int, [Image, Image, Image, Image, C.POINTER(PoseF), C.POINTER(PoseF),
                                 Intrinsics, _P, _P, _P, _P, _P, _P, _P]),
+    "sicp_icp_step_fp32": (C.c_int, [Image, Image, Image, Image, C.POINTER(PoseF),
+                                     C.POINTER(PoseF), Intrinsics, _P, _P, _P, _P, _P, _P, _P]),
     "sicp_rgb_step": (C.c_int, [Image, Image, Image, Image, Image, Image, C.POINTER(PoseF),
                                 C.POINTER(PoseF), Intrinsics, _P, C.c_size_t, _P, _P, _P, _P,
                                 _P, _P, _P, _P]),

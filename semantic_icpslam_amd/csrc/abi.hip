@@ -273,10 +273,10 @@ int sicp_resize_map(sicp_image vsrc, sicp_image nsrc, sicp_image vdst, sicp_imag
 // -------------------------------------------------------------------- ICP --
 size_t sicp_icp_partials_size(void) { return sizeof(double) * kIcpMaxBlocks * kIcpSums + 256; }
 
-int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nlast,
-                  const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
-                  void* partials_dev, float* sums_dev, float* host_sums, double* A_host,
-                  double* b_host, float* residual_host, void* stream) {
+static int icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nlast,
+                    const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
+                    void* partials_dev, float* sums_dev, float* host_sums, double* A_host,
+                    double* b_host, float* residual_host, bool k32, void* stream) {
     const int c = vcurr.cols, r = vcurr.rows;
     if (!image_ok(vcurr, 16, c, r) || !image_ok(vlast, 16, c, r) || !image_ok(ncurr, 16, c, r) ||
         !image_ok(nlast, 16, c, r))
@@ -286,7 +286,7 @@ int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_ima
     hipStream_t s = (hipStream_t)stream;
     int rc = launch_icp_sums(view<float4>(vcurr), view<float4>(vlast), view<float4>(ncurr),
                              view<float4>(nlast), make_pose_arg(curr, nullptr),
-                             make_pose_arg(last, nullptr), K, sc, sums_dev, s);
+                             make_pose_arg(last, nullptr), K, sc, sums_dev, k32, s);
     if (rc) return rc;
     if (host_sums || A_host || b_host || residual_host) {
         float h[kIcpSums];
@@ -303,6 +303,22 @@ int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_ima
         }
     }
     return SICP_OK;
+}
+
+int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nlast,
+                  const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
+                  void* partials_dev, float* sums_dev, float* host_sums, double* A_host,
+                  double* b_host, float* residual_host, void* stream) {
+    return icp_step(vcurr, vlast, ncurr, nlast, curr, last, K, partials_dev, sums_dev, host_sums,
+                    A_host, b_host, residual_host, false, stream);
+}
+
+int sicp_icp_step_fp32(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nlast,
+                       const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
+                       void* partials_dev, float* sums_dev, float* host_sums, double* A_host,
+                       double* b_host, float* residual_host, void* stream) {
+    return icp_step(vcurr, vlast, ncurr, nlast, curr, last, K, partials_dev, sums_dev, host_sums,
+                    A_host, b_host, residual_host, true, stream);
 }
 
 int sicp_so3_step(sicp_image next_image, sicp_image last_image, sicp_image didx,

@@ -181,39 +181,87 @@ SICP_DEV uint2 block_sum_u32x2(uint32_t a, uint32_t b) {
     return r;
 }
 
-// Workgroup reduction of every thread's 29 FP64 sums into this workgroup's
+// The sums of the reduction, one 8-byte word each at every level (thread,
+// workgroup partial row, total).
+// SumF64 (the plain form): the FP32 products accumulated in FP64.
+// SumK32 (the precision form): FP32 with compensation, no FP64 arithmetic.
+// Every addition of a sum — a thread adding a product, two partial sums
+// meeting in the workgroup's transposed rows, the lane butterflies and the
+// partial rows of the workgroups — is Knuth's TwoSum, whose rounding error
+// r is exact; the error terms are added in FP32 and the value s + e is
+// rounded once, at the end (the cascaded summation of Ogita, Rump and
+// Oishi). Kahan's cheaper update, e = y - ((s + y) - s), is not exact when a
+// product outweighs the running sum, as happens with few pixels per thread.
+struct SumF64 {
+    double v;
+    SICP_DEV void zero() { v = 0.0; }
+    SICP_DEV void add(float p) { v += (double)p; }
+    SICP_DEV void merge(const SumF64& o) { v += o.v; }
+    SICP_DEV SumF64 shfl_xor(int m) const { return SumF64{__shfl_xor(v, m, 64)}; }
+    SICP_DEV double value() const { return v; }
+};
+struct SumK32 {
+    float s, e;  // running sum and its error term
+    SICP_DEV void zero() {
+        s = 0.f;
+        e = 0.f;
+    }
+    SICP_DEV void add(float p) {
+        const float t = s + p;
+        const float z = t - s;
+        const float r = (s - (t - z)) + (p - z);
+        e = e + r;
+        s = t;
+    }
+    SICP_DEV void merge(const SumK32& o) {
+        const float t = s + o.s;
+        const float z = t - s;
+        const float r = (s - (t - z)) + (o.s - z);
+        e = (e + o.e) + r;
+        s = t;
+    }
+    SICP_DEV SumK32 shfl_xor(int m) const {
+        return SumK32{__shfl_xor(s, m, 64), __shfl_xor(e, m, 64)};
+    }
+    SICP_DEV double value() const { return (double)(s + e); }
+};
+static_assert(sizeof(SumF64) == 8 && sizeof(SumK32) == 8, "one 8-byte partial per sum");
+
+// Workgroup reduction of every thread's NS sums into this workgroup's
 // partial row (column-major: partials[j * kIcpMaxBlocks + workgroup]), by
 // transposition through LDS: every thread stores its sums as one row; thread
 // (j, k) then adds rows k, k+8, ... of column j and eight lanes fold with
 // three butterflies. Fixed order; much cheaper than folding 29 doubles
 // across 64 lanes with cross-lane permutes.
-template <int NS>
-SICP_DEV void block_rows_to_partials(const double (&acc)[NS], double* partials) {
-    __shared__ double tile[kIcpThreads][NS + 1];
+template <int NS, class Acc>
+SICP_DEV void block_rows_to_partials(const Acc (&acc)[NS], double* partials) {
+    __shared__ Acc tile[kIcpThreads][NS + 1];
 #pragma unroll
     for (int j = 0; j < NS; ++j) tile[threadIdx.x][j] = acc[j];
     __syncthreads();
     const int t = threadIdx.x;
     if (t < NS * 8) {
         const int j = t >> 3, k = t & 7;
-        double s = 0.0;
+        Acc s;
+        s.zero();
 #pragma unroll 8
-        for (int r = k; r < kIcpThreads; r += 8) s += tile[r][j];
-        s += __shfl_xor(s, 1, 64);
-        s += __shfl_xor(s, 2, 64);
-        s += __shfl_xor(s, 4, 64);
+        for (int r = k; r < kIcpThreads; r += 8) s.merge(tile[r][j]);
+        s.merge(s.shfl_xor(1));
+        s.merge(s.shfl_xor(2));
+        s.merge(s.shfl_xor(4));
         // column-major: sum j of every workgroup is contiguous
-        if (k == 0) partials[j * kIcpMaxBlocks + blockIdx.x] = s;
+        if (k == 0) reinterpret_cast<Acc*>(partials)[j * kIcpMaxBlocks + blockIdx.x] = s;
     }
 }
 
-SICP_DEV void add_row(double acc[kIcpSums], const float row[7]) {
+template <class Acc>
+SICP_DEV void add_row(Acc acc[kIcpSums], const float row[7]) {
     int q = 0;
 #pragma unroll
     for (int a = 0; a < 7; ++a)
 #pragma unroll
-        for (int b = a; b < 7; ++b) acc[q++] += (double)(row[a] * row[b]);
-    acc[28] += 1.0;
+        for (int b = a; b < 7; ++b) acc[q++].add(row[a] * row[b]);
+    acc[28].add(1.0f);
 }
 
 // Per-thread accumulation + workgroup reduction into this workgroup's
@@ -222,7 +270,7 @@ SICP_DEV void add_row(double acc[kIcpSums], const float row[7]) {
 // kRgb: the same pass also runs RGBStep's correspondence test
 // (FindCorresp, Reduction.cu:409-461) on every pixel, writing its record and
 // the workgroup's int count and sum of squared differences.
-template <bool kRgb, class PoseFn>
+template <bool kRgb, class Acc, class PoseFn>
 SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlast,
                              const View<float4>& ncurr, const View<float4>& nlast,
                              const IcpGeom& g, double* partials, const RgbPass& rp,
@@ -233,9 +281,9 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
     icp_load_batch<kRgb>(bt, base, stride, vcurr, ncurr, rp, g);
     PoseF cur, last;
     if (!get_pose(cur, last)) return;
-    double acc[kIcpSums];
+    Acc acc[kIcpSums];
 #pragma unroll
-    for (int j = 0; j < kIcpSums; ++j) acc[j] = 0.0;
+    for (int j = 0; j < kIcpSums; ++j) acc[j].zero();
     uint32_t cnt = 0, sq = 0;
     while (base < g.n) {
         float4 vl[kPixPerThread], nl[kPixPerThread], rvl[kPixPerThread];
@@ -292,7 +340,7 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
 // int sums (RGBStep, Reduction.cu:618-620; every workgroup forms the same
 // value), then GetRow of every record into this workgroup's partial row.
 // Workgroup 0 also stores the count, the sum and sigma in `rs.totals`.
-template <class PoseFn>
+template <class Acc, class PoseFn>
 SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
                        const View<int16_t>& dy, const IcpGeom& g, const RgbScratch& rs,
                        int nb_match, PoseFn get_pose) {
@@ -322,9 +370,9 @@ SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
         rs.totals[1] = sum_sq;
         rs.totals[2] = __float_as_int(sigma);
     }
-    double acc[kIcpSums];
+    Acc acc[kIcpSums];
 #pragma unroll
-    for (int j = 0; j < kIcpSums; ++j) acc[j] = 0.0;
+    for (int j = 0; j < kIcpSums; ++j) acc[j].zero();
     while (base < g.n) {
         float4 vl[kPixPerThread];
         int gx[kPixPerThread], gy[kPixPerThread];
@@ -355,43 +403,47 @@ SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
 // Sum of the partial rows of `nb` workgroups into totals (LDS), in a fixed
 // order: thread (j, k) adds rows k, k+8, ..., k+504 of column j (every load
 // issued before the first add), then eight lanes fold. NS sums per row.
-template <int NS = kIcpSums>
-SICP_DEV void icp_totals_issue(const double* partials, int nb, double& s) {
+template <int NS = kIcpSums, class Acc = SumF64>
+SICP_DEV void icp_totals_issue(const double* partials, int nb, Acc& s) {
     const int t = threadIdx.x;
-    s = 0.0;
+    s.zero();
     if (t < NS * 8) {
         const int j = t >> 3, k = t & 7;
-        const double* col = partials + j * kIcpMaxBlocks;
-        double v[kIcpMaxBlocks / 8];
+        const Acc* col = reinterpret_cast<const Acc*>(partials) + j * kIcpMaxBlocks;
+        Acc v[kIcpMaxBlocks / 8];
 #pragma unroll
         for (int r = 0; r < kIcpMaxBlocks / 8; ++r) {
             const int bi = k + 8 * r;
-            v[r] = bi < nb ? col[bi] : 0.0;
+            if (bi < nb)
+                v[r] = col[bi];
+            else
+                v[r].zero();
         }
 #pragma unroll
-        for (int r = 0; r < kIcpMaxBlocks / 8; ++r) s += v[r];
+        for (int r = 0; r < kIcpMaxBlocks / 8; ++r) s.merge(v[r]);
     }
 }
-template <int NS = kIcpSums>
-SICP_DEV void icp_totals_fold(double s, double* totals) {
+template <int NS = kIcpSums, class Acc = SumF64>
+SICP_DEV void icp_totals_fold(Acc s, double* totals) {
     const int t = threadIdx.x;
-    s += __shfl_xor(s, 1, 64);
-    s += __shfl_xor(s, 2, 64);
-    s += __shfl_xor(s, 4, 64);
-    if (t < NS * 8 && (t & 7) == 0) totals[t >> 3] = s;
+    s.merge(s.shfl_xor(1));
+    s.merge(s.shfl_xor(2));
+    s.merge(s.shfl_xor(4));
+    if (t < NS * 8 && (t & 7) == 0) totals[t >> 3] = s.value();
 }
-template <int NS = kIcpSums>
+template <int NS = kIcpSums, class Acc = SumF64>
 SICP_DEV void icp_totals(const double* partials, int nb, double* totals) {
-    double s;
-    icp_totals_issue<NS>(partials, nb, s);
-    icp_totals_fold<NS>(s, totals);
+    Acc s;
+    icp_totals_issue<NS, Acc>(partials, nb, s);
+    icp_totals_fold<NS, Acc>(s, totals);
     __syncthreads();
 }
 
+template <class Acc>
 __global__ __launch_bounds__(kIcpThreads) void icp_partials_kernel(
     View<float4> vcurr, View<float4> vlast, View<float4> ncurr, View<float4> nlast, PoseArg pc,
     PoseArg pl, IcpGeom g, double* partials) {
-    icp_accumulate<false>(vcurr, vlast, ncurr, nlast, g, partials, RgbPass{},
+    icp_accumulate<false, Acc>(vcurr, vlast, ncurr, nlast, g, partials, RgbPass{},
                           [&](PoseF& cur, PoseF& last) {
                               cur = load_pose(pc);
                               last = load_pose(pl);
@@ -399,12 +451,12 @@ __global__ __launch_bounds__(kIcpThreads) void icp_partials_kernel(
                           });
 }
 
-template <int NS = kIcpSums>
+template <int NS = kIcpSums, class Acc = SumF64>
 __global__ __launch_bounds__(kIcpThreads) void icp_sums_finalize_kernel(const double* partials,
                                                                         int nb,
                                                                         float* sums_out) {
     __shared__ double totals[NS];
-    icp_totals<NS>(partials, nb, totals);
+    icp_totals<NS, Acc>(partials, nb, totals);
     if (threadIdx.x < NS) sums_out[threadIdx.x] = (float)totals[threadIdx.x];
 }
 
@@ -418,9 +470,9 @@ __global__ __launch_bounds__(kIcpThreads) void so3_partials_kernel(
     View<uint8_t> inext, View<uint8_t> ilast, View<int16_t> dx, View<int16_t> dy, PoseArg pc,
     PoseArg pl, IcpGeom g, double* partials) {
     const PoseF cur = load_pose(pc), last = load_pose(pl);
-    double acc[kSo3Sums];
+    SumF64 acc[kSo3Sums];
 #pragma unroll
-    for (int j = 0; j < kSo3Sums; ++j) acc[j] = 0.0;
+    for (int j = 0; j < kSo3Sums; ++j) acc[j].zero();
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x) {
         const int y = i / g.cols, x = i - y * g.cols;
         const v3 vl = v3{((float)x - g.cx) / g.fx, ((float)y - g.cy) / g.fy, 1.0f};
@@ -443,8 +495,8 @@ __global__ __launch_bounds__(kIcpThreads) void so3_partials_kernel(
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int b = a; b < 4; ++b) acc[q++] += (double)(row[a] * row[b]);
-        acc[10] += 1.0;
+            for (int b = a; b < 4; ++b) acc[q++].add(row[a] * row[b]);
+        acc[10].add(1.0f);
     }
     block_rows_to_partials(acc, partials);
 }
@@ -601,7 +653,7 @@ static_assert(kStateWords <= kIcpThreads, "TrackState copied one word per thread
 // workgroup waits for another; workgroup 0 publishes it to `st_out`.
 // With `rgb_prev` (the previous iteration's photometric partial rows) the
 // update folds those too; `totals` then holds 2 * kIcpSums values.
-template <bool kFirst>
+template <bool kFirst, class Acc>
 SICP_DEV void icp_state(TrackState* S, double* totals, const TrackState* st_in,
                         const FrameMeta* last, const TrackParams& tp,
                         const double* partials_prev, const double* rgb_prev, int nb_prev,
@@ -612,12 +664,13 @@ SICP_DEV void icp_state(TrackState* S, double* totals, const TrackState* st_in,
     } else {
         const unsigned long long* src = reinterpret_cast<const unsigned long long*>(st_in);
         unsigned long long w = threadIdx.x < kStateWords ? src[threadIdx.x] : 0ull;
-        double s, sr = 0.0;
-        icp_totals_issue(partials_prev, nb_prev, s);
-        if (rgb_prev) icp_totals_issue(rgb_prev, nb_prev, sr);
+        Acc s, sr;
+        sr.zero();
+        icp_totals_issue<kIcpSums, Acc>(partials_prev, nb_prev, s);
+        if (rgb_prev) icp_totals_issue<kIcpSums, Acc>(rgb_prev, nb_prev, sr);
         if (threadIdx.x < kStateWords) reinterpret_cast<unsigned long long*>(S)[threadIdx.x] = w;
-        icp_totals_fold(s, totals);
-        if (rgb_prev) icp_totals_fold(sr, totals + kIcpSums);
+        icp_totals_fold<kIcpSums, Acc>(s, totals);
+        if (rgb_prev) icp_totals_fold<kIcpSums, Acc>(sr, totals + kIcpSums);
         __syncthreads();
         if (threadIdx.x == 0 && !S->done)
             gauss_newton_step(S, totals, rgb_prev ? totals + kIcpSums : nullptr, min_count_prev);
@@ -632,7 +685,8 @@ SICP_DEV void icp_state(TrackState* S, double* totals, const TrackState* st_in,
 // iteration, then this iteration's pixel pass (ICPReduce at the state's pose).
 // kRgb: the photometric term is on; its correspondence pass runs here and
 // the previous iteration's photometric rows (`rgb_prev`) join the update.
-template <bool kFirst, bool kRgb>
+// Acc: the arithmetic of the sums (SumF64, or SumK32 in the precision form).
+template <bool kFirst, bool kRgb, class Acc>
 __global__ __launch_bounds__(kIcpThreads) void icp_iteration_kernel(
     View<float4> vcurr, View<float4> vlast, View<float4> ncurr, View<float4> nlast, IcpGeom g,
     const TrackState* st_in, const FrameMeta* last, TrackParams tp, const double* partials_prev,
@@ -640,24 +694,25 @@ __global__ __launch_bounds__(kIcpThreads) void icp_iteration_kernel(
     double* partials, RgbPass rp) {
     __shared__ TrackState S;
     __shared__ double totals[2 * kIcpSums];
-    icp_accumulate<kRgb>(vcurr, vlast, ncurr, nlast, g, partials, rp,
-                         [&](PoseF& cur, PoseF& lst) {
-                             icp_state<kFirst>(&S, totals, st_in, last, tp, partials_prev,
-                                               kRgb ? rgb_prev : nullptr, nb_prev,
-                                               min_count_prev, st_out);
-                             if (S.done) return false;
-                             cur = to_pose(S.curr_f);
-                             lst = to_pose(S.last_f);
-                             return true;
-                         });
+    icp_accumulate<kRgb, Acc>(vcurr, vlast, ncurr, nlast, g, partials, rp,
+                              [&](PoseF& cur, PoseF& lst) {
+                                  icp_state<kFirst, Acc>(&S, totals, st_in, last, tp,
+                                                         partials_prev, kRgb ? rgb_prev : nullptr,
+                                                         nb_prev, min_count_prev, st_out);
+                                  if (S.done) return false;
+                                  cur = to_pose(S.curr_f);
+                                  lst = to_pose(S.last_f);
+                                  return true;
+                              });
 }
 
 // RGBStep's row pass. The pose is the iteration state's (`st`, published by
 // the iteration's pixel pass) or, without a state, `pc` / `pl`.
+template <class Acc>
 __global__ __launch_bounds__(kIcpThreads) void rgb_rows_kernel(
     View<float4> vlast, View<int16_t> dx, View<int16_t> dy, IcpGeom g, RgbScratch rs,
     int nb_match, const TrackState* st, PoseArg pc, PoseArg pl) {
-    rgb_rows(vlast, dx, dy, g, rs, nb_match, [&](PoseF& cur, PoseF& lst) {
+    rgb_rows<Acc>(vlast, dx, dy, g, rs, nb_match, [&](PoseF& cur, PoseF& lst) {
         if (st) {
             if (st->done) return false;
             cur = to_pose(st->curr_f);
@@ -704,7 +759,7 @@ __global__ __launch_bounds__(kIcpThreads) void rgb_match_kernel(View<float4> vcu
 // motion / error gate, the frame pose; when the frame is not tracked, the
 // last frame's level-0 model maps are copied into the new frame so the frame
 // swap keeps the last model (Tracking.cc:98-117 keeps lastFrame on failure).
-template <bool kFirst>
+template <bool kFirst, class Acc>
 __global__ __launch_bounds__(kIcpThreads) void track_finish_kernel(
     const TrackState* st_in, const FrameMeta* last, TrackParams tp, const double* partials_prev,
     const double* rgb_prev, int nb_prev, int min_count_prev, TrackState* st_final,
@@ -712,7 +767,7 @@ __global__ __launch_bounds__(kIcpThreads) void track_finish_kernel(
     View<float4> nnext, FrameImages images) {
     __shared__ TrackState S;
     __shared__ double totals[2 * kIcpSums];
-    icp_state<kFirst>(&S, totals, st_in, last, tp, partials_prev, rgb_prev, nb_prev,
+    icp_state<kFirst, Acc>(&S, totals, st_in, last, tp, partials_prev, rgb_prev, nb_prev,
                       min_count_prev, nullptr);
     if (threadIdx.x == 0) track_end_state(&S);
     __syncthreads();
@@ -771,13 +826,21 @@ inline int icp_blocks(int n) {
 
 int launch_icp_sums(View<float4> vcurr, View<float4> vlast, View<float4> ncurr,
                     View<float4> nlast, PoseArg curr, PoseArg last, sicp_intrinsics K,
-                    IcpScratch scratch, float* sums_out, hipStream_t s) {
+                    IcpScratch scratch, float* sums_out, bool k32, hipStream_t s) {
     IcpGeom g{vcurr.cols, vcurr.rows, vcurr.cols * vcurr.rows, K.fx, K.fy, K.cx, K.cy};
     const int nb = icp_blocks(g.n);
-    icp_partials_kernel<<<nb, kIcpThreads, 0, s>>>(vcurr, vlast, ncurr, nlast, curr, last, g,
-                                                     scratch.partials);
-    SICP_LAUNCH_CHECK("icp_partials_kernel");
-    icp_sums_finalize_kernel<><<<1, kIcpThreads, 0, s>>>(scratch.partials, nb, sums_out);
+    if (k32) {
+        icp_partials_kernel<SumK32><<<nb, kIcpThreads, 0, s>>>(vcurr, vlast, ncurr, nlast, curr,
+                                                               last, g, scratch.partials);
+        SICP_LAUNCH_CHECK("icp_partials_kernel");
+        icp_sums_finalize_kernel<kIcpSums, SumK32><<<1, kIcpThreads, 0, s>>>(scratch.partials,
+                                                                              nb, sums_out);
+    } else {
+        icp_partials_kernel<SumF64><<<nb, kIcpThreads, 0, s>>>(vcurr, vlast, ncurr, nlast, curr,
+                                                               last, g, scratch.partials);
+        SICP_LAUNCH_CHECK("icp_partials_kernel");
+        icp_sums_finalize_kernel<><<<1, kIcpThreads, 0, s>>>(scratch.partials, nb, sums_out);
+    }
     SICP_LAUNCH_CHECK("icp_sums_finalize_kernel");
     return SICP_OK;
 }
@@ -790,10 +853,11 @@ IcpGeom geom_of(const IcpLevel& l) {
 }
 }  // namespace
 
-int launch_icp_iteration(const IcpLevel& cur, const IcpLevel* prev, const FrameMeta* last,
-                         TrackParams tp, const TrackState* st_in, TrackState* st_out,
-                         const double* partials_prev, double* partials, const RgbScratch* rgb,
-                         hipStream_t s) {
+template <class Acc>
+static void icp_iteration_launch(const IcpLevel& cur, const IcpLevel* prev,
+                                 const FrameMeta* last, TrackParams tp, const TrackState* st_in,
+                                 TrackState* st_out, const double* partials_prev,
+                                 double* partials, const RgbScratch* rgb, hipStream_t s) {
     const IcpGeom g = geom_of(cur);
     const int nb = icp_blocks(g.n);
     const int nb_prev = prev ? icp_blocks(prev->vcurr.cols * prev->vcurr.rows) : 0;
@@ -805,30 +869,47 @@ int launch_icp_iteration(const IcpLevel& cur, const IcpLevel* prev, const FrameM
         rgb_prev = rgb->partials;
     }
     if (prev && rgb)
-        icp_iteration_kernel<false, true><<<nb, kIcpThreads, 0, s>>>(
+        icp_iteration_kernel<false, true, Acc><<<nb, kIcpThreads, 0, s>>>(
             cur.vcurr, cur.vlast, cur.ncurr, cur.nlast, g, st_in, last, tp, partials_prev,
             rgb_prev, nb_prev, min_prev, st_out, partials, rp);
     else if (prev)
-        icp_iteration_kernel<false, false><<<nb, kIcpThreads, 0, s>>>(
+        icp_iteration_kernel<false, false, Acc><<<nb, kIcpThreads, 0, s>>>(
             cur.vcurr, cur.vlast, cur.ncurr, cur.nlast, g, st_in, last, tp, partials_prev,
             nullptr, nb_prev, min_prev, st_out, partials, rp);
     else if (rgb)
-        icp_iteration_kernel<true, true><<<nb, kIcpThreads, 0, s>>>(
+        icp_iteration_kernel<true, true, Acc><<<nb, kIcpThreads, 0, s>>>(
             cur.vcurr, cur.vlast, cur.ncurr, cur.nlast, g, nullptr, last, tp, nullptr, nullptr,
             0, 0, st_out, partials, rp);
     else
-        icp_iteration_kernel<true, false><<<nb, kIcpThreads, 0, s>>>(
+        icp_iteration_kernel<true, false, Acc><<<nb, kIcpThreads, 0, s>>>(
             cur.vcurr, cur.vlast, cur.ncurr, cur.nlast, g, nullptr, last, tp, nullptr, nullptr,
             0, 0, st_out, partials, rp);
+}
+
+int launch_icp_iteration(const IcpLevel& cur, const IcpLevel* prev, const FrameMeta* last,
+                         TrackParams tp, const TrackState* st_in, TrackState* st_out,
+                         const double* partials_prev, double* partials, const RgbScratch* rgb,
+                         hipStream_t s) {
+    if (tp.sums_k32)
+        icp_iteration_launch<SumK32>(cur, prev, last, tp, st_in, st_out, partials_prev, partials,
+                                     rgb, s);
+    else
+        icp_iteration_launch<SumF64>(cur, prev, last, tp, st_in, st_out, partials_prev, partials,
+                                     rgb, s);
     SICP_LAUNCH_CHECK("icp_iteration_kernel");
     return SICP_OK;
 }
 
-int launch_rgb_rows(const IcpLevel& cur, const TrackState* st, RgbScratch rgb, hipStream_t s) {
+int launch_rgb_rows(const IcpLevel& cur, const TrackState* st, RgbScratch rgb, bool k32,
+                    hipStream_t s) {
     const IcpGeom g = geom_of(cur);
     const int nb = icp_blocks(g.n);
-    rgb_rows_kernel<<<nb, kIcpThreads, 0, s>>>(cur.vlast, cur.rgb.dx, cur.rgb.dy, g, rgb, nb, st,
-                                                PoseArg{}, PoseArg{});
+    if (k32)
+        rgb_rows_kernel<SumK32><<<nb, kIcpThreads, 0, s>>>(cur.vlast, cur.rgb.dx, cur.rgb.dy, g,
+                                                           rgb, nb, st, PoseArg{}, PoseArg{});
+    else
+        rgb_rows_kernel<SumF64><<<nb, kIcpThreads, 0, s>>>(cur.vlast, cur.rgb.dx, cur.rgb.dy, g,
+                                                           rgb, nb, st, PoseArg{}, PoseArg{});
     SICP_LAUNCH_CHECK("rgb_rows_kernel");
     return SICP_OK;
 }
@@ -844,11 +925,16 @@ int launch_track_finish(const IcpLevel* prev, const FrameMeta* last, TrackParams
     if (images) im = *images;
     if (prev) {
         const int nb_prev = icp_blocks(prev->vcurr.cols * prev->vcurr.rows);
-        track_finish_kernel<false><<<blocks, kIcpThreads, 0, s>>>(
-            st_in, last, tp, partials_prev, rgb ? rgb->partials : nullptr, nb_prev,
-            prev->min_count, st_final, next, vlast, nlast, vnext, nnext, im);
+        if (tp.sums_k32)
+            track_finish_kernel<false, SumK32><<<blocks, kIcpThreads, 0, s>>>(
+                st_in, last, tp, partials_prev, rgb ? rgb->partials : nullptr, nb_prev,
+                prev->min_count, st_final, next, vlast, nlast, vnext, nnext, im);
+        else
+            track_finish_kernel<false, SumF64><<<blocks, kIcpThreads, 0, s>>>(
+                st_in, last, tp, partials_prev, rgb ? rgb->partials : nullptr, nb_prev,
+                prev->min_count, st_final, next, vlast, nlast, vnext, nnext, im);
     } else {
-        track_finish_kernel<true><<<blocks, kIcpThreads, 0, s>>>(
+        track_finish_kernel<true, SumF64><<<blocks, kIcpThreads, 0, s>>>(
             nullptr, last, tp, nullptr, nullptr, 0, 0, st_final, next, vlast, nlast, vnext,
             nnext, im);
     }
@@ -874,7 +960,8 @@ int launch_rgb_sums(View<uint8_t> inext, View<uint8_t> ilast, View<float4> vnext
     rgb_match_kernel<<<nb, kIcpThreads, 0, s>>>(vnext, vlast, g, curr, last,
                                                  RgbPass{kv, ilast, rgb.rec, rgb.ipart});
     SICP_LAUNCH_CHECK("rgb_match_kernel");
-    rgb_rows_kernel<<<nb, kIcpThreads, 0, s>>>(vlast, dx, dy, g, rgb, nb, nullptr, curr, last);
+    rgb_rows_kernel<SumF64><<<nb, kIcpThreads, 0, s>>>(vlast, dx, dy, g, rgb, nb, nullptr, curr,
+                                                       last);
     SICP_LAUNCH_CHECK("rgb_rows_kernel");
     icp_sums_finalize_kernel<><<<1, kIcpThreads, 0, s>>>(rgb.partials, nb, sums_out);
     SICP_LAUNCH_CHECK("icp_sums_finalize_kernel");

@@ -72,16 +72,18 @@ struct TrackParams {
     float thresh;      // thresh_icp
     float rgb_weight;  // w (Tracking.cc:392)
     int rgb;           // !icpOnly: RGBStep runs before ICPStep in every iteration
+    int sums_k32;      // the precision form: FP32 compensated sums (else FP64)
 };
 
 struct IcpScratch {
     double* partials;  // kIcpSums * kIcpMaxBlocks, column-major
 };
 
-// ICPStep: the 29 sums of one level into `sums_out` (device).
+// ICPStep: the 29 sums of one level into `sums_out` (device); `k32`: the
+// precision form's FP32 compensated sums instead of FP64 accumulation.
 int launch_icp_sums(View<float4> vcurr, View<float4> vlast, View<float4> ncurr,
                     View<float4> nlast, PoseArg curr, PoseArg last, sicp_intrinsics K,
-                    IcpScratch scratch, float* sums_out, hipStream_t s);
+                    IcpScratch scratch, float* sums_out, bool k32, hipStream_t s);
 
 // The photometric inputs of one level (RGBStep, Reduction.h:81-90): the
 // current frame's candidate key (its intensity where the pose-independent
@@ -122,7 +124,8 @@ int launch_icp_iteration(const IcpLevel& cur, const IcpLevel* prev, const FrameM
 // RGBStep's second pass for the iteration whose state is `st`: sigma from the
 // correspondence counts, then the 29 weighted photometric sums into
 // `rgb.partials`.
-int launch_rgb_rows(const IcpLevel& cur, const TrackState* st, RgbScratch rgb, hipStream_t s);
+int launch_rgb_rows(const IcpLevel& cur, const TrackState* st, RgbScratch rgb, bool k32,
+                    hipStream_t s);
 // The last iteration's update, the final gate, the new frame's pose and,
 // when the frame is not tracked, the copy of the last frame's level-0 model
 // maps (and, with the photometric term, its intensity pyramid) into the new

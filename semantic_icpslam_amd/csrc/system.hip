@@ -211,6 +211,8 @@ int sicp_system_create(const sicp_system_config* cfg, sicp_system** out) {
         return invalid("system_create: icp_only must be 0 or 1");
     if (!cfg->icp_only && !std::isfinite(cfg->rgb_weight))
         return invalid("system_create: rgb_weight must be finite");
+    if (cfg->icp_sums_fp32 != 0 && cfg->icp_sums_fp32 != 1)
+        return invalid("system_create: icp_sums_fp32 must be 0 or 1");
     const bool rgb = !cfg->icp_only;
     sicp_system* s = new sicp_system();
     s->cfg = *cfg;
@@ -309,7 +311,8 @@ static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
         TrackState* iter = s->st + 1;
         const size_t psz = (size_t)kIcpMaxBlocks * kIcpSums;
         const bool rgb = !s->cfg.icp_only;
-        const TrackParams tp{s->cfg.icp_thresh, s->cfg.rgb_weight, rgb ? 1 : 0};
+        const TrackParams tp{s->cfg.icp_thresh, s->cfg.rgb_weight, rgb ? 1 : 0,
+                             s->cfg.icp_sums_fp32};
         IcpLevel prev{};
         int k = 0;
         for (int l = 2; l >= 0; --l) {
@@ -324,7 +327,9 @@ static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
                                           s->partials + psz * (k & 1), rgb ? &s->rgb : nullptr,
                                           st);
                 if (rc) return rc;
-                if (rgb && (rc = launch_rgb_rows(cur, iter + (k & 1), s->rgb, st))) return rc;
+                if (rgb && (rc = launch_rgb_rows(cur, iter + (k & 1), s->rgb,
+                                                 tp.sums_k32 != 0, st)))
+                    return rc;
                 stage_end(s, kStageIcpIter, ek, st);
                 prev = cur;
             }

@@ -25,7 +25,7 @@ def _maps(c, frame, seed=0):
     return v, O.compute_nmap(v)
 
 
-def _gpu_sums(dev, vc, vl, nc, nl, pc, pl, k):
+def _gpu_sums(dev, vc, vl, nc, nl, pc, pl, k, fp32=False):
     t = [to_dev(a, dev) for a in (vc, vl, nc, nl)]
     partials = torch.zeros(B.lib().sicp_icp_partials_size(), dtype=torch.uint8, device=dev)
     sums = torch.zeros(29, dtype=torch.float32, device=dev)
@@ -34,9 +34,10 @@ def _gpu_sums(dev, vc, vl, nc, nl, pc, pl, k):
     b = np.zeros(6)
     res = np.zeros(2, np.float32)
     p = lambda a: a.ctypes.data_as(B.C.c_void_p)
-    B.check(B.lib().sicp_icp_step(B.image(t[0]), B.image(t[1]), B.image(t[2]), B.image(t[3]),
-                                  B.C.byref(pc), B.C.byref(pl), k, partials.data_ptr(),
-                                  sums.data_ptr(), p(hs), p(A), p(b), p(res), B.stream_ptr()))
+    fn = B.lib().sicp_icp_step_fp32 if fp32 else B.lib().sicp_icp_step
+    B.check(fn(B.image(t[0]), B.image(t[1]), B.image(t[2]), B.image(t[3]), B.C.byref(pc),
+               B.C.byref(pl), k, partials.data_ptr(), sums.data_ptr(), p(hs), p(A), p(b), p(res),
+               B.stream_ptr()))
     return hs, A.reshape(6, 6), b, res, sums
 
 
@@ -72,6 +73,36 @@ def test_icp_sums_vs_oracle(dev, c, level):
     assert np.array_equal(A, Ao) and np.array_equal(b, bo)
 
 
+@pytest.mark.parametrize("c", [CFG_SMALL, CFG_VGA])
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_icp_sums_fp32_compensated_bit_exact(dev, c, level):
+    """sicp_icp_step_fp32 (the precision form's FP32 compensated sums) against
+    the oracle's restatement in the same order: bit-exact. Against the exact
+    sums of the same FP32 products (math.fsum): within half an FP32 ulp of the
+    sum plus 2^-40 of the sum of magnitudes, for every one of the 29."""
+    import math
+    v0, n0 = _maps(c, 0)
+    v1, n1 = _maps(c, 3)
+    for _ in range(level):
+        v0, n0 = O.resize_map(v0, n0)
+        v1, n1 = O.resize_map(v1, n1)
+    pc = B.pose_f(synth(c).pose(3))
+    pl = B.pose_f(np.eye(4))
+    hs, A, b, res, _ = _gpu_sums(dev, v1, v0, n1, n0, pc, pl, K(c, level), fp32=True)
+    _, rows = O.icp_sums(v1, v0, n1, n0, pc, pl, K(c, level), rows_out=True)
+    ok = O.sums_k32(rows)
+    assert np.array_equal(hs.view(np.uint32), ok.view(np.uint32)), np.abs(hs - ok).max()
+    assert rows[:, 28].sum() > 500
+    r64 = rows.astype(np.float64)
+    for j in range(29):
+        exact = math.fsum(r64[:, j])
+        mag = math.fsum(np.abs(r64[:, j]))
+        assert abs(float(hs[j]) - exact) <= 2.0 ** -24 * abs(exact) + 2.0 ** -40 * mag, j
+    # the plain FP64 step agrees to the test's tolerance
+    hd = _gpu_sums(dev, v1, v0, n1, n0, pc, pl, K(c, level))[0]
+    _check_sums(hs, hd.astype(np.float64))
+
+
 def test_icp_identical_frames(dev):
     """Zero motion on identical maps: r = 0 for every match, so b = 0, r^2 = 0
     and every valid vertex with a valid normal is a correspondence."""
@@ -93,9 +124,10 @@ def test_icp_deterministic(dev):
     v1, n1 = _maps(c, 2)
     pc = B.pose_f(synth(c).pose(2))
     pl = B.pose_f(np.eye(4))
-    a = _gpu_sums(dev, v1, v0, n1, n0, pc, pl, K(c))[0]
-    b = _gpu_sums(dev, v1, v0, n1, n0, pc, pl, K(c))[0]
-    assert np.array_equal(a, b)
+    for fp32 in (False, True):
+        a = _gpu_sums(dev, v1, v0, n1, n0, pc, pl, K(c), fp32)[0]
+        b = _gpu_sums(dev, v1, v0, n1, n0, pc, pl, K(c), fp32)[0]
+        assert np.array_equal(a, b)
 
 
 def test_se3_update_host_matches_oracle():

@@ -165,15 +165,18 @@ def test_so3_step_vs_oracle(dev, c, level):
     assert np.array_equal(hs, hp)
 
 
-def _rgb_cfg(c):
+def _rgb_cfg(c, k32=0):
     cfg = sys_cfg(c)
     cfg.icp_only = 0
+    cfg.icp_sums_fp32 = k32
     return cfg
 
 
-@pytest.mark.parametrize("c,frames", [(CFG_SMALL, 12), (CFG_VGA, 4)])
-def test_system_rgb_vs_oracle(dev, c, frames):
-    cfg = _rgb_cfg(c)
+# k32 = 1: the precision form's FP32 compensated sums (photometric and ICP)
+# on both sides.
+@pytest.mark.parametrize("c,frames,k32", [(CFG_SMALL, 12, 0), (CFG_VGA, 4, 0), (CFG_SMALL, 8, 1)])
+def test_system_rgb_vs_oracle(dev, c, frames, k32):
+    cfg = _rgb_cfg(c, k32)
     g = B.System(cfg)
     o = O.OracleSystem(cfg)
     s = synth(c)

@@ -15,9 +15,14 @@ from oracle import binding as O
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("c,frames,half", [(CFG_SMALL, 12, 0), (CFG_VGA, 4, 0), (CFG_SMALL, 8, 1)])
-def test_system_vs_oracle(dev, c, frames, half):
+# half = 1: the precision form's map (sdf as half, weights up to 128);
+# k32 = 1: its FP32 compensated Gauss-Newton sums, on both sides.
+@pytest.mark.parametrize("c,frames,half,k32", [(CFG_SMALL, 12, 0, 0), (CFG_VGA, 4, 0, 0),
+                                               (CFG_SMALL, 8, 1, 0), (CFG_SMALL, 8, 1, 1),
+                                               (CFG_VGA, 4, 1, 1)])
+def test_system_vs_oracle(dev, c, frames, half, k32):
     cfg = sys_cfg(c, **(dict(tsdf_half=1, max_weight=128) if half else {}))
+    cfg.icp_sums_fp32 = k32
     g = B.System(cfg)
     o = O.OracleSystem(cfg)
     s = synth(c)

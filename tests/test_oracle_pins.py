@@ -278,6 +278,68 @@ def _photometric_A(img, dx, dy, v, ok):
     return A, Jsum
 
 
+# ------------------------------------------- compensated FP32 sums --------
+def _k32_rows(n, col0):
+    r = np.zeros((n, 29), np.float32)
+    r[:, 28] = 1.0
+    for i, v in col0.items():
+        r[i, 0] = v
+    return r
+
+
+def test_k32_sums_known_answers():
+    """2^24 + 1 + 1 is 16777218 exactly; plain FP32 loses both ones (each
+    addition ties back to 2^24). Pixels 0, 1, 2 meet in the workgroup's
+    butterfly (TwoSum); pixels 0, 256, 512 are one thread's (Kahan)."""
+    big = float(2 ** 24)
+    for n, idx in ((3, (0, 1, 2)), (513, (0, 256, 512))):
+        r = _k32_rows(n, {idx[0]: big, idx[1]: 1.0, idx[2]: 1.0})
+        s = O.sums_k32(r)
+        assert s[0] == 16777218.0 and s[28] == n
+        assert np.all(s[1:28] == 0)
+        assert O.sums_k32(r, compensate=False)[0] == big
+    # rows whose slot 28 is 0 do not contribute
+    r = _k32_rows(40, {3: 5.0, 7: 2.0})
+    r[3, 28] = 0.0
+    s = O.sums_k32(r)
+    assert s[0] == 2.0 and s[28] == 39
+
+
+def _exact_bound_ratio(rows, s):
+    """max over the 29 sums of |s - exact| / (2^-24 |exact| + 2^-40 sum |x|)."""
+    import math
+    r64 = rows.astype(np.float64)
+    worst = 0.0
+    for j in range(29):
+        col = r64[:, j][rows[:, 28] != 0]
+        exact = math.fsum(col)
+        bound = 2.0 ** -24 * abs(exact) + 2.0 ** -40 * math.fsum(np.abs(col))
+        err = abs(float(s[j]) - exact)
+        worst = max(worst, err / bound if bound > 0 else (0.0 if err == 0 else np.inf))
+    return worst
+
+
+@pytest.mark.parametrize("case", ["icp", "cancel"])
+def test_k32_sums_within_half_ulp_of_exact(case):
+    """The compensated sum is the exact sum of the FP32 inputs rounded once
+    (within 2^-40 of the sum of magnitudes), over several workgroups; the same
+    order without compensation (a planted mistake) misses that bound."""
+    if case == "icp":
+        c = CFG_SMALL
+        v0, n0 = _frame_maps(c, 0)
+        v1, n1 = _frame_maps(c, 3)
+        _, rows = O.icp_sums(v1, v0, n1, n0, B.pose_f(synth(c).pose(3)), B.pose_f(np.eye(4)),
+                             K(c), rows_out=True)
+    else:
+        rng = np.random.default_rng(11)
+        n = 5 * 1024 + 37
+        rows = (rng.standard_normal((n, 29)) * 10.0 ** rng.integers(-3, 4, (n, 29))).astype(
+            np.float32)
+        rows[:, 28] = (rng.random(n) < 0.9).astype(np.float32)
+    assert _exact_bound_ratio(rows, O.sums_k32(rows)) <= 1.0
+    assert _exact_bound_ratio(rows, O.sums_k32(rows, compensate=False)) > 1.0
+
+
 def test_rgb_identical_frames_textbook_jacobian():
     """Zero motion, identical frames with scattered zero intensities: the
     count is the number of pixels passing the candidate tests, every
