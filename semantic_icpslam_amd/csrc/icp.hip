@@ -506,95 +506,178 @@ SICP_DEV void copy16(double* d, const double* s) {
     for (int i = 0; i < 16; ++i) d[i] = s[i];
 }
 
+// Lane j's value of v, in every lane (j uniform, every lane of the wave
+// active).
+SICP_DEV double lane_bcast(double v, int j) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, j);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), j);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// pose_to_float spread over lanes: lane e writes R[e] and Rinv[e] (e < 9)
+// and t[e] (e < 3).
+SICP_DEV void pose_to_float_lanes(const double* T, sicp_pose_f* out, int e) {
+    if (e < 9) {
+        const int r = e / 3, c = e - 3 * r;
+        out->R[e] = (float)T[r * 4 + c];
+        out->Rinv[e] = (float)T[c * 4 + r];
+    }
+    if (e < 3) out->t[e] = (float)T[e * 4 + 3];
+}
+
 // The update of one iteration of the inner loop of ComputeSE3
-// (Tracking.cc:400-514), on one thread, from the iteration's 29 ICP sums and,
-// with the photometric term, its 29 RGB sums (`rgb_totals`, else null). The
-// state lives in LDS; the poses are read into registers once and written
-// back once.
-SICP_DEV void gauss_newton_step(TrackState* st, const double* totals, const double* rgb_totals,
-                                int min_count) {
-    float sums[kIcpSums];
-#pragma unroll
-    for (int j = 0; j < kIcpSums; ++j) sums[j] = (float)totals[j];  // the float download
-    double last_pose[16], next_pose[16], delta[16];
-    copy16(last_pose, st->last_pose);
-    copy16(next_pose, st->next_pose);
-    copy16(delta, st->delta);
+// (Tracking.cc:400-514), from the iteration's 29 ICP sums and, with the
+// photometric term, its 29 RGB sums (`rgb_totals`, else null), run by the
+// 64 lanes of one wavefront. The arithmetic is se3_math.hpp's —
+// create_matrix, ldlt_solve6, se3_exp, mat4_mul, rigid_inverse — every
+// value formed by the same sequence of operations, so the result is the
+// same to the bit; only the work is spread: lane i < 6 owns row i of A and
+// of L (column j of the factorisation is one step for every row), lane
+// e < 16 owns entry e of the 4x4 products. The state lives in LDS; `tmp`
+// is 32 doubles of LDS scratch. Every lane computes the uniform values
+// (errors, pivots, the twist, its exponential); lane 0 writes the scalars.
+SICP_DEV void gauss_newton_wave(TrackState* st, const double* totals, const double* rgb_totals,
+                                int min_count, double* tmp) {
+    const int lane = threadIdx.x & 63;
     float rgb_error = 0.f;
     if (rgb_totals) {
         // rgbError and "track rgb failed" (Tracking.cc:425-432), before ICPStep's
         const float q0 = (float)rgb_totals[27], q1 = (float)rgb_totals[28];
         rgb_error = sqrtf(q0) / q1;
         const int rgb_count = (int)q1;
-        st->rgb_error = rgb_error;
-        st->rgb_count = rgb_count;
+        if (lane == 0) {
+            st->rgb_error = rgb_error;
+            st->rgb_count = rgb_count;
+        }
         if (isnan_(rgb_error) || rgb_count < min_count) {
-            copy16(st->next_pose, last_pose);
-            st->failed = 1;
-            st->done = 1;
-            pose_to_float(last_pose, &st->curr_f);
+            if (lane < 16) st->next_pose[lane] = st->last_pose[lane];
+            if (lane == 0) {
+                st->failed = 1;
+                st->done = 1;
+            }
+            pose_to_float_lanes(st->last_pose, &st->curr_f, lane);
             return;
         }
     }
     int break_count = st->break_count;
     const float last_error = st->last_error;
-    const float r0 = sums[27], r1 = sums[28];
+    const float r0 = (float)totals[27], r1 = (float)totals[28];  // the float download
     const float icp_error = sqrtf(r0) / r1;
     const int icp_count = (int)r1;
-    st->icp_count = icp_count;
+    if (lane == 0) st->icp_count = icp_count;
     if (isnan_(icp_error) || icp_count < min_count) {
         // "track icp failed": roll back (Tracking.cc:452-456)
-        copy16(st->next_pose, last_pose);
-        st->icp_error = icp_error;
-        st->failed = 1;
-        st->done = 1;
-        pose_to_float(last_pose, &st->curr_f);
+        if (lane < 16) st->next_pose[lane] = st->last_pose[lane];
+        if (lane == 0) {
+            st->icp_error = icp_error;
+            st->failed = 1;
+            st->done = 1;
+        }
+        pose_to_float_lanes(st->last_pose, &st->curr_f, lane);
         return;
     }
     const float curr_error = rgb_error * st->rgb_weight + icp_error;  // Tracking.cc:458
     if (last_error < curr_error) {
         break_count++;
-        if (break_count == 1) copy16(st->candidate, next_pose);
-        st->candidate_icp = icp_error;  // assigned on every increase (Q11)
+        if (break_count == 1 && lane < 16) st->candidate[lane] = st->next_pose[lane];
+        if (lane == 0) st->candidate_icp = icp_error;  // assigned on every increase (Q11)
     } else {
         break_count = 0;
     }
-    st->break_count = break_count;
+    if (lane == 0) st->break_count = break_count;
     if (break_count > 2) {
-        double cand[16];
-        copy16(cand, st->candidate);
-        copy16(st->next_pose, cand);
-        st->icp_error = st->candidate_icp;
-        st->done = 1;
-        pose_to_float(cand, &st->curr_f);
+        if (lane < 16) st->next_pose[lane] = st->candidate[lane];
+        if (lane == 0) {
+            st->icp_error = st->candidate_icp;
+            st->done = 1;
+        }
+        pose_to_float_lanes(st->candidate, &st->curr_f, lane);
         return;
     }
-    st->last_error = curr_error;
-    st->icp_error = icp_error;
-    double A[36], b[6], x[6], dT[16], inv[16];
-    create_matrix(sums, A, b);
-    if (rgb_totals) {
-        // matA = matA_rgb * w + matA_icp, vecb likewise (Tracking.cc:484-487)
-        float rs[kIcpSums];
-#pragma unroll
-        for (int j = 0; j < kIcpSums; ++j) rs[j] = (float)rgb_totals[j];
-        double Ar[36], br[6];
-        create_matrix(rs, Ar, br);
-        const double w = (double)st->rgb_weight;
-#pragma unroll
-        for (int k = 0; k < 36; ++k) A[k] = Ar[k] * w + A[k];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) b[k] = br[k] * w + b[k];
+    if (lane == 0) {
+        st->last_error = curr_error;
+        st->icp_error = icp_error;
     }
-    ldlt_solve6(A, b, x);
+    // CreateMatrix<6,7>: row i of A and b_i from the float sums; entry (i, j),
+    // j >= i, is sum 7i - i(i-1)/2 + (j - i), mirrored below the diagonal.
+    // With the photometric term, A = A_rgb * w + A (Tracking.cc:484-487).
+    const int i = lane < 6 ? lane : 5;
+    double a[6], bi;
+    {
+        const int di = 7 * i - i * (i - 1) / 2;
+        int q[7];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) q[j] = j >= i ? di + (j - i) : 7 * j - j * (j - 1) / 2 + (i - j);
+        q[6] = di + 6 - i;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) a[j] = (double)(float)totals[q[j]];
+        bi = (double)(float)totals[q[6]];
+        if (rgb_totals) {
+            const double w = (double)st->rgb_weight;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) a[j] = (double)(float)rgb_totals[q[j]] * w + a[j];
+            bi = (double)(float)rgb_totals[q[6]] * w + bi;
+        }
+    }
+    // ldlt_solve6: column j of L and pivot j, every row at once.
+    double L[6], D[6], Dinv[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double s = a[j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) s -= L[k] * lane_bcast(L[k], j) * D[k];
+        const double d = lane_bcast(s, j);
+        D[j] = d;
+        Dinv[j] = (d != 0.0) ? 1.0 / d : 0.0;
+        L[j] = i > j ? s * Dinv[j] : 1.0;
+    }
+    double zs = bi;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const double zk = lane_bcast(zs, k);
+        if (i > k) zs -= L[k] * zk;
+    }
+    double dinv_i = Dinv[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k)
+        if (i == k) dinv_i = Dinv[k];
+    zs = zs * dinv_i;
+    double x[6];
+#pragma unroll
+    for (int r = 5; r >= 0; --r) {
+        double s = lane_bcast(zs, r);
+#pragma unroll
+        for (int k = r + 1; k < 6; ++k) s -= lane_bcast(L[r], k) * x[k];
+        x[r] = s;
+    }
+    double dT[16];
     se3_exp(x, dT);
-    mat4_mul(dT, delta, delta);
-    rigid_inverse(delta, inv);
-    mat4_mul(last_pose, inv, next_pose);
-    copy16(st->delta, delta);
-    copy16(st->next_pose, next_pose);
-    st->applied++;
-    pose_to_float(next_pose, &st->curr_f);
+    // delta = exp(x) * delta; next = last * delta^-1; lane e owns entry e
+    if (lane == 0) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) tmp[e] = dT[e];
+    }
+    const int e = lane & 15, r = e >> 2, c = e & 3;
+    const double nd = tmp[r * 4 + 0] * st->delta[0 * 4 + c] + tmp[r * 4 + 1] * st->delta[1 * 4 + c] +
+                      tmp[r * 4 + 2] * st->delta[2 * 4 + c] + tmp[r * 4 + 3] * st->delta[3 * 4 + c];
+    if (lane < 16) st->delta[e] = nd;
+    double inv;
+    if (r < 3 && c < 3)
+        inv = st->delta[c * 4 + r];
+    else if (r < 3)
+        inv = -(st->delta[0 * 4 + r] * st->delta[3] + st->delta[1 * 4 + r] * st->delta[7] +
+                st->delta[2 * 4 + r] * st->delta[11]);
+    else
+        inv = c == 3 ? 1.0 : 0.0;
+    if (lane < 16) tmp[16 + e] = inv;
+    const double np = st->last_pose[r * 4 + 0] * tmp[16 + 0 * 4 + c] +
+                      st->last_pose[r * 4 + 1] * tmp[16 + 1 * 4 + c] +
+                      st->last_pose[r * 4 + 2] * tmp[16 + 2 * 4 + c] +
+                      st->last_pose[r * 4 + 3] * tmp[16 + 3 * 4 + c];
+    if (lane < 16) st->next_pose[e] = np;
+    if (lane == 0) st->applied++;
+    pose_to_float_lanes(st->next_pose, &st->curr_f, lane);
 }
 
 // ComputeSE3's set-up (Tracking.cc:369-431 with TrackFrame :157-173): the
@@ -672,8 +755,10 @@ SICP_DEV void icp_state(TrackState* S, double* totals, const TrackState* st_in,
         icp_totals_fold<kIcpSums, Acc>(s, totals);
         if (rgb_prev) icp_totals_fold<kIcpSums, Acc>(sr, totals + kIcpSums);
         __syncthreads();
-        if (threadIdx.x == 0 && !S->done)
-            gauss_newton_step(S, totals, rgb_prev ? totals + kIcpSums : nullptr, min_count_prev);
+        __shared__ double gn_tmp[32];
+        if (threadIdx.x < 64 && !S->done)
+            gauss_newton_wave(S, totals, rgb_prev ? totals + kIcpSums : nullptr, min_count_prev,
+                              gn_tmp);
         __syncthreads();
     }
     if (st_out && blockIdx.x == 0 && threadIdx.x < kStateWords)
