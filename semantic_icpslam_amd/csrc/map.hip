@@ -569,8 +569,12 @@ __global__ __launch_bounds__(256) void fuse_kernel(MapDev m, const MapCounters* 
     const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
     const float T = m.trunc;
     uint32_t updated = 0;
+    // Grid-stride over the visible blocks; the next block's record is loaded
+    // while the current block is fused.
+    sicp_block nblk = wave0 < nvis ? visible[wave0] : sicp_block{};
     for (uint32_t vi = wave0; vi < nvis; vi += nwaves) {
-        const sicp_block blk = visible[vi];
+        const sicp_block blk = nblk;
+        if (vi + nwaves < nvis) nblk = visible[vi + nwaves];
         const size_t vbase = (size_t)blk.slot * kBlockVoxels + lane;
         const int bx = blk.x * kBlockSide + lx, by = blk.y * kBlockSide + ly;
         const int bz = blk.z * kBlockSide;

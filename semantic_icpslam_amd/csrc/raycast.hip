@@ -219,20 +219,20 @@ SICP_DEV int corner_slot(const CornerBlocks& cb, const AxisPair& X, const AxisPa
     return dz ? b1 : b0;
 }
 
-// Rendering::readSdfInterped (RenderScene.cu:310-343) from per-axis pairs.
-// Corner order, NaN for a missing block, `valid` of the last corner's lookup
-// and the blend order are the reference's. When all eight corners lie in the
-// cached block (the common case) no lookup is made.
-SICP_DEV float interp_axes(const MapDev& m, AxisPair X, AxisPair Y, AxisPair Z, BlockCache& c,
-                           bool& valid) {
-    float sd[8];
+// Rendering::readSdfInterped (RenderScene.cu:310-343) in three parts: the
+// voxel indices of the eight corners (-1 for a missing block) with `valid`
+// of the last corner's lookup, the eight loads, the blend. Corner order,
+// NaN for a missing block and the blend order are the reference's. When all
+// eight corners lie in the cached block (the common case) no lookup is made.
+SICP_DEV void interp_corners(const MapDev& m, const AxisPair& X, const AxisPair& Y,
+                             const AxisPair& Z, const BlockCache& c, int (&idx)[8], bool& valid) {
     if (cache_has(c, X.v0 >> 3, Y.v0 >> 3, Z.v0 >> 3) && (X.v1 >> 3) == c.x &&
         (Y.v1 >> 3) == c.y && (Z.v1 >> 3) == c.z) {
-        const size_t base = (size_t)c.slot * kBlockVoxels;
+        const int base = c.slot * kBlockVoxels;
 #pragma unroll
         for (int i = 0; i < 8; ++i)
-            sd[i] = vox_sdf(m, base + local_of((i & 1) ? X.v1 : X.v0, (i & 2) ? Y.v1 : Y.v0,
-                                               (i & 4) ? Z.v1 : Z.v0));
+            idx[i] = base + local_of((i & 1) ? X.v1 : X.v0, (i & 2) ? Y.v1 : Y.v0,
+                                     (i & 4) ? Z.v1 : Z.v0);
         valid = true;
     } else {
         CornerBlocks cb;
@@ -240,13 +240,23 @@ SICP_DEV float interp_axes(const MapDev& m, AxisPair X, AxisPair Y, AxisPair Z, 
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int sl = corner_slot(cb, X, Y, Z, i & 1, (i >> 1) & 1, i >> 2);
-            sd[i] = sl < 0 ? qnanf()
-                           : vox_sdf(m, (size_t)sl * kBlockVoxels +
-                                            local_of((i & 1) ? X.v1 : X.v0, (i & 2) ? Y.v1 : Y.v0,
-                                                     (i & 4) ? Z.v1 : Z.v0));
+            idx[i] = sl < 0 ? -1
+                            : sl * kBlockVoxels + local_of((i & 1) ? X.v1 : X.v0,
+                                                           (i & 2) ? Y.v1 : Y.v0,
+                                                           (i & 4) ? Z.v1 : Z.v0);
         }
         valid = corner_slot(cb, X, Y, Z, 1, 1, 1) >= 0;
     }
+}
+// Every load issued (a missing corner reads voxel 0 and is replaced by NaN).
+SICP_DEV void interp_loads(const MapDev& m, const int (&idx)[8], float (&sd)[8]) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) sd[i] = vox_sdf(m, (size_t)(idx[i] < 0 ? 0 : idx[i]));
+#pragma unroll
+    for (int i = 0; i < 8; ++i) sd[i] = idx[i] < 0 ? qnanf() : sd[i];
+}
+SICP_DEV float interp_blend(const float (&sd)[8], const AxisPair& X, const AxisPair& Y,
+                            const AxisPair& Z) {
     const float fx = X.frac, fy = Y.frac, fz = Z.frac;
     float r0, r1, r2, r3;
     r0 = (1.0f - fx) * sd[0] + fx * sd[1];
@@ -257,33 +267,73 @@ SICP_DEV float interp_axes(const MapDev& m, AxisPair X, AxisPair Y, AxisPair Z, 
     r3 = (1.0f - fy) * r0 + fy * r1;
     return (1.0f - fz) * r2 + fz * r3;
 }
+SICP_DEV float interp_axes(const MapDev& m, AxisPair X, AxisPair Y, AxisPair Z, BlockCache& c,
+                           bool& valid) {
+    int idx[8];
+    float sd[8];
+    interp_corners(m, X, Y, Z, c, idx, valid);
+    interp_loads(m, idx, sd);
+    return interp_blend(sd, X, Y, Z);
+}
 
 SICP_DEV float read_sdf_interped(const MapDev& m, v3 pt, BlockCache& c, bool& valid) {
     return interp_axes(m, axis_pair(pt.x), axis_pair(pt.y), axis_pair(pt.z), c, valid);
 }
 
+// Three interpolated reads with their 24 loads issued together.
+SICP_DEV void interp3(const MapDev& m, const BlockCache& c, const AxisPair (&P)[3][3],
+                      float (&s)[3], bool (&v)[3]) {
+    int i0[8], i1[8], i2[8];
+    interp_corners(m, P[0][0], P[0][1], P[0][2], c, i0, v[0]);
+    interp_corners(m, P[1][0], P[1][1], P[1][2], c, i1, v[1]);
+    interp_corners(m, P[2][0], P[2][1], P[2][2], c, i2, v[2]);
+    float a0[8], a1[8], a2[8];
+    interp_loads(m, i0, a0);
+    interp_loads(m, i1, a1);
+    interp_loads(m, i2, a2);
+    s[0] = interp_blend(a0, P[0][0], P[0][1], P[0][2]);
+    s[1] = interp_blend(a1, P[1][0], P[1][1], P[1][2]);
+    s[2] = interp_blend(a2, P[2][0], P[2][1], P[2][2]);
+}
+
 // Rendering::readNormal (RenderScene.cu:345-378): six interpolated reads at
 // pt +- 1 voxel along each axis (pt.a + 1.0f, pt.a + -1.0f), in order, the
 // first failing one deciding. The three centre values per axis are
-// decomposed once and shared by the six reads.
+// decomposed once and shared by the six reads. The reads have no side
+// effect, so they run three at a time (their loads in flight together) and
+// the tests follow in the reference's order: the same result.
 SICP_DEV bool read_normal(const MapDev& m, v3 pt, BlockCache& c, const PoseF& pose, v3& n) {
     const AxisPair xm = axis_pair(pt.x + -1.0f), x0 = axis_pair(pt.x), xp = axis_pair(pt.x + 1.0f);
     const AxisPair ym = axis_pair(pt.y + -1.0f), y0 = axis_pair(pt.y), yp = axis_pair(pt.y + 1.0f);
     const AxisPair zm = axis_pair(pt.z + -1.0f), z0 = axis_pair(pt.z), zp = axis_pair(pt.z + 1.0f);
-    bool valid;
     float s[6];
-    s[0] = interp_axes(m, xp, y0, z0, c, valid);
-    if (isnan_(s[0]) || s[0] == 1.0f || !valid) return false;
-    s[1] = interp_axes(m, xm, y0, z0, c, valid);
-    if (isnan_(s[1]) || s[1] == 1.0f || !valid) return false;
-    s[2] = interp_axes(m, x0, yp, z0, c, valid);
-    if (isnan_(s[2]) || s[2] == 1.0f || !valid) return false;
-    s[3] = interp_axes(m, x0, ym, z0, c, valid);
-    if (isnan_(s[3]) || s[3] == 1.0f || !valid) return false;
-    s[4] = interp_axes(m, x0, y0, zp, c, valid);
-    if (isnan_(s[4]) || s[4] == 1.0f || !valid) return false;
-    s[5] = interp_axes(m, x0, y0, zm, c, valid);
-    if (isnan_(s[5]) || s[5] == 1.0f || !valid) return false;
+    bool ok[6];
+    {
+        const AxisPair P[3][3] = {{xp, y0, z0}, {xm, y0, z0}, {x0, yp, z0}};
+        float t[3];
+        bool v[3];
+        interp3(m, c, P, t, v);
+        for (int i = 0; i < 3; ++i) {
+            s[i] = t[i];
+            ok[i] = v[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (isnan_(s[i]) || s[i] == 1.0f || !ok[i]) return false;
+    {
+        const AxisPair P[3][3] = {{x0, ym, z0}, {x0, y0, zp}, {x0, y0, zm}};
+        float t[3];
+        bool v[3];
+        interp3(m, c, P, t, v);
+        for (int i = 0; i < 3; ++i) {
+            s[3 + i] = t[i];
+            ok[3 + i] = v[i];
+        }
+    }
+#pragma unroll
+    for (int i = 3; i < 6; ++i)
+        if (isnan_(s[i]) || s[i] == 1.0f || !ok[i]) return false;
     n = normalised(mv(pose.Rinv, mk(s[0] - s[1], s[2] - s[3], s[4] - s[5])));
     return true;
 }
