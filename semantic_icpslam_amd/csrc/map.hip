@@ -236,13 +236,43 @@ __global__ __launch_bounds__(256) void alloc_emit_kernel(MapDev m, AllocScratch 
     });
 }
 
+// Clears the voxels of the blocks this frame creates (empty sdf, weight 0,
+// zero semantic record), over `nwg` workgroups starting at `wg`.
+SICP_DEV void clear_new_blocks(const MapDev& m, const MapCounters* ctr, uint32_t wg,
+                               uint32_t nwg) {
+    const uint32_t total = ctr->new_total;
+    const uint32_t base = ctr->heap_used;
+    uint32_t created = 0;
+    if (base < (uint32_t)m.num_blocks) created = min(total, (uint32_t)m.num_blocks - base);
+    const uint2 g = make_uint2(0x7fffffffu, 0u);
+    const uint3 z = make_uint3(0u, 0u, 0u);
+    for (uint32_t r = wg; r < created; r += nwg) {
+        size_t off = (size_t)(base + r) * kBlockVoxels;
+        for (int v = threadIdx.x; v < kBlockVoxels; v += blockDim.x) {
+            if (m.half)
+                m.geom_h[off + v] = kEmptyHalfVoxel;
+            else
+                ((uint2*)m.geom)[off + v] = g;
+            ((uint3*)m.sem)[off + v] = z;
+        }
+    }
+}
+
+// Workgroups below kLinkBlocks link the new requests into their buckets'
+// pending lists and write the block records; the others clear the new
+// blocks' voxels (disjoint memory, so one launch serves both).
+constexpr int kLinkBlocks = 512, kClearBlocks = 1024;
 __global__ __launch_bounds__(256) void alloc_link_kernel(MapDev m, const MapCounters* ctr,
                                                          AllocScratch a, const int* gate) {
     if (gate && *gate == 0) return;
+    if (blockIdx.x >= kLinkBlocks) {
+        clear_new_blocks(m, ctr, blockIdx.x - kLinkBlocks, gridDim.x - kLinkBlocks);
+        return;
+    }
     const uint32_t total = ctr->new_total;
     const uint32_t base = ctr->heap_used;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < total;
-         r += gridDim.x * blockDim.x) {
+         r += kLinkBlocks * blockDim.x) {
         uint32_t slot = base + r;
         if (slot >= (uint32_t)m.num_blocks) continue;  // heap exhausted
         i3 p = unpack_key(a.new_keys[r]);
@@ -316,28 +346,6 @@ __global__ __launch_bounds__(256) void alloc_insert_kernel(MapDev m, const MapCo
 
 // New blocks start empty: sdf NaN, weight 0, labels 0 (Voxel::release,
 // Mapping/DeviceMap.h:76-87). One workgroup of 256 per block.
-__global__ __launch_bounds__(256) void alloc_clear_voxels_kernel(MapDev m,
-                                                                 const MapCounters* ctr,
-                                                                 const int* gate) {
-    if (gate && *gate == 0) return;
-    const uint32_t total = ctr->new_total;
-    const uint32_t base = ctr->heap_used;
-    uint32_t created = 0;
-    if (base < (uint32_t)m.num_blocks) created = min(total, (uint32_t)m.num_blocks - base);
-    const uint2 g = make_uint2(0x7fffffffu, 0u);
-    const uint3 z = make_uint3(0u, 0u, 0u);
-    for (uint32_t r = blockIdx.x; r < created; r += gridDim.x) {
-        size_t off = (size_t)(base + r) * kBlockVoxels;
-        for (int v = threadIdx.x; v < kBlockVoxels; v += blockDim.x) {
-            if (m.half)
-                m.geom_h[off + v] = kEmptyHalfVoxel;
-            else
-                ((uint2*)m.geom)[off + v] = g;
-            ((uint3*)m.sem)[off + v] = z;
-        }
-    }
-}
-
 __global__ void alloc_finish_kernel(MapDev m, MapCounters* ctr, const int* gate) {
     if (gate && *gate == 0) {
         ctr->new_blocks = 0;
@@ -512,12 +520,15 @@ __global__ __launch_bounds__(kScanThreads) void vis_flag_kernel(MapDev m, const 
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
+// Also stores the visible count (the scan's total; 0 on a lost frame).
 __global__ __launch_bounds__(kScanThreads) void vis_compact_kernel(MapDev m, MapCounters* ctr,
                                                                    AllocScratch a,
                                                                    sicp_block* visible,
                                                                    const uint32_t* partials,
+                                                                   const uint32_t* total,
                                                                    const int* gate) {
     const bool off = gate && *gate == 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->visible = off ? 0u : *total;
     const uint32_t n = off ? 0u : ctr->heap_used;
     const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
     uint32_t f[kScanPerThread], s = 0;
@@ -532,10 +543,6 @@ __global__ __launch_bounds__(kScanThreads) void vis_compact_kernel(MapDev m, Map
     for (int k = 0; k < kScanPerThread; ++k) {
         if (f[k]) visible[run++] = m.blocks[base + k];
     }
-}
-
-__global__ void vis_total_kernel(MapCounters* ctr, const uint32_t* total, const int* gate) {
-    ctr->visible = (gate && *gate == 0) ? 0u : *total;
 }
 
 // ---------------------------------------------------------------- fusion --
@@ -739,9 +746,31 @@ __global__ __launch_bounds__(256) void fuse_kernel(MapDev m, const MapCounters* 
 
 // Sums the per-wavefront counts of fuse_kernel into the map's counters; a
 // gated-off frame fused nothing.
+// Workgroup 0: the sum of the fusion's per-wavefront counts. With `rr.zmin`
+// set, the other workgroups also clear the raycast's zRange images and
+// workgroup 0 its tile and queue counters — work the prepass and the raycast
+// would otherwise each start with a launch of their own.
+struct RenderReset {
+    View<float> zmin, zmax;
+    uint32_t* rq_count;
+};
 __global__ __launch_bounds__(1024) void fuse_end_kernel(MapCounters* ctr,
                                                         const uint32_t* wave_counts, int n,
-                                                        const int* gate) {
+                                                        const int* gate, RenderReset rr) {
+    if (blockIdx.x > 0) {
+        const int nz = rr.zmin.cols * rr.zmin.rows;
+        for (int i = (blockIdx.x - 1) * blockDim.x + threadIdx.x; i < nz;
+             i += (gridDim.x - 1) * blockDim.x) {
+            const int y = i / rr.zmin.cols, x = i - y * rr.zmin.cols;
+            rr.zmin.at(y, x) = 100.f;
+            rr.zmax.at(y, x) = 0.f;
+        }
+        return;
+    }
+    if (rr.zmin.ptr && threadIdx.x == 0) {
+        ctr->render_tiles = 0u;
+        *rr.rq_count = 0u;
+    }
     if (gate && *gate == 0) {
         if (threadIdx.x == 0) ctr->updated_voxels = 0u;
         return;
@@ -803,12 +832,10 @@ int launch_allocate(MapDev m, MapCounters* ctr, AllocScratch a, View<float> dept
     if (rc) return rc;
     alloc_emit_kernel<<<grid, 256, 0, s>>>(m, a, depth, pose, g, gate);
     SICP_LAUNCH_CHECK("alloc_emit_kernel");
-    alloc_link_kernel<<<512, 256, 0, s>>>(m, ctr, a, gate);
+    alloc_link_kernel<<<kLinkBlocks + kClearBlocks, 256, 0, s>>>(m, ctr, a, gate);
     SICP_LAUNCH_CHECK("alloc_link_kernel");
     alloc_insert_kernel<<<512, 256, 0, s>>>(m, ctr, a, gate);
     SICP_LAUNCH_CHECK("alloc_insert_kernel");
-    alloc_clear_voxels_kernel<<<1024, 256, 0, s>>>(m, ctr, gate);
-    SICP_LAUNCH_CHECK("alloc_clear_voxels_kernel");
     alloc_finish_kernel<<<1, 1, 0, s>>>(m, ctr, gate);
     SICP_LAUNCH_CHECK("alloc_finish_kernel");
     return SICP_OK;
@@ -826,10 +853,9 @@ int launch_visibility(MapDev m, MapCounters* ctr, AllocScratch a, sicp_block* vi
     uint32_t* total = a.scan_partials + kScanMaxChunks;
     scan_partials_kernel<<<1, 1024, 0, s>>>(a.scan_partials, nchunks, total);
     SICP_LAUNCH_CHECK("scan_partials_kernel");
-    vis_compact_kernel<<<nchunks, kScanThreads, 0, s>>>(m, ctr, a, visible, a.scan_partials, gate);
+    vis_compact_kernel<<<nchunks, kScanThreads, 0, s>>>(m, ctr, a, visible, a.scan_partials, total,
+                                                       gate);
     SICP_LAUNCH_CHECK("vis_compact_kernel");
-    vis_total_kernel<<<1, 1, 0, s>>>(ctr, total, gate);
-    SICP_LAUNCH_CHECK("vis_total_kernel");
     return SICP_OK;
 }
 
@@ -845,8 +871,11 @@ int launch_fuse(MapDev m, MapCounters* ctr, const sicp_block* visible, View<floa
 }
 
 int launch_fuse_count(MapCounters* ctr, const uint32_t* wave_counts, const int* gate,
-                      hipStream_t s) {
-    fuse_end_kernel<<<1, 1024, 0, s>>>(ctr, wave_counts, kFuseBlocks * 4, gate);
+                      hipStream_t s, View<float> zmin, View<float> zmax, uint32_t* rq_count) {
+    const RenderReset rr{zmin, zmax, rq_count};
+    const int nz = zmin.ptr ? zmin.cols * zmin.rows : 0;
+    const int blocks = 1 + (int)div_up(nz, 1024);
+    fuse_end_kernel<<<blocks, 1024, 0, s>>>(ctr, wave_counts, kFuseBlocks * 4, gate, rr);
     SICP_LAUNCH_CHECK("fuse_end_kernel");
     return SICP_OK;
 }

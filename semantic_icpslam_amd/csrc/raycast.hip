@@ -949,24 +949,31 @@ size_t raycast_adjoint_scratch_bytes() { return sizeof(double) * (6 * kAdjMaxBlo
 
 int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible, View<float> zmin,
                           View<float> zmax, PoseArg pose, sicp_intrinsics K, float dmax,
-                          float dmin, const int* gate, hipStream_t s) {
+                          float dmin, const int* gate, hipStream_t s, bool zrange_cleared) {
     int n = zmin.cols * zmin.rows;
-    zrange_init_kernel<<<div_up(n, 256) > 0 ? div_up(n, 256) : 1, 256, 0, s>>>(zmin, zmax, ctr);
-    SICP_LAUNCH_CHECK("zrange_init_kernel");
+    if (!zrange_cleared) {
+        zrange_init_kernel<<<div_up(n, 256) > 0 ? div_up(n, 256) : 1, 256, 0, s>>>(zmin, zmax,
+                                                                                  ctr);
+        SICP_LAUNCH_CHECK("zrange_init_kernel");
+    }
     PrepassArgs g{K.fx, K.fy, K.cx, K.cy, dmax, dmin, zmin.cols, zmin.rows};
     prepass_kernel<<<512, 256, 0, s>>>(m, ctr, visible, zmin, zmax, pose, g, gate);
     SICP_LAUNCH_CHECK("prepass_kernel");
     return SICP_OK;
 }
 
-// Both passes; the queue count is cleared first.
+// Both passes; the queue count is cleared first (unless the caller's
+// launch_fuse_count already did).
 int launch_raycast_gated(MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin,
                          View<float> zmax, PoseArg pose, float invfx, float invfy, float cx,
-                         float cy, const int* gate, const uint32_t* nonempty, hipStream_t s) {
+                         float cy, const int* gate, const uint32_t* nonempty, hipStream_t s,
+                         bool queue_cleared) {
     if ((size_t)vmap.cols * vmap.rows > m.rq_cap) return invalid("raycast: image above the map's max_cols x max_rows");
     RayArgs g{invfx, invfy, cx, cy, vmap.cols, vmap.rows};
-    ray_queue_reset_kernel<<<1, 1, 0, s>>>(m.rq_count);
-    SICP_LAUNCH_CHECK("ray_queue_reset_kernel");
+    if (!queue_cleared) {
+        ray_queue_reset_kernel<<<1, 1, 0, s>>>(m.rq_count);
+        SICP_LAUNCH_CHECK("ray_queue_reset_kernel");
+    }
     dim3 grid(div_up(vmap.cols, 8), div_up(vmap.rows, 8));
     raycast_kernel<<<grid, 64, 0, s>>>(m, vmap, nmap, zmin, zmax, pose, g, gate, nonempty);
     SICP_LAUNCH_CHECK("raycast_kernel");
@@ -979,7 +986,7 @@ int launch_raycast(MapDev m, View<float4> vmap, View<float4> nmap, View<float> z
                    View<float> zmax, PoseArg pose, float invfx, float invfy, float cx, float cy,
                    const int* gate, hipStream_t s) {
     return launch_raycast_gated(m, vmap, nmap, zmin, zmax, pose, invfx, invfy, cx, cy, gate,
-                                nullptr, s);
+                                nullptr, s, false);
 }
 
 }  // namespace sicp

@@ -23,9 +23,12 @@ int map_alloc(const sicp_map_config& cfg, sicp_map** out);
 void map_free(sicp_map* m);
 // The neighbour label pass over the map's visible list (unify.hip).
 int launch_unify(sicp_map* map, const int* gate, hipStream_t s);
+// `queue_cleared`: the hand-over queue count was cleared by launch_fuse_count
+// on the same stream.
 int launch_raycast_gated(MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin,
                          View<float> zmax, PoseArg pose, float invfx, float invfy, float cx,
-                         float cy, const int* gate, const uint32_t* nonempty, hipStream_t s);
+                         float cy, const int* gate, const uint32_t* nonempty, hipStream_t s,
+                         bool queue_cleared = false);
 
 // A pitched device image owned by the library.
 template <class T> struct DevImage {

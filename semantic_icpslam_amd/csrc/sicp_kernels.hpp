@@ -232,12 +232,18 @@ int launch_fuse(MapDev m, MapCounters* ctr, const sicp_block* visible, View<floa
                 PoseArg pose, sicp_intrinsics K, uint32_t* wave_counts, const int* gate,
                 hipStream_t s);
 // The per-wavefront counts of launch_fuse into the map's counters (always run
-// after it).
+// after it). With `zmin` set, the same launch also clears the zRange images,
+// the rendering-tile count and the raycast queue count (`rq_count`), so the
+// prepass and the raycast that follow can skip their own clearing launches.
 int launch_fuse_count(MapCounters* ctr, const uint32_t* wave_counts, const int* gate,
-                      hipStream_t s);
+                      hipStream_t s, View<float> zmin = View<float>{},
+                      View<float> zmax = View<float>{}, uint32_t* rq_count = nullptr);
+// `zrange_cleared`: the zRange images and the tile count were cleared by
+// launch_fuse_count on the same stream.
 int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible, View<float> zmin,
                           View<float> zmax, PoseArg pose, sicp_intrinsics K, float dmax,
-                          float dmin, const int* gate, hipStream_t s);
+                          float dmin, const int* gate, hipStream_t s,
+                          bool zrange_cleared = false);
 int launch_raycast(MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin,
                    View<float> zmax, PoseArg pose, float invfx, float invfy, float cx, float cy,
                    const int* gate, hipStream_t s);

@@ -373,21 +373,25 @@ static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
         return rc;
     stage_end(s, kStageFuse, ev, st);
     ev = stage_begin(s, st);
-    if ((rc = launch_fuse_count(m->ctr, m->alloc.fuse_counts, gate, st))) return rc;
+    // the fusion count also clears the zRange images and the raycast queue
+    if ((rc = launch_fuse_count(m->ctr, m->alloc.fuse_counts, gate, st, s->zmin.v(),
+                                s->zmax.v(), m->dev.rq_count)))
+        return rc;
     if (m->cfg.unify_labels && labels && (rc = launch_unify(m, gate, st))) return rc;
     stage_end(s, kStageFuseCount, ev, st);
     // Mapping::RayTrace (Mapping.cc:142-161): prepass, then the raycast into
     // the frame's level-0 model maps when any block produced a tile.
     ev = stage_begin(s, st);
     if ((rc = launch_render_prepass(m->dev, m->ctr, m->visible, s->zmin.v(), s->zmax.v(), pose,
-                                    s->K[0], m->cfg.depth_max, m->cfg.depth_min, gate, st)))
+                                    s->K[0], m->cfg.depth_max, m->cfg.depth_min, gate, st,
+                                    true)))
         return rc;
     stage_end(s, kStagePrepass, ev, st);
     ev = stage_begin(s, st);
     rc = launch_raycast_gated(m->dev, f.vmap[0].v(), f.nmap[0].v(), s->zmin.v(), s->zmax.v(),
                               pose, (float)(1.0 / (double)s->K[0].fx),
                               (float)(1.0 / (double)s->K[0].fy), s->K[0].cx, s->K[0].cy, gate,
-                              &m->ctr->render_tiles, st);
+                              &m->ctr->render_tiles, st, true);
     stage_end(s, kStageRaycast, ev, st);
     return rc;
 }
