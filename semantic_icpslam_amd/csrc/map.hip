@@ -497,6 +497,10 @@ SICP_DEV bool block_visible(const sicp_block& b, float s, const PoseF& pose, con
     return false;
 }
 
+// P heap slots per thread (a chunk of 256 P slots per workgroup): one slot
+// per thread spreads the allocated heap over four times the workgroups,
+// when the partial sums of the smaller chunks fit the scan.
+template <int P>
 __global__ __launch_bounds__(kScanThreads) void vis_flag_kernel(MapDev m, const MapCounters* ctr,
                                                                 AllocScratch a, PoseArg pa,
                                                                 VisArgs g, uint32_t* partials,
@@ -504,10 +508,10 @@ __global__ __launch_bounds__(kScanThreads) void vis_flag_kernel(MapDev m, const 
     const bool off = gate && *gate == 0;
     const uint32_t n = off ? 0u : ctr->heap_used;
     const PoseF pose = load_pose(pa);
-    const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
+    const uint32_t base = blockIdx.x * (kScanThreads * P) + threadIdx.x * P;
     uint32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanPerThread; ++k) {
+    for (int k = 0; k < P; ++k) {
         uint32_t slot = base + k;
         if (slot < n) {
             uint8_t f = block_visible(m.blocks[slot], m.block_width, pose, g) ? 1 : 0;
@@ -521,6 +525,7 @@ __global__ __launch_bounds__(kScanThreads) void vis_flag_kernel(MapDev m, const 
 }
 
 // Also stores the visible count (the scan's total; 0 on a lost frame).
+template <int P>
 __global__ __launch_bounds__(kScanThreads) void vis_compact_kernel(MapDev m, MapCounters* ctr,
                                                                    AllocScratch a,
                                                                    sicp_block* visible,
@@ -530,17 +535,17 @@ __global__ __launch_bounds__(kScanThreads) void vis_compact_kernel(MapDev m, Map
     const bool off = gate && *gate == 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr->visible = off ? 0u : *total;
     const uint32_t n = off ? 0u : ctr->heap_used;
-    const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
-    uint32_t f[kScanPerThread], s = 0;
+    const uint32_t base = blockIdx.x * (kScanThreads * P) + threadIdx.x * P;
+    uint32_t f[P], s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanPerThread; ++k) {
+    for (int k = 0; k < P; ++k) {
         f[k] = (base + k < n) ? a.vis_flags[base + k] : 0u;
         s += f[k];
     }
     uint32_t tot;
     uint32_t run = partials[blockIdx.x] + block_exclusive_scan(s, &tot);
 #pragma unroll
-    for (int k = 0; k < kScanPerThread; ++k) {
+    for (int k = 0; k < P; ++k) {
         if (f[k]) visible[run++] = m.blocks[base + k];
     }
 }
@@ -847,14 +852,26 @@ int launch_visibility(MapDev m, MapCounters* ctr, AllocScratch a, sicp_block* vi
     uint32_t nchunks = div_up(m.num_blocks, kScanChunk);
     if (nchunks == 0) nchunks = 1;
     if (nchunks > kScanMaxChunks) return invalid("heap larger than 4M blocks");
+    const uint32_t nsmall = div_up(m.num_blocks, kScanThreads);
+    const bool one = nsmall <= (uint32_t)kScanMaxChunks;  // one slot per thread
+    if (one) nchunks = nsmall > 0 ? nsmall : 1;
     VisArgs g{K.fx, K.fy, K.cx, K.cy, cols, rows, dmin, dmax};
-    vis_flag_kernel<<<nchunks, kScanThreads, 0, s>>>(m, ctr, a, pose, g, a.scan_partials, gate);
+    if (one)
+        vis_flag_kernel<1><<<nchunks, kScanThreads, 0, s>>>(m, ctr, a, pose, g, a.scan_partials,
+                                                            gate);
+    else
+        vis_flag_kernel<kScanPerThread><<<nchunks, kScanThreads, 0, s>>>(m, ctr, a, pose, g,
+                                                                         a.scan_partials, gate);
     SICP_LAUNCH_CHECK("vis_flag_kernel");
     uint32_t* total = a.scan_partials + kScanMaxChunks;
     scan_partials_kernel<<<1, 1024, 0, s>>>(a.scan_partials, nchunks, total);
     SICP_LAUNCH_CHECK("scan_partials_kernel");
-    vis_compact_kernel<<<nchunks, kScanThreads, 0, s>>>(m, ctr, a, visible, a.scan_partials, total,
-                                                       gate);
+    if (one)
+        vis_compact_kernel<1><<<nchunks, kScanThreads, 0, s>>>(m, ctr, a, visible,
+                                                               a.scan_partials, total, gate);
+    else
+        vis_compact_kernel<kScanPerThread><<<nchunks, kScanThreads, 0, s>>>(
+            m, ctr, a, visible, a.scan_partials, total, gate);
     SICP_LAUNCH_CHECK("vis_compact_kernel");
     return SICP_OK;
 }
