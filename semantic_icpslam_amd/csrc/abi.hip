@@ -97,7 +97,7 @@ int map_alloc(const sicp_map_config& cfg, sicp_map** out) {
     AllocScratch& a = m->alloc;
     if ((rc = dmalloc(m, &a.req_keys, q)) || (rc = dmalloc(m, &a.req_first, q)) ||
         (rc = dmalloc(m, &a.pix_mask, px)) || (rc = dmalloc(m, &a.pix_count, px)) ||
-        (rc = dmalloc(m, &a.pix_base, px)) || (rc = dmalloc(m, &a.scan_partials, 4096 + 64)) ||
+        (rc = dmalloc(m, &a.scan_partials, 4096 + 64)) ||
         (rc = dmalloc(m, &a.new_keys, q)) || (rc = dmalloc(m, &a.new_req_slot, q)) ||
         (rc = dmalloc(m, &a.bucket_leader, nb)) || (rc = dmalloc(m, &a.bucket_head, nb)) ||
         (rc = dmalloc(m, &a.pending_next, q)) || (rc = dmalloc(m, &a.vis_flags, nblk)) ||

@@ -196,44 +196,78 @@ SICP_DEV void for_owned_steps(const MapDev& m, const AllocScratch& a, const View
     }
 }
 
+SICP_DEV uint32_t block_exclusive_scan(uint32_t v, uint32_t* total);
+SICP_DEV uint2 chunk_offset(const uint32_t* sums, uint32_t w, uint32_t n);
+
+// The request counts and keys of the owned steps run over raster chunks of
+// 256 P pixels per workgroup (P per thread), the chunks of the exclusive
+// scan that ranks the requests in raster order: the count kernel leaves
+// each chunk's sum in `partials`, and the emit kernel forms every pixel's
+// first rank itself (the sum of the earlier chunks plus the prefix within
+// the chunk) and stores the total — no separate scan launches.
+template <int P>
 __global__ __launch_bounds__(256) void alloc_count_kernel(MapDev m, AllocScratch a,
                                                           View<float> depth, PoseArg pa,
-                                                          AllocArgs g, const int* gate) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= g.cols || y >= g.rows) return;
-    const int i = y * g.cols + x;
-    if (gate && *gate == 0) {
-        a.pix_count[i] = 0;
-        return;
+                                                          AllocArgs g, uint32_t* partials,
+                                                          const int* gate) {
+    const bool off = gate && *gate == 0;
+    const int n = g.cols * g.rows;
+    const int base = (int)blockIdx.x * (kScanThreads * P) + (int)threadIdx.x * P;
+    const PoseF pose = load_pose(pa);
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const int i = base + k;
+        if (i >= n) continue;
+        uint32_t cnt = 0;
+        const unsigned long long mask = off ? 0ull : a.pix_mask[i];
+        if (mask) {
+            const int y = i / g.cols, x = i - y * g.cols;
+            for_owned_steps(m, a, depth, x, y, i, mask, pose, g, [&](uint64_t, int) { ++cnt; });
+        }
+        a.pix_count[i] = cnt;
+        s += cnt;
     }
-    unsigned long long mask = a.pix_mask[i];
-    uint32_t cnt = 0;
-    if (mask) {
-        const PoseF pose = load_pose(pa);
-        for_owned_steps(m, a, depth, x, y, i, mask, pose, g,
-                        [&](uint64_t, int) { ++cnt; });
-    }
-    a.pix_count[i] = cnt;
+    uint32_t tot;
+    block_exclusive_scan(s, &tot);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(256) void alloc_emit_kernel(MapDev m, AllocScratch a,
-                                                         View<float> depth, PoseArg pa,
-                                                         AllocArgs g, const int* gate) {
-    if (gate && *gate == 0) return;
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= g.cols || y >= g.rows) return;
-    const int i = y * g.cols + x;
-    unsigned long long mask = a.pix_mask[i];
-    if (!mask) return;
+template <int P>
+__global__ __launch_bounds__(256) void alloc_emit_kernel(MapDev m, MapCounters* ctr,
+                                                         AllocScratch a, View<float> depth,
+                                                         PoseArg pa, AllocArgs g,
+                                                         const uint32_t* partials,
+                                                         uint32_t nchunks, const int* gate) {
+    if (gate && *gate == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctr->new_total = 0u;
+        return;
+    }
+    const int n = g.cols * g.rows;
+    const int base = (int)blockIdx.x * (kScanThreads * P) + (int)threadIdx.x * P;
+    uint32_t c[P], s = 0;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        c[k] = base + k < n ? a.pix_count[base + k] : 0u;
+        s += c[k];
+    }
+    const uint2 off = chunk_offset(partials, blockIdx.x, nchunks);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->new_total = off.y;
+    uint32_t tot;
+    uint32_t r = off.x + block_exclusive_scan(s, &tot);
     const PoseF pose = load_pose(pa);
-    uint32_t r = a.pix_base[i];
-    for_owned_steps(m, a, depth, x, y, i, mask, pose, g, [&](uint64_t key, int slot) {
-        a.new_keys[r] = key;
-        a.new_req_slot[r] = (uint32_t)slot;
-        ++r;
-    });
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const int i = base + k;
+        if (c[k] == 0) continue;
+        const unsigned long long mask = a.pix_mask[i];
+        const int y = i / g.cols, x = i - y * g.cols;
+        for_owned_steps(m, a, depth, x, y, i, mask, pose, g, [&](uint64_t key, int slot) {
+            a.new_keys[r] = key;
+            a.new_req_slot[r] = (uint32_t)slot;
+            ++r;
+        });
+    }
 }
 
 // Clears the voxels of the blocks this frame creates (empty sdf, weight 0,
@@ -384,6 +418,38 @@ SICP_DEV uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
     return wbase + inc - v;
 }
 
+// The sum of the chunk sums before chunk `w`, and of all `n` of them, formed
+// by the whole workgroup (integer sums: any order gives the same values).
+// Every workgroup of a pass forms its own offset, which costs a few loads
+// per thread instead of a launch that scans the chunk sums.
+SICP_DEV uint2 chunk_offset(const uint32_t* sums, uint32_t w, uint32_t n) {
+    __shared__ uint32_t red[2][kScanThreads / 64];
+    uint32_t a = 0, b = 0;
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+        const uint32_t v = sums[j];
+        a += j < w ? v : 0u;
+        b += v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    __syncthreads();  // `red` may still be read by an earlier call
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    uint2 r = make_uint2(0u, 0u);
+#pragma unroll
+    for (int k = 0; k < kScanThreads / 64; ++k) {
+        r.x += red[0][k];
+        r.y += red[1][k];
+    }
+    return r;
+}
+
 __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* in,
                                                                    const uint32_t* n_dev,
                                                                    uint32_t cap,
@@ -524,17 +590,20 @@ __global__ __launch_bounds__(kScanThreads) void vis_flag_kernel(MapDev m, const 
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
-// Also stores the visible count (the scan's total; 0 on a lost frame).
+// Also stores the visible count (the sum of every chunk's count; 0 on a lost
+// frame). Workgroups past the allocated heap leave at once.
 template <int P>
 __global__ __launch_bounds__(kScanThreads) void vis_compact_kernel(MapDev m, MapCounters* ctr,
                                                                    AllocScratch a,
                                                                    sicp_block* visible,
                                                                    const uint32_t* partials,
-                                                                   const uint32_t* total,
+                                                                   uint32_t nchunks,
                                                                    const int* gate) {
     const bool off = gate && *gate == 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->visible = off ? 0u : *total;
     const uint32_t n = off ? 0u : ctr->heap_used;
+    if (blockIdx.x > 0 && blockIdx.x * (uint32_t)(kScanThreads * P) >= n) return;
+    const uint2 co = chunk_offset(partials, blockIdx.x, nchunks);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->visible = off ? 0u : co.y;
     const uint32_t base = blockIdx.x * (kScanThreads * P) + threadIdx.x * P;
     uint32_t f[P], s = 0;
 #pragma unroll
@@ -543,7 +612,7 @@ __global__ __launch_bounds__(kScanThreads) void vis_compact_kernel(MapDev m, Map
         s += f[k];
     }
     uint32_t tot;
-    uint32_t run = partials[blockIdx.x] + block_exclusive_scan(s, &tot);
+    uint32_t run = co.x + block_exclusive_scan(s, &tot);
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         if (f[k]) visible[run++] = m.blocks[base + k];
@@ -830,12 +899,25 @@ int launch_allocate(MapDev m, MapCounters* ctr, AllocScratch a, View<float> dept
     dim3 grid(div_up(cols, 64), div_up(rows, 4));
     alloc_mark_kernel<<<grid, 256, 0, s>>>(m, ctr, a, depth, pose, g, gate);
     SICP_LAUNCH_CHECK("alloc_mark_kernel");
-    alloc_count_kernel<<<grid, 256, 0, s>>>(m, a, depth, pose, g, gate);
+    // raster chunks of 256 pixels (one per thread) while their sums fit the
+    // partial scan, else of 1024
+    const uint32_t npx = (uint32_t)(cols * rows);
+    const bool one = div_up(npx, kScanThreads) <= (uint32_t)kScanMaxChunks;
+    const uint32_t nchunks = (uint32_t)div_up(npx, one ? kScanThreads : kScanChunk);
+    if (nchunks > (uint32_t)kScanMaxChunks) return invalid("image above 4M pixels");
+    if (one)
+        alloc_count_kernel<1><<<nchunks, kScanThreads, 0, s>>>(m, a, depth, pose, g,
+                                                               a.scan_partials, gate);
+    else
+        alloc_count_kernel<kScanPerThread><<<nchunks, kScanThreads, 0, s>>>(
+            m, a, depth, pose, g, a.scan_partials, gate);
     SICP_LAUNCH_CHECK("alloc_count_kernel");
-    int rc = launch_scan_u32(a.pix_count, a.pix_base, nullptr, (uint32_t)(cols * rows),
-                             a.scan_partials, &ctr->new_total, s);
-    if (rc) return rc;
-    alloc_emit_kernel<<<grid, 256, 0, s>>>(m, a, depth, pose, g, gate);
+    if (one)
+        alloc_emit_kernel<1><<<nchunks, kScanThreads, 0, s>>>(m, ctr, a, depth, pose, g,
+                                                              a.scan_partials, nchunks, gate);
+    else
+        alloc_emit_kernel<kScanPerThread><<<nchunks, kScanThreads, 0, s>>>(
+            m, ctr, a, depth, pose, g, a.scan_partials, nchunks, gate);
     SICP_LAUNCH_CHECK("alloc_emit_kernel");
     alloc_link_kernel<<<kLinkBlocks + kClearBlocks, 256, 0, s>>>(m, ctr, a, gate);
     SICP_LAUNCH_CHECK("alloc_link_kernel");
@@ -863,15 +945,12 @@ int launch_visibility(MapDev m, MapCounters* ctr, AllocScratch a, sicp_block* vi
         vis_flag_kernel<kScanPerThread><<<nchunks, kScanThreads, 0, s>>>(m, ctr, a, pose, g,
                                                                          a.scan_partials, gate);
     SICP_LAUNCH_CHECK("vis_flag_kernel");
-    uint32_t* total = a.scan_partials + kScanMaxChunks;
-    scan_partials_kernel<<<1, 1024, 0, s>>>(a.scan_partials, nchunks, total);
-    SICP_LAUNCH_CHECK("scan_partials_kernel");
     if (one)
         vis_compact_kernel<1><<<nchunks, kScanThreads, 0, s>>>(m, ctr, a, visible,
-                                                               a.scan_partials, total, gate);
+                                                               a.scan_partials, nchunks, gate);
     else
         vis_compact_kernel<kScanPerThread><<<nchunks, kScanThreads, 0, s>>>(
-            m, ctr, a, visible, a.scan_partials, total, gate);
+            m, ctr, a, visible, a.scan_partials, nchunks, gate);
     SICP_LAUNCH_CHECK("vis_compact_kernel");
     return SICP_OK;
 }

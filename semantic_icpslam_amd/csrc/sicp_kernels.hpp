@@ -207,7 +207,6 @@ struct AllocScratch {
     uint32_t req_mask;         // capacity - 1 (power of two)
     unsigned long long* pix_mask;  // per pixel: steps that missed the table
     uint32_t* pix_count;       // per pixel: new blocks first seen there
-    uint32_t* pix_base;        // exclusive scan of pix_count
     uint32_t* scan_partials;   // scan workspace
     uint64_t* new_keys;        // rank -> key
     uint32_t* new_req_slot;    // rank -> request-set slot
