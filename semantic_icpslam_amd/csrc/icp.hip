@@ -20,6 +20,9 @@ namespace sicp {
 namespace {
 
 constexpr int kPixPerThread = 4;
+// Every kernel here runs kIcpThreads threads per workgroup; the constant is
+// used instead of blockDim, whose value the kernel would otherwise load from
+// the dispatch packet (a memory round trip before the first address).
 
 struct IcpGeom {
     int cols, rows, n;
@@ -275,8 +278,8 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
                              const View<float4>& ncurr, const View<float4>& nlast,
                              const IcpGeom& g, double* partials, const RgbPass& rp,
                              PoseFn get_pose) {
-    const int stride = gridDim.x * blockDim.x;
-    int base = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * kIcpThreads;
+    int base = blockIdx.x * kIcpThreads + threadIdx.x;
     IcpBatch bt;
     icp_load_batch<kRgb>(bt, base, stride, vcurr, ncurr, rp, g);
     PoseF cur, last;
@@ -344,8 +347,8 @@ template <class Acc, class PoseFn>
 SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
                        const View<int16_t>& dy, const IcpGeom& g, const RgbScratch& rs,
                        int nb_match, PoseFn get_pose) {
-    const int stride = gridDim.x * blockDim.x;
-    int base = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * kIcpThreads;
+    int base = blockIdx.x * kIcpThreads + threadIdx.x;
     int2 rc[kPixPerThread];
     auto load_records = [&]() {
 #pragma unroll
@@ -356,7 +359,7 @@ SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
     };
     load_records();
     uint32_t c = 0, q = 0;
-    for (int b = threadIdx.x; b < nb_match; b += blockDim.x) {
+    for (int b = threadIdx.x; b < nb_match; b += kIcpThreads) {
         c += rs.ipart[b];
         q += rs.ipart[kIcpMaxBlocks + b];
     }
@@ -473,7 +476,7 @@ __global__ __launch_bounds__(kIcpThreads) void so3_partials_kernel(
     SumF64 acc[kSo3Sums];
 #pragma unroll
     for (int j = 0; j < kSo3Sums; ++j) acc[j].zero();
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x) {
+    for (int i = blockIdx.x * kIcpThreads + threadIdx.x; i < g.n; i += gridDim.x * kIcpThreads) {
         const int y = i / g.cols, x = i - y * g.cols;
         const v3 vl = v3{((float)x - g.cx) / g.fx, ((float)y - g.cy) / g.fy, 1.0f};
         const v3 p = mv(cur.Rinv, mv(last.R, vl));
@@ -818,7 +821,7 @@ __global__ __launch_bounds__(kIcpThreads) void rgb_match_kernel(View<float4> vcu
                                                                 RgbPass rp) {
     const PoseF cur = load_pose(pc), last = load_pose(pl);
     uint32_t cnt = 0, sq = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x) {
+    for (int i = blockIdx.x * kIcpThreads + threadIdx.x; i < g.n; i += gridDim.x * kIcpThreads) {
         const int y = i / g.cols, x = i - y * g.cols;
         const int key = rp.key.at(y, x);
         const float4 vc = vcurr.at(y, x);
@@ -865,7 +868,7 @@ __global__ __launch_bounds__(kIcpThreads) void track_finish_kernel(
     }
     if (S.tracked) return;
     const int n = vnext.cols * vnext.rows;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = blockIdx.x * kIcpThreads + threadIdx.x; i < n; i += gridDim.x * kIcpThreads) {
         int y = i / vnext.cols, x = i - y * vnext.cols;
         vnext.at(y, x) = vlast.at(y, x);
         nnext.at(y, x) = nlast.at(y, x);
@@ -876,8 +879,8 @@ __global__ __launch_bounds__(kIcpThreads) void track_finish_kernel(
         const View<uint8_t>& dst = images.next[l];
         if (!dst.ptr) continue;
         const int m = dst.cols * dst.rows;
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m;
-             i += gridDim.x * blockDim.x) {
+        for (int i = blockIdx.x * kIcpThreads + threadIdx.x; i < m;
+             i += gridDim.x * kIcpThreads) {
             int y = i / dst.cols, x = i - y * dst.cols;
             dst.at(y, x) = src.at(y, x);
         }
