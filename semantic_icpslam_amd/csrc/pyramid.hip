@@ -371,22 +371,43 @@ template <class T> struct TileSrc {
     SICP_DEV T operator()(int y, int x) const { return p[(y - y0) * 20 + (x - x0)]; }
 };
 
+// The level-0 window of a tile: the 44 x 44 pixels the 5x5 taps of its
+// 20 x 20 level-1 window reach (clamped taps stay inside it), staged in LDS
+// with every load in flight at once.
+constexpr int kWin0 = 44;
+template <class T> struct WinSrc {
+    const T* p;
+    int x0, y0;
+    SICP_DEV T operator()(int y, int x) const { return p[(y - y0) * kWin0 + (x - x0)]; }
+};
+
 __global__ __launch_bounds__(256) void pyrdown_levels_kernel(View<float> d0, View<uint8_t> g0,
                                                              View<float> d1, View<uint8_t> g1,
                                                              View<float> d2, View<uint8_t> g2) {
+    __shared__ float w0d[kWin0 * kWin0];
+    __shared__ uint8_t w0g[kWin0 * kWin0];
     __shared__ float rd[400];
     __shared__ uint8_t rg[400];
     const int X0 = blockIdx.x * 8, Y0 = blockIdx.y * 8;  // level-2 tile origin
     const int rx0 = 2 * X0 - 2, ry0 = 2 * Y0 - 2;        // level-1 window origin
+    const int wx0 = 2 * rx0 - 2, wy0 = 2 * ry0 - 2;      // level-0 window origin
+    for (int i = threadIdx.x; i < kWin0 * kWin0; i += 256) {
+        const int wx = wx0 + i % kWin0, wy = wy0 + i / kWin0;
+        if (wx < 0 || wy < 0 || wx >= d0.cols || wy >= d0.rows) continue;
+        w0d[i] = d0.at(wy, wx);
+        w0g[i] = g0.at(wy, wx);
+    }
+    __syncthreads();
     // Level-1 pixels owned here: [2*X0, 2*X0 + 16), the last tile column
     // (row) also the odd remainder up to the image edge.
     const int ox1 = blockIdx.x == gridDim.x - 1 ? d1.cols : 2 * X0 + 16;
     const int oy1 = blockIdx.y == gridDim.y - 1 ? d1.rows : 2 * Y0 + 16;
-    for (int i = threadIdx.x; i < 400; i += blockDim.x) {
+    for (int i = threadIdx.x; i < 400; i += 256) {
         const int rx = rx0 + i % 20, ry = ry0 + i / 20;
         if (rx < 0 || ry < 0 || rx >= d1.cols || ry >= d1.rows) continue;
-        const float v = pyrdown_px<float>(GlobalSrc<float>{d0}, d0.cols, d0.rows, rx, ry);
-        const uint8_t u = pyrdown_px<uint8_t>(GlobalSrc<uint8_t>{g0}, g0.cols, g0.rows, rx, ry);
+        const float v = pyrdown_px<float>(WinSrc<float>{w0d, wx0, wy0}, d0.cols, d0.rows, rx, ry);
+        const uint8_t u =
+            pyrdown_px<uint8_t>(WinSrc<uint8_t>{w0g, wx0, wy0}, g0.cols, g0.rows, rx, ry);
         rd[i] = v;
         rg[i] = u;
         if (rx >= 2 * X0 && rx < ox1 && ry >= 2 * Y0 && ry < oy1) {
