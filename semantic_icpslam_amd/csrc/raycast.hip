@@ -127,11 +127,11 @@ struct BlockCache {
     int x, y, z, slot;  // slot -1: empty
 };
 
-// make_int3 of one coordinate (VectorMath.h:36-42), as floor_i3.
-SICP_DEV int floor_i(float a) {
-    const int b = (int)a;
-    return ((float)b > a) ? b - 1 : b;
-}
+// make_int3 of one coordinate (VectorMath.h:36-42): the truncation, one less
+// where it exceeds the value — the floor, for every float (NaN and values
+// past the int range convert as the truncation does). v_floor_f32 and one
+// conversion, the floor shared with the fraction of an interpolated read.
+SICP_DEV int floor_i(float a) { return (int)floorf(a); }
 
 SICP_DEV int local_of(int vx, int vy, int vz) { return ((vz & 7) << 6) | ((vy & 7) << 3) | (vx & 7); }
 
