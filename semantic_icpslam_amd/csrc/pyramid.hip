@@ -334,7 +334,8 @@ SICP_DEV bool take_tile(int& b, int cols, int rows, int& x, int& y) {
 __global__ __launch_bounds__(256) void frame_level0_kernel(View<uint16_t> d16, View<uint8_t> rgb,
                                                            View<float> range, View<float> depth0,
                                                            View<uint8_t> gray0, float scale_inv,
-                                                           float cutoff, ModelPyramid mp) {
+                                                           float cutoff, ModelPyramid mp,
+                                                           FrameReset fr) {
     int b = blockIdx.x, x, y;
     if (take_tile(b, d16.cols, d16.rows, x, y)) {
         if (x < d16.cols && y < d16.rows) {
@@ -362,6 +363,20 @@ __global__ __launch_bounds__(256) void frame_level0_kernel(View<uint16_t> d16, V
             mp.v[2].at(y, x) = vo;
             mp.n[2].at(y, x) = no;
         }
+        return;
+    }
+    // the frame's resets, in the workgroups after the tiles
+    const int nz = fr.zmin.cols * fr.zmin.rows;
+    const int i = b * 256 + (int)(threadIdx.y * kTX + threadIdx.x);
+    if (i < nz) {
+        const int zy = i / fr.zmin.cols, zx = i - zy * fr.zmin.cols;
+        fr.zmin.at(zy, zx) = 100.f;
+        fr.zmax.at(zy, zx) = 0.f;
+    }
+    if (b == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+        *fr.render_tiles = 0u;
+        *fr.rq_count = 0u;
+        *fr.updated_voxels = 0u;
     }
 }
 
@@ -514,12 +529,14 @@ int launch_resize_map(View<float4> vsrc, View<float4> nsrc, View<float4> vdst, V
 
 int launch_frame_pyramid(View<uint16_t> depth_u16, View<uint8_t> rgb, View<float> range,
                          const FramePyramid& P, float scale_inv, float cutoff,
-                         const ModelPyramid& last_model, hipStream_t s) {
+                         const ModelPyramid& last_model, const FrameReset& reset, hipStream_t s) {
     const ModelPyramid& mp = last_model;
+    static_assert(kTX * kTY == 256, "reset workgroups of 256 threads");
+    const int nr = (int)div_up(reset.zmin.cols * reset.zmin.rows, 256);
     const int n0 = tiles(depth_u16.cols, depth_u16.rows) + tiles(mp.v[1].cols, mp.v[1].rows) +
-                   tiles(mp.v[2].cols, mp.v[2].rows);
+                   tiles(mp.v[2].cols, mp.v[2].rows) + (nr > 0 ? nr : 1);
     frame_level0_kernel<<<n0, dim3(kTX, kTY), 0, s>>>(depth_u16, rgb, range, P.depth[0],
-                                                      P.image[0], scale_inv, cutoff, mp);
+                                                      P.image[0], scale_inv, cutoff, mp, reset);
     SICP_LAUNCH_CHECK("frame_level0_kernel");
     const dim3 g2((unsigned)div_up(P.depth[2].cols, 8), (unsigned)div_up(P.depth[2].rows, 8));
     pyrdown_levels_kernel<<<g2, 256, 0, s>>>(P.depth[0], P.image[0], P.depth[1], P.image[1],

@@ -81,7 +81,26 @@ SICP_DEV bool project_block(const sicp_block& b, const PoseF& pose, const Prepas
 __global__ __launch_bounds__(256) void prepass_kernel(MapDev m, MapCounters* ctr,
                                                       const sicp_block* visible, View<float> zmin,
                                                       View<float> zmax, PoseArg pa,
-                                                      PrepassArgs g, const int* gate) {
+                                                      PrepassArgs g, const int* gate,
+                                                      const uint32_t* fuse_counts) {
+    if (fuse_counts && !(gate && *gate == 0)) {
+        // the fusion's sums (fuse_end_kernel's, updated_voxels reset before
+        // the fusion): 256 per-wavefront counts per workgroup, integer sums
+        uint32_t c = 0;
+        for (int i = blockIdx.x * 256 + threadIdx.x; i < kFuseBlocks * 4; i += gridDim.x * 256)
+#pragma unroll
+            for (int j = 0; j < 256; j += 64) c += fuse_counts[i + j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+        if (threadIdx.x == 0 && c) {
+            atomicAdd(&ctr->updated_voxels, c);
+            atomicAdd(&ctr->updated_total, (unsigned long long)c);
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            ctr->visible_total += ctr->visible;
+            ctr->fusions += 1ull;
+        }
+    }
     if (gate && *gate == 0) return;
     const uint32_t nvis = ctr->visible;
     const PoseF pose = load_pose(pa);
@@ -949,7 +968,8 @@ size_t raycast_adjoint_scratch_bytes() { return sizeof(double) * (6 * kAdjMaxBlo
 
 int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible, View<float> zmin,
                           View<float> zmax, PoseArg pose, sicp_intrinsics K, float dmax,
-                          float dmin, const int* gate, hipStream_t s, bool zrange_cleared) {
+                          float dmin, const int* gate, hipStream_t s, bool zrange_cleared,
+                          const uint32_t* fuse_counts) {
     int n = zmin.cols * zmin.rows;
     if (!zrange_cleared) {
         zrange_init_kernel<<<div_up(n, 256) > 0 ? div_up(n, 256) : 1, 256, 0, s>>>(zmin, zmax,
@@ -960,7 +980,8 @@ int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible,
     // One-wavefront workgroups: the visible blocks (a thread each) spread over
     // more CUs than 256-thread groups would give them (16.5 against 19.8 µs
     // per launch on the 640x480 sequence).
-    prepass_kernel<<<2048, 64, 0, s>>>(m, ctr, visible, zmin, zmax, pose, g, gate);
+    static_assert(kFuseBlocks * 4 % 256 == 0, "whole 256-count chunks");
+    prepass_kernel<<<2048, 64, 0, s>>>(m, ctr, visible, zmin, zmax, pose, g, gate, fuse_counts);
     SICP_LAUNCH_CHECK("prepass_kernel");
     return SICP_OK;
 }

@@ -35,11 +35,20 @@ struct FramePyramid {
 struct ModelPyramid {
     View<float4> v[3], n[3];
 };
+// What the frame's first launch also resets for the frame's fusion count and
+// render: the zRange images (zmin 100, zmax 0), the render tile count, the
+// raycast queue count and the count of voxels the fusion updated.
+struct FrameReset {
+    View<float> zmin, zmax;
+    uint32_t* render_tiles;
+    uint32_t* rq_count;
+    uint32_t* updated_voxels;
+};
 // FillImages of the new frame and ResizeImages of the last one, in three
 // launches; pixel for pixel the same as the single-image launches above.
 int launch_frame_pyramid(View<uint16_t> depth_u16, View<uint8_t> rgb, View<float> range,
                          const FramePyramid& P, float scale_inv, float cutoff,
-                         const ModelPyramid& last_model, hipStream_t s);
+                         const ModelPyramid& last_model, const FrameReset& reset, hipStream_t s);
 
 // ---------------------------------------------------------------- ICP -----
 constexpr int kIcpThreads = 256;
@@ -237,12 +246,14 @@ int launch_fuse(MapDev m, MapCounters* ctr, const sicp_block* visible, View<floa
 int launch_fuse_count(MapCounters* ctr, const uint32_t* wave_counts, const int* gate,
                       hipStream_t s, View<float> zmin = View<float>{},
                       View<float> zmax = View<float>{}, uint32_t* rq_count = nullptr);
-// `zrange_cleared`: the zRange images and the tile count were cleared by
-// launch_fuse_count on the same stream.
+// `zrange_cleared`: the zRange images and the tile count were cleared earlier
+// on the same stream (FrameReset). `fuse_counts` (kFuseBlocks * 4 per-wavefront
+// counts of the fusion before it, with updated_voxels reset by FrameReset): the
+// prepass also does launch_fuse_count's sums.
 int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible, View<float> zmin,
                           View<float> zmax, PoseArg pose, sicp_intrinsics K, float dmax,
                           float dmin, const int* gate, hipStream_t s,
-                          bool zrange_cleared = false);
+                          bool zrange_cleared = false, const uint32_t* fuse_counts = nullptr);
 int launch_raycast(MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin,
                    View<float> zmax, PoseArg pose, float invfx, float invfy, float cx, float cy,
                    const int* gate, hipStream_t s);

@@ -148,9 +148,12 @@ static int frame_pyramid(sicp_system* s, FrameBuf& f, FrameBuf& last,
         mp.v[i] = last.vmap[i].v();
         mp.n[i] = last.nmap[i].v();
     }
+    // the frame's fusion count and render start from these
+    const FrameReset fr{s->zmin.v(), s->zmax.v(), &s->map->ctr->render_tiles,
+                        s->map->dev.rq_count, &s->map->ctr->updated_voxels};
     return launch_frame_pyramid(view<uint16_t>(depth_u16), view<uint8_t>(rgb), f.range.v(), P,
                                 (float)(1.0 / (double)s->cfg.depth_scale), s->cfg.depth_cutoff,
-                                mp, st);
+                                mp, fr, st);
 }
 
 static void system_free(sicp_system* s) {
@@ -372,11 +375,9 @@ static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
                           m->alloc.fuse_counts, gate, st)))
         return rc;
     stage_end(s, kStageFuse, ev, st);
+    // The fusion's counts are summed by the render prepass; the zRange images,
+    // tile and queue counts were reset by the frame's first launch.
     ev = stage_begin(s, st);
-    // the fusion count also clears the zRange images and the raycast queue
-    if ((rc = launch_fuse_count(m->ctr, m->alloc.fuse_counts, gate, st, s->zmin.v(),
-                                s->zmax.v(), m->dev.rq_count)))
-        return rc;
     if (m->cfg.unify_labels && labels && (rc = launch_unify(m, gate, st))) return rc;
     stage_end(s, kStageFuseCount, ev, st);
     // Mapping::RayTrace (Mapping.cc:142-161): prepass, then the raycast into
@@ -384,7 +385,7 @@ static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
     ev = stage_begin(s, st);
     if ((rc = launch_render_prepass(m->dev, m->ctr, m->visible, s->zmin.v(), s->zmax.v(), pose,
                                     s->K[0], m->cfg.depth_max, m->cfg.depth_min, gate, st,
-                                    true)))
+                                    true, m->alloc.fuse_counts)))
         return rc;
     stage_end(s, kStagePrepass, ev, st);
     ev = stage_begin(s, st);

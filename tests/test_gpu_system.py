@@ -63,6 +63,12 @@ def test_system_first_frame_model_bit_exact(dev):
     ov, _ = o.model(0)
     from helpers import bits_equal, mismatch_report
     assert bits_equal(buf, ov), mismatch_report(buf, ov)
+    # The fusion's counters (summed by the render prepass in this path) equal
+    # the oracle's fused-voxel and visible-block counts of the same frame.
+    st, w = B.map_stats(g.map_handle()), o.work()
+    assert st["fusions"] == 1
+    assert st["updated_voxels"] == st["updated_total"] == w["updated_voxels"] > 0
+    assert st["visible_total"] == w["visible_blocks"] > 0
 
 
 @pytest.mark.parametrize("c", [CFG_SMALL, dict(CFG_SMALL, cols=317, rows=239, cx=158.0, cy=119.0)])
