@@ -904,6 +904,9 @@ __global__ void track_init_kernel(FrameMeta* next, FrameMeta* last, TrackState* 
     st->rgb_error = 0.f;
 }
 
+// Workgroups of the tracking finish.
+constexpr int kFinishBlocks = 64;
+
 // Four pixels per thread, at most kIcpMaxBlocks workgroups.
 inline int icp_blocks(int n) {
     int b = (int)div_up(n, kIcpThreads * kPixPerThread);
@@ -1007,8 +1010,10 @@ int launch_track_finish(const IcpLevel* prev, const FrameMeta* last, TrackParams
                         const RgbScratch* rgb, TrackState* st_final, FrameMeta* next,
                         View<float4> vlast, View<float4> nlast, View<float4> vnext,
                         View<float4> nnext, const FrameImages* images, hipStream_t s) {
+    // Every workgroup redoes the final update (reading all partial rows), so
+    // the grid is kept small; the copy of a lost frame's maps is grid-stride.
     const int n = vnext.cols * vnext.rows;
-    const int blocks = (int)div_up(n, kIcpThreads * 4);
+    const int blocks = std::min((int)div_up(n, kIcpThreads * 4), kFinishBlocks);
     FrameImages im{};
     if (images) im = *images;
     if (prev) {
