@@ -10,6 +10,8 @@ namespace sicp {
 namespace {
 
 constexpr int kTX = 64, kTY = 4;
+constexpr int kModelTX = kTX;  // model pyramid tile: kModelTX x 8 level-1 pixels
+static_assert(kTY == 4 && kModelTX == 64, "two level-1 rows per thread, 128 level-2 pixels");
 
 inline dim3 grid2d(int cols, int rows) { return dim3(div_up(cols, kTX), div_up(rows, kTY)); }
 
@@ -344,27 +346,45 @@ __global__ __launch_bounds__(256) void frame_level0_kernel(View<uint16_t> d16, V
         }
         return;
     }
-    if (take_tile(b, mp.v[1].cols, mp.v[1].rows, x, y)) {
-        if (x < mp.v[1].cols && y < mp.v[1].rows) {
-            float4 vo, no;
-            resize_from(mp.v[0], mp.n[0], x, y, vo, no);
-            mp.v[1].at(y, x) = vo;
-            mp.n[1].at(y, x) = no;
+    // The model pyramid in tiles of 64 x 8 level-1 pixels (kept in LDS) and
+    // the 32 x 4 level-2 pixels made of them, so level 0 is read once.
+    __shared__ float4 lv[8][kModelTX], ln[8][kModelTX];
+    const int nx1 = (mp.v[1].cols + kModelTX - 1) / kModelTX, ny1 = (mp.v[1].rows + 7) / 8;
+    if (b < nx1 * ny1) {
+        const int X1 = (b % nx1) * kModelTX, Y1 = (b / nx1) * 8;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int ty = (int)threadIdx.y + 4 * r;
+            const int x1 = X1 + (int)threadIdx.x, y1 = Y1 + ty;
+            if (x1 < mp.v[1].cols && y1 < mp.v[1].rows) {
+                float4 vo, no;
+                resize_from(mp.v[0], mp.n[0], x1, y1, vo, no);
+                mp.v[1].at(y1, x1) = vo;
+                mp.n[1].at(y1, x1) = no;
+                lv[ty][threadIdx.x] = vo;
+                ln[ty][threadIdx.x] = no;
+            }
         }
-        return;
-    }
-    if (take_tile(b, mp.v[2].cols, mp.v[2].rows, x, y)) {
-        if (x < mp.v[2].cols && y < mp.v[2].rows) {
+        __syncthreads();
+        const int t = (int)(threadIdx.y * kTX + threadIdx.x);
+        const int x2l = t & 31, y2l = t >> 5;
+        const int x2 = X1 / 2 + x2l, y2 = Y1 / 2 + y2l;
+        // level-2 pixel (x2, y2) reads level-1 pixels 2 x2 + {0,1}, 2 y2 + {0,1},
+        // inside level 1 and inside this tile
+        if (t < 128 && x2 < mp.v[2].cols && y2 < mp.v[2].rows) {
             float4 v[4], n[4], vo, no;
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                resize_from(mp.v[0], mp.n[0], 2 * x + (k & 1), 2 * y + (k >> 1), v[k], n[k]);
+            for (int k = 0; k < 4; ++k) {
+                v[k] = lv[2 * y2l + (k >> 1)][2 * x2l + (k & 1)];
+                n[k] = ln[2 * y2l + (k >> 1)][2 * x2l + (k & 1)];
+            }
             resize_px(v, n, vo, no);
-            mp.v[2].at(y, x) = vo;
-            mp.n[2].at(y, x) = no;
+            mp.v[2].at(y2, x2) = vo;
+            mp.n[2].at(y2, x2) = no;
         }
         return;
     }
+    b -= nx1 * ny1;
     // the frame's resets, in the workgroups after the tiles
     const int nz = fr.zmin.cols * fr.zmin.rows;
     const int i = b * 256 + (int)(threadIdx.y * kTX + threadIdx.x);
@@ -533,8 +553,8 @@ int launch_frame_pyramid(View<uint16_t> depth_u16, View<uint8_t> rgb, View<float
     const ModelPyramid& mp = last_model;
     static_assert(kTX * kTY == 256, "reset workgroups of 256 threads");
     const int nr = (int)div_up(reset.zmin.cols * reset.zmin.rows, 256);
-    const int n0 = tiles(depth_u16.cols, depth_u16.rows) + tiles(mp.v[1].cols, mp.v[1].rows) +
-                   tiles(mp.v[2].cols, mp.v[2].rows) + (nr > 0 ? nr : 1);
+    const int nm = (int)(div_up(mp.v[1].cols, kModelTX) * div_up(mp.v[1].rows, 8));
+    const int n0 = tiles(depth_u16.cols, depth_u16.rows) + nm + (nr > 0 ? nr : 1);
     frame_level0_kernel<<<n0, dim3(kTX, kTY), 0, s>>>(depth_u16, rgb, range, P.depth[0],
                                                       P.image[0], scale_inv, cutoff, mp, reset);
     SICP_LAUNCH_CHECK("frame_level0_kernel");
