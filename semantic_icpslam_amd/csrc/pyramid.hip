@@ -426,11 +426,29 @@ __global__ __launch_bounds__(256) void pyrdown_levels_kernel(View<float> d0, Vie
     const int X0 = blockIdx.x * 8, Y0 = blockIdx.y * 8;  // level-2 tile origin
     const int rx0 = 2 * X0 - 2, ry0 = 2 * Y0 - 2;        // level-1 window origin
     const int wx0 = 2 * rx0 - 2, wy0 = 2 * ry0 - 2;      // level-0 window origin
-    for (int i = threadIdx.x; i < kWin0 * kWin0; i += 256) {
+    // Every load of the window is issued before the first LDS store (a loop
+    // of load-then-store waits out one memory round trip per element);
+    // entries outside the image are loaded from (0, 0) and not stored.
+    constexpr int kWinIters = (kWin0 * kWin0 + 255) / 256;
+    float dv[kWinIters];
+    uint8_t gv[kWinIters];
+    bool in[kWinIters];
+#pragma unroll
+    for (int j = 0; j < kWinIters; ++j) {
+        const int i = (int)threadIdx.x + 256 * j;
         const int wx = wx0 + i % kWin0, wy = wy0 + i / kWin0;
-        if (wx < 0 || wy < 0 || wx >= d0.cols || wy >= d0.rows) continue;
-        w0d[i] = d0.at(wy, wx);
-        w0g[i] = g0.at(wy, wx);
+        in[j] = i < kWin0 * kWin0 && wx >= 0 && wy >= 0 && wx < d0.cols && wy < d0.rows;
+        const int cx = in[j] ? wx : 0, cy = in[j] ? wy : 0;
+        dv[j] = d0.at(cy, cx);
+        gv[j] = g0.at(cy, cx);
+    }
+#pragma unroll
+    for (int j = 0; j < kWinIters; ++j) {
+        const int i = (int)threadIdx.x + 256 * j;
+        if (in[j]) {
+            w0d[i] = dv[j];
+            w0g[i] = gv[j];
+        }
     }
     __syncthreads();
     // Level-1 pixels owned here: [2*X0, 2*X0 + 16), the last tile column
