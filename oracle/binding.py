@@ -81,6 +81,9 @@ _SIGS = {
     "oracle_system_grab": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(FrameResult)]),
     "oracle_system_model": (None, [_P, C.c_int, _P, _P]),
     "oracle_system_work": (None, [_P, _P]),
+    "oracle_se3_control_trace": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_float, C.c_int, _P,
+                                           C.POINTER(C.c_float)]),
+    "oracle_se3_gate": (C.c_int, [C.c_float, C.c_float, _P, _P]),
 }
 
 
@@ -303,6 +306,30 @@ def euler_sin_norm(Ta, Tb):
     tn = C.c_double()
     lib().oracle_euler_sin_norm(_p(a), _p(b), C.byref(rn), C.byref(tn))
     return rn.value, tn.value
+
+
+def se3_control_trace(icp_error, icp_count, iters=(10, 5, 3), min_count=(2000, 1000, 100),
+                      rgb_error=None, rgb_count=None, w=1e-4):
+    """ComputeSE3's loop decisions on a stream of per-iteration results:
+    (final pose marker, failed, updates applied, iterations run, final ICP
+    error). Marker 0 is the initial pose, k + 1 the pose after iteration k."""
+    e = _c(icp_error, np.float32)
+    c = _c(icp_count, np.int32)
+    it = _c(iters, np.int32)
+    mc = _c(min_count, np.int32)
+    re = _c(rgb_error, np.float32) if rgb_error is not None else None
+    rc = _c(rgb_count, np.int32) if rgb_count is not None else None
+    out = np.zeros(4, np.int32)
+    err = C.c_float()
+    if lib().oracle_se3_control_trace(_p(it), _p(mc), _p(e), _p(c), _p(re), _p(rc), w, len(e),
+                                      _p(out), C.byref(err)) != 0:
+        raise ValueError("stream shorter than the loop")
+    return int(out[0]), int(out[1]), int(out[2]), int(out[3]), err.value
+
+
+def se3_gate(icp_error, thresh, pose0, pose):
+    return bool(lib().oracle_se3_gate(icp_error, thresh, _p(_c(pose0, np.float64)),
+                                      _p(_c(pose, np.float64))))
 
 
 class OracleMap:

@@ -144,6 +144,17 @@ void* oracle_system_map(void* sys);
  * interpolated reads, raycast block lookups, allocation probes, voxels fused,
  * visible blocks fused. */
 void oracle_system_work(void* sys, uint64_t* out8);
+/* ComputeSE3's loop decisions (Tracking.cc:398-503) on a given stream of
+ * per-iteration ICP errors / counts (and, when rgb_error is non-NULL, the
+ * photometric error / count checked first), with the pose replaced by a
+ * marker: 0 the initial pose, k + 1 the pose after iteration k's update.
+ * out4 = {final marker, failed, updates applied, iterations run};
+ * out_error = the final ICP error. Returns -1 if the stream is too short. */
+int oracle_se3_control_trace(const int* iters3, const int* min_count3, const float* icp_error,
+                             const int* icp_count, const float* rgb_error, const int* rgb_count,
+                             float w, int n, int* out4, float* out_error);
+/* The final gate of ComputeSE3 (Tracking.cc:505-516): 1 = tracked. */
+int oracle_se3_gate(float icp_error, float thresh, const double* pose0, const double* pose);
 /* Copies the last frame's level-l model vertex / normal maps (after raycast). */
 void oracle_system_model(void* sys, int level, float* vmap, float* nmap);
 

@@ -60,6 +60,49 @@ void mat4_mul(const double* A, const double* B, double* C);
 void rigid_inverse(const double* T, double* Ti);
 void euler_sin_norm(const double* T, double* rot_norm, double* trans_norm);
 
+// The decisions of ComputeSE3's inner loop (Tracking.cc:449-481), one call
+// per Gauss-Newton iteration after its ICP sums (and, with the photometric
+// term, after RGBStep's failure test, :425-432): the count / NaN failure,
+// the error-increase counter with its candidate pose, the break-out after a
+// third increase in a row. `capture` is set when the caller must store its
+// current pose as the candidate (the first increase of a run). The
+// reference's `candidateICP = icpError` sits outside the braces of
+// `if (breakCount == 1)`, so it is taken on every increase (Q11). The state
+// carries across pyramid levels, as the reference's locals do.
+enum { kSe3Apply = 0, kSe3Fail = 1, kSe3Restore = 2 };
+template <class R> struct Se3Control {
+    R last_error = (R)100.f;  // Tracking.cc:391
+    R candidate_icp = (R)0;
+    int break_count = 0;
+    bool capture = false;
+    int step(R icp_error, int icp_count, R rgb_error, R w, int min_count) {
+        capture = false;
+        if (isnan_(icp_error) || icp_count < min_count) return kSe3Fail;  // :452-456
+        const R curr = rgb_error * w + icp_error;                          // :458
+        if (last_error < curr) {
+            ++break_count;
+            if (break_count == 1) capture = true;
+            candidate_icp = icp_error;
+        } else {
+            break_count = 0;
+        }
+        if (break_count > 2) return kSe3Restore;  // :469-478
+        last_error = curr;
+        return kSe3Apply;
+    }
+};
+// The final gate of ComputeSE3 (Tracking.cc:505-516): the frame is tracked
+// when the last ICP error is below the threshold and the motion from the
+// frame's initial pose `pose0` to `pose` is small (sine norm of the XYZ Euler
+// angles and translation norm at most 0.1).
+template <class R> bool se3_gate(R icp_error, R thresh, const double* pose0, const double* pose) {
+    double inv0[16], p[16], rn, tn;
+    rigid_inverse(pose0, inv0);
+    mat4_mul(inv0, pose, p);
+    euler_sin_norm(p, &rn, &tn);
+    return icp_error < thresh && rn <= 0.1 && tn <= 0.1;
+}
+
 // ---- map ----
 struct Voxel {
     double sdf;

@@ -114,15 +114,17 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
     std::memcpy(pose0, nf.pose, sizeof(pose0));
     for (int i = 0; i < 16; ++i) delta[i] = (i % 5 == 0) ? 1.0 : 0.0;
     std::memcpy(candidate, nf.pose, sizeof(candidate));
-    R icp_error = 0, last_error = (R)100.f, curr_error = 0, candidate_icp = 0, rgb_error = 0;
-    int icp_count = 0, break_count = 0, applied = 0;
+    R icp_error = 0, rgb_error = 0;
+    int icp_count = 0, applied = 0;
+    Se3Control<R> ctl;
+    bool broke = false;
     const bool icp_only = s.cfg.icp_only != 0;
     const float w = s.cfg.rgb_weight;
     // The precision form's FP32 compensated sums (the FP64 restatement keeps
     // its own arithmetic).
     const bool k32 = std::is_same<R, float>::value && s.cfg.icp_sums_fp32 != 0;
     std::vector<float> rows(k32 ? (size_t)nf.cols[0] * nf.rows[0] * 29 : 0);
-    for (int i = 2; i >= 0; --i) {
+    for (int i = 2; i >= 0 && !broke; --i) {
         for (int j = 0; j < s.cfg.icp_iters[i]; ++j) {
             Pose<R> pc = pose_from_d<R>(nf.pose);
             Pose<R> pl = pose_from_d<R>(lf.pose);
@@ -165,26 +167,20 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
             icp_error = std::sqrt(r0) / r1;
             icp_count = (int)r1;
             res->icp_count = icp_count;
-            if (isnan_(icp_error) || icp_count < s.cfg.icp_min_count[i]) {
+            const int act = ctl.step(icp_error, icp_count, rgb_error, (R)w, s.cfg.icp_min_count[i]);
+            if (act == kSe3Fail) {
                 std::memcpy(nf.pose, last_pose, sizeof(last_pose));
                 res->icp_error = (float)icp_error;
                 res->iterations = applied;
                 return false;
             }
-            curr_error = rgb_error * (R)w + icp_error;  // Tracking.cc:458
-            if (last_error < curr_error) {
-                break_count++;
-                if (break_count == 1) std::memcpy(candidate, nf.pose, sizeof(candidate));
-                candidate_icp = icp_error;  // brace-less in the reference (Q11)
-            } else {
-                break_count = 0;
-            }
-            if (break_count > 2) {
+            if (ctl.capture) std::memcpy(candidate, nf.pose, sizeof(candidate));
+            if (act == kSe3Restore) {
                 std::memcpy(nf.pose, candidate, sizeof(candidate));
-                icp_error = candidate_icp;
+                icp_error = ctl.candidate_icp;
+                broke = true;
                 break;
             }
-            last_error = curr_error;
             if (!icp_only)  // matA = matA_rgb * w + matA_icp (Tracking.cc:484-487)
                 for (int k = 0; k < 36; ++k) {
                     A[k] = A_rgb[k] * (double)w + A[k];
@@ -198,15 +194,10 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
             mat4_mul(last_pose, inv, nf.pose);
             applied++;
         }
-        if (break_count > 2) break;
     }
     res->icp_error = (float)icp_error;
     res->iterations = applied;
-    double inv0[16], p[16], rn, tn;
-    rigid_inverse(pose0, inv0);
-    mat4_mul(inv0, nf.pose, p);
-    euler_sin_norm(p, &rn, &tn);
-    if (icp_error < (R)s.cfg.icp_thresh && rn <= 0.1 && tn <= 0.1) return true;
+    if (se3_gate<R>(icp_error, (R)s.cfg.icp_thresh, pose0, nf.pose)) return true;
     std::memcpy(nf.pose, last_pose, sizeof(last_pose));
     return false;
 }
@@ -327,6 +318,69 @@ void* oracle_system_map(void* sys) {
     // Only the FP32 system's map is exposed through the map handle API.
     (void)sys;
     return nullptr;
+}
+
+// ComputeSE3's loop (Tracking.cc:398-503) driven by a given stream of
+// per-iteration results instead of images: the same controller as the
+// system restatement above, with the pose replaced by a marker (0: the
+// frame's initial pose; k + 1: the pose after the update of iteration k).
+// out4 = {final marker, failed, updates applied, iterations run}; the
+// final ICP error (the candidate's after a break-out) into out_error.
+// Returns 0, or -1 when the stream runs out before the loop ends.
+int oracle_se3_control_trace(const int* iters3, const int* min_count3, const float* icp_error,
+                             const int* icp_count, const float* rgb_error, const int* rgb_count,
+                             float w, int n, int* out4, float* out_error) {
+    Se3Control<float> ctl;
+    int marker = 0, candidate = 0, applied = 0, k = 0;
+    float err = 0.f, rgb = 0.f;
+    bool broke = false;
+    for (int l = 2; l >= 0 && !broke; --l) {
+        for (int j = 0; j < iters3[l]; ++j, ++k) {
+            if (k >= n) return -1;
+            if (rgb_error) {
+                rgb = rgb_error[k];
+                if (isnan_(rgb) || rgb_count[k] < min_count3[l]) {
+                    out4[0] = 0;
+                    out4[1] = 1;
+                    out4[2] = applied;
+                    out4[3] = k + 1;
+                    *out_error = err;
+                    return 0;
+                }
+            }
+            err = icp_error[k];
+            const int act = ctl.step(err, icp_count[k], rgb, w, min_count3[l]);
+            if (act == kSe3Fail) {
+                out4[0] = 0;
+                out4[1] = 1;
+                out4[2] = applied;
+                out4[3] = k + 1;
+                *out_error = err;
+                return 0;
+            }
+            if (ctl.capture) candidate = marker;
+            if (act == kSe3Restore) {
+                marker = candidate;
+                err = ctl.candidate_icp;
+                broke = true;
+                ++k;
+                break;
+            }
+            marker = k + 1;
+            ++applied;
+        }
+    }
+    out4[0] = marker;
+    out4[1] = 0;
+    out4[2] = applied;
+    out4[3] = k;
+    *out_error = err;
+    return 0;
+}
+
+// The final gate of ComputeSE3 (Tracking.cc:505-516) on given poses.
+int oracle_se3_gate(float icp_error, float thresh, const double* pose0, const double* pose) {
+    return se3_gate<float>(icp_error, thresh, pose0, pose) ? 1 : 0;
 }
 
 void oracle_system_model(void* sys, int level, float* vmap, float* nmap) {
