@@ -1,12 +1,14 @@
 """Benchmark of the per-frame dense SLAM path (track + fuse + raycast).
 
 Workload: the 640x480 sequence of the seeded procedural room (fx = fy = 525,
-c = (319.5, 239.5)), uint16 depth at 1 mm with sensor noise and 2 % dropout,
-uint8 RGB and a 10-class label image fused as per-voxel labels; 5 mm voxels,
-15 mm truncation, 2^20 buckets x 2, a 400k-block heap; ICP 10/5/3, FP32.
-One step = one frame (System::GrabImage). All frames are resident in HBM
-before the timed region; W warm-up frames run first, then K frames are timed
-between two synchronisations. Prints one JSON line.
+c = (319.5, 239.5)), 300 frames of uint16 depth at 1 mm with sensor noise and
+2 % dropout, uint8 RGB and a 10-class label image fused as per-voxel labels;
+5 mm voxels, 15 mm truncation, 2^20 buckets x 2, a 400k-block heap; ICP
+10/5/3, FP32. One step = one frame (System::GrabImage). Every pass runs the
+whole sequence on a fresh system, so the map reaches its full size; the last
+K frames (--steps) are timed between two synchronisations, after at least W
+(--warmup) untimed frames. All frames are resident in HBM before the timed
+region. Prints one JSON line.
 """
 import argparse
 import json
@@ -21,24 +23,50 @@ sys.path.insert(0, ROOT)
 
 WORKLOAD = dict(cols=640, rows=480, f=525.0, cx=319.5, cy=239.5, voxel=0.005, trunc=0.015,
                 buckets=1 << 20, blocks=400000, seed=0)
+SEQ_FRAMES = 300  # length of the 640x480 sequence
 STAGES = ["pyramid", "track", "allocate", "visibility", "fuse", "prepass", "raycast",
-          "fuse_count", "frame", "icp_iteration", "track_finish"]
+          "unify", "frame", "icp_iteration", "track_finish"]
 HBM_PEAK_GBS = 8000.0
-# Bytes of one fused voxel: 8 B geometry + 12 B semantic record read and
-# written, plus its depth (4), RGB (3), normal (16) and label (1) gathers.
+# Algorithmic bytes (each input read once, each output written once).
+# Fusion: per fused voxel the 8 B geometry and 12 B semantic records read and
+# written, plus its depth (4), RGB (3), normal (16) and label (1) gathers;
+# per visible block its list record.
 FUSE_BYTES_PER_VOXEL = 2 * (8 + 12) + 4 + 3 + 16 + 1
-FUSE_BYTES_PER_BLOCK = 16  # visible-list record
+FUSE_BYTES_PER_BLOCK = 16
 # Raycast: per ray the zRange pair (8) and the vmap + nmap writes (32); per
 # march step the voxel's sdf + weight (5); per interpolated read eight sdf
-# values (32); per block lookup one two-entry bucket (32).
+# values (32); per block lookup one hash entry (16).
 RAY_BYTES_PER_RAY = 40
 RAY_BYTES_PER_STEP = 5
 RAY_BYTES_PER_INTERP = 32
-RAY_BYTES_PER_LOOKUP = 32
+RAY_BYTES_PER_LOOKUP = 16
 # ICP: per pixel of a level, current vertex + normal and the gathered last
 # vertex + normal (float4 each).
 ICP_BYTES_PER_PIXEL = 64
+# Allocation: one two-entry bucket (32 B) per table probe, the depth (4 B) of
+# every pixel.
+ALLOC_BYTES_PER_PROBE = 32
+ALLOC_BYTES_PER_PIXEL = 4
+# Visibility: the block record (16 B) of every allocated heap slot, the
+# record of every visible block written to the list.
+VIS_BYTES_PER_SLOT = 16
+VIS_BYTES_PER_VISIBLE = 16
+# Prepass: the visible list read (16 B a block), both zRange images written.
+PREPASS_BYTES_PER_BLOCK = 16
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_vga_5mm_sequence.json")
+
+
+def pyramid_bytes(cols, rows):
+    """The frame pyramid's three launches per frame: (1) u16 depth and RGB in,
+    raw + filtered depth and intensity out, the last frame's level-0 model
+    maps in and levels 1-2 out; (2) filtered depth and intensity in, levels
+    1-2 of both out; (3) per level depth and intensity in, vertex, normal and
+    the two derivative images out."""
+    p = [(cols >> lv) * (rows >> lv) for lv in range(3)]
+    b1 = p[0] * (2 + 3 + 4 + 4 + 1 + 32) + 32 * (p[1] + p[2])
+    b2 = p[0] * (4 + 1) + 5 * (p[1] + p[2])
+    b3 = sum(q * (4 + 1 + 16 + 16 + 2 + 2) for q in p)
+    return b1 + b2 + b3
 
 
 def log(*a):
@@ -56,11 +84,12 @@ def make_frames(n, w):
     return syn, frames
 
 
-def cpu_baseline(w, frames, first_timed, seconds=20.0):
+def cpu_baseline(w, frames, first_timed, seconds=60.0):
     """The oracle (scalar C++ restatement, OpenMP over pixels and blocks) on the
-    same frames, on all host cores, for a bounded prefix of the sequence; also
-    its work counters over the GPU-timed frames (for algorithmic bytes) and a
-    one-thread time on a short sample."""
+    same frames, on all host cores: the whole sequence when it fits the time
+    bound (its work counters over the GPU-timed frames then give the exact
+    algorithmic bytes), else a prefix; also a one-thread time on a short
+    sample."""
     from oracle import binding as O
     from semantic_icpslam_amd import binding as B
     cores = O.set_threads(0)
@@ -84,6 +113,7 @@ def cpu_baseline(w, frames, first_timed, seconds=20.0):
     if work0 is not None and n > first_timed:
         work = {k: work1[k] - work0[k] for k in work1}
         work["frames"] = n - first_timed
+    del sysc
     # one thread, a few frames of the same sequence (fresh map)
     O.set_threads(1)
     sys1 = O.OracleSystem(cfg)
@@ -142,30 +172,102 @@ def cpu_small():
     return out
 
 
-def pose_error_vs_fp64(w, frames, cfgs, m=30):
-    """Per-frame pose of the GPU system, for each configuration in `cfgs`,
-    against the FP64 oracle on the first m frames (an untimed pass): max
-    rotation angle (rad) of T_fp64^-1 T and max translation difference (m)."""
+def _rot_trans_err(a, b):
+    """Rotation angle (rad) of b^-1 a and translation difference (m)."""
+    dR = b[:3, :3].T @ a[:3, :3]
+    return (float(np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1))),
+            float(np.linalg.norm(a[:3, 3] - b[:3, 3])))
+
+
+def _download_image(im, shape, dtype):
+    import ctypes
+    import torch
+    torch.cuda.synchronize()
+    buf = np.zeros(shape, dtype)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy2D.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int]
+    width = buf.strides[0]
+    if hip.hipMemcpy2D(buf.ctypes.data, width, im.ptr, im.pitch, width, shape[0], 2) != 0:
+        raise RuntimeError("hipMemcpy2D failed")
+    return buf
+
+
+def pose_precision(w, frames, syn, cfgs, sample_every=10):
+    """Pose precision of the GPU system against FP64 (BASELINE's C3 report),
+    for each configuration in `cfgs` (name -> system config), over the whole
+    sequence (an untimed pass per configuration):
+    - one step: at every `sample_every`-th frame, the GPU's tracked pose
+      against the FP64 transcription of ComputeSE3 run on identical inputs —
+      the frame's measured pyramid, the model pyramid the GPU tracked against
+      (downloaded) and the GPU's previous pose;
+    - drift: the GPU trajectory against the FP64 oracle system run on the
+      same frames (each tracking against its own model), and each against the
+      ground-truth trajectory, at the last frame."""
     import torch
     from oracle import binding as O
     from semantic_icpslam_amd import binding as B
     dev = torch.device("cuda:0")
-    o = O.OracleSystem(cfgs[0], fp64=True)
-    ref = [o.grab(d, rgb, lab)["pose"] for d, rgb, lab in frames[:m]]
-    out = []
-    for cfg in cfgs:
+    n = len(frames)
+    cols, rows = w["cols"], w["rows"]
+    first = next(iter(cfgs.values()))
+    log("[bench] FP64 oracle system over the sequence")
+    o64 = O.OracleSystem(first, fp64=True)
+    ref = [o64.grab(d, rgb, lab)["pose"] for d, rgb, lab in frames]
+    del o64
+    out = {"what": "one_step: at every %d-th frame, the tracked pose against the FP64 "
+                   "transcription of ComputeSE3 on the same measured pyramid, model pyramid and "
+                   "previous pose (max / median over the samples); drift_vs_fp64: the trajectory "
+                   "against the FP64 oracle system's over the same %d frames, at the last "
+                   "frame; drift_vs_ground_truth: each at the last frame against the "
+                   "synthetic trajectory" % (sample_every, n)}
+    gt = syn.pose(n - 1)
+    r, t = _rot_trans_err(ref[-1], gt)
+    out["fp64_oracle_drift_vs_ground_truth"] = {"rotation_rad": r, "translation_m": t}
+    for name, cfg in cfgs.items():
         g = B.System(cfg)
-        rot, tr = [], []
-        for i, (d, rgb, lab) in enumerate(frames[:m]):
+        steps, fp32_steps = [], []
+        poses = []
+        prev = None
+        for i, (d, rgb, lab) in enumerate(frames):
             g.grab(torch.from_numpy(d).to(dev), torch.from_numpy(rgb).to(dev),
                    torch.from_numpy(lab).to(dev))
-            a, b = g.result()["pose"], ref[i]
-            dR = b[:3, :3].T @ a[:3, :3]
-            rot.append(float(np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1))))
-            tr.append(float(np.linalg.norm(a[:3, 3] - b[:3, 3])))
+            res = g.result()
+            poses.append(res["pose"])
+            if i > 0 and i % sample_every == 0:
+                model = []
+                for lv in range(3):
+                    shp = (rows >> lv, cols >> lv, 4)
+                    model.append((_download_image(g.frame_image(16 + 0, lv), shp, np.float32),
+                                  _download_image(g.frame_image(16 + 1, lv), shp, np.float32)))
+                meas = O.measured_pyramid(d, w["f"], w["f"], w["cx"], w["cy"])
+                p64, ok64 = O.track_frame(cfg, meas, model, prev, fp64=True)
+                if ok64 and res["tracked"]:
+                    steps.append(_rot_trans_err(res["pose"], p64))
+                    p32, ok32 = O.track_frame(cfg, meas, model, prev, fp64=False)
+                    if ok32:
+                        fp32_steps.append(_rot_trans_err(p32, p64))
+            prev = res["pose"]
         g.close()
-        out.append({"frames": m, "max_rotation_rad": max(rot), "max_translation_m": max(tr),
-                    "last_translation_m": tr[-1]})
+        e = {}
+        if steps:
+            a = np.array(steps)
+            e["one_step"] = {"samples": len(steps),
+                             "max_rotation_rad": float(a[:, 0].max()),
+                             "median_rotation_rad": float(np.median(a[:, 0])),
+                             "max_translation_m": float(a[:, 1].max()),
+                             "median_translation_m": float(np.median(a[:, 1]))}
+        if fp32_steps:
+            a = np.array(fp32_steps)
+            e["one_step_fp32_oracle"] = {
+                "what": "the FP32 transcription (the reference's arithmetic) on the same inputs",
+                "max_rotation_rad": float(a[:, 0].max()),
+                "max_translation_m": float(a[:, 1].max())}
+        r, t = _rot_trans_err(poses[-1], ref[-1])
+        e["drift_vs_fp64"] = {"frames": n, "rotation_rad": r, "translation_m": t}
+        r, t = _rot_trans_err(poses[-1], gt)
+        e["drift_vs_ground_truth"] = {"rotation_rad": r, "translation_m": t}
+        out[name] = e
     return out
 
 
@@ -250,8 +352,14 @@ def precision_report(w, frames, syn, cfg32, cfg16, m=30):
                               invf, invf, w["cx"], w["cy"])
     for name, dpt in zip(("fp32", "half"), depths):
         ok = ~np.isnan(dpt) & ~np.isnan(v64[..., 2])
-        out[f"depth_rms_{name}_mm"] = float(np.sqrt(np.mean((dpt[ok] - v64[..., 2][ok]) ** 2)) * 1e3)
+        err = np.abs(dpt[ok] - v64[..., 2][ok]) * 1e3
+        out[f"depth_rms_{name}_mm"] = float(np.sqrt(np.mean(err ** 2)))
+        out[f"depth_p99_{name}_mm"] = float(np.percentile(err, 99))
+        out[f"depth_max_{name}_mm"] = float(err.max())
+        out[f"depth_pixels_over_1mm_{name}"] = int((err > 1.0).sum())
         out[f"depth_pixels_{name}"] = int(ok.sum())
+        # hit / no-hit disagreements with the FP64 render
+        out[f"depth_hit_mismatch_{name}"] = int((np.isnan(dpt) != np.isnan(v64[..., 2])).sum())
     for g in maps:
         g.close()
     return out
@@ -496,7 +604,7 @@ def main():
     ap.add_argument("--steps", type=int, default=290)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=60.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB first (counter calibration)")
     ap.add_argument("--no-pose-check", action="store_true")
@@ -540,12 +648,18 @@ def main():
 
     w = WORKLOAD
     dev = torch.device("cuda:0")
-    n = args.warmup + args.steps
+    if args.steps < 1:
+        print("--steps must be at least 1", file=sys.stderr)
+        sys.exit(2)
+    # The whole sequence every pass: the untimed prefix (at least --warmup
+    # frames) builds the map, the last --steps frames are timed.
+    n = max(SEQ_FRAMES, args.warmup + args.steps)
+    first_timed = n - args.steps
     log(f"[bench] rendering {n} frames of {w['cols']}x{w['rows']}")
     syn, frames = make_frames(n, w)
     depth = [torch.from_numpy(d).to(dev) for d, _, _ in frames]
     rgb = [torch.from_numpy(c).to(dev) for _, c, _ in frames]
-    lab = [torch.from_numpy(l).to(dev) for _, _, l in frames]
+    lab = [torch.from_numpy(lb).to(dev) for _, _, lb in frames]
     torch.cuda.synchronize()
 
     cfg = B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"], w["voxel"],
@@ -553,10 +667,11 @@ def main():
     stream = torch.cuda.current_stream()
 
     def run_pass(timing, graph=False, system_cfg=None):
-        """Warm-up frames, then the K timed frames on a fresh system; with
-        `timing`, HIP events bracket every stage and kernel launch; with
-        `graph`, every frame is copied into the system's staged inputs and
-        runs as one hipGraph launch."""
+        """The whole sequence on a fresh system: frames before `first_timed`
+        untimed, the rest timed; with `timing`, HIP events bracket every
+        stage and kernel launch of the timed frames; with `graph`, every
+        frame is copied into the system's staged inputs and runs as one
+        hipGraph launch."""
         sysg = B.System(system_cfg or cfg)
 
         def grab(i):
@@ -565,7 +680,7 @@ def main():
             else:
                 sysg.grab(depth[i], rgb[i], lab[i], stream)
 
-        for i in range(args.warmup):
+        for i in range(first_timed):
             grab(i)
         torch.cuda.synchronize()
         map_h = sysg.map_handle()
@@ -574,7 +689,7 @@ def main():
             B.check(B.lib().sicp_system_set_timing(sysg.h, 1))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for i in range(args.warmup, n):
+        for i in range(first_timed, n):
             grab(i)
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
@@ -595,10 +710,10 @@ def main():
         sysg.close()
         return out
 
-    # Timed region: K frames with no instrumentation (the reported value),
-    # then the same frames again on a fresh system with per-stage HIP events
-    # (stage times, per-frame percentiles, per-kernel rooflines).
-    log(f"[bench] warmup {args.warmup} + timing {args.steps} frames")
+    # Timed region: the last K frames with no instrumentation (the reported
+    # value); then the same sequence again on a fresh system with per-stage
+    # HIP events (stage times, per-frame percentiles, per-kernel rooflines).
+    log(f"[bench] {first_timed} untimed + {args.steps} timed frames per pass")
     clean = run_pass(False)
     graphed = None
     if not args.no_graph:
@@ -627,7 +742,7 @@ def main():
     tot, cnt, fm = inst["tot"], inst["cnt"], inst["frame_ms"]
     gt = syn.pose(n - 1)
     ms_per_frame = 1000.0 * clean["elapsed"] / args.steps
-    stage_ms = {s: round(float(tot[k] / max(cnt[k], 1)), 4) for k, s in enumerate(STAGES)}
+    stage_ms = {s_: round(float(tot[k] / max(cnt[k], 1)), 4) for k, s_ in enumerate(STAGES)}
     stage_ms["icp_iteration"] = round(float(tot[STAGES.index("icp_iteration")] / args.steps), 4)
     log("[bench] per-stage ms/frame: " + json.dumps(stage_ms))
 
@@ -637,6 +752,7 @@ def main():
     # over the summed fuse-kernel event time.
     upd = st1["updated_total"] - st0["updated_total"]
     vis = st1["visible_total"] - st0["visible_total"]
+    heap = st1["heap_scanned_total"] - st0["heap_scanned_total"]
     fus = max(st1["fusions"] - st0["fusions"], 1)
     fuse_bytes = upd * FUSE_BYTES_PER_VOXEL + vis * FUSE_BYTES_PER_BLOCK
     kernels["fuse_kernel"] = roofline_entry("fuse_kernel", fuse_bytes,
@@ -648,15 +764,19 @@ def main():
     cpu, work = None, None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle)")
-        cpu, work = cpu_baseline(w, frames, args.warmup, args.cpu_seconds)
+        cpu, work = cpu_baseline(w, frames, first_timed, args.cpu_seconds)
+    ray_bytes = alloc_bytes = None
     if work and work["raycasts"]:
-        # Raycast: the oracle's counts over the same frames (deterministic;
-        # scaled by frames when the CPU leg covered only a prefix of them).
+        # Raycast and allocation: the oracle's counts over the same frames
+        # (deterministic; scaled by frames when the CPU leg covered only a
+        # prefix of the timed frames).
         scale = args.steps / work["frames"]
         ray_bytes = scale * (RAY_BYTES_PER_RAY * work["rays"] +
                              RAY_BYTES_PER_STEP * work["march_steps"] +
                              RAY_BYTES_PER_INTERP * work["interp_reads"] +
                              RAY_BYTES_PER_LOOKUP * work["ray_lookups"])
+        alloc_bytes = (scale * ALLOC_BYTES_PER_PROBE * work["alloc_probes"] +
+                       args.steps * ALLOC_BYTES_PER_PIXEL * w["cols"] * w["rows"])
         ray_traffic = None
         if "raycast_kernel" in traffic and "raycast_long_kernel" in traffic:
             ray_traffic = traffic["raycast_kernel"] + traffic["raycast_long_kernel"]
@@ -667,8 +787,13 @@ def main():
             k: round(work[k] / max(work["rays"], 1), 3)
             for k in ("march_steps", "interp_reads", "ray_lookups")}
         kernels["raycast"]["work_frames"] = work["frames"]
+        if ray_traffic:
+            # the same launches' measured HBM traffic (KiB per launch) over their time
+            kernels["raycast"]["traffic_gbs"] = round(
+                ray_traffic * 1024 * cnt[STAGES.index("raycast")] /
+                max(tot[STAGES.index("raycast")] / 1000.0, 1e-12) / 1e9, 1)
     # ICP: every Gauss-Newton launch reads 64 B per pixel of its level.
-    lv = [(w["cols"] >> l) * (w["rows"] >> l) for l in range(3)]
+    lv = [(w["cols"] >> l_) * (w["rows"] >> l_) for l_ in range(3)]
     icp_launches = int(cnt[STAGES.index("icp_iteration")])
     icp_bytes = ICP_BYTES_PER_PIXEL * args.steps * (10 * lv[0] + 5 * lv[1] + 3 * lv[2])
     kernels["icp_iteration_kernel"] = roofline_entry(
@@ -678,18 +803,55 @@ def main():
     # The dominant kernel: the most time per frame (summed over its launches).
     dom = max(kernels, key=lambda k: kernels[k]["avg_launch_us"] *
               kernels[k].get("launches_per_frame", 1.0))
-    roofline = kernels[dom]
+    roofline = dict(kernels[dom])
+    roofline["traffic_source"] = (
+        "profiles/pmc_traffic_vga_5mm_sequence.json: rocprofv3 FETCH_SIZE and WRITE_SIZE "
+        "passes over bench.py --no-graph on this 300-frame workload, each scaled by the same "
+        "counter on a plain device copy")
+
+    # Every stage of the frame: algorithmic bytes per frame over the stage's
+    # event time (the instrumented pass, timed frames).
+    zc, zr = (w["cols"] + 7) // 8, (w["rows"] + 7) // 8
+    stage_bytes = {
+        "pyramid": args.steps * pyramid_bytes(w["cols"], w["rows"]),
+        "track": icp_bytes,
+        "allocate": alloc_bytes,
+        "visibility": VIS_BYTES_PER_SLOT * heap + VIS_BYTES_PER_VISIBLE * vis,
+        "fuse": fuse_bytes,
+        "prepass": PREPASS_BYTES_PER_BLOCK * vis + args.steps * 8 * zc * zr,
+        "raycast": ray_bytes,
+    }
+    stages = {}
+    for name, nbytes in stage_bytes.items():
+        ms = tot[STAGES.index(name)] / args.steps
+        e = {"ms_per_frame": round(float(ms), 4)}
+        if nbytes:
+            gbs = nbytes / args.steps / (ms * 1e-3) / 1e9
+            e.update(bytes_per_frame=int(nbytes / args.steps), gbs=round(gbs, 1),
+                     frac=round(gbs / HBM_PEAK_GBS, 4))
+        stages[name] = e
+    known = [b for b in stage_bytes.values() if b]
+    if len(known) == len(stage_bytes):
+        fr_ms = tot[STAGES.index("frame")] / args.steps
+        gbs = sum(known) / args.steps / (fr_ms * 1e-3) / 1e9
+        stages["frame"] = {"ms_per_frame": round(float(fr_ms), 4),
+                           "bytes_per_frame": int(sum(known) / args.steps),
+                           "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    stages["what"] = ("algorithmic bytes per frame (each input read once, each output written "
+                      "once) over the stage's HIP-event time in the instrumented pass; "
+                      "allocation and raycast counts from the oracle on the same frames; the "
+                      "track stage's time includes its finishing launch")
 
     small = None
     if not args.no_cpu_baseline:
         log("[bench] CPU reference method at the small size")
         small = cpu_small()
-    pose_check = pose_half = prec = None
+    pose_check = prec = None
     if not args.no_pose_check:
-        log("[bench] pose check against the FP64 oracle (both forms)")
-        pose_check, pose_half = pose_error_vs_fp64(w, frames, [cfg, cfg16])
+        log("[bench] pose precision against FP64 (both forms, whole sequence)")
+        pose_check = pose_precision(w, frames, syn, {"fp32_form": cfg, "precision_form": cfg16})
         log("[bench] TSDF and depth of both forms against FP64")
-        prec = precision_report(w, frames, syn, cfg, cfg16)
+        prec = precision_report(w, frames, syn, cfg, cfg16, m=min(30, n))
 
     out = {
         "metric": "ms/frame track+fuse+raycast 640x480 (plain FP32)",
@@ -706,8 +868,9 @@ def main():
         "data": "synthetic (seeded procedural room, seed 0)",
         "config": {"workload": "640x480 / 5 mm voxels / T 15 mm / 2^20x2 buckets / "
                                "400k-block heap / ICP 10-5-3 / RGB + 10-class labels",
-                   "frames": n, "global_batch": 1},
+                   "frames": n, "timed_frames": [first_timed, n - 1], "global_batch": 1},
         "stage_ms_per_frame": stage_ms,
+        "stages": stages,
         "allocated_blocks": st1["allocated_blocks"],
         "last_frame_tracked": res["tracked"],
         "last_frame_translation_vs_ground_truth_m": round(
@@ -725,11 +888,12 @@ def main():
                     "summation (TwoSum at every addition; FP64 only in the 6x6 solve and the "
                     "pose composition), one hipGraph launch per frame",
             "voxel_bytes": {"fp32_form": 8 + 12, "half_form": 4 + 12},
+            "allocated_blocks": precise["st1"]["allocated_blocks"],
             "last_frame_tracked": precise["res"]["tracked"],
             "last_frame_translation_vs_ground_truth_m": round(
                 float(np.linalg.norm(precise["res"]["pose"][:3, 3] - gt[:3, 3])), 5),
-            "pose_vs_fp64_oracle": pose_half,
             "vs_fp64": prec},
+        "pose_precision": pose_check,
         "photometric_tracking": None if photometric is None else {
             "ms_per_step": round(1000.0 * photometric["elapsed"] / args.steps, 4),
             "what": "the same frames with the reference's default tracking, ComputeSE3(false): "
@@ -742,12 +906,10 @@ def main():
         "frame_ms_p99": round(float(np.percentile(fm, 99)), 4) if fm.size else None,
         "roofline": roofline,
         "roofline_kernels": kernels,
-        "pose_vs_fp64_oracle": pose_check,
         "cpu_baseline": cpu,
         "cpu_small": small,
     }
     print(json.dumps(out), flush=True)
-
 
 if __name__ == "__main__":
     main()

@@ -141,6 +141,9 @@ typedef struct sicp_map_stats {
     uint64_t updated_total;     /* voxels written, summed since reset        */
     uint64_t visible_total;     /* visible blocks fused, summed since reset  */
     uint64_t fusions;           /* fusion passes since reset                 */
+    uint64_t heap_scanned_total;/* allocated blocks at each fusion (the heap
+                                   slots its visibility pass scanned), summed
+                                   since reset                              */
 } sicp_map_stats;
 
 typedef struct sicp_map sicp_map; /* opaque; owns every map buffer (Mapping.cc:11-51) */
@@ -261,11 +264,13 @@ int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_ima
                   double* A_host, double* b_host, float* residual_host, void* stream);
 size_t sicp_icp_partials_size(void);
 /* ICPStep in the precision form: the same 29 sums, arguments and outputs as
- * sicp_icp_step, accumulated in FP32 with compensated summation (Kahan's
- * update per product, TwoSum where partial sums meet) instead of FP64 — the
- * reference's own sums are plain FP32 (BlockReduce / Reduce<float, 29>,
- * Reduction.cu:29-60, 315, 374). Bitwise reproducible for a given image
- * size. */
+ * sicp_icp_step, accumulated in FP32 with compensated summation instead of
+ * FP64: every addition (a thread adding a product, partial sums meeting in
+ * the workgroup and across workgroups) is Knuth's TwoSum, the error terms
+ * are summed in FP32 and each value is rounded once at the end (cascaded
+ * summation) — the reference's own sums are plain FP32 (BlockReduce /
+ * Reduce<float, 29>, Reduction.cu:29-60, 315, 374). Bitwise reproducible for
+ * a given image size. */
 int sicp_icp_step_fp32(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nlast,
                        const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
                        void* partials_dev, float* sums_dev, float* host_sums,
@@ -397,8 +402,8 @@ typedef struct sicp_system_config {
     float rgb_weight;        /* w of the photometric term, 1e-4 (Tracking.cc:392) */
     int32_t icp_sums_fp32;   /* 0: the Gauss-Newton sums (ICP and photometric)
                                 accumulated in FP64; 1: the precision form, FP32
-                                with compensated summation (Kahan per thread,
-                                TwoSum where partial sums meet). The reference
+                                with compensated summation (TwoSum at every
+                                addition, as sicp_icp_step_fp32). The reference
                                 sums in plain FP32 (BlockReduce / Reduce<float,
                                 29>, Reduction.cu:29-60, 315, 374). */
 } sicp_system_config;

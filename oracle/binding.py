@@ -84,6 +84,8 @@ _SIGS = {
     "oracle_se3_control_trace": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_float, C.c_int, _P,
                                            C.POINTER(C.c_float)]),
     "oracle_se3_gate": (C.c_int, [C.c_float, C.c_float, _P, _P]),
+    "oracle_track_frame": (C.c_int, [C.POINTER(SystemConfig), C.c_int, _P, _P, _P, _P, _P, _P,
+                                     C.POINTER(FrameResult)]),
 }
 
 
@@ -330,6 +332,55 @@ def se3_control_trace(icp_error, icp_count, iters=(10, 5, 3), min_count=(2000, 1
 def se3_gate(icp_error, thresh, pose0, pose):
     return bool(lib().oracle_se3_gate(icp_error, thresh, _p(_c(pose0, np.float64)),
                                       _p(_c(pose, np.float64))))
+
+
+def measured_pyramid(depth_u16, fx, fy, cx, cy, scale=1000.0, cutoff=3.0):
+    """The three (vmap, nmap) levels of a frame as Frame::FillImages makes
+    them (Core/Frame.cc:68-92): bilateral filter, PyrDown of the filtered
+    depth, vertex and normal maps with K / 2^l."""
+    _, d = filter_depth(depth_u16, scale, cutoff)
+    out = []
+    for lvl in range(3):
+        if lvl:
+            d = pyrdown_f32(d)
+        s = float(np.float32(1 << lvl))
+        v = compute_vmap(d, float(np.float32(fx) / np.float32(s)), float(np.float32(fy) / np.float32(s)),
+                         float(np.float32(cx) / np.float32(s)), float(np.float32(cy) / np.float32(s)),
+                         cutoff)
+        out.append((v, compute_nmap(v)))
+    return out
+
+
+def model_pyramid(vmap0, nmap0):
+    """The model maps at three levels: the raycast's level 0 and ResizeMap
+    below it (Frame::ResizeImages, Core/Frame.cc:94-98)."""
+    out = [(np.ascontiguousarray(vmap0, np.float32), np.ascontiguousarray(nmap0, np.float32))]
+    for _ in range(2):
+        out.append(resize_map(*out[-1]))
+    return out
+
+
+def track_frame(cfg, next_maps, last_maps, last_pose, fp64=True):
+    """One ComputeSE3 (point-to-plane) on given pyramids: next_maps and
+    last_maps are lists of three (vmap, nmap) float32 pairs (levels 0-2).
+    Returns (pose after tracking, tracked)."""
+    keep = []
+
+    def arr(maps, k):
+        a = (C.c_void_p * 3)()
+        for lvl in range(3):
+            x = _c(maps[lvl][k], np.float32)
+            keep.append(x)
+            a[lvl] = x.ctypes.data
+        return a
+    nv, nn, lv, ln = arr(next_maps, 0), arr(next_maps, 1), arr(last_maps, 0), arr(last_maps, 1)
+    lp = _c(last_pose, np.float64)
+    out = np.zeros(16, np.float64)
+    res = FrameResult()
+    ok = lib().oracle_track_frame(C.byref(cfg), 1 if fp64 else 0, C.cast(nv, _P),
+                                  C.cast(nn, _P), C.cast(lv, _P), C.cast(ln, _P), _p(lp),
+                                  _p(out), C.byref(res))
+    return out.reshape(4, 4), bool(ok)
 
 
 class OracleMap:

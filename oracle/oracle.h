@@ -153,6 +153,16 @@ void oracle_system_work(void* sys, uint64_t* out8);
 int oracle_se3_control_trace(const int* iters3, const int* min_count3, const float* icp_error,
                              const int* icp_count, const float* rgb_error, const int* rgb_count,
                              float w, int n, int* out4, float* out_error);
+/* One ComputeSE3 (point-to-plane only) on given pyramids: next_* the next
+ * frame's measured vertex / normal maps, last_* the last frame's model maps,
+ * three levels each (float4, cols/2^l x rows/2^l), from `last_pose`
+ * (4x4 row-major); `fp64` selects the double transcription (else FP32 in
+ * the reference's order, with cfg->icp_sums_fp32 honoured). The pose after
+ * tracking (the last pose when lost) into pose_out; returns 1 if tracked. */
+int oracle_track_frame(const sicp_system_config* cfg, int fp64, const float* const* next_vmap,
+                       const float* const* next_nmap, const float* const* last_vmap,
+                       const float* const* last_nmap, const double* last_pose, double* pose_out,
+                       sicp_frame_result* res);
 /* The final gate of ComputeSE3 (Tracking.cc:505-516): 1 = tracked. */
 int oracle_se3_gate(float icp_error, float thresh, const double* pose0, const double* pose);
 /* Copies the last frame's level-l model vertex / normal maps (after raycast). */

@@ -106,9 +106,9 @@ static void compensated_sums(const std::vector<float>& rows, int n, double* sums
 // Tracker::ComputeSE3 (Tracking.cc:369-517): the point-to-plane term alone
 // (icpOnly, :488-490) or with the weighted photometric term (:400-433,
 // :484-487), as the configuration says.
-template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res) {
-    OFrame<R>& nf = *s.next;
-    OFrame<R>& lf = *s.last;
+template <class R>
+static bool compute_se3(const sicp_system_config& cfg, const Intr<R>* K, OFrame<R>& nf,
+                        OFrame<R>& lf, sicp_frame_result* res) {
     double last_pose[16], pose0[16], delta[16], candidate[16];
     std::memcpy(last_pose, lf.pose, sizeof(last_pose));
     std::memcpy(pose0, nf.pose, sizeof(pose0));
@@ -118,14 +118,14 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
     int icp_count = 0, applied = 0;
     Se3Control<R> ctl;
     bool broke = false;
-    const bool icp_only = s.cfg.icp_only != 0;
-    const float w = s.cfg.rgb_weight;
+    const bool icp_only = cfg.icp_only != 0;
+    const float w = cfg.rgb_weight;
     // The precision form's FP32 compensated sums (the FP64 restatement keeps
     // its own arithmetic).
-    const bool k32 = std::is_same<R, float>::value && s.cfg.icp_sums_fp32 != 0;
+    const bool k32 = std::is_same<R, float>::value && cfg.icp_sums_fp32 != 0;
     std::vector<float> rows(k32 ? (size_t)nf.cols[0] * nf.rows[0] * 29 : 0);
     for (int i = 2; i >= 0 && !broke; --i) {
-        for (int j = 0; j < s.cfg.icp_iters[i]; ++j) {
+        for (int j = 0; j < cfg.icp_iters[i]; ++j) {
             Pose<R> pc = pose_from_d<R>(nf.pose);
             Pose<R> pl = pose_from_d<R>(lf.pose);
             double A_rgb[36], b_rgb[6];
@@ -135,7 +135,7 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
                 int res2[2];
                 rgb_sums<R>(nf.image[i].data(), lf.image[i].data(), nf.vmap[i].data(),
                             lf.vmap[i].data(), nf.dx[i].data(), nf.dy[i].data(), nf.cols[i],
-                            nf.rows[i], pc, pl, s.K[i], rs, res2, nullptr,
+                            nf.rows[i], pc, pl, K[i], rs, res2, nullptr,
                             k32 ? rows.data() : nullptr);
                 if (k32)
                     compensated_sums(rows, nf.cols[i] * nf.rows[i], rs);
@@ -147,7 +147,7 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
                 const int rgb_count = (int)q1;
                 res->rgb_error = (float)rgb_error;
                 res->rgb_count = rgb_count;
-                if (isnan_(rgb_error) || rgb_count < s.cfg.icp_min_count[i]) {
+                if (isnan_(rgb_error) || rgb_count < cfg.icp_min_count[i]) {
                     std::memcpy(nf.pose, last_pose, sizeof(last_pose));
                     res->icp_error = (float)icp_error;  // the previous iteration's
                     res->iterations = applied;
@@ -156,7 +156,7 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
             }
             double sums[29], A[36], b[6];
             icp_sums<R>(nf.vmap[i].data(), lf.vmap[i].data(), nf.nmap[i].data(),
-                        lf.nmap[i].data(), nf.cols[i], nf.rows[i], pc, pl, s.K[i], sums,
+                        lf.nmap[i].data(), nf.cols[i], nf.rows[i], pc, pl, K[i], sums,
                         k32 ? rows.data() : nullptr);
             if (k32)
                 compensated_sums(rows, nf.cols[i] * nf.rows[i], sums);
@@ -167,7 +167,7 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
             icp_error = std::sqrt(r0) / r1;
             icp_count = (int)r1;
             res->icp_count = icp_count;
-            const int act = ctl.step(icp_error, icp_count, rgb_error, (R)w, s.cfg.icp_min_count[i]);
+            const int act = ctl.step(icp_error, icp_count, rgb_error, (R)w, cfg.icp_min_count[i]);
             if (act == kSe3Fail) {
                 std::memcpy(nf.pose, last_pose, sizeof(last_pose));
                 res->icp_error = (float)icp_error;
@@ -197,7 +197,7 @@ template <class R> static bool compute_se3(OSystem<R>& s, sicp_frame_result* res
     }
     res->icp_error = (float)icp_error;
     res->iterations = applied;
-    if (se3_gate<R>(icp_error, (R)s.cfg.icp_thresh, pose0, nf.pose)) return true;
+    if (se3_gate<R>(icp_error, (R)cfg.icp_thresh, pose0, nf.pose)) return true;
     std::memcpy(nf.pose, last_pose, sizeof(last_pose));
     return false;
 }
@@ -236,7 +236,7 @@ static int system_grab(OSystem<R>& s, const uint16_t* depth, const uint8_t* rgb,
         tracked = true;
     } else {
         std::memcpy(s.next->pose, s.last->pose, sizeof(s.next->pose));  // TrackFrame (:166)
-        tracked = compute_se3(s, res);
+        tracked = compute_se3(s.cfg, s.K, *s.next, *s.last, res);
         if (tracked) std::swap(s.next, s.last);
     }
     res->tracked = tracked ? 1 : 0;
@@ -269,6 +269,42 @@ static int system_grab(OSystem<R>& s, const uint16_t* depth, const uint8_t* rgb,
     res->allocated_blocks = s.map.heap_used;
     std::memcpy(res->pose, s.last->pose, sizeof(res->pose));
     return SICP_OK;
+}
+
+// One ComputeSE3 (Tracking.cc:369-517, point-to-plane only) on given
+// pyramids: the next frame's measured vertex / normal maps and the last
+// frame's model maps at three levels (float4, level l of cols/2^l x
+// rows/2^l), from the last frame's pose; arithmetic as `fp64` says. Writes
+// the resulting pose (the last pose when the frame is not tracked) and the
+// frame result; returns the tracked flag.
+template <class R>
+static int track_frame(const sicp_system_config& cfg, const float* const* nv,
+                       const float* const* nn, const float* const* lv, const float* const* ln,
+                       const double* last_pose, double* pose_out, sicp_frame_result* res) {
+    OFrame<R> nf, lf;
+    nf.create(cfg.cols, cfg.rows);
+    lf.create(cfg.cols, cfg.rows);
+    Intr<R> K[3];
+    for (int i = 0; i < 3; ++i) {
+        const float d = (float)(1 << i);  // Frame::SetK (Core/Frame.cc:580-588)
+        K[i] = Intr<R>{(R)(cfg.K.fx / d), (R)(cfg.K.fy / d), (R)(cfg.K.cx / d),
+                       (R)(cfg.K.cy / d)};
+        const size_t n = (size_t)nf.cols[i] * nf.rows[i] * 4;
+        for (size_t k = 0; k < n; ++k) {
+            nf.vmap[i][k] = (R)nv[i][k];
+            nf.nmap[i][k] = (R)nn[i][k];
+            lf.vmap[i][k] = (R)lv[i][k];
+            lf.nmap[i][k] = (R)ln[i][k];
+        }
+    }
+    std::memcpy(lf.pose, last_pose, sizeof(lf.pose));
+    std::memcpy(nf.pose, last_pose, sizeof(nf.pose));  // TrackFrame (Tracking.cc:166)
+    *res = sicp_frame_result{};
+    const bool ok = compute_se3<R>(cfg, K, nf, lf, res);
+    res->tracked = ok ? 1 : 0;
+    std::memcpy(pose_out, nf.pose, sizeof(nf.pose));
+    std::memcpy(res->pose, nf.pose, sizeof(nf.pose));
+    return ok ? 1 : 0;
 }
 
 }  // namespace oracle
@@ -376,6 +412,18 @@ int oracle_se3_control_trace(const int* iters3, const int* min_count3, const flo
     out4[3] = k;
     *out_error = err;
     return 0;
+}
+
+int oracle_track_frame(const sicp_system_config* cfg, int fp64, const float* const* next_vmap,
+                       const float* const* next_nmap, const float* const* last_vmap,
+                       const float* const* last_nmap, const double* last_pose, double* pose_out,
+                       sicp_frame_result* res) {
+    sicp_system_config c = *cfg;
+    c.icp_only = 1;
+    return fp64 ? track_frame<double>(c, next_vmap, next_nmap, last_vmap, last_nmap, last_pose,
+                                      pose_out, res)
+                : track_frame<float>(c, next_vmap, next_nmap, last_vmap, last_nmap, last_pose,
+                                     pose_out, res);
 }
 
 // The final gate of ComputeSE3 (Tracking.cc:505-516) on given poses.

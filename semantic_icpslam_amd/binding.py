@@ -51,7 +51,8 @@ class MapStats(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in (
         "allocated_blocks", "visible_blocks", "new_blocks", "dropped_heap",
         "dropped_requests", "dropped_range", "updated_voxels", "render_tiles", "ray_handover")] + [
-        (n, C.c_uint64) for n in ("updated_total", "visible_total", "fusions")]
+        (n, C.c_uint64) for n in ("updated_total", "visible_total", "fusions",
+                                  "heap_scanned_total")]
 
 
 class SystemConfig(C.Structure):

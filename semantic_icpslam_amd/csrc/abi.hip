@@ -653,6 +653,7 @@ int sicp_map_get_stats(sicp_map* map, sicp_map_stats* out, void* stream) {
     out->updated_total = c.updated_total;
     out->visible_total = c.visible_total;
     out->fusions = c.fusions;
+    out->heap_scanned_total = c.heap_total;
     return SICP_OK;
 }
 
@@ -730,7 +731,7 @@ struct SnapshotHeader {
     uint64_t table_entries, blocks, visible;
 };
 constexpr char kSnapMagic[8] = {'S', 'I', 'C', 'P', 'M', 'A', 'P', '\0'};
-constexpr uint32_t kSnapVersion = 1;
+constexpr uint32_t kSnapVersion = 2;  // 2: MapCounters with heap_total
 
 struct File {
     FILE* f;
@@ -746,11 +747,79 @@ template <class T> int put(FILE* f, const T* dev, size_t n, const char* what) {
     if (fwrite(h.data(), sizeof(T), n, f) != n) return invalid(what);
     return SICP_OK;
 }
-template <class T> int get(FILE* f, T* dev, size_t n, const char* what) {
-    if (!n) return SICP_OK;
-    std::vector<T> h(n);
-    if (fread(h.data(), sizeof(T), n, f) != n) return invalid(what);
-    SICP_HIP(hipMemcpy(dev, h.data(), n * sizeof(T), hipMemcpyHostToDevice));
+template <class T> int read_all(FILE* f, std::vector<T>& h, size_t n) {
+    h.resize(n);
+    if (n && fread(h.data(), sizeof(T), n, f) != n) return invalid("map_load: truncated file");
+    return SICP_OK;
+}
+template <class T> int upload(T* dev, const std::vector<T>& h) {
+    if (!h.empty())
+        SICP_HIP(hipMemcpy(dev, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return SICP_OK;
+}
+
+// The block key and bucket of the device hash (map_common.hpp pack_key /
+// bucket_of), on the host.
+uint64_t host_pack_key(int x, int y, int z) {
+    const uint64_t msk = (1ull << 21) - 1;
+    return (((uint64_t)(uint32_t)x & msk) << 42) | (((uint64_t)(uint32_t)y & msk) << 21) |
+           ((uint64_t)(uint32_t)z & msk);
+}
+uint32_t host_bucket(const sicp_block& b, uint32_t nb) {
+    const uint32_t h = ((uint32_t)b.x * 73856093u) ^ ((uint32_t)b.y * 19349669u) ^
+                       ((uint32_t)b.z * 83492791u);
+    return h % nb;
+}
+
+// Everything a loaded map's kernels index by: every entry's slot and chain
+// link inside the used heap, each overflow chain acyclic and owned by one
+// bucket, each entry's key that of the block record at its slot, the block
+// records at their own slots, the visible list a subset of the records, and
+// the counters equal to the section sizes.
+int validate_snapshot(const SnapshotHeader& h, const std::vector<sicp_hash_entry>& table,
+                      const std::vector<sicp_hash_entry>& excess,
+                      const std::vector<sicp_block>& blocks,
+                      const std::vector<sicp_block>& visible) {
+    const int64_t nb = (int64_t)h.blocks;
+    if (h.ctr.heap_used != h.blocks || h.ctr.visible != h.visible)
+        return invalid("map_load: counters disagree with the snapshot's sections");
+    auto in_heap = [&](int32_t v) { return v >= -1 && (int64_t)v < nb; };
+    const uint32_t nbk = (uint32_t)(table.size() / 2);
+    // an entry's key is its block record's, and the block hashes to `bucket`
+    auto key_ok = [&](const sicp_hash_entry& e, size_t bucket) {
+        if (e.slot < 0) return true;
+        const sicp_block& b = blocks[(size_t)e.slot];
+        return e.key == host_pack_key(b.x, b.y, b.z) && host_bucket(b, nbk) == bucket;
+    };
+    for (int64_t i = 0; i < nb; ++i)
+        if (blocks[(size_t)i].slot != (int32_t)i)
+            return invalid("map_load: a block record is not at its own heap slot");
+    for (const sicp_hash_entry& e : table)
+        if (!in_heap(e.slot) || !in_heap(e.next))
+            return invalid("map_load: a table entry points outside the heap");
+    for (const sicp_hash_entry& e : excess)
+        if (!in_heap(e.slot) || !in_heap(e.next))
+            return invalid("map_load: an overflow entry points outside the heap");
+    for (size_t i = 0; i < table.size(); ++i)
+        if (!key_ok(table[i], i / 2)) return invalid("map_load: a table entry's key is wrong");
+    // every chain from a bucket's second entry: each overflow entry visited
+    // at most once over all chains (no cycle, no shared tail)
+    std::vector<uint8_t> seen((size_t)nb, 0);
+    for (size_t b = 1; b < table.size(); b += 2) {
+        for (int32_t n = table[b].next; n >= 0; n = excess[(size_t)n].next) {
+            if (seen[(size_t)n]) return invalid("map_load: an overflow chain loops or is shared");
+            seen[(size_t)n] = 1;
+            if (!key_ok(excess[(size_t)n], b / 2))
+                return invalid("map_load: an overflow entry's key is wrong");
+        }
+    }
+    for (const sicp_block& v : visible) {
+        if (v.slot < 0 || (int64_t)v.slot >= nb)
+            return invalid("map_load: a visible block is outside the heap");
+        const sicp_block& b = blocks[(size_t)v.slot];
+        if (b.x != v.x || b.y != v.y || b.z != v.z)
+            return invalid("map_load: a visible block differs from its record");
+    }
     return SICP_OK;
 }
 }  // namespace
@@ -786,12 +855,12 @@ int sicp_map_save(sicp_map* map, const char* path) {
     return SICP_OK;
 }
 
-int sicp_map_load(sicp_map* map, const char* path) {
-    if (!map || !path) return invalid("map_load: null argument");
-    File file{fopen(path, "rb")};
-    if (!file.f) return invalid("map_load: cannot open the file");
+}  // extern "C"
+
+namespace {
+int map_load_file(sicp_map* map, FILE* f) {
     SnapshotHeader h;
-    if (fread(&h, sizeof(h), 1, file.f) != 1) return invalid("map_load: truncated header");
+    if (fread(&h, sizeof(h), 1, f) != 1) return invalid("map_load: truncated header");
     if (std::memcmp(h.magic, kSnapMagic, sizeof(h.magic)) != 0)
         return invalid("map_load: not a map snapshot");
     if (h.version != kSnapVersion || h.header_bytes != sizeof(SnapshotHeader))
@@ -806,20 +875,51 @@ int sicp_map_load(sicp_map* map, const char* path) {
     if (h.blocks > (uint64_t)a.num_blocks || h.visible > h.blocks ||
         h.table_entries != 2 * (uint64_t)a.num_buckets)
         return invalid("map_load: the snapshot does not fit this map");
+    // the whole file into host memory and checked before any upload
+    const size_t nb = h.blocks;
+    std::vector<sicp_hash_entry> table, excess;
+    std::vector<sicp_block> blocks, visible;
+    std::vector<uint32_t> geom_h;
+    std::vector<sicp_voxel_geom> geom;
+    std::vector<sicp_voxel_sem> sem;
+    int rc;
+    if ((rc = read_all(f, table, h.table_entries)) || (rc = read_all(f, excess, nb)) ||
+        (rc = read_all(f, blocks, nb)) ||
+        (map->dev.half ? (rc = read_all(f, geom_h, nb * kBlockVoxels))
+                       : (rc = read_all(f, geom, nb * kBlockVoxels))) ||
+        (rc = read_all(f, sem, nb * kBlockVoxels)) || (rc = read_all(f, visible, h.visible)))
+        return rc;
+    if (fgetc(f) != EOF) return invalid("map_load: trailing bytes after the snapshot");
+    if ((rc = validate_snapshot(h, table, excess, blocks, visible))) return rc;
+    if ((rc = upload(map->dev.table, table)) || (rc = upload(map->dev.excess, excess)) ||
+        (rc = upload(map->dev.blocks, blocks)) ||
+        (map->dev.half ? (rc = upload(map->dev.geom_h, geom_h))
+                       : (rc = upload(map->dev.geom, geom))) ||
+        (rc = upload(map->dev.sem, sem)) || (rc = upload(map->visible, visible)))
+        return rc;
+    SICP_HIP(hipMemcpy(map->ctr, &h.ctr, sizeof(h.ctr), hipMemcpyHostToDevice));
+    return SICP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+// The map is reset first and left reset when any check or copy fails.
+int sicp_map_load(sicp_map* map, const char* path) {
+    if (!map || !path) return invalid("map_load: null argument");
+    File file{fopen(path, "rb")};
+    if (!file.f) return invalid("map_load: cannot open the file");
     int rc = launch_map_reset(map->dev, map->ctr, map->alloc, nullptr);
     if (rc) return rc;
     SICP_HIP(hipDeviceSynchronize());
-    const size_t nb = h.blocks;
-    if ((rc = get(file.f, map->dev.table, h.table_entries, "map_load: truncated file")) ||
-        (rc = get(file.f, map->dev.excess, nb, "map_load: truncated file")) ||
-        (rc = get(file.f, map->dev.blocks, nb, "map_load: truncated file")) ||
-        (map->dev.half
-             ? (rc = get(file.f, map->dev.geom_h, nb * kBlockVoxels, "map_load: truncated file"))
-             : (rc = get(file.f, map->dev.geom, nb * kBlockVoxels, "map_load: truncated file"))) ||
-        (rc = get(file.f, map->dev.sem, nb * kBlockVoxels, "map_load: truncated file")) ||
-        (rc = get(file.f, map->visible, (size_t)h.visible, "map_load: truncated file")))
+    rc = map_load_file(map, file.f);
+    if (rc) {
+        const std::string msg = sicp_last_error();
+        if (launch_map_reset(map->dev, map->ctr, map->alloc, nullptr) == SICP_OK)
+            (void)hipDeviceSynchronize();
+        set_error(msg);  // keep the load's own message
         return rc;
-    SICP_HIP(hipMemcpy(map->ctr, &h.ctr, sizeof(h.ctr), hipMemcpyHostToDevice));
+    }
     return SICP_OK;
 }
 

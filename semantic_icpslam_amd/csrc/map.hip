@@ -862,6 +862,7 @@ __global__ __launch_bounds__(1024) void fuse_end_kernel(MapCounters* ctr,
         ctr->updated_voxels = t;
         ctr->updated_total += t;
         ctr->visible_total += ctr->visible;
+        ctr->heap_total += ctr->heap_used;
         ctr->fusions += 1ull;
     }
 }

@@ -85,7 +85,8 @@ __global__ __launch_bounds__(256) void prepass_kernel(MapDev m, MapCounters* ctr
                                                       const uint32_t* fuse_counts) {
     if (fuse_counts && !(gate && *gate == 0)) {
         // the fusion's sums (fuse_end_kernel's, updated_voxels reset before
-        // the fusion): 256 per-wavefront counts per workgroup, integer sums
+        // the fusion): 256 per-wavefront counts per one-wavefront workgroup
+        // (lane t adds entries t, t + 64, t + 128, t + 192), integer sums
         uint32_t c = 0;
         for (int i = blockIdx.x * 256 + threadIdx.x; i < kFuseBlocks * 4; i += gridDim.x * 256)
 #pragma unroll
@@ -98,6 +99,7 @@ __global__ __launch_bounds__(256) void prepass_kernel(MapDev m, MapCounters* ctr
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             ctr->visible_total += ctr->visible;
+            ctr->heap_total += ctr->heap_used;
             ctr->fusions += 1ull;
         }
     }

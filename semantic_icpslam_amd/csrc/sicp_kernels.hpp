@@ -208,6 +208,7 @@ struct MapCounters {
     unsigned long long updated_total;  // voxels written, summed since reset
     unsigned long long visible_total;  // visible blocks, summed since reset
     unsigned long long fusions;        // fusion passes since reset
+    unsigned long long heap_total;     // allocated blocks at each fusion, summed since reset
 };
 
 struct AllocScratch {

@@ -47,7 +47,7 @@ def _compare_maps(gm, om):
     gs, os_ = gm.stats(), om.stats()
     n = os_["allocated_blocks"]
     assert gs["allocated_blocks"] == n
-    g = gm.download(n_blocks=0)
+    g = gm.download(n_blocks=n)
     o = om.export()
     assert np.array_equal(g["table"], o["table"]), "hash table differs"
     assert np.array_equal(g["excess"][:n], o["excess"][:n]), "overflow chains differ"
@@ -89,6 +89,24 @@ def test_fusion_vga_bit_exact(dev, half):
     _fuse_both(dev, c, [0, 3], gm, om)
     n = _compare_maps(gm, om)
     assert n > 2000
+
+
+@pytest.mark.parametrize("frames", [30, 300])
+def test_fusion_vga_sequence_bit_exact(dev, frames):
+    """The 640x480 / 5 mm sequence fused frame after frame at its ground-truth
+    poses, with labels: after 30 frames (label votes accumulated over every
+    frame) and after the whole 300-frame sequence, the table, the overflow
+    chains, the block list, both voxel planes with every label and count, and
+    the last visible list equal the oracle's."""
+    c = CFG_VGA
+    cfg = map_cfg(c, num_blocks=60000)
+    gm, om = B.Map(cfg), O.OracleMap(cfg)
+    _fuse_both(dev, c, range(frames), gm, om)
+    n = _compare_maps(gm, om)
+    assert n > (7000 if frames == 30 else 20000)
+    sem = om.export()["sem"][: n * 512]
+    assert sem["count"].max() >= min(frames, 255) // 2  # votes accumulated over the frames
+    assert (sem["label2"] > 0).sum() > 0
 
 
 def test_fusion_overflow_chains_and_heap_exhaustion(dev):
@@ -212,3 +230,63 @@ def test_map_snapshot_round_trip(dev, tmp_path, half):
     assert np.array_equal(a["sem"][: na * 512], b["sem"][: na * 512])
     other = B.Map(map_cfg(dict(c, voxel=0.02)))
     assert B.lib().sicp_map_load(other.h, path) == B.SICP_ERR_INVALID_ARGUMENT
+
+
+def test_map_load_rejects_corrupt_snapshots(dev, tmp_path):
+    """sicp_map_load checks a snapshot on the host before uploading it: a
+    truncated file, an entry whose heap slot lies past the used heap, an
+    overflow chain that loops and a counter that disagrees with the sections
+    are refused with SICP_ERR_INVALID_ARGUMENT, and the map is left reset."""
+    c = CFG_SMALL
+    cfg = map_cfg(c, num_buckets=1 << 8, num_blocks=4000)  # small table: overflow chains
+    gm, om = B.Map(cfg), O.OracleMap(cfg)
+    _fuse_both(dev, c, [0, 3], gm, om)
+    path = tmp_path / "map.bin"
+    B.check(B.lib().sicp_map_save(gm.h, str(path).encode()))
+    raw = bytearray(path.read_bytes())
+    hdr = int(np.frombuffer(raw[12:16], np.uint32)[0])  # header_bytes
+    n = gm.stats()["allocated_blocks"]
+    nb2 = 2 * cfg.num_buckets
+    table = np.frombuffer(raw[hdr: hdr + 16 * nb2], B.HASH_ENTRY)
+    excess_off = hdr + 16 * nb2
+    excess = np.frombuffer(raw[excess_off: excess_off + 16 * n], B.HASH_ENTRY)
+    # a good copy loads
+    good = B.Map(cfg)
+    B.check(B.lib().sicp_map_load(good.h, str(path).encode()))
+    assert good.stats()["allocated_blocks"] == n
+
+    def refused(data, name):
+        p = tmp_path / name
+        p.write_bytes(bytes(data))
+        m = B.Map(cfg)
+        rc = B.lib().sicp_map_load(m.h, str(p).encode())
+        st = m.stats()
+        assert rc == B.SICP_ERR_INVALID_ARGUMENT, (name, rc)
+        assert st["allocated_blocks"] == 0 and st["visible_blocks"] == 0, name
+        assert m.download(n_blocks=4, voxels=False)["table"]["slot"].max() == -1, name
+
+    refused(raw[: len(raw) - 100], "truncated.bin")
+    refused(raw + b"x", "trailing.bin")
+    # an entry pointing past the used heap
+    used = np.nonzero(table["slot"] >= 0)[0][0]
+    bad = bytearray(raw)
+    off = hdr + 16 * used + 8
+    bad[off: off + 4] = np.int32(n + 5).tobytes()
+    refused(bad, "slot.bin")
+    # an overflow chain that loops back on itself
+    head = table["next"][1::2]
+    chained = np.nonzero(head >= 0)[0]
+    assert len(chained) > 0
+    e = int(head[chained[0]])
+    bad = bytearray(raw)
+    off = excess_off + 16 * e + 12
+    bad[off: off + 4] = np.int32(e).tobytes()
+    refused(bad, "loop.bin")
+    # heap_used in the header's counters disagrees with the block section
+    bad = bytearray(raw)
+    # magic, version, header_bytes, the map config, then the counters
+    # (8-byte aligned; heap_used first)
+    ctr_off = (16 + B.C.sizeof(B.MapConfig) + 7) // 8 * 8
+    bad[ctr_off: ctr_off + 4] = np.uint32(n + 1).tobytes()
+    refused(bad, "counter.bin")
+    assert excess["slot"][e] == e  # the entry corrupted above was a real chain link
