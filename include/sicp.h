@@ -263,6 +263,22 @@ int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_ima
                   void* partials_dev, float* sums_dev, float* host_sums,
                   double* A_host, double* b_host, float* residual_host, void* stream);
 size_t sicp_icp_partials_size(void);
+/* ICPStep with no workspace argument and no allocation: the same kernels,
+ * arguments (less `partials_dev`) and results as sicp_icp_step, with the
+ * workgroups' partial rows in a 116 KiB array the library holds on each GPU
+ * for the life of the process. No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * serving loops, harnesses that forbid hipMalloc). Every image size up to
+ * 2^30 pixels fits; larger ones are rejected (SICP_ERR_INVALID_ARGUMENT).
+ * The call makes no hipMalloc / hipFree and no allocating library call; the
+ * array is found once per device (hipGetSymbolAddress) on the first call
+ * there. All calls on one GPU share its array: calls that overlap in time on
+ * one GPU (other streams, other threads) race and must be serialised by the
+ * caller. */
+int sicp_icp_step_static(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nlast,
+                         const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
+                         float* sums_dev, float* host_sums, double* A_host, double* b_host,
+                         float* residual_host, void* stream);
 /* ICPStep in the precision form: the same 29 sums, arguments and outputs as
  * sicp_icp_step, accumulated in FP32 with compensated summation instead of
  * FP64: every addition (a thread adding a product, partial sums meeting in

@@ -126,6 +126,8 @@ _SIGS = {
     "sicp_mesh_tables": (C.c_int, [_P, _P, _P]),
     "sicp_mesh_write_ply": (C.c_int, [C.c_char_p, _P, _P, _P, C.c_uint32]),
     "sicp_icp_partials_size": (C.c_size_t, []),
+    "sicp_icp_step_static": (C.c_int, [Image, Image, Image, Image, C.POINTER(PoseF),
+                                       C.POINTER(PoseF), Intrinsics, _P, _P, _P, _P, _P, _P]),
     "sicp_se3_update_host": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "sicp_map_config_default": (None, [C.POINTER(MapConfig), C.c_float, C.c_float, C.c_int32,
                                        C.c_int32]),

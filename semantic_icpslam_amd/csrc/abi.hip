@@ -273,7 +273,10 @@ int sicp_resize_map(sicp_image vsrc, sicp_image nsrc, sicp_image vdst, sicp_imag
 // -------------------------------------------------------------------- ICP --
 size_t sicp_icp_partials_size(void) { return sizeof(double) * kIcpMaxBlocks * kIcpSums + 256; }
 
-static int icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nlast,
+}  // extern "C"
+
+namespace sicp {
+int icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nlast,
                     const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
                     void* partials_dev, float* sums_dev, float* host_sums, double* A_host,
                     double* b_host, float* residual_host, bool k32, void* stream) {
@@ -304,6 +307,9 @@ static int icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_i
     }
     return SICP_OK;
 }
+}  // namespace sicp
+
+extern "C" {
 
 int sicp_icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nlast,
                   const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,

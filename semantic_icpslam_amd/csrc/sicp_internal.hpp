@@ -30,6 +30,16 @@ int launch_raycast_gated(MapDev m, View<float4> vmap, View<float4> nmap, View<fl
                          float cy, const int* gate, const uint32_t* nonempty, hipStream_t s,
                          bool queue_cleared = false);
 
+// ICPStep over the four maps with `partials_dev` as the workgroups' partial
+// rows (sicp_icp_step and its variants).
+int icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nlast,
+             const sicp_pose_f* curr, const sicp_pose_f* last, sicp_intrinsics K,
+             void* partials_dev, float* sums_dev, float* host_sums, double* A_host,
+             double* b_host, float* residual_host, bool k32, void* stream);
+// The largest image sicp_icp_step_static takes (the kernels index pixels
+// with 32-bit integers).
+constexpr int64_t kIcpStaticMaxPixels = (int64_t)1 << 30;
+
 // A pitched device image owned by the library.
 template <class T> struct DevImage {
     void* ptr = nullptr;
