@@ -20,10 +20,9 @@
 namespace sicp {
 
 // ------------------------------------------------------------- tables -----
-// Corner k of a voxel cube and edge j (its two corners), in the reference's
-// numbering (MeshScene.cu:200-291).
-static const int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
-                                  {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+// Edge j of a voxel cube (its two corners), in the reference's numbering
+// (MeshScene.cu:200-291); corner k sits at (x, y, z) = (0,0,0), (1,0,0),
+// (1,1,0), (0,1,0), then the same four at z = 1.
 static const int kEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
                                  {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
 
@@ -33,146 +32,28 @@ struct MeshTables {
     int32_t nvert[256];     // 3 x triangles
 };
 
-// The triangulation of every sign case, generated from two rules:
-//  * on each cube face, walking its corners counter-clockwise as seen from
-//    outside the cube, every run of inside corners (sdf < 0) is cut off by
-//    one segment from the edge where the walk enters the run to the edge
-//    where it leaves it; a face with two diagonal inside corners therefore
-//    separates them, a rule that depends only on the face's own corners, so
-//    two cubes sharing a face cut it the same way (the surface is closed);
-//  * the directed segments chain into loops (every cut edge starts one
-//    segment and ends one), and each loop is fanned from the first vertex
-//    whose diagonals stay off the cube's faces, so no triangle edge lies in
-//    a face where the neighbouring cube could produce it too.
-// Every triangle is oriented with its normal towards the outside
-// (sdf > 0). The edge table is the set of cut edges. The reference uses the
-// published table of the same name (Mapping/Constant.h); the cut edges,
-// vertex positions, normals and colours are the same for any table, the
-// triangle connectivity is this table's.
+// The triangle table is the reference's (the classic published
+// marching-cubes table, Mapping/Constant.h:39-296), so the triangles of a
+// voxel, and the corner order within each, are the reference's; it is held
+// as one hexadecimal string of edge indices per sign case (mc_table.inc,
+// written by gen_mc_table.py). The edge table (the cut edges: corners of
+// different sign) and the vertex counts follow from it.
+#include "mc_table.inc"
+
 static void build_tables(MeshTables& t) {
-    static const int face[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4},
-                                   {3, 2, 6, 7}, {0, 3, 7, 4}, {1, 2, 6, 5}};
-    auto edge_of = [](int a, int b) {
-        for (int j = 0; j < 12; ++j)
-            if ((kEdge[j][0] == a && kEdge[j][1] == b) || (kEdge[j][0] == b && kEdge[j][1] == a))
-                return j;
-        return -1;
-    };
-    // Faces counter-clockwise from outside: (c1 - c0) x (c2 - c1) points out.
-    int cyc[6][4];
-    for (int f = 0; f < 6; ++f) {
-        const int* c = face[f];
-        double p[4][3];
-        for (int i = 0; i < 4; ++i)
-            for (int a = 0; a < 3; ++a) p[i][a] = kCorner[c[i]][a];
-        double u[3], v[3], n[3], out[3];
-        for (int a = 0; a < 3; ++a) {
-            u[a] = p[1][a] - p[0][a];
-            v[a] = p[2][a] - p[1][a];
-            out[a] = 0.25 * (p[0][a] + p[1][a] + p[2][a] + p[3][a]) - 0.5;
-        }
-        n[0] = u[1] * v[2] - u[2] * v[1];
-        n[1] = u[2] * v[0] - u[0] * v[2];
-        n[2] = u[0] * v[1] - u[1] * v[0];
-        const bool ccw = n[0] * out[0] + n[1] * out[1] + n[2] * out[2] > 0;
-        for (int i = 0; i < 4; ++i) cyc[f][i] = ccw ? c[i] : c[3 - i];
-    }
-    bool on_face[12][6];
-    for (int j = 0; j < 12; ++j)
-        for (int f = 0; f < 6; ++f) {
-            bool a = false, b = false;
-            for (int i = 0; i < 4; ++i) {
-                a = a || cyc[f][i] == kEdge[j][0];
-                b = b || cyc[f][i] == kEdge[j][1];
-            }
-            on_face[j][f] = a && b;
-        }
-    auto share_face = [&](int e1, int e2) {
-        for (int f = 0; f < 6; ++f)
-            if (on_face[e1][f] && on_face[e2][f]) return true;
-        return false;
-    };
     for (int cs = 0; cs < 256; ++cs) {
-        auto inside = [&](int k) { return (cs >> k) & 1; };
-        int next[12];
-        for (int j = 0; j < 12; ++j) next[j] = -1;
         int emask = 0;
-        for (int f = 0; f < 6; ++f) {
-            int cut_e[4], cut_entry[4], nc = 0;
-            for (int i = 0; i < 4; ++i) {
-                const int a = cyc[f][i], b = cyc[f][(i + 1) & 3];
-                if (inside(a) != inside(b)) {
-                    cut_e[nc] = edge_of(a, b);
-                    cut_entry[nc] = inside(b);
-                    ++nc;
-                }
-            }
-            for (int i = 0; i < nc; ++i) {
-                emask |= 1 << cut_e[i];
-                if (!cut_entry[i]) continue;
-                for (int s = 1; s <= nc; ++s) {
-                    const int k = (i + s) % nc;
-                    if (!cut_entry[k]) {
-                        next[cut_e[i]] = cut_e[k];
-                        break;
-                    }
-                }
-            }
-        }
+        for (int j = 0; j < 12; ++j)
+            if (((cs >> kEdge[j][0]) & 1) != ((cs >> kEdge[j][1]) & 1)) emask |= 1 << j;
         t.edge[cs] = emask;
-        int nt = 0;
-        for (int j = 0; j < 16; ++j) t.tri[cs][j] = -1;
-        bool seen[12] = {false};
-        for (int s0 = 0; s0 < 12; ++s0) {
-            if (next[s0] < 0 || seen[s0]) continue;
-            int loop[12], m = 0;
-            for (int v = s0; !seen[v]; v = next[v]) {
-                seen[v] = true;
-                loop[m++] = v;
-            }
-            int apex = 0;
-            for (int k = 0; k < m; ++k) {
-                bool ok = true;
-                for (int j = 0; j < m && ok; ++j) {
-                    if (j == k || j == (k + 1) % m || j == (k + m - 1) % m) continue;
-                    ok = !share_face(loop[k], loop[j]);
-                }
-                if (ok) {
-                    apex = k;
-                    break;
-                }
-            }
-            for (int i = 1; i + 1 < m; ++i) {
-                t.tri[cs][3 * nt + 0] = (int8_t)loop[apex];
-                t.tri[cs][3 * nt + 1] = (int8_t)loop[(apex + i) % m];
-                t.tri[cs][3 * nt + 2] = (int8_t)loop[(apex + i + 1) % m];
-                ++nt;
-            }
+        const char* row = kMcTriangles[cs];
+        const int n = (int)strlen(row);
+        for (int j = 0; j < 16; ++j) {
+            const char ch = j < n ? row[j] : 0;
+            t.tri[cs][j] = j < n ? (int8_t)(ch <= '9' ? ch - '0' : ch - 'a' + 10) : (int8_t)-1;
         }
-        t.nvert[cs] = 3 * nt;
+        t.nvert[cs] = n;
     }
-    // Orientation: case 1 (corner 0 inside) must face away from corner 0.
-    auto mid = [](int e, double* p) {
-        for (int a = 0; a < 3; ++a) p[a] = 0.5 * (kCorner[kEdge[e][0]][a] + kCorner[kEdge[e][1]][a]);
-    };
-    double p0[3], p1[3], p2[3];
-    mid(t.tri[1][0], p0);
-    mid(t.tri[1][1], p1);
-    mid(t.tri[1][2], p2);
-    double u[3], v[3];
-    for (int a = 0; a < 3; ++a) {
-        u[a] = p1[a] - p0[a];
-        v[a] = p2[a] - p0[a];
-    }
-    const double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2],
-                         u[0] * v[1] - u[1] * v[0]};
-    if (n[0] * p0[0] + n[1] * p0[1] + n[2] * p0[2] < 0)
-        for (int cs = 0; cs < 256; ++cs)
-            for (int i = 0; i < t.nvert[cs]; i += 3) {
-                const int8_t x = t.tri[cs][i + 1];
-                t.tri[cs][i + 1] = t.tri[cs][i + 2];
-                t.tri[cs][i + 2] = x;
-            }
 }
 
 static const MeshTables& tables() {
@@ -231,7 +112,7 @@ struct Cell {
     int cube;  // sign case, 0 when the voxel makes no triangles
 };
 
-// kCorner on the device, as closed forms of the corner number.
+// The corner coordinates on the device, as closed forms of the corner number.
 SICP_DEV int d_corner(int k, int axis) {
     return axis == 0 ? ((k == 1 || k == 2 || k == 5 || k == 6) ? 1 : 0)
                      : (axis == 1 ? ((k == 2 || k == 3 || k == 6 || k == 7) ? 1 : 0) : (k >> 2));

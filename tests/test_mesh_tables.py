@@ -1,16 +1,18 @@
 """The marching-cubes tables of sicp_mesh_scene and the PLY writer, on the
 CPU (host functions of the library; no GPU call).
 
-The reference takes its tables as arguments (Mapping/MeshScene.cu:308-318);
-this build generates its own (face rule + fan, see mesh.hip). They are
-pinned here by what any correct table must satisfy:
+The triangle table is the reference's (the classic published table,
+Mapping/Constant.h:39-296, carried over as mc_table.inc); the edge table and
+the vertex counts are derived. Pinned here by:
+  * rows of the published table (Lorensen and Cline's cases as Bourke lists
+    them): cases 1, 3, 5, 6, 7 and their complements' triangle counts;
   * the edge table is exactly the set of edges whose corners differ in sign;
   * every triangle uses cut edges only, and every cut edge is used;
   * over random sign fields the triangles form a closed surface in which
-    every mesh edge is used once in each direction (watertight and
-    consistently oriented), also across the ambiguous faces;
-  * a sphere meshes to a closed surface of Euler characteristic 2 whose
-    triangles face outwards (positive signed volume, normal towards sdf > 0).
+    every mesh edge is used once in each direction;
+  * a sphere meshes to a closed surface of Euler characteristic 2 wound
+    with its normals towards the inside (sdf < 0), the classic table's
+    winding (negative signed volume).
 """
 import itertools
 import os
@@ -86,7 +88,7 @@ def test_random_fields_give_closed_oriented_surfaces(tables):
         assert tris and _closed_and_oriented(tris)
 
 
-def test_sphere_is_a_closed_outward_surface(tables):
+def test_sphere_is_a_closed_inward_wound_surface(tables):
     n = 14
     g = np.arange(n) - (n - 1) / 2
     zz, yy, xx = np.meshgrid(g, g, g, indexing="ij")
@@ -104,7 +106,18 @@ def test_sphere_is_a_closed_outward_surface(tables):
         a = v0 / (v0 - v1)
         pos[k] = np.array([x0 + a * (x1 - x0), y0 + a * (y1 - y0), z0 + a * (z1 - z0)]) - (n - 1) / 2
     vol = sum(np.dot(pos[t[0]], np.cross(pos[t[1]], pos[t[2]])) for t in tris) / 6
-    assert abs(vol - 4 / 3 * np.pi * 4.3 ** 3) < 0.1 * 4 / 3 * np.pi * 4.3 ** 3
+    assert abs(vol + 4 / 3 * np.pi * 4.3 ** 3) < 0.1 * 4 / 3 * np.pi * 4.3 ** 3
+
+
+def test_rows_of_the_published_table(tables):
+    edge, tri, nvert = tables
+    want = {1: [0, 8, 3], 3: [1, 8, 3, 9, 8, 1], 5: [0, 8, 3, 1, 2, 10], 6: [9, 2, 10, 0, 2, 9],
+            7: [2, 8, 3, 2, 10, 8, 10, 9, 8], 8: [3, 11, 2], 15: [9, 8, 10, 10, 8, 11]}
+    for case, row in want.items():
+        assert list(tri[case][: nvert[case]]) == row, case
+    # a case and its complement cut the same edges
+    for case in range(256):
+        assert edge[case] == edge[255 - case]
 
 
 def test_ply_writer(tmp_path):
