@@ -396,11 +396,12 @@ BATCHED = (
 
 
 def run_batched(args):
-    """Eight independent sequences, each with its own system, every frame one
-    hipGraph launch; one round advances every sequence by a frame. Three
-    passes over the same frames on fresh systems: one stream per sequence
-    (the value), all on one stream (for comparison), and one stream per
-    sequence synchronised after every round (round latency)."""
+    """Eight independent sequences, each with its own system; one round
+    advances every sequence by a frame. Passes over the same frames on fresh
+    systems: the batch (every tracking iteration of all eight sequences in one
+    launch, a round as one hipGraph launch) is the value; for comparison one
+    stream per sequence with one hipGraph per frame, and all on one stream;
+    and the batch synchronised after every round (round latency)."""
     import torch
     from semantic_icpslam_amd import binding as B
     nfr = args.warmup + args.steps
@@ -414,19 +415,27 @@ def run_batched(args):
                                       for f in frames]))
     torch.cuda.synchronize()
 
-    def run(concurrent, sync_rounds):
-        """Fresh systems; warm-up rounds, then the timed rounds. Concurrent:
-        one stream per sequence; otherwise all on one stream."""
+    def run(mode, sync_rounds=False, tuning=None, autotune=False):
+        """Fresh systems (raycast launch shapes from `tuning`); warm-up rounds,
+        then the timed rounds. With `autotune`, the warm-up ends with every
+        system's autotuner and the shapes it chose are returned instead."""
         systems, streams = [], []
         shared = torch.cuda.Stream()
-        for q in seqs:
+        for k, q in enumerate(seqs):
             w = q["w"]
             systems.append(B.System(B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"],
                                                     w["cy"], w["voxel"], w["trunc"], w["buckets"],
                                                     w["blocks"])))
-            streams.append(torch.cuda.Stream() if concurrent else shared)
+            if tuning:
+                systems[-1].set_tuning(tuning[k]["ray_first_pass_samples"],
+                                       tuning[k]["ray_second_pass_workgroups"])
+            streams.append(torch.cuda.Stream() if mode == "streams" else shared)
+        batch = B.Batch(systems) if mode == "batch" else None
 
         def round_(i):
+            if batch:
+                batch.grab_staged([q["frames"][i] for q in seqs], True, shared)
+                return
             for q, g, st in zip(seqs, systems, streams):
                 d, c, l = q["frames"][i]
                 g.grab_staged(d, c, l, True, st)
@@ -434,6 +443,18 @@ def run_batched(args):
         for i in range(args.warmup):
             round_(i)
         torch.cuda.synchronize()
+        if autotune:
+            chosen = []
+            for g in systems:
+                try:
+                    chosen.append(g.autotune(repeats=5, stream=shared))
+                except B.SicpError as e:  # last warm-up frame lost: keep the defaults
+                    chosen.append(dict(g.tuning(), note=str(e)))
+            if batch:
+                batch.close()
+            for g in systems:
+                g.close()
+            return chosen
         lat = []
         t0 = time.perf_counter()
         for i in range(args.warmup, nfr):
@@ -445,20 +466,27 @@ def run_batched(args):
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         tracked = [g.result(st)["tracked"] for g, st in zip(systems, streams)]
+        if batch:
+            batch.close()
         for g in systems:
             g.close()
         return elapsed, lat, tracked
 
-    log("[bench] batched: concurrent streams")
-    thr_s, _, tracked = run(True, False)
-    log("[bench] batched: one stream")
-    ser_s, _, _ = run(False, False)
-    log("[bench] batched: concurrent, synchronised per round")
-    _, lat, _ = run(True, True)
+    log("[bench] batched: autotuning each sequence's raycast launch shape after the warm-up")
+    tuning = run("batch", autotune=True)
+    log("[bench] batched: one launch per iteration for all sequences, a round per graph")
+    thr_s, _, tracked = run("batch", tuning=tuning)
+    log("[bench] batched: one stream per sequence, one graph per frame")
+    str_s, _, _ = run("streams", tuning=tuning)
+    log("[bench] batched: one stream, one graph per frame")
+    ser_s, _, _ = run("serial", tuning=tuning)
+    log("[bench] batched: the batch synchronised per round")
+    _, lat, _ = run("batch", True, tuning=tuning)
     frames_done = args.steps * len(seqs)
     out = {
         "metric": "frames/s, 8 independent sequences (3x640x480 @5 mm, 3x320x240 @8 mm, "
-                  "2x1280x720 @4 mm), one hipGraph per frame",
+                  "2x1280x720 @4 mm), each Gauss-Newton iteration of all eight in one launch, "
+                  "a round of eight frames as one hipGraph",
         "value": round(frames_done / thr_s, 1),
         "unit": "frames/s",
         "n_gpus": 1,
@@ -471,12 +499,19 @@ def run_batched(args):
         "dtype": "f32",
         "data": "synthetic (seeded procedural rooms, seeds 0-7)",
         "config": {"workload": "batched8", "sequences": len(seqs), "frames_per_sequence": nfr},
+        "streams_frames_per_s": round(frames_done / str_s, 1),
+        "streams_what": "one stream per sequence, one hipGraph per frame",
         "one_stream_frames_per_s": round(frames_done / ser_s, 1),
         "round_ms_p50": round(float(np.percentile(lat, 50)), 4),
         "round_ms_p99": round(float(np.percentile(lat, 99)), 4),
         "round_what": "one round = every sequence advances one frame (step); p50/p99 from a "
-                      "pass synchronised after every round",
+                      "pass of the batch synchronised after every round",
         "last_frame_tracked": tracked,
+        "autotune": [dict(sequence=f"{q['w']['cols']}x{q['w']['rows']}@{q['w']['voxel'] * 1000:g}mm",
+                          **t) for q, t in zip(seqs, tuning)],
+        "autotune_what": "sicp_system_autotune after the warm-up rounds: the raycast's first-pass "
+                         "samples and second-pass workgroups with the lowest median raycast time "
+                         "(raycast_us; previous_us = defaults 48/1024); applied to every pass",
     }
     print(json.dumps(out), flush=True)
 

@@ -477,6 +477,54 @@ int sicp_system_staging(sicp_system* sys, sicp_image* depth_u16, sicp_image* rgb
                         sicp_image* labels_u8);
 int sicp_system_grab_staged(sicp_system* sys, int num_detection, int use_graph, void* stream);
 
+/* GrabImage of several independent systems (sequences of any resolutions)
+ * as one batch, on one stream (no counterpart in the reference, whose System
+ * serves one camera). Each system's pyramid, then the tracking of every
+ * system past its first frame in shared launches — one per Gauss-Newton
+ * iteration, each system on its own range of workgroups with exactly the
+ * arithmetic of its own launches — then each system's fusion and raycast.
+ * Results equal n separate sicp_system_grab_image calls. At most 8 systems,
+ * each distinct, point-to-plane tracking (icp_only = 1) with one form of the
+ * sums and the same iteration counts; labels_u8 and num_detection may be NULL
+ * (no labels). Stage timing is not recorded for batched frames. */
+int sicp_system_grab_batch(sicp_system* const* systems, int32_t n, const sicp_image* depth_u16,
+                           const sicp_image* rgb_u8x3, const sicp_image* labels_u8,
+                           const int32_t* num_detection, void* stream);
+/* The graph form of batches: a batch holds the systems (not owned) and, once
+ * every system is past its first frame, runs a round — one frame of every
+ * system, from each system's staged inputs (sicp_system_staging) — as one
+ * hipGraph launch (two graphs, one per frame-buffer parity pattern, recorded
+ * on first use and again when the patterns or num_detection change). Results
+ * equal the eager rounds. */
+typedef struct sicp_batch sicp_batch;
+int sicp_batch_create(sicp_system* const* systems, int32_t n, sicp_batch** out);
+int sicp_batch_destroy(sicp_batch* batch);
+int sicp_batch_grab_staged(sicp_batch* batch, const int32_t* num_detection, int use_graph,
+                           void* stream);
+
+/* Launch shape of the system's raycast (Mapping::RayTrace, Mapping.cc:142-161;
+ * the reference's RayCastKernel launch is fixed at 4 x 8 threads per block,
+ * RenderScene.cu:528-533): the samples one
+ * first-pass lane marches before handing its ray to the second pass, and the
+ * second pass's 256-thread workgroups. Every valid choice gives bit-identical
+ * model maps. Defaults 48 and 1024; set_tuning takes 1..4096 and 1..65536.
+ * A change re-records the system's (and its batches') graphs on next use. */
+typedef struct sicp_tuning {
+    int32_t ray_first_pass_samples;
+    int32_t ray_second_pass_workgroups;
+} sicp_tuning;
+int sicp_system_get_tuning(const sicp_system* sys, sicp_tuning* out);
+int sicp_system_set_tuning(sicp_system* sys, const sicp_tuning* tuning);
+/* The autotuner (no counterpart in the reference): re-runs the last frame's
+ * raycast into scratch maps under each candidate shape (the tuning in effect,
+ * then 16..128 samples x 256..2048 workgroups), `repeats` (1..64) interleaved
+ * rounds, keeps the lowest median and reports it with the previous tuning's
+ * median (microseconds; either pointer may be NULL). The frame's own maps
+ * are untouched. Needs a processed frame that was tracked and rendered;
+ * synchronises `stream`; allocates scratch for the call. */
+int sicp_system_autotune(sicp_system* sys, int32_t repeats, sicp_tuning* chosen,
+                         float* chosen_us, float* previous_us, void* stream);
+
 /* Frame buffers of the system, for inspection (the public members of Frame,
  * Core/Frame.h): which = 0 vmap, 1 nmap, 2 raw depth (level 0 only),
  * 3 filtered depth, 4 intensity, 5 d/dx, 6 d/dy at pyramid `level`, of the

@@ -55,6 +55,11 @@ class MapStats(C.Structure):
                                   "heap_scanned_total")]
 
 
+class Tuning(C.Structure):
+    """sicp_tuning: the raycast's launch shape."""
+    _fields_ = [("ray_first_pass_samples", C.c_int32), ("ray_second_pass_workgroups", C.c_int32)]
+
+
 class SystemConfig(C.Structure):
     _fields_ = [
         ("cols", C.c_int32), ("rows", C.c_int32), ("K", Intrinsics),
@@ -168,6 +173,14 @@ _SIGS = {
     "sicp_system_read_timing_detail": (C.c_int, [_P, C.c_int, _P, C.c_int, C.POINTER(C.c_int),
                                                  _P]),
     "sicp_system_frame_image": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Image)]),
+    "sicp_system_grab_batch": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P]),
+    "sicp_batch_create": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
+    "sicp_batch_destroy": (C.c_int, [_P]),
+    "sicp_batch_grab_staged": (C.c_int, [_P, _P, C.c_int, _P]),
+    "sicp_system_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "sicp_system_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "sicp_system_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(Tuning), C.POINTER(C.c_float),
+                                       C.POINTER(C.c_float), _P]),
     "sicp_synth_create": (C.c_int, [C.POINTER(SynthConfig), C.POINTER(_P)]),
     "sicp_synth_destroy": (C.c_int, [_P]),
     "sicp_synth_pose": (C.c_int, [_P, C.c_int, _P]),
@@ -464,3 +477,77 @@ class System:
         im = Image()
         check(lib().sicp_system_frame_image(self.h, which, level, C.byref(im)), "frame_image")
         return im
+
+    def tuning(self):
+        t = Tuning()
+        check(lib().sicp_system_get_tuning(self.h, C.byref(t)), "system_get_tuning")
+        return dict(ray_first_pass_samples=t.ray_first_pass_samples,
+                    ray_second_pass_workgroups=t.ray_second_pass_workgroups)
+
+    def set_tuning(self, ray_first_pass_samples, ray_second_pass_workgroups):
+        t = Tuning(ray_first_pass_samples, ray_second_pass_workgroups)
+        check(lib().sicp_system_set_tuning(self.h, C.byref(t)), "system_set_tuning")
+
+    def autotune(self, repeats=5, stream=None):
+        """Times the last frame's raycast under each candidate launch shape and
+        keeps the fastest (sicp_system_autotune); returns the choice with its
+        median time and the previous tuning's, in microseconds."""
+        t = Tuning()
+        us, prev = C.c_float(), C.c_float()
+        check(lib().sicp_system_autotune(self.h, repeats, C.byref(t), C.byref(us), C.byref(prev),
+                                         stream_ptr(stream)), "system_autotune")
+        return dict(ray_first_pass_samples=t.ray_first_pass_samples,
+                    ray_second_pass_workgroups=t.ray_second_pass_workgroups,
+                    raycast_us=round(us.value, 2), previous_us=round(prev.value, 2))
+
+
+def grab_batch(systems, frames, stream=None):
+    """One frame of every system in shared tracking launches
+    (sicp_system_grab_batch); `frames` holds one (depth, rgb, labels or None)
+    tuple of device tensors per system."""
+    n = len(systems)
+    hs = (C.c_void_p * n)(*[g.h for g in systems])
+    depth = (Image * n)(*[image(f[0]) for f in frames])
+    rgb = (Image * n)(*[image(f[1]) for f in frames])
+    lab = (Image * n)(*[image(f[2]) for f in frames])
+    nd = (C.c_int32 * n)(*[1 if f[2] is not None else 0 for f in frames])
+    check(lib().sicp_system_grab_batch(hs, n, depth, rgb, lab, nd, stream_ptr(stream)),
+          "sicp_system_grab_batch")
+
+
+class Batch:
+    """Systems advanced together, each round (one frame of every system) as
+    one hipGraph launch once every system is past its first frame."""
+
+    def __init__(self, systems):
+        self.systems = list(systems)
+        n = len(self.systems)
+        h = C.c_void_p()
+        check(lib().sicp_batch_create((C.c_void_p * n)(*[g.h for g in self.systems]), n,
+                                      C.byref(h)), "sicp_batch_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().sicp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def grab_staged(self, frames, use_graph=True, stream=None):
+        """Copies every system's frame into its staged inputs on `stream` and
+        runs the round."""
+        for g, f in zip(self.systems, frames):
+            if not hasattr(g, "_staged"):
+                g._staged = g.staging()
+            for dst, src in zip(g._staged, f):
+                if src is not None:
+                    copy2d(dst, src, stream)
+        n = len(self.systems)
+        nd = (C.c_int32 * n)(*[1 if f[2] is not None else 0 for f in frames])
+        check(lib().sicp_batch_grab_staged(self.h, nd, 1 if use_graph else 0,
+                                           stream_ptr(stream)), "sicp_batch_grab_staged")

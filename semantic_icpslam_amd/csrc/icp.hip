@@ -29,6 +29,47 @@ struct IcpGeom {
     float fx, fy, cx, cy;
 };
 
+// A workgroup's place among the workgroups that share one image: the launch
+// grid itself, or, in a launch serving several sequences, the sequence's own
+// range of workgroups.
+struct Wg {
+    int id, n;
+};
+SICP_DEV Wg grid_wg() { return Wg{(int)blockIdx.x, (int)gridDim.x}; }
+
+// Kernel arguments of the launches that serve several sequences.
+struct IcpSeqArgs {
+    View<float4> vcurr, vlast, ncurr, nlast;
+    IcpGeom g;
+    TrackParams tp;
+    const TrackState* st_in;
+    const FrameMeta* last;
+    TrackState* st_out;
+    const double* partials_prev;
+    double* partials;
+    int nb_prev, min_count_prev;
+};
+struct IcpBatchArgs {
+    int n;
+    int wg0[kMaxBatch + 1];  // sequence q: workgroups wg0[q] .. wg0[q + 1] - 1
+    IcpSeqArgs seq[kMaxBatch];
+};
+struct FinishSeqArgs {
+    TrackParams tp;
+    const TrackState* st_in;
+    const FrameMeta* last;
+    const double* partials_prev;
+    TrackState* st_final;
+    FrameMeta* next;
+    View<float4> vlast, nlast, vnext, nnext;
+    int nb_prev, min_count_prev;
+};
+struct FinishBatchArgs {
+    int n;
+    int wg0[kMaxBatch + 1];
+    FinishSeqArgs seq[kMaxBatch];
+};
+
 // ICPReduce::searchPoint up to the gather (Tracking/Reduction.cu:231-243):
 // current vertex to world, into the last camera, projected with the
 // reference's FP64 promotions (invz = 1.0 / z in double stored to float,
@@ -237,7 +278,7 @@ static_assert(sizeof(SumF64) == 8 && sizeof(SumK32) == 8, "one 8-byte partial pe
 // three butterflies. Fixed order; much cheaper than folding 29 doubles
 // across 64 lanes with cross-lane permutes.
 template <int NS, class Acc>
-SICP_DEV void block_rows_to_partials(const Acc (&acc)[NS], double* partials) {
+SICP_DEV void block_rows_to_partials(const Acc (&acc)[NS], double* partials, Wg wg) {
     __shared__ Acc tile[kIcpThreads][NS + 1];
 #pragma unroll
     for (int j = 0; j < NS; ++j) tile[threadIdx.x][j] = acc[j];
@@ -253,7 +294,7 @@ SICP_DEV void block_rows_to_partials(const Acc (&acc)[NS], double* partials) {
         s.merge(s.shfl_xor(2));
         s.merge(s.shfl_xor(4));
         // column-major: sum j of every workgroup is contiguous
-        if (k == 0) reinterpret_cast<Acc*>(partials)[j * kIcpMaxBlocks + blockIdx.x] = s;
+        if (k == 0) reinterpret_cast<Acc*>(partials)[j * kIcpMaxBlocks + wg.id] = s;
     }
 }
 
@@ -276,10 +317,10 @@ SICP_DEV void add_row(Acc acc[kIcpSums], const float row[7]) {
 template <bool kRgb, class Acc, class PoseFn>
 SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlast,
                              const View<float4>& ncurr, const View<float4>& nlast,
-                             const IcpGeom& g, double* partials, const RgbPass& rp,
+                             const IcpGeom& g, double* partials, const RgbPass& rp, Wg wg,
                              PoseFn get_pose) {
-    const int stride = gridDim.x * kIcpThreads;
-    int base = blockIdx.x * kIcpThreads + threadIdx.x;
+    const int stride = wg.n * kIcpThreads;
+    int base = wg.id * kIcpThreads + threadIdx.x;
     IcpBatch bt;
     icp_load_batch<kRgb>(bt, base, stride, vcurr, ncurr, rp, g);
     PoseF cur, last;
@@ -331,11 +372,11 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
     if (kRgb) {
         const uint2 r = block_sum_u32x2(cnt, sq);
         if (threadIdx.x == 0) {
-            rp.ipart[blockIdx.x] = r.x;
-            rp.ipart[kIcpMaxBlocks + blockIdx.x] = r.y;
+            rp.ipart[wg.id] = r.x;
+            rp.ipart[kIcpMaxBlocks + wg.id] = r.y;
         }
     }
-    block_rows_to_partials(acc, partials);
+    block_rows_to_partials(acc, partials, wg);
 }
 
 // RGBStep's row pass over the records of the correspondence pass of
@@ -347,8 +388,9 @@ template <class Acc, class PoseFn>
 SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
                        const View<int16_t>& dy, const IcpGeom& g, const RgbScratch& rs,
                        int nb_match, PoseFn get_pose) {
-    const int stride = gridDim.x * kIcpThreads;
-    int base = blockIdx.x * kIcpThreads + threadIdx.x;
+    const Wg wg = grid_wg();
+    const int stride = wg.n * kIcpThreads;
+    int base = wg.id * kIcpThreads + threadIdx.x;
     int2 rc[kPixPerThread];
     auto load_records = [&]() {
 #pragma unroll
@@ -400,7 +442,7 @@ SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
         base += stride * kPixPerThread;
         if (base < g.n) load_records();
     }
-    block_rows_to_partials(acc, rs.partials);
+    block_rows_to_partials(acc, rs.partials, wg);
 }
 
 // Sum of the partial rows of `nb` workgroups into totals (LDS), in a fixed
@@ -446,7 +488,7 @@ template <class Acc>
 __global__ __launch_bounds__(kIcpThreads) void icp_partials_kernel(
     View<float4> vcurr, View<float4> vlast, View<float4> ncurr, View<float4> nlast, PoseArg pc,
     PoseArg pl, IcpGeom g, double* partials) {
-    icp_accumulate<false, Acc>(vcurr, vlast, ncurr, nlast, g, partials, RgbPass{},
+    icp_accumulate<false, Acc>(vcurr, vlast, ncurr, nlast, g, partials, RgbPass{}, grid_wg(),
                           [&](PoseF& cur, PoseF& last) {
                               cur = load_pose(pc);
                               last = load_pose(pl);
@@ -501,7 +543,7 @@ __global__ __launch_bounds__(kIcpThreads) void so3_partials_kernel(
             for (int b = a; b < 4; ++b) acc[q++].add(row[a] * row[b]);
         acc[10].add(1.0f);
     }
-    block_rows_to_partials(acc, partials);
+    block_rows_to_partials(acc, partials, grid_wg());
 }
 
 SICP_DEV void copy16(double* d, const double* s) {
@@ -743,7 +785,7 @@ template <bool kFirst, class Acc>
 SICP_DEV void icp_state(TrackState* S, double* totals, const TrackState* st_in,
                         const FrameMeta* last, const TrackParams& tp,
                         const double* partials_prev, const double* rgb_prev, int nb_prev,
-                        int min_count_prev, TrackState* st_out) {
+                        int min_count_prev, TrackState* st_out, Wg wg) {
     if (kFirst) {
         if (threadIdx.x == 0) track_begin_state(S, last, tp);
         __syncthreads();
@@ -764,7 +806,7 @@ SICP_DEV void icp_state(TrackState* S, double* totals, const TrackState* st_in,
                               gn_tmp);
         __syncthreads();
     }
-    if (st_out && blockIdx.x == 0 && threadIdx.x < kStateWords)
+    if (st_out && wg.id == 0 && threadIdx.x < kStateWords)
         reinterpret_cast<unsigned long long*>(st_out)[threadIdx.x] =
             reinterpret_cast<const unsigned long long*>(S)[threadIdx.x];
 }
@@ -775,23 +817,49 @@ SICP_DEV void icp_state(TrackState* S, double* totals, const TrackState* st_in,
 // the previous iteration's photometric rows (`rgb_prev`) join the update.
 // Acc: the arithmetic of the sums (SumF64, or SumK32 in the precision form).
 template <bool kFirst, bool kRgb, class Acc>
-__global__ __launch_bounds__(kIcpThreads) void icp_iteration_kernel(
-    View<float4> vcurr, View<float4> vlast, View<float4> ncurr, View<float4> nlast, IcpGeom g,
-    const TrackState* st_in, const FrameMeta* last, TrackParams tp, const double* partials_prev,
-    const double* rgb_prev, int nb_prev, int min_count_prev, TrackState* st_out,
-    double* partials, RgbPass rp) {
+SICP_DEV void icp_iteration_body(const View<float4>& vcurr, const View<float4>& vlast,
+                                 const View<float4>& ncurr, const View<float4>& nlast,
+                                 const IcpGeom& g, const TrackState* st_in, const FrameMeta* last,
+                                 const TrackParams& tp, const double* partials_prev,
+                                 const double* rgb_prev, int nb_prev, int min_count_prev,
+                                 TrackState* st_out, double* partials, const RgbPass& rp, Wg wg) {
     __shared__ TrackState S;
     __shared__ double totals[2 * kIcpSums];
-    icp_accumulate<kRgb, Acc>(vcurr, vlast, ncurr, nlast, g, partials, rp,
+    icp_accumulate<kRgb, Acc>(vcurr, vlast, ncurr, nlast, g, partials, rp, wg,
                               [&](PoseF& cur, PoseF& lst) {
                                   icp_state<kFirst, Acc>(&S, totals, st_in, last, tp,
                                                          partials_prev, kRgb ? rgb_prev : nullptr,
-                                                         nb_prev, min_count_prev, st_out);
+                                                         nb_prev, min_count_prev, st_out, wg);
                                   if (S.done) return false;
                                   cur = to_pose(S.curr_f);
                                   lst = to_pose(S.last_f);
                                   return true;
                               });
+}
+
+template <bool kFirst, bool kRgb, class Acc>
+__global__ __launch_bounds__(kIcpThreads) void icp_iteration_kernel(
+    View<float4> vcurr, View<float4> vlast, View<float4> ncurr, View<float4> nlast, IcpGeom g,
+    const TrackState* st_in, const FrameMeta* last, TrackParams tp, const double* partials_prev,
+    const double* rgb_prev, int nb_prev, int min_count_prev, TrackState* st_out,
+    double* partials, RgbPass rp) {
+    icp_iteration_body<kFirst, kRgb, Acc>(vcurr, vlast, ncurr, nlast, g, st_in, last, tp,
+                                          partials_prev, rgb_prev, nb_prev, min_count_prev,
+                                          st_out, partials, rp, grid_wg());
+}
+
+// One Gauss-Newton iteration of several sequences in one launch: sequence q
+// owns workgroups wg0[q] .. wg0[q + 1] - 1 and runs exactly the arithmetic of
+// its own icp_iteration_kernel launch on them (the point-to-plane term).
+template <bool kFirst, class Acc>
+__global__ __launch_bounds__(kIcpThreads) void icp_iteration_batch_kernel(IcpBatchArgs a) {
+    int q = 0;
+    while (q + 1 < a.n && (int)blockIdx.x >= a.wg0[q + 1]) ++q;
+    const IcpSeqArgs& e = a.seq[q];
+    const Wg wg{(int)blockIdx.x - a.wg0[q], a.wg0[q + 1] - a.wg0[q]};
+    icp_iteration_body<kFirst, false, Acc>(e.vcurr, e.vlast, e.ncurr, e.nlast, e.g, e.st_in,
+                                           e.last, e.tp, e.partials_prev, nullptr, e.nb_prev,
+                                           e.min_count_prev, e.st_out, e.partials, RgbPass{}, wg);
 }
 
 // RGBStep's row pass. The pose is the iteration state's (`st`, published by
@@ -843,23 +911,22 @@ __global__ __launch_bounds__(kIcpThreads) void rgb_match_kernel(View<float4> vcu
     }
 }
 
-// The end of ComputeSE3 and TrackFrame: the last iteration's update, the
-// motion / error gate, the frame pose; when the frame is not tracked, the
-// last frame's level-0 model maps are copied into the new frame so the frame
-// swap keeps the last model (Tracking.cc:98-117 keeps lastFrame on failure).
+// The finishing step of a frame's tracking on one workgroup range (the body
+// of track_finish_kernel below).
 template <bool kFirst, class Acc>
-__global__ __launch_bounds__(kIcpThreads) void track_finish_kernel(
-    const TrackState* st_in, const FrameMeta* last, TrackParams tp, const double* partials_prev,
-    const double* rgb_prev, int nb_prev, int min_count_prev, TrackState* st_final,
-    FrameMeta* next, View<float4> vlast, View<float4> nlast, View<float4> vnext,
-    View<float4> nnext, FrameImages images) {
+SICP_DEV void track_finish_body(const TrackState* st_in, const FrameMeta* last,
+                                const TrackParams& tp, const double* partials_prev,
+                                const double* rgb_prev, int nb_prev, int min_count_prev,
+                                TrackState* st_final, FrameMeta* next, const View<float4>& vlast,
+                                const View<float4>& nlast, const View<float4>& vnext,
+                                const View<float4>& nnext, const FrameImages& images, Wg wg) {
     __shared__ TrackState S;
     __shared__ double totals[2 * kIcpSums];
     icp_state<kFirst, Acc>(&S, totals, st_in, last, tp, partials_prev, rgb_prev, nb_prev,
-                      min_count_prev, nullptr);
+                           min_count_prev, nullptr, wg);
     if (threadIdx.x == 0) track_end_state(&S);
     __syncthreads();
-    if (blockIdx.x == 0) {
+    if (wg.id == 0) {
         if (threadIdx.x < kStateWords)
             reinterpret_cast<unsigned long long*>(st_final)[threadIdx.x] =
                 reinterpret_cast<const unsigned long long*>(&S)[threadIdx.x];
@@ -868,7 +935,7 @@ __global__ __launch_bounds__(kIcpThreads) void track_finish_kernel(
     }
     if (S.tracked) return;
     const int n = vnext.cols * vnext.rows;
-    for (int i = blockIdx.x * kIcpThreads + threadIdx.x; i < n; i += gridDim.x * kIcpThreads) {
+    for (int i = wg.id * kIcpThreads + threadIdx.x; i < n; i += wg.n * kIcpThreads) {
         int y = i / vnext.cols, x = i - y * vnext.cols;
         vnext.at(y, x) = vlast.at(y, x);
         nnext.at(y, x) = nlast.at(y, x);
@@ -879,12 +946,39 @@ __global__ __launch_bounds__(kIcpThreads) void track_finish_kernel(
         const View<uint8_t>& dst = images.next[l];
         if (!dst.ptr) continue;
         const int m = dst.cols * dst.rows;
-        for (int i = blockIdx.x * kIcpThreads + threadIdx.x; i < m;
-             i += gridDim.x * kIcpThreads) {
+        for (int i = wg.id * kIcpThreads + threadIdx.x; i < m; i += wg.n * kIcpThreads) {
             int y = i / dst.cols, x = i - y * dst.cols;
             dst.at(y, x) = src.at(y, x);
         }
     }
+}
+
+// The end of ComputeSE3 and TrackFrame: the last iteration's update, the
+// motion / error gate, the frame pose; when the frame is not tracked, the
+// last frame's level-0 model maps are copied into the new frame so the frame
+// swap keeps the last model (Tracking.cc:98-117 keeps lastFrame on failure).
+template <bool kFirst, class Acc>
+__global__ __launch_bounds__(kIcpThreads) void track_finish_kernel(
+    const TrackState* st_in, const FrameMeta* last, TrackParams tp, const double* partials_prev,
+    const double* rgb_prev, int nb_prev, int min_count_prev, TrackState* st_final,
+    FrameMeta* next, View<float4> vlast, View<float4> nlast, View<float4> vnext,
+    View<float4> nnext, FrameImages images) {
+    track_finish_body<kFirst, Acc>(st_in, last, tp, partials_prev, rgb_prev, nb_prev,
+                                   min_count_prev, st_final, next, vlast, nlast, vnext, nnext,
+                                   images, grid_wg());
+}
+
+// The tracking finish of several sequences in one launch (point-to-plane
+// term), sequence q on workgroups wg0[q] .. wg0[q + 1] - 1.
+template <class Acc>
+__global__ __launch_bounds__(kIcpThreads) void track_finish_batch_kernel(FinishBatchArgs a) {
+    int q = 0;
+    while (q + 1 < a.n && (int)blockIdx.x >= a.wg0[q + 1]) ++q;
+    const FinishSeqArgs& e = a.seq[q];
+    const Wg wg{(int)blockIdx.x - a.wg0[q], a.wg0[q + 1] - a.wg0[q]};
+    track_finish_body<false, Acc>(e.st_in, e.last, e.tp, e.partials_prev, nullptr, e.nb_prev,
+                                  e.min_count_prev, e.st_final, e.next, e.vlast, e.nlast, e.vnext,
+                                  e.nnext, FrameImages{}, wg);
 }
 
 __global__ void track_init_kernel(FrameMeta* next, FrameMeta* last, TrackState* st) {
@@ -988,6 +1082,74 @@ int launch_icp_iteration(const IcpLevel& cur, const IcpLevel* prev, const FrameM
         icp_iteration_launch<SumF64>(cur, prev, last, tp, st_in, st_out, partials_prev, partials,
                                      rgb, s);
     SICP_LAUNCH_CHECK("icp_iteration_kernel");
+    return SICP_OK;
+}
+
+int launch_icp_iteration_batch(const IcpBatchEntry* e, int n, hipStream_t s) {
+    if (n < 1 || n > kMaxBatch) return invalid("icp batch: 1 to 8 sequences");
+    IcpBatchArgs a{};
+    a.n = n;
+    int wg = 0;
+    const bool first = e[0].prev == nullptr;
+    const int k32 = e[0].tp.sums_k32;
+    for (int q = 0; q < n; ++q) {
+        if ((e[q].prev == nullptr) != first || e[q].tp.sums_k32 != k32 || e[q].tp.rgb)
+            return invalid("icp batch: sequences at different iterations or forms");
+        const IcpGeom g = geom_of(e[q].cur);
+        a.wg0[q] = wg;
+        wg += icp_blocks(g.n);
+        const IcpLevel* p = e[q].prev;
+        a.seq[q] = IcpSeqArgs{e[q].cur.vcurr,
+                              e[q].cur.vlast,
+                              e[q].cur.ncurr,
+                              e[q].cur.nlast,
+                              g,
+                              e[q].tp,
+                              e[q].st_in,
+                              e[q].last,
+                              e[q].st_out,
+                              e[q].partials_prev,
+                              e[q].partials,
+                              p ? icp_blocks(p->vcurr.cols * p->vcurr.rows) : 0,
+                              p ? p->min_count : 0};
+    }
+    a.wg0[n] = wg;
+    if (first && k32)
+        icp_iteration_batch_kernel<true, SumK32><<<wg, kIcpThreads, 0, s>>>(a);
+    else if (first)
+        icp_iteration_batch_kernel<true, SumF64><<<wg, kIcpThreads, 0, s>>>(a);
+    else if (k32)
+        icp_iteration_batch_kernel<false, SumK32><<<wg, kIcpThreads, 0, s>>>(a);
+    else
+        icp_iteration_batch_kernel<false, SumF64><<<wg, kIcpThreads, 0, s>>>(a);
+    SICP_LAUNCH_CHECK("icp_iteration_batch_kernel");
+    return SICP_OK;
+}
+
+int launch_track_finish_batch(const FinishBatchEntry* e, int n, hipStream_t s) {
+    if (n < 1 || n > kMaxBatch) return invalid("finish batch: 1 to 8 sequences");
+    FinishBatchArgs a{};
+    a.n = n;
+    int wg = 0;
+    const int k32 = e[0].tp.sums_k32;
+    for (int q = 0; q < n; ++q) {
+        if (!e[q].prev || e[q].tp.sums_k32 != k32 || e[q].tp.rgb)
+            return invalid("finish batch: sequences in different forms");
+        const int npx = e[q].vnext.cols * e[q].vnext.rows;
+        a.wg0[q] = wg;
+        wg += std::min((int)div_up(npx, kIcpThreads * 4), kFinishBlocks);
+        const IcpLevel* p = e[q].prev;
+        a.seq[q] = FinishSeqArgs{e[q].tp,       e[q].st_in,  e[q].last,  e[q].partials_prev,
+                                 e[q].st_final, e[q].next,   e[q].vlast, e[q].nlast,
+                                 e[q].vnext,    e[q].nnext,
+                                 icp_blocks(p->vcurr.cols * p->vcurr.rows), p->min_count};
+    }
+    a.wg0[n] = wg;
+    if (k32)
+        track_finish_batch_kernel<SumK32><<<wg, kIcpThreads, 0, s>>>(a);
+    else
+        track_finish_batch_kernel<SumF64><<<wg, kIcpThreads, 0, s>>>(a);
+    SICP_LAUNCH_CHECK("track_finish_batch_kernel");
     return SICP_OK;
 }
 

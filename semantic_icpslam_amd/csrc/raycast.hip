@@ -146,7 +146,36 @@ __global__ __launch_bounds__(256) void prepass_kernel(MapDev m, MapCounters* ctr
 // DeviceMap.cu:169-201, for every int), and one cache compare per block.
 struct BlockCache {
     int x, y, z, slot;  // slot -1: empty
+    // The first pass's per-wavefront table of recent lookups (null elsewhere).
+    unsigned long long* lds = nullptr;
 };
+
+// A wavefront's 64 rays cross the same few blocks, so the first pass keeps a
+// direct-mapped table of its recent lookups in LDS, hits and misses alike:
+// 256 entries of one 64-bit word {tag = 1 + the block coordinates in 14 bits
+// each, slot + 1 (0: no block)}, written by atomic exchange so that lanes
+// storing the same entry at once never leave a torn word. Blocks outside
+// +-8192 per axis, and heaps above 2^21 - 2 blocks, go to the table directly.
+// The map is read-only during the raycast, so a cached answer is the table's.
+constexpr int kRayLdsEntries = 256;
+SICP_DEV int lds_lookup(const MapDev& m, unsigned long long* t, int bx, int by, int bz) {
+    const bool fits = (unsigned)(bx + 8192) < 16384u && (unsigned)(by + 8192) < 16384u &&
+                      (unsigned)(bz + 8192) < 16384u;
+    if (!fits) return map_lookup(m, bx, by, bz);
+    const unsigned long long tag = 1ull + (((unsigned long long)(bx & 0x3fff) << 28) |
+                                           ((unsigned long long)(by & 0x3fff) << 14) |
+                                           (unsigned long long)(bz & 0x3fff));
+    const uint32_t i = (((uint32_t)bx * 0x9E3779B1u) ^ ((uint32_t)by * 0x85EBCA77u) ^
+                        ((uint32_t)bz * 0xC2B2AE3Du)) >> 24;
+    const unsigned long long e = t[i];
+    if ((e >> 21) == tag) return (int)(e & 0x1fffffull) - 1;
+    const int s = map_lookup(m, bx, by, bz);
+    atomicExch(&t[i], (tag << 21) | (unsigned long long)(s + 1));
+    return s;
+}
+SICP_DEV int cached_lookup(const MapDev& m, const BlockCache& c, int bx, int by, int bz) {
+    return c.lds ? lds_lookup(m, c.lds, bx, by, bz) : map_lookup(m, bx, by, bz);
+}
 
 // make_int3 of one coordinate (VectorMath.h:36-42): the truncation, one less
 // where it exceeds the value — the floor, for every float (NaN and values
@@ -169,13 +198,13 @@ SICP_DEV long find_voxel_i(const MapDev& m, int vx, int vy, int vz, BlockCache& 
         valid = true;
         return (long)c.slot * kBlockVoxels + local_of(vx, vy, vz);
     }
-    const int slot = map_lookup(m, bx, by, bz);
+    const int slot = cached_lookup(m, c, bx, by, bz);
     if (slot < 0) {
         valid = false;
         return -1;
     }
     valid = true;
-    c = BlockCache{bx, by, bz, slot};
+    c = BlockCache{bx, by, bz, slot, c.lds};
     return (long)slot * kBlockVoxels + local_of(vx, vy, vz);
 }
 
@@ -207,7 +236,7 @@ SICP_DEV void corner_blocks(const MapDev& m, const AxisPair& X, const AxisPair& 
         if (dx > nx || dy > ny || dz > nz) continue;
         const int bx = dx ? (X.v1 >> 3) : bx0, by = dy ? (Y.v1 >> 3) : by0,
                   bz = dz ? (Z.v1 >> 3) : bz0;
-        const int sl = cache_has(c, bx, by, bz) ? c.slot : map_lookup(m, bx, by, bz);
+        const int sl = cache_has(c, bx, by, bz) ? c.slot : cached_lookup(m, c, bx, by, bz);
         s0 = b == 0 ? sl : s0;
         s1 = b == 1 ? sl : s1;
         s2 = b == 2 ? sl : s2;
@@ -362,6 +391,7 @@ SICP_DEV bool read_normal(const MapDev& m, v3 pt, BlockCache& c, const PoseF& po
 struct RayArgs {
     float invfx, invfy, cx, cy;
     int cols, rows;
+    int first_pass_samples;  // the first pass's hand-over point (raycast_kernel only)
 };
 
 // The ray of pixel (x, y) (RenderScene.cu:393-421): start point and unit
@@ -391,11 +421,10 @@ SICP_DEV bool ray_setup(const MapDev& m, const View<float>& zmin, const View<flo
     return true;
 }
 
-// Samples one ray marches in the first pass; rays still marching then are
-// handed to the second pass. Most rays end well before; the few that graze a
-// surface for hundreds of one-voxel steps would otherwise set the length of
-// the whole launch.
-constexpr int kRayIters = 48;
+// RayArgs::first_pass_samples is the number of samples one ray marches in
+// the first pass; rays still marching then are handed to the second pass.
+// Most rays end well before; the few that graze a surface for hundreds of
+// one-voxel steps would otherwise set the length of the whole launch.
 
 // First pass: one lane per ray, 8x8-pixel wavefronts (the zRange tile).
 __global__ __launch_bounds__(64) void raycast_kernel(MapDev m, View<float4> vmap,
@@ -406,6 +435,12 @@ __global__ __launch_bounds__(64) void raycast_kernel(MapDev m, View<float4> vmap
     if (nonempty && *nonempty == 0) return;  // CreateRenderingBlocks returned false
     const int x = blockIdx.x * 8 + (threadIdx.x & 7);
     const int y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    // One wavefront per workgroup: the table is cleared before any lane uses
+    // it, and its accesses are ordered by the wavefront's own instruction
+    // order (no barrier needed).
+    __shared__ unsigned long long lookup_table[kRayLdsEntries];
+#pragma unroll
+    for (int i = 0; i < kRayLdsEntries / 64; ++i) lookup_table[threadIdx.x + 64 * i] = 0ull;
     if (x >= g.cols || y >= g.rows) return;
     const float nan = qnanf();
     float4 vout = make_float4(nan, nan, nan, nan);
@@ -420,10 +455,11 @@ __global__ __launch_bounds__(64) void raycast_kernel(MapDev m, View<float4> vmap
         dist_s = ray.dist_s;
         bool valid_sdf = false, back_face = false;
         float step, prev_sdf;
-        BlockCache cache{0, 0, 0, -1};
+        BlockCache cache{0, 0, 0, -1,
+                         m.num_blocks < (1 << 21) - 2 ? lookup_table : nullptr};
         int iters = 0;
         while (dist_s < ray.dist_e) {
-            if (iters == kRayIters) {
+            if (iters == g.first_pass_samples) {
                 deferred = true;
                 break;
             }
@@ -955,7 +991,7 @@ int launch_raycast_adjoint(MapDev m, View<float> zmin, View<float> zmax, PoseArg
                            const int32_t* pixels, int n, const float* grad_v,
                            const float* grad_n, float* grad_sdf, float* grad_px,
                            double* partials, double* grad_pose, hipStream_t s) {
-    AdjointArgs a{RayArgs{invfx, invfy, cx, cy, cols, rows}, pixels, n, grad_v, grad_n,
+    AdjointArgs a{RayArgs{invfx, invfy, cx, cy, cols, rows, 0}, pixels, n, grad_v, grad_n,
                   grad_sdf, grad_px, partials};
     int nb = (int)div_up(n > 0 ? n : 1, kAdjThreads);
     if (nb > kAdjMaxBlocks) nb = kAdjMaxBlocks;
@@ -993,9 +1029,12 @@ int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible,
 int launch_raycast_gated(MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin,
                          View<float> zmax, PoseArg pose, float invfx, float invfy, float cx,
                          float cy, const int* gate, const uint32_t* nonempty, hipStream_t s,
-                         bool queue_cleared) {
+                         bool queue_cleared, RayTuning tune) {
     if ((size_t)vmap.cols * vmap.rows > m.rq_cap) return invalid("raycast: image above the map's max_cols x max_rows");
-    RayArgs g{invfx, invfy, cx, cy, vmap.cols, vmap.rows};
+    if (tune.first_pass_samples < 1 || tune.first_pass_samples > kRayTuneMaxSamples ||
+        tune.second_pass_workgroups < 1 || tune.second_pass_workgroups > kRayTuneMaxWorkgroups)
+        return invalid("raycast: tuning outside 1..4096 samples, 1..65536 workgroups");
+    RayArgs g{invfx, invfy, cx, cy, vmap.cols, vmap.rows, tune.first_pass_samples};
     if (!queue_cleared) {
         ray_queue_reset_kernel<<<1, 1, 0, s>>>(m.rq_count);
         SICP_LAUNCH_CHECK("ray_queue_reset_kernel");
@@ -1003,7 +1042,8 @@ int launch_raycast_gated(MapDev m, View<float4> vmap, View<float4> nmap, View<fl
     dim3 grid(div_up(vmap.cols, 8), div_up(vmap.rows, 8));
     raycast_kernel<<<grid, 64, 0, s>>>(m, vmap, nmap, zmin, zmax, pose, g, gate, nonempty);
     SICP_LAUNCH_CHECK("raycast_kernel");
-    raycast_long_kernel<<<1024, 256, 0, s>>>(m, vmap, nmap, zmin, zmax, pose, g, gate, nonempty);
+    raycast_long_kernel<<<tune.second_pass_workgroups, 256, 0, s>>>(m, vmap, nmap, zmin, zmax,
+                                                                    pose, g, gate, nonempty);
     SICP_LAUNCH_CHECK("raycast_long_kernel");
     return SICP_OK;
 }
@@ -1012,7 +1052,7 @@ int launch_raycast(MapDev m, View<float4> vmap, View<float4> nmap, View<float> z
                    View<float> zmax, PoseArg pose, float invfx, float invfy, float cx, float cy,
                    const int* gate, hipStream_t s) {
     return launch_raycast_gated(m, vmap, nmap, zmin, zmax, pose, invfx, invfy, cx, cy, gate,
-                                nullptr, s, false);
+                                nullptr, s, false, RayTuning{});
 }
 
 }  // namespace sicp

@@ -28,7 +28,7 @@ int launch_unify(sicp_map* map, const int* gate, hipStream_t s);
 int launch_raycast_gated(MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin,
                          View<float> zmax, PoseArg pose, float invfx, float invfy, float cx,
                          float cy, const int* gate, const uint32_t* nonempty, hipStream_t s,
-                         bool queue_cleared = false);
+                         bool queue_cleared = false, RayTuning tune = RayTuning{});
 
 // ICPStep over the four maps with `partials_dev` as the workgroups' partial
 // rows (sicp_icp_step and its variants).

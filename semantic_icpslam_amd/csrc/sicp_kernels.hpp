@@ -148,6 +148,34 @@ int launch_track_finish(const IcpLevel* prev, const FrameMeta* last, TrackParams
                         View<float4> vlast, View<float4> nlast, View<float4> vnext,
                         View<float4> nnext, const FrameImages* images, hipStream_t s);
 
+// Several sequences' tracking in the same launches (point-to-plane term):
+// at Gauss-Newton iteration k every sequence of a batch is at the same level
+// and iteration, so one launch serves them all, each on its own range of
+// workgroups with exactly its own launch's arithmetic.
+constexpr int kMaxBatch = 8;
+struct IcpBatchEntry {
+    IcpLevel cur;
+    const IcpLevel* prev;  // the previous iteration's level; null for the first
+    const FrameMeta* last;
+    TrackParams tp;
+    const TrackState* st_in;
+    TrackState* st_out;
+    const double* partials_prev;
+    double* partials;
+};
+int launch_icp_iteration_batch(const IcpBatchEntry* e, int n, hipStream_t s);
+struct FinishBatchEntry {
+    const IcpLevel* prev;  // the last iteration's level
+    const FrameMeta* last;
+    TrackParams tp;
+    const TrackState* st_in;
+    const double* partials_prev;
+    TrackState* st_final;
+    FrameMeta* next;
+    View<float4> vlast, nlast, vnext, nnext;
+};
+int launch_track_finish_batch(const FinishBatchEntry* e, int n, hipStream_t s);
+
 // RGBStep on its own (the C ABI): key pass, correspondence pass, row pass,
 // then the 29 sums into `sums_out` (device) and the correspondence count,
 // sum of squared differences and sigma into `rgb.totals`.
@@ -255,6 +283,15 @@ int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible,
                           View<float> zmax, PoseArg pose, sicp_intrinsics K, float dmax,
                           float dmin, const int* gate, hipStream_t s,
                           bool zrange_cleared = false, const uint32_t* fuse_counts = nullptr);
+// Launch shape of the two-pass raycast. Any valid choice gives the same
+// vertex and normal maps bit for bit: the first pass hands a ray over with
+// its complete loop state, and the second pass continues that loop exactly.
+struct RayTuning {
+    int first_pass_samples = 48;     // samples a first-pass lane marches before the hand-over
+    int second_pass_workgroups = 1024;  // 256-thread workgroups draining the hand-over queue
+};
+constexpr int kRayTuneMaxSamples = 4096;
+constexpr int kRayTuneMaxWorkgroups = 65536;
 int launch_raycast(MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin,
                    View<float> zmax, PoseArg pose, float invfx, float invfy, float cx, float cy,
                    const int* gate, hipStream_t s);

@@ -11,6 +11,7 @@
 // puts the last model maps and pose into the frame that becomes "last", so
 // the next frame tracks against the same model as in the reference, where
 // the swap is skipped.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -94,6 +95,11 @@ struct sicp_system {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int graph_nd = -1;  // num_detection the graphs were captured with
     hipStream_t capture_stream = nullptr;
+    // Launch shape of the raycast (set or autotuned); graphs recorded under
+    // an older tuning epoch are recorded again.
+    sicp::RayTuning tuning;
+    uint64_t tuning_epoch = 0;
+    uint64_t graph_epoch = ~0ull;
 };
 
 namespace sicp {
@@ -285,24 +291,46 @@ int sicp_system_grab_image(sicp_system* s, sicp_image depth_u16, sicp_image rgb_
     return rc;
 }
 
-static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
-                      sicp_image labels_u8, int num_detection, void* stream) {
+// Input checks and Tracker::GrabFrame's images: the last frame's model
+// pyramid and the new frame's pyramid.
+static int grab_begin(sicp_system* s, const sicp_image& depth_u16, const sicp_image& rgb_u8x3,
+                      const sicp_image& labels_u8, int num_detection, hipStream_t st) {
     const int c = s->cfg.cols, r = s->cfg.rows;
     if (!image_ok(depth_u16, 2, c, r) || !image_ok(rgb_u8x3, 3, c, r))
         return invalid("system_grab_image: depth / rgb shape mismatch");
     const bool labels = num_detection > 0 && labels_u8.ptr;
     if (labels && !image_ok(labels_u8, 1, c, r))
         return invalid("system_grab_image: label image shape mismatch");
-    hipStream_t st = (hipStream_t)stream;
-    int rc;
-    size_t ev;
-    // Tracker::GrabFrame: model pyramid of the last frame, then the new frame.
-    ev = stage_begin(s, st);
-    if ((rc = frame_pyramid(s, *s->next, *s->last, depth_u16, rgb_u8x3, st))) return rc;
+    const size_t ev = stage_begin(s, st);
+    const int rc = frame_pyramid(s, *s->next, *s->last, depth_u16, rgb_u8x3, st);
+    if (rc) return rc;
     stage_end(s, kStagePyramid, ev, st);
+    return SICP_OK;
+}
+
+static TrackParams track_params(const sicp_system* s) {
+    return TrackParams{s->cfg.icp_thresh, s->cfg.rgb_weight, s->cfg.icp_only ? 0 : 1,
+                       s->cfg.icp_sums_fp32};
+}
+
+// The level-l maps of the tracking pair (next frame against the last).
+static IcpLevel icp_level(const sicp_system* s, int l) {
+    const FrameBuf& nf = *s->next;
+    const FrameBuf& lf = *s->last;
+    IcpLevel cur{nf.vmap[l].v(), lf.vmap[l].v(), nf.nmap[l].v(), lf.nmap[l].v(),
+                 s->K[l], s->cfg.icp_min_count[l], RgbLevel{}};
+    if (!s->cfg.icp_only)
+        cur.rgb = RgbLevel{nf.key[l].v(), nf.dx[l].v(), nf.dy[l].v(), lf.image[l].v()};
+    return cur;
+}
+
+// Tracker::TrackFrame / ComputeSE3 of one frame (Tracking.cc:157-173,
+// 369-517), or InitTracking on the first frame.
+static int track_single(sicp_system* s, hipStream_t st) {
+    int rc;
     FrameBuf& nf = *s->next;
     FrameBuf& lf = *s->last;
-    ev = stage_begin(s, st);
+    const size_t ev = stage_begin(s, st);
     if (s->frame_index == 0) {
         // InitTracking (Tracking.cc:233-240): both poses identity.
         if ((rc = launch_track_init(nf.meta, lf.meta, s->st, st))) return rc;
@@ -314,14 +342,11 @@ static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
         TrackState* iter = s->st + 1;
         const size_t psz = (size_t)kIcpMaxBlocks * kIcpSums;
         const bool rgb = !s->cfg.icp_only;
-        const TrackParams tp{s->cfg.icp_thresh, s->cfg.rgb_weight, rgb ? 1 : 0,
-                             s->cfg.icp_sums_fp32};
+        const TrackParams tp = track_params(s);
         IcpLevel prev{};
         int k = 0;
         for (int l = 2; l >= 0; --l) {
-            IcpLevel cur{nf.vmap[l].v(), lf.vmap[l].v(), nf.nmap[l].v(), lf.nmap[l].v(),
-                         s->K[l], s->cfg.icp_min_count[l], RgbLevel{}};
-            if (rgb) cur.rgb = RgbLevel{nf.key[l].v(), nf.dx[l].v(), nf.dy[l].v(), lf.image[l].v()};
+            const IcpLevel cur = icp_level(s, l);
             for (int j = 0; j < s->cfg.icp_iters[l]; ++j, ++k) {
                 const size_t ek = stage_begin(s, st);
                 rc = launch_icp_iteration(cur, k ? &prev : nullptr, lf.meta, tp,
@@ -352,6 +377,17 @@ static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
         stage_end(s, kStageTrackFinish, ef, st);
     }
     stage_end(s, kStageTrack, ev, st);
+    return SICP_OK;
+}
+
+// SwapFrame, then Mapping::FuseColor and Mapping::RayTrace of the tracked
+// frame (Mapping.cc:76-97,142-161), each gated on the device-side decision.
+static int grab_end(sicp_system* s, const sicp_image& rgb_u8x3, const sicp_image& labels_u8,
+                    int num_detection, hipStream_t st) {
+    int rc;
+    size_t ev;
+    const int c = s->cfg.cols, r = s->cfg.rows;
+    const bool labels = num_detection > 0 && labels_u8.ptr;
     std::swap(s->next, s->last);  // SwapFrame (Tracking.cc:318-320)
     s->frame_index++;
     FrameBuf& f = *s->last;
@@ -392,9 +428,108 @@ static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
     rc = launch_raycast_gated(m->dev, f.vmap[0].v(), f.nmap[0].v(), s->zmin.v(), s->zmax.v(),
                               pose, (float)(1.0 / (double)s->K[0].fx),
                               (float)(1.0 / (double)s->K[0].fy), s->K[0].cx, s->K[0].cy, gate,
-                              &m->ctr->render_tiles, st, true);
+                              &m->ctr->render_tiles, st, true, s->tuning);
     stage_end(s, kStageRaycast, ev, st);
     return rc;
+}
+
+static int grab_frame(sicp_system* s, sicp_image depth_u16, sicp_image rgb_u8x3,
+                      sicp_image labels_u8, int num_detection, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = grab_begin(s, depth_u16, rgb_u8x3, labels_u8, num_detection, st))) return rc;
+    if ((rc = track_single(s, st))) return rc;
+    return grab_end(s, rgb_u8x3, labels_u8, num_detection, st);
+}
+
+// The tracking of several systems' frames, each past its first frame, in
+// shared launches: one per Gauss-Newton iteration and one finishing launch.
+static int track_batch(sicp_system* const* sys, int n, hipStream_t st) {
+    const size_t psz = (size_t)kIcpMaxBlocks * kIcpSums;
+    IcpLevel prev[kMaxBatch], cur[kMaxBatch];
+    IcpBatchEntry e[kMaxBatch];
+    int k = 0, rc;
+    for (int l = 2; l >= 0; --l) {
+        for (int q = 0; q < n; ++q) cur[q] = icp_level(sys[q], l);
+        for (int j = 0; j < sys[0]->cfg.icp_iters[l]; ++j, ++k) {
+            for (int q = 0; q < n; ++q) {
+                sicp_system* s = sys[q];
+                TrackState* iter = s->st + 1;
+                e[q] = IcpBatchEntry{cur[q],
+                                     k ? &prev[q] : nullptr,
+                                     s->last->meta,
+                                     track_params(s),
+                                     iter + ((k + 1) & 1),
+                                     iter + (k & 1),
+                                     s->partials + psz * ((k + 1) & 1),
+                                     s->partials + psz * (k & 1)};
+            }
+            if ((rc = launch_icp_iteration_batch(e, n, st))) return rc;
+            for (int q = 0; q < n; ++q) prev[q] = cur[q];
+        }
+    }
+    FinishBatchEntry f[kMaxBatch];
+    for (int q = 0; q < n; ++q) {
+        sicp_system* s = sys[q];
+        f[q] = FinishBatchEntry{&prev[q],
+                                s->last->meta,
+                                track_params(s),
+                                s->st + 1 + ((k + 1) & 1),
+                                s->partials + psz * ((k + 1) & 1),
+                                s->st,
+                                s->next->meta,
+                                s->last->vmap[0].v(),
+                                s->last->nmap[0].v(),
+                                s->next->vmap[0].v(),
+                                s->next->nmap[0].v()};
+    }
+    return launch_track_finish_batch(f, n, st);
+}
+
+// Checks that the systems can share tracking launches: distinct, at most
+// kMaxBatch, point-to-plane tracking with one form of the sums and the same
+// iteration counts.
+static int batch_ok(sicp_system* const* sys, int n) {
+    if (!sys || n < 1 || n > kMaxBatch) return invalid("system batch: 1 to 8 systems");
+    for (int q = 0; q < n; ++q) {
+        if (!sys[q]) return invalid("system batch: null system");
+        for (int p = 0; p < q; ++p)
+            if (sys[p] == sys[q]) return invalid("system batch: a system appears twice");
+        if (!sys[q]->cfg.icp_only)
+            return invalid("system batch: the photometric term is not batched (icp_only = 1)");
+        if (sys[q]->cfg.icp_sums_fp32 != sys[0]->cfg.icp_sums_fp32)
+            return invalid("system batch: all systems need the same form of the sums");
+        for (int l = 0; l < 3; ++l)
+            if (sys[q]->cfg.icp_iters[l] != sys[0]->cfg.icp_iters[l])
+                return invalid("system batch: all systems need the same iteration counts");
+    }
+    return SICP_OK;
+}
+
+// One frame of every system of a batch: the pyramids, the tracking of the
+// systems past their first frame in shared launches (the others initialise),
+// then each system's swap, fusion and raycast; all in stream order.
+static int grab_batch(sicp_system* const* sys, int n, const sicp_image* depth,
+                      const sicp_image* rgb, const sicp_image* labels, const int32_t* nd,
+                      hipStream_t st) {
+    int rc;
+    sicp_system* tracked[kMaxBatch];
+    int nt = 0;
+    for (int q = 0; q < n; ++q) {
+        const sicp_image lab = labels ? labels[q] : sicp_image{nullptr, 0, 0, 0};
+        if ((rc = grab_begin(sys[q], depth[q], rgb[q], lab, nd ? nd[q] : 0, st))) return rc;
+        if (sys[q]->frame_index == 0) {
+            if ((rc = track_single(sys[q], st))) return rc;
+        } else {
+            tracked[nt++] = sys[q];
+        }
+    }
+    if (nt && (rc = track_batch(tracked, nt, st))) return rc;
+    for (int q = 0; q < n; ++q) {
+        const sicp_image lab = labels ? labels[q] : sicp_image{nullptr, 0, 0, 0};
+        if ((rc = grab_end(sys[q], rgb[q], lab, nd ? nd[q] : 0, st))) return rc;
+    }
+    return SICP_OK;
 }
 
 // Stage timing of the frames grabbed since it was enabled. Stages: 0 pyramid,
@@ -530,6 +665,7 @@ static int capture_graphs(sicp_system* s, int nd) {
     s->frame_index = index0;
     s->timing = timing;
     s->graph_nd = rc == SICP_OK ? nd : -1;
+    s->graph_epoch = rc == SICP_OK ? s->tuning_epoch : ~0ull;
     return rc;
 }
 
@@ -541,7 +677,7 @@ int sicp_system_grab_staged(sicp_system* s, int num_detection, int use_graph, vo
     // The first frame (tracking initialisation) and timed frames run eagerly.
     if (!use_graph || s->frame_index == 0 || s->timing)
         return sicp_system_grab_image(s, s->in_depth.img(), s->in_rgb.img(), lab, nd, stream);
-    if (!s->graph[0] || s->graph_nd != nd) {
+    if (!s->graph[0] || s->graph_nd != nd || s->graph_epoch != s->tuning_epoch) {
         const int rc = capture_graphs(s, nd);
         if (rc) return rc;
     }
@@ -549,6 +685,180 @@ int sicp_system_grab_staged(sicp_system* s, int num_detection, int use_graph, vo
     SICP_HIP(hipGraphLaunch(s->graph[p], (hipStream_t)stream));
     std::swap(s->next, s->last);  // what the recorded frame's host code does
     s->frame_index++;
+    return SICP_OK;
+}
+
+int sicp_system_grab_batch(sicp_system* const* systems, int32_t n, const sicp_image* depth_u16,
+                           const sicp_image* rgb_u8x3, const sicp_image* labels_u8,
+                           const int32_t* num_detection, void* stream) {
+    int rc = batch_ok(systems, n);
+    if (rc) return rc;
+    if (!depth_u16 || !rgb_u8x3) return invalid("system_grab_batch: null image array");
+    return grab_batch(systems, n, depth_u16, rgb_u8x3, labels_u8, num_detection,
+                      (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// A batch of systems advanced together, with the graph form of a round: the
+// frame-buffer parities of the systems flip together, so a round has two
+// parity patterns and one instantiated graph each.
+struct sicp_batch {
+    std::vector<sicp_system*> sys;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    uint64_t pattern[2] = {0, 0};
+    std::vector<int32_t> nd;  // num_detection the graphs were captured with
+    std::vector<uint64_t> epochs;  // the systems' tuning epochs at capture
+    hipStream_t capture_stream = nullptr;
+};
+
+namespace sicp {
+static std::vector<uint64_t> batch_epochs(const sicp_batch* b) {
+    std::vector<uint64_t> e;
+    for (const sicp_system* s : b->sys) e.push_back(s->tuning_epoch);
+    return e;
+}
+}  // namespace sicp
+
+namespace sicp {
+
+// Bit q: system q's `next` frame buffer is its first one.
+static uint64_t batch_pattern(const sicp_batch* b) {
+    uint64_t p = 0;
+    for (size_t q = 0; q < b->sys.size(); ++q)
+        if (b->sys[q]->next == &b->sys[q]->fa) p |= 1ull << q;
+    return p;
+}
+
+static void batch_drop_graphs(sicp_batch* b) {
+    for (hipGraphExec_t& g : b->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// Records a round of the batch from the systems' staged inputs for the
+// current parity pattern and its complement, as capture_graphs does for one
+// system; the host state of every system is restored afterwards.
+static int batch_capture(sicp_batch* b, const int32_t* nd) {
+    if (!b->capture_stream)
+        SICP_HIP(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
+    batch_drop_graphs(b);
+    const int n = (int)b->sys.size();
+    std::vector<FrameBuf*> next0(n), last0(n);
+    std::vector<int> index0(n);
+    std::vector<bool> timing0(n);
+    std::vector<sicp_image> depth(n), rgb(n), lab(n);
+    for (int q = 0; q < n; ++q) {
+        sicp_system* s = b->sys[q];
+        next0[q] = s->next;
+        last0[q] = s->last;
+        index0[q] = s->frame_index;
+        timing0[q] = s->timing;
+        s->timing = false;
+        depth[q] = s->in_depth.img();
+        rgb[q] = s->in_rgb.img();
+        lab[q] = nd[q] > 0 ? s->in_labels.img() : sicp_image{nullptr, 0, 0, 0};
+    }
+    const uint64_t cur = batch_pattern(b);
+    int rc = SICP_OK;
+    for (int p = 0; p < 2 && rc == SICP_OK; ++p) {
+        for (int q = 0; q < n; ++q) {
+            sicp_system* s = b->sys[q];
+            const bool first_is_next = ((cur >> q) & 1) != (uint64_t)p;
+            s->next = first_is_next ? &s->fa : &s->fb;
+            s->last = first_is_next ? &s->fb : &s->fa;
+            s->frame_index = index0[q] > 0 ? index0[q] : 1;
+        }
+        b->pattern[p] = batch_pattern(b);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "hipStreamBeginCapture");
+            break;
+        }
+        rc = grab_batch(b->sys.data(), n, depth.data(), rgb.data(), lab.data(), nd,
+                        b->capture_stream);
+        e = hipStreamEndCapture(b->capture_stream, &g);
+        if (rc == SICP_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+        if (rc == SICP_OK) {
+            e = hipGraphInstantiate(&b->graph[p], g, nullptr, nullptr, 0);
+            if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+        }
+        if (g) (void)hipGraphDestroy(g);
+    }
+    for (int q = 0; q < n; ++q) {
+        sicp_system* s = b->sys[q];
+        s->next = next0[q];
+        s->last = last0[q];
+        s->frame_index = index0[q];
+        s->timing = timing0[q];
+    }
+    if (rc == SICP_OK) {
+        b->nd.assign(nd, nd + n);
+        b->epochs = batch_epochs(b);
+    } else {
+        batch_drop_graphs(b);
+    }
+    return rc;
+}
+
+}  // namespace sicp
+
+extern "C" {
+
+int sicp_batch_create(sicp_system* const* systems, int32_t n, sicp_batch** out) {
+    if (!out) return invalid("batch_create: null argument");
+    int rc = batch_ok(systems, n);
+    if (rc) return rc;
+    for (int q = 0; q < n; ++q)
+        if ((rc = sicp_system_staging(systems[q], nullptr, nullptr, nullptr))) return rc;
+    sicp_batch* b = new sicp_batch();
+    b->sys.assign(systems, systems + n);
+    *out = b;
+    return SICP_OK;
+}
+
+int sicp_batch_destroy(sicp_batch* b) {
+    if (!b) return SICP_OK;
+    batch_drop_graphs(b);
+    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    delete b;
+    return SICP_OK;
+}
+
+int sicp_batch_grab_staged(sicp_batch* b, const int32_t* num_detection, int use_graph,
+                           void* stream) {
+    if (!b) return invalid("batch_grab_staged: null batch");
+    const int n = (int)b->sys.size();
+    std::vector<int32_t> nd(n, 0);
+    std::vector<sicp_image> depth(n), rgb(n), lab(n);
+    bool eager = !use_graph;
+    for (int q = 0; q < n; ++q) {
+        sicp_system* s = b->sys[q];
+        nd[q] = num_detection && num_detection[q] > 0 ? num_detection[q] : 0;
+        depth[q] = s->in_depth.img();
+        rgb[q] = s->in_rgb.img();
+        lab[q] = nd[q] > 0 ? s->in_labels.img() : sicp_image{nullptr, 0, 0, 0};
+        // first frames (tracking initialisation) and timed frames run eagerly
+        if (s->frame_index == 0 || s->timing) eager = true;
+    }
+    if (eager)
+        return grab_batch(b->sys.data(), n, depth.data(), rgb.data(), lab.data(), nd.data(),
+                          (hipStream_t)stream);
+    const uint64_t cur = batch_pattern(b);
+    int p = (b->graph[0] && b->pattern[0] == cur) ? 0 : ((b->graph[1] && b->pattern[1] == cur) ? 1 : -1);
+    if (p < 0 || b->nd != nd || b->epochs != batch_epochs(b)) {
+        const int rc = batch_capture(b, nd.data());
+        if (rc) return rc;
+        p = b->pattern[0] == cur ? 0 : 1;
+    }
+    SICP_HIP(hipGraphLaunch(b->graph[p], (hipStream_t)stream));
+    for (sicp_system* s : b->sys) {  // what the recorded host code does
+        std::swap(s->next, s->last);
+        s->frame_index++;
+    }
     return SICP_OK;
 }
 
@@ -565,6 +875,127 @@ int sicp_system_frame_image(sicp_system* s, int which, int level, sicp_image* ou
         case 6: *out = f.dy[level].img(); break;
         default: return invalid("system_frame_image: unknown image");
     }
+    return SICP_OK;
+}
+
+}  // extern "C"
+
+// ---- raycast launch tuning -------------------------------------------------
+// The raycast's two launch-shape parameters leave its output unchanged, so
+// they are chosen by measurement: sicp_system_autotune re-runs the last
+// frame's raycast (same map, zRange images and pose, into scratch maps)
+// under every candidate, interleaved over `repeats` rounds, and keeps the
+// candidate with the lowest median launch time.
+
+namespace sicp {
+
+struct TuneScratch {
+    DevImage<float4> vmap, nmap;
+    std::vector<hipEvent_t> events;
+    ~TuneScratch() {
+        vmap.release();
+        nmap.release();
+        for (hipEvent_t e : events)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+static bool tuning_ok(const sicp_tuning& t) {
+    return t.ray_first_pass_samples >= 1 && t.ray_first_pass_samples <= kRayTuneMaxSamples &&
+           t.ray_second_pass_workgroups >= 1 && t.ray_second_pass_workgroups <= kRayTuneMaxWorkgroups;
+}
+
+static void set_tuning(sicp_system* s, const RayTuning& t) {
+    if (t.first_pass_samples == s->tuning.first_pass_samples &&
+        t.second_pass_workgroups == s->tuning.second_pass_workgroups)
+        return;
+    s->tuning = t;
+    s->tuning_epoch++;
+}
+
+}  // namespace sicp
+
+extern "C" {
+
+int sicp_system_get_tuning(const sicp_system* s, sicp_tuning* out) {
+    if (!s || !out) return invalid("system_get_tuning: null argument");
+    out->ray_first_pass_samples = s->tuning.first_pass_samples;
+    out->ray_second_pass_workgroups = s->tuning.second_pass_workgroups;
+    return SICP_OK;
+}
+
+int sicp_system_set_tuning(sicp_system* s, const sicp_tuning* t) {
+    if (!s || !t) return invalid("system_set_tuning: null argument");
+    if (!tuning_ok(*t))
+        return invalid("system_set_tuning: samples outside 1..4096 or workgroups outside 1..65536");
+    set_tuning(s, RayTuning{t->ray_first_pass_samples, t->ray_second_pass_workgroups});
+    return SICP_OK;
+}
+
+int sicp_system_autotune(sicp_system* s, int32_t repeats, sicp_tuning* chosen, float* chosen_us,
+                         float* previous_us, void* stream) {
+    if (!s) return invalid("system_autotune: null system");
+    if (repeats < 1 || repeats > 64) return invalid("system_autotune: repeats outside 1..64");
+    if (s->frame_index == 0) return invalid("system_autotune: no frame processed yet");
+    hipStream_t st = (hipStream_t)stream;
+    sicp_map* m = s->map;
+    TrackState t;
+    uint32_t tiles = 0;
+    SICP_HIP(hipMemcpyAsync(&t, s->st, sizeof(t), hipMemcpyDeviceToHost, st));
+    SICP_HIP(hipMemcpyAsync(&tiles, &m->ctr->render_tiles, sizeof(tiles), hipMemcpyDeviceToHost, st));
+    SICP_HIP(hipStreamSynchronize(st));
+    if (!t.tracked || tiles == 0)
+        return invalid("system_autotune: the last frame raycast nothing (lost, or no visible block)");
+    // Candidate 0 is the tuning in effect.
+    std::vector<RayTuning> cand{s->tuning};
+    for (int smp : {16, 24, 32, 48, 64, 96, 128})
+        for (int wg : {256, 512, 1024, 2048})
+            if (smp != s->tuning.first_pass_samples || wg != s->tuning.second_pass_workgroups)
+                cand.push_back(RayTuning{smp, wg});
+    const int nc = (int)cand.size();
+    TuneScratch ts;
+    int rc;
+    if ((rc = ts.vmap.create(s->cfg.cols, s->cfg.rows)) ||
+        (rc = ts.nmap.create(s->cfg.cols, s->cfg.rows)))
+        return rc;
+    ts.events.assign((size_t)2 * nc * repeats, nullptr);
+    for (hipEvent_t& e : ts.events) SICP_HIP(hipEventCreate(&e));
+    const FrameBuf& f = *s->last;  // the frame whose raycast is re-run
+    const PoseArg pose = make_pose_arg(nullptr, &f.meta->pose_f);
+    const float ifx = (float)(1.0 / (double)s->K[0].fx), ify = (float)(1.0 / (double)s->K[0].fy);
+    auto run = [&](const RayTuning& c) {
+        return launch_raycast_gated(m->dev, ts.vmap.v(), ts.nmap.v(), s->zmin.v(), s->zmax.v(),
+                                    pose, ifx, ify, s->K[0].cx, s->K[0].cy, &s->st->tracked,
+                                    &m->ctr->render_tiles, st, false, c);
+    };
+    if ((rc = run(s->tuning))) return rc;  // warm the caches
+    for (int r = 0; r < repeats; ++r)
+        for (int c = 0; c < nc; ++c) {
+            hipEvent_t* e = &ts.events[(size_t)2 * (r * nc + c)];
+            SICP_HIP(hipEventRecord(e[0], st));
+            if ((rc = run(cand[c]))) return rc;
+            SICP_HIP(hipEventRecord(e[1], st));
+        }
+    SICP_HIP(hipStreamSynchronize(st));
+    std::vector<float> med(nc), x(repeats);
+    for (int c = 0; c < nc; ++c) {
+        for (int r = 0; r < repeats; ++r) {
+            const hipEvent_t* e = &ts.events[(size_t)2 * (r * nc + c)];
+            SICP_HIP(hipEventElapsedTime(&x[r], e[0], e[1]));
+        }
+        std::sort(x.begin(), x.end());
+        med[c] = x[repeats / 2];
+    }
+    int best = 0;
+    for (int c = 1; c < nc; ++c)
+        if (med[c] < med[best]) best = c;
+    // Leave the hand-over queue count as the chosen launch shape leaves it.
+    if ((rc = run(cand[best]))) return rc;
+    SICP_HIP(hipStreamSynchronize(st));
+    set_tuning(s, cand[best]);
+    if (chosen) (void)sicp_system_get_tuning(s, chosen);
+    if (chosen_us) *chosen_us = 1000.0f * med[best];
+    if (previous_us) *previous_us = 1000.0f * med[0];
     return SICP_OK;
 }
 

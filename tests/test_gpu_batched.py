@@ -55,34 +55,87 @@ def _run(systems, frames, streams):
     return out
 
 
-def test_eight_sequences_concurrent_equal_alone(dev):
+@pytest.fixture(scope="module")
+def alone(dev):
+    """Each sequence run by itself: per-frame results and final map."""
     frames = [_frames(w, dev) for w in SEQS]
-    tracked = 0
     torch.cuda.synchronize()
+    out = []
+    for k, w in enumerate(SEQS):
+        g = B.System(_config(w))
+        res = _run([g], [frames[k]], [torch.cuda.Stream()])[0]
+        out.append((res, _map_state(g)))
+        g.close()
+    return frames, out
+
+
+def _same(k, got, want):
+    (res_g, (ng, dg)), (res_w, (nw, dw)) = got, want
+    for i, (a, b) in enumerate(zip(res_g, res_w)):
+        assert np.array_equal(a["pose"], b["pose"]), (k, i)
+        for f in ("tracked", "icp_count", "iterations", "visible_blocks", "allocated_blocks"):
+            assert a[f] == b[f], (k, i, f)
+        assert a["icp_error"] == b["icp_error"] or (np.isnan(a["icp_error"]) and
+                                                    np.isnan(b["icp_error"]))
+    assert ng == nw and ng > 0
+    for f in ("table", "excess", "blocks"):
+        assert np.array_equal(dg[f], dw[f]), (k, f)
+    assert np.array_equal(dg["geom"].view(np.uint8), dw["geom"].view(np.uint8)), k
+    assert np.array_equal(dg["sem"], dw["sem"]), k
+
+
+def test_eight_sequences_concurrent_equal_alone(alone):
+    frames, want = alone
     systems = [B.System(_config(w)) for w in SEQS]
     streams = [torch.cuda.Stream() for _ in SEQS]
     together = _run(systems, frames, streams)
-    maps_together = [_map_state(g) for g in systems]
+    got = [(together[k], _map_state(g)) for k, g in enumerate(systems)]
     for g in systems:
         g.close()
-    for k, w in enumerate(SEQS):
-        print(f"sequence {k}: {w['cols']}x{w['rows']} seed {w['seed']}", flush=True)
-        g = B.System(_config(w))
-        alone = _run([g], [frames[k]], [torch.cuda.Stream()])[0]
-        n, d = _map_state(g)
-        g.close()
-        for i, (a, b) in enumerate(zip(together[k], alone)):
-            assert np.array_equal(a["pose"], b["pose"]), (k, i)
-            for f in ("tracked", "icp_count", "iterations", "visible_blocks",
-                      "allocated_blocks"):
-                assert a[f] == b[f], (k, i, f)
-            assert a["icp_error"] == b["icp_error"] or (np.isnan(a["icp_error"]) and
-                                                        np.isnan(b["icp_error"]))
-        nt, dt = maps_together[k]
-        assert n == nt and n > 0
-        for f in ("table", "excess", "blocks"):
-            assert np.array_equal(d[f], dt[f]), (k, f)
-        assert np.array_equal(d["geom"].view(np.uint8), dt["geom"].view(np.uint8)), k
-        assert np.array_equal(d["sem"], dt["sem"]), k
-        tracked += sum(r["tracked"] for r in alone[1:])
+    tracked = 0
+    for k in range(len(SEQS)):
+        print(f"sequence {k}", flush=True)
+        _same(k, got[k], want[k])
+        tracked += sum(r["tracked"] for r in want[k][0][1:])
     assert tracked >= len(SEQS) * (FRAMES - 1) * 3 // 4, tracked
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_eight_sequences_one_launch_per_iteration_equal_alone(alone, graph):
+    """The eight sequences advanced as one batch: their tracking iterations
+    share launches (sicp_system_grab_batch), eagerly or as one hipGraph per
+    round (sicp_batch_grab_staged)."""
+    frames, want = alone
+    systems = [B.System(_config(w)) for w in SEQS]
+    batch = B.Batch(systems) if graph else None
+    res = [[] for _ in SEQS]
+    for i in range(FRAMES):
+        fr = [frames[k][i] for k in range(len(SEQS))]
+        if graph:
+            batch.grab_staged(fr, True)
+        else:
+            B.grab_batch(systems, fr)
+        for k, g in enumerate(systems):
+            res[k].append(g.result())
+    torch.cuda.synchronize()
+    got = [(res[k], _map_state(g)) for k, g in enumerate(systems)]
+    if batch:
+        batch.close()
+    for g in systems:
+        g.close()
+    for k in range(len(SEQS)):
+        print(f"sequence {k} ({'graph' if graph else 'eager'})", flush=True)
+        _same(k, got[k], want[k])
+
+
+def test_batch_rejects_mixed_forms(dev):
+    w = SEQS[3]
+    a = B.System(_config(w))
+    cfg = _config(w)
+    cfg.icp_only = 0
+    b = B.System(cfg)
+    hs = (B.C.c_void_p * 2)(a.h, b.h)
+    h = B.C.c_void_p()
+    assert B.lib().sicp_batch_create(hs, 2, B.C.byref(h)) == B.SICP_ERR_INVALID_ARGUMENT
+    hs = (B.C.c_void_p * 2)(a.h, a.h)
+    assert B.lib().sicp_batch_create(hs, 2, B.C.byref(h)) == B.SICP_ERR_INVALID_ARGUMENT
