@@ -18,7 +18,8 @@ from semantic_icpslam_amd.binding import (  # noqa: E402
     BLOCK, HASH_ENTRY, VOXEL_GEOM, VOXEL_SEM, FrameResult, Intrinsics, MapConfig, MapStats,
     PoseF, SystemConfig)
 
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ORACLE_LIB names another build of the same sources (the sanitizer builds).
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 _P = C.c_void_p
 

@@ -156,3 +156,30 @@ def test_gate_thresholds():
     T = np.eye(4)
     T[:3, :3] = two
     assert not O.se3_gate(5e-5, 1e-4, T0, T0 @ T)
+
+
+# ------------------------------------------------ forced motion-gate failure --
+def test_pose_jump_fails_the_motion_gate():
+    """Frame 0 then frame 9 of the synthetic trajectory (true motion 0.104 m):
+    tracking converges (ICP error below the threshold, all 18 iterations) yet
+    the frame is lost and the pose restored, which only the translation gate
+    (|t| > 0.1 m, Tracking.cc:505-516) can do; frame 8 (0.093 m) tracks."""
+    from helpers import CFG_SMALL, sys_cfg, synth
+    c = CFG_SMALL
+    cfg = sys_cfg(c)
+    s = synth(c)
+    for jump, tracked in ((9, 0), (8, 1)):
+        motion = np.linalg.norm((np.linalg.inv(s.pose(0)) @ s.pose(jump))[:3, 3])
+        assert (motion > 0.1) == (tracked == 0)
+        o = O.OracleSystem(cfg)
+        r0 = o.grab(*s.render(0))
+        r = o.grab(*s.render(jump))
+        assert r["tracked"] == tracked, (jump, r)
+        assert 0 < r["icp_error"] < cfg.icp_thresh and r["icp_count"] > 2000
+        if tracked:
+            est = np.linalg.norm((np.linalg.inv(r0["pose"]) @ r["pose"])[:3, 3])
+            assert abs(est - motion) < 0.01 and r["visible_blocks"] > 0
+        else:
+            assert r["iterations"] == 18
+            assert np.array_equal(r["pose"], r0["pose"])
+            assert r["visible_blocks"] == 0 and r["allocated_blocks"] == r0["allocated_blocks"]
