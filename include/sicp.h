@@ -515,15 +515,23 @@ typedef struct sicp_tuning {
 } sicp_tuning;
 int sicp_system_get_tuning(const sicp_system* sys, sicp_tuning* out);
 int sicp_system_set_tuning(sicp_system* sys, const sicp_tuning* tuning);
-/* The autotuner (no counterpart in the reference): re-runs the last frame's
- * raycast into scratch maps under each candidate shape (the tuning in effect,
- * then 16..128 samples x 256..2048 workgroups), `repeats` (1..64) interleaved
- * rounds, keeps the lowest median and reports it with the previous tuning's
- * median (microseconds; either pointer may be NULL). The frame's own maps
- * are untouched. Needs a processed frame that was tracked and rendered;
- * synchronises `stream`; allocates scratch for the call. */
+/* The autotuner (no counterpart in the reference). Each call re-runs the
+ * last frame's raycast into scratch maps under each of 28 candidate shapes
+ * (the default 48 / 1024 first, then 16..128 samples x 256..2048
+ * workgroups), `repeats` (1..64) interleaved rounds, and adds each
+ * candidate's median time to the system's running sums; the candidate with
+ * the lowest sum becomes the tuning. Call it on several frames of a sequence
+ * to choose for the sequence. Reports the chosen and the default shape's
+ * mean time per call (microseconds; either pointer may be NULL). The
+ * frame's own maps are untouched. Needs a processed frame that was tracked
+ * and rendered; synchronises `stream`; allocates scratch for the call.
+ * autotune_reset clears the sums; autotune_results lists every candidate
+ * with its mean (up to `cap`; `count` = 28, or 0 before any call). */
 int sicp_system_autotune(sicp_system* sys, int32_t repeats, sicp_tuning* chosen,
-                         float* chosen_us, float* previous_us, void* stream);
+                         float* chosen_us, float* default_us, void* stream);
+int sicp_system_autotune_reset(sicp_system* sys);
+int sicp_system_autotune_results(const sicp_system* sys, sicp_tuning* shapes, float* mean_us,
+                                 int32_t cap, int32_t* count);
 
 /* Frame buffers of the system, for inspection (the public members of Frame,
  * Core/Frame.h): which = 0 vmap, 1 nmap, 2 raw depth (level 0 only),

@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsicp.so")
+# SICP_LIB names another build of the same sources (the profiling build).
+LIB_PATH = os.environ.get("SICP_LIB") or os.path.join(_HERE, "libsicp.so")
 
 SICP_OK = 0
 SICP_ERR_INVALID_ARGUMENT = 1
@@ -181,6 +182,8 @@ _SIGS = {
     "sicp_system_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "sicp_system_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(Tuning), C.POINTER(C.c_float),
                                        C.POINTER(C.c_float), _P]),
+    "sicp_system_autotune_reset": (C.c_int, [_P]),
+    "sicp_system_autotune_results": (C.c_int, [_P, _P, _P, C.c_int32, C.POINTER(C.c_int32)]),
     "sicp_synth_create": (C.c_int, [C.POINTER(SynthConfig), C.POINTER(_P)]),
     "sicp_synth_destroy": (C.c_int, [_P]),
     "sicp_synth_pose": (C.c_int, [_P, C.c_int, _P]),
@@ -489,16 +492,25 @@ class System:
         check(lib().sicp_system_set_tuning(self.h, C.byref(t)), "system_set_tuning")
 
     def autotune(self, repeats=5, stream=None):
-        """Times the last frame's raycast under each candidate launch shape and
-        keeps the fastest (sicp_system_autotune); returns the choice with its
-        median time and the previous tuning's, in microseconds."""
+        """Times the last frame's raycast under each candidate launch shape,
+        adds the medians to the system's sums and keeps the lowest sum
+        (sicp_system_autotune); returns the choice with its mean time per
+        call and the default shape's, in microseconds."""
         t = Tuning()
-        us, prev = C.c_float(), C.c_float()
-        check(lib().sicp_system_autotune(self.h, repeats, C.byref(t), C.byref(us), C.byref(prev),
+        us, dflt = C.c_float(), C.c_float()
+        check(lib().sicp_system_autotune(self.h, repeats, C.byref(t), C.byref(us), C.byref(dflt),
                                          stream_ptr(stream)), "system_autotune")
         return dict(ray_first_pass_samples=t.ray_first_pass_samples,
                     ray_second_pass_workgroups=t.ray_second_pass_workgroups,
-                    raycast_us=round(us.value, 2), previous_us=round(prev.value, 2))
+                    raycast_us=round(us.value, 2), default_us=round(dflt.value, 2))
+
+    def autotune_results(self):
+        """Every autotuner candidate with its mean raycast time (microseconds)."""
+        shapes, us, n = (Tuning * 64)(), (C.c_float * 64)(), C.c_int32()
+        check(lib().sicp_system_autotune_results(self.h, shapes, us, 64, C.byref(n)),
+              "system_autotune_results")
+        return [(shapes[i].ray_first_pass_samples, shapes[i].ray_second_pass_workgroups,
+                 round(us[i], 2)) for i in range(n.value)]
 
 
 def grab_batch(systems, frames, stream=None):

@@ -1,21 +1,29 @@
 // icp.hip — point-to-plane ICP reduction and the device-resident
 // Gauss-Newton loop of Tracker::ComputeSE3, for gfx950.
 //
-// Two launches per iteration. The pixel kernel gives each thread four pixels
-// with every memory access of a phase in flight together: the current frame's
-// vertex and normal (coalesced), then the projective gathers from the last
-// frame. The 29 per-pixel values (27 upper A|b entries, r^2, valid) are FP32
-// products, as in the reference, accumulated in FP64; a workgroup reduces
-// them by transposition through LDS into one partial row. A one-workgroup
-// kernel then sums the partial rows in a fixed order — results are bitwise
-// reproducible — and, in the tracking mode, solves the 6x6 system in FP64 and
-// updates the pose on the device. (On MI355X the kernel boundary is cheaper
-// than an in-kernel last-workgroup hand-off, whose agent-scope fences write
-// back and invalidate the per-XCD L2 caches.)
+// One launch per Gauss-Newton iteration. Every workgroup first redoes the
+// previous iteration's update: it sums the previous launch's partial rows in
+// a fixed order (results are bitwise reproducible) and runs the FP64 6x6
+// solve and the SE(3) update on one wavefront; workgroup 0 publishes the
+// state. Then its pixel pass gives each thread four pixels with every memory
+// access of a phase in flight together: the current frame's vertex and
+// normal (coalesced, issued before the update), then the projective gathers
+// from the last frame. The 29 per-pixel values (27 upper A|b entries, r^2,
+// valid) are FP32 products, as in the reference, accumulated in FP64 (or
+// compensated FP32); a workgroup reduces them by transposition through LDS
+// into one partial row. No workgroup waits for another: on MI355X the kernel
+// boundary is cheaper than an in-kernel hand-off, whose agent-scope fences
+// write back and invalidate the per-XCD L2 caches.
 #include "se3_math.hpp"
 #include "sicp_kernels.hpp"
 
 namespace sicp {
+
+#ifdef SICP_ICP_PHASES
+// Profiling build: stamps of launch k of the tracking loop, workgroup w.
+constexpr int kPhaseLaunches = 24, kPhaseStamps = 8;
+__device__ unsigned long long g_icp_phase[kPhaseLaunches][kIcpMaxBlocks][kPhaseStamps];
+#endif
 
 namespace {
 
@@ -36,6 +44,24 @@ struct Wg {
     int id, n;
 };
 SICP_DEV Wg grid_wg() { return Wg{(int)blockIdx.x, (int)gridDim.x}; }
+
+// ---- phase stamps (profiling build only) -----------------------------------
+// Built with -DSICP_ICP_PHASES (`make phases`), every workgroup of the
+// tracking loop's launches records eight wall-clock stamps (s_memrealtime,
+// 100 MHz): 0 start, 1 partial rows folded, 6 6x6 system solved, 7 SE(3)
+// exponential done, 2 Gauss-Newton done, 3 state ready, 4 pixel pass done,
+// 5 end. The product build compiles none of it.
+#ifdef SICP_ICP_PHASES
+SICP_DEV long long* phase_lds() {
+    __shared__ long long v[kPhaseStamps];
+    return v;
+}
+SICP_DEV void phase_mark(int slot) {
+    if (threadIdx.x == 0) phase_lds()[slot] = wall_clock64();
+}
+#else
+SICP_DEV void phase_mark(int) {}
+#endif
 
 // Kernel arguments of the launches that serve several sequences.
 struct IcpSeqArgs {
@@ -324,7 +350,10 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
     IcpBatch bt;
     icp_load_batch<kRgb>(bt, base, stride, vcurr, ncurr, rp, g);
     PoseF cur, last;
-    if (!get_pose(cur, last)) return;
+    if (!get_pose(cur, last)) {
+        phase_mark(4);
+        return;
+    }
     Acc acc[kIcpSums];
 #pragma unroll
     for (int j = 0; j < kIcpSums; ++j) acc[j].zero();
@@ -369,6 +398,7 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
         base += stride * kPixPerThread;
         if (base < g.n) icp_load_batch<kRgb>(bt, base, stride, vcurr, ncurr, rp, g);
     }
+    phase_mark(4);
     if (kRgb) {
         const uint2 r = block_sum_u32x2(cnt, sq);
         if (threadIdx.x == 0) {
@@ -696,8 +726,10 @@ SICP_DEV void gauss_newton_wave(TrackState* st, const double* totals, const doub
         for (int k = r + 1; k < 6; ++k) s -= lane_bcast(L[r], k) * x[k];
         x[r] = s;
     }
+    phase_mark(6);
     double dT[16];
     se3_exp(x, dT);
+    phase_mark(7);
     // delta = exp(x) * delta; next = last * delta^-1; lane e owns entry e
     if (lane == 0) {
 #pragma unroll
@@ -787,8 +819,10 @@ SICP_DEV void icp_state(TrackState* S, double* totals, const TrackState* st_in,
                         const double* partials_prev, const double* rgb_prev, int nb_prev,
                         int min_count_prev, TrackState* st_out, Wg wg) {
     if (kFirst) {
+        phase_mark(1);
         if (threadIdx.x == 0) track_begin_state(S, last, tp);
         __syncthreads();
+        phase_mark(2);
     } else {
         const unsigned long long* src = reinterpret_cast<const unsigned long long*>(st_in);
         unsigned long long w = threadIdx.x < kStateWords ? src[threadIdx.x] : 0ull;
@@ -800,11 +834,13 @@ SICP_DEV void icp_state(TrackState* S, double* totals, const TrackState* st_in,
         icp_totals_fold<kIcpSums, Acc>(s, totals);
         if (rgb_prev) icp_totals_fold<kIcpSums, Acc>(sr, totals + kIcpSums);
         __syncthreads();
+        phase_mark(1);
         __shared__ double gn_tmp[32];
         if (threadIdx.x < 64 && !S->done)
             gauss_newton_wave(S, totals, rgb_prev ? totals + kIcpSums : nullptr, min_count_prev,
                               gn_tmp);
         __syncthreads();
+        phase_mark(2);
     }
     if (st_out && wg.id == 0 && threadIdx.x < kStateWords)
         reinterpret_cast<unsigned long long*>(st_out)[threadIdx.x] =
@@ -825,16 +861,24 @@ SICP_DEV void icp_iteration_body(const View<float4>& vcurr, const View<float4>& 
                                  TrackState* st_out, double* partials, const RgbPass& rp, Wg wg) {
     __shared__ TrackState S;
     __shared__ double totals[2 * kIcpSums];
+    phase_mark(0);
     icp_accumulate<kRgb, Acc>(vcurr, vlast, ncurr, nlast, g, partials, rp, wg,
                               [&](PoseF& cur, PoseF& lst) {
                                   icp_state<kFirst, Acc>(&S, totals, st_in, last, tp,
                                                          partials_prev, kRgb ? rgb_prev : nullptr,
                                                          nb_prev, min_count_prev, st_out, wg);
+                                  phase_mark(3);
                                   if (S.done) return false;
                                   cur = to_pose(S.curr_f);
                                   lst = to_pose(S.last_f);
                                   return true;
                               });
+    phase_mark(5);
+#ifdef SICP_ICP_PHASES
+    if (threadIdx.x == 0 && tp.launch >= 0 && tp.launch < kPhaseLaunches)
+        for (int i = 0; i < kPhaseStamps; ++i)
+            g_icp_phase[tp.launch][wg.id][i] = (unsigned long long)phase_lds()[i];
+#endif
 }
 
 template <bool kFirst, bool kRgb, class Acc>
@@ -1255,3 +1299,15 @@ void se3_update_host(const double* A, const double* b, double* delta, const doub
 }
 
 }  // namespace sicp
+
+#ifdef SICP_ICP_PHASES
+// Profiling build: copies the stamps (kPhaseLaunches x kIcpMaxBlocks x 8
+// words, 100 MHz) to `host` after synchronising the device.
+extern "C" int sicp_profile_icp_phases(unsigned long long* host) {
+    if (!host) return sicp::invalid("profile_icp_phases: null argument");
+    SICP_HIP(hipDeviceSynchronize());
+    SICP_HIP(hipMemcpyFromSymbol(host, HIP_SYMBOL(sicp::g_icp_phase), sizeof(sicp::g_icp_phase),
+                                 0, hipMemcpyDeviceToHost));
+    return SICP_OK;
+}
+#endif

@@ -82,6 +82,7 @@ struct TrackParams {
     float rgb_weight;  // w (Tracking.cc:392)
     int rgb;           // !icpOnly: RGBStep runs before ICPStep in every iteration
     int sums_k32;      // the precision form: FP32 compensated sums (else FP64)
+    int launch;        // the launch's index in the frame's tracking loop (phase stamps)
 };
 
 struct IcpScratch {

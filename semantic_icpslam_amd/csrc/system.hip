@@ -100,6 +100,12 @@ struct sicp_system {
     sicp::RayTuning tuning;
     uint64_t tuning_epoch = 0;
     uint64_t graph_epoch = ~0ull;
+    std::vector<double> tune_sum;  // per autotuner candidate: summed median ms over calls
+    int tune_calls = 0;
+    // Batched frames: the system's own stages run on its side stream, forked
+    // from and joined back to the batch's stream with these events.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace sicp {
@@ -168,6 +174,9 @@ static void system_free(sicp_system* s) {
     for (hipGraphExec_t& g : s->graph)
         if (g) (void)hipGraphExecDestroy(g);
     if (s->capture_stream) (void)hipStreamDestroy(s->capture_stream);
+    if (s->side) (void)hipStreamDestroy(s->side);
+    if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+    if (s->ev_join) (void)hipEventDestroy(s->ev_join);
     s->in_depth.release();
     s->in_rgb.release();
     s->in_labels.release();
@@ -308,9 +317,9 @@ static int grab_begin(sicp_system* s, const sicp_image& depth_u16, const sicp_im
     return SICP_OK;
 }
 
-static TrackParams track_params(const sicp_system* s) {
+static TrackParams track_params(const sicp_system* s, int launch = 0) {
     return TrackParams{s->cfg.icp_thresh, s->cfg.rgb_weight, s->cfg.icp_only ? 0 : 1,
-                       s->cfg.icp_sums_fp32};
+                       s->cfg.icp_sums_fp32, launch};
 }
 
 // The level-l maps of the tracking pair (next frame against the last).
@@ -342,13 +351,14 @@ static int track_single(sicp_system* s, hipStream_t st) {
         TrackState* iter = s->st + 1;
         const size_t psz = (size_t)kIcpMaxBlocks * kIcpSums;
         const bool rgb = !s->cfg.icp_only;
-        const TrackParams tp = track_params(s);
+        TrackParams tp = track_params(s);
         IcpLevel prev{};
         int k = 0;
         for (int l = 2; l >= 0; --l) {
             const IcpLevel cur = icp_level(s, l);
             for (int j = 0; j < s->cfg.icp_iters[l]; ++j, ++k) {
                 const size_t ek = stage_begin(s, st);
+                tp.launch = k;
                 rc = launch_icp_iteration(cur, k ? &prev : nullptr, lf.meta, tp,
                                           iter + ((k + 1) & 1), iter + (k & 1),
                                           s->partials + psz * ((k + 1) & 1),
@@ -458,7 +468,7 @@ static int track_batch(sicp_system* const* sys, int n, hipStream_t st) {
                 e[q] = IcpBatchEntry{cur[q],
                                      k ? &prev[q] : nullptr,
                                      s->last->meta,
-                                     track_params(s),
+                                     track_params(s, k),
                                      iter + ((k + 1) & 1),
                                      iter + (k & 1),
                                      s->partials + psz * ((k + 1) & 1),
@@ -473,7 +483,7 @@ static int track_batch(sicp_system* const* sys, int n, hipStream_t st) {
         sicp_system* s = sys[q];
         f[q] = FinishBatchEntry{&prev[q],
                                 s->last->meta,
-                                track_params(s),
+                                track_params(s, k),
                                 s->st + 1 + ((k + 1) & 1),
                                 s->partials + psz * ((k + 1) & 1),
                                 s->st,
@@ -506,28 +516,59 @@ static int batch_ok(sicp_system* const* sys, int n) {
     return SICP_OK;
 }
 
+// Side stream of a batched system: its work after fork_to runs after what
+// was queued on `st` before, and `st`'s work after join_from runs after it.
+static int side_init(sicp_system* s) {
+    if (!s->side) SICP_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+    if (!s->ev_fork) SICP_HIP(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+    if (!s->ev_join) SICP_HIP(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
+    return SICP_OK;
+}
+
+static int fork_to(sicp_system* s, hipStream_t st) {
+    SICP_HIP(hipEventRecord(s->ev_fork, st));
+    SICP_HIP(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+    return SICP_OK;
+}
+
+static int join_from(sicp_system* s, hipStream_t st) {
+    SICP_HIP(hipEventRecord(s->ev_join, s->side));
+    SICP_HIP(hipStreamWaitEvent(st, s->ev_join, 0));
+    return SICP_OK;
+}
+
 // One frame of every system of a batch: the pyramids, the tracking of the
-// systems past their first frame in shared launches (the others initialise),
-// then each system's swap, fusion and raycast; all in stream order.
+// systems past their first frame in shared launches on `st` (the others
+// initialise), then each system's swap, fusion and raycast. A system's own
+// stages run on its side stream, so the systems' pyramids overlap, and so do
+// their fusions and raycasts; in a captured round these are the graph's
+// parallel branches.
 static int grab_batch(sicp_system* const* sys, int n, const sicp_image* depth,
                       const sicp_image* rgb, const sicp_image* labels, const int32_t* nd,
                       hipStream_t st) {
     int rc;
     sicp_system* tracked[kMaxBatch];
     int nt = 0;
+    for (int q = 0; q < n; ++q)
+        if ((rc = side_init(sys[q])) || (rc = fork_to(sys[q], st))) return rc;
     for (int q = 0; q < n; ++q) {
+        sicp_system* s = sys[q];
         const sicp_image lab = labels ? labels[q] : sicp_image{nullptr, 0, 0, 0};
-        if ((rc = grab_begin(sys[q], depth[q], rgb[q], lab, nd ? nd[q] : 0, st))) return rc;
-        if (sys[q]->frame_index == 0) {
-            if ((rc = track_single(sys[q], st))) return rc;
+        if ((rc = grab_begin(s, depth[q], rgb[q], lab, nd ? nd[q] : 0, s->side))) return rc;
+        if (s->frame_index == 0) {
+            if ((rc = track_single(s, s->side))) return rc;
         } else {
-            tracked[nt++] = sys[q];
+            tracked[nt++] = s;
         }
+        if ((rc = join_from(s, st))) return rc;
     }
     if (nt && (rc = track_batch(tracked, nt, st))) return rc;
     for (int q = 0; q < n; ++q) {
+        sicp_system* s = sys[q];
         const sicp_image lab = labels ? labels[q] : sicp_image{nullptr, 0, 0, 0};
-        if ((rc = grab_end(sys[q], rgb[q], lab, nd ? nd[q] : 0, st))) return rc;
+        if ((rc = fork_to(s, st)) || (rc = grab_end(s, rgb[q], lab, nd ? nd[q] : 0, s->side)) ||
+            (rc = join_from(s, st)))
+            return rc;
     }
     return SICP_OK;
 }
@@ -882,12 +923,22 @@ int sicp_system_frame_image(sicp_system* s, int which, int level, sicp_image* ou
 
 // ---- raycast launch tuning -------------------------------------------------
 // The raycast's two launch-shape parameters leave its output unchanged, so
-// they are chosen by measurement: sicp_system_autotune re-runs the last
-// frame's raycast (same map, zRange images and pose, into scratch maps)
-// under every candidate, interleaved over `repeats` rounds, and keeps the
-// candidate with the lowest median launch time.
+// they are chosen by measurement. Each sicp_system_autotune call re-runs the
+// last frame's raycast (same map, zRange images and pose, into scratch maps)
+// under every candidate, interleaved over `repeats` rounds, and adds each
+// candidate's median time to the system's running sums; the candidate with
+// the lowest sum so far becomes the tuning. Calls on several frames of a
+// sequence therefore choose for the sequence, not for one view.
 
 namespace sicp {
+
+// Candidate 0 is the default shape.
+static const RayTuning kTuneCandidates[] = {
+    {48, 1024},  {16, 256},  {16, 512},  {16, 1024}, {16, 2048},  {24, 256},  {24, 512},
+    {24, 1024},  {24, 2048}, {32, 256},  {32, 512},  {32, 1024},  {32, 2048}, {48, 256},
+    {48, 512},   {48, 2048}, {64, 256},  {64, 512},  {64, 1024},  {64, 2048}, {96, 256},
+    {96, 512},   {96, 1024}, {96, 2048}, {128, 256}, {128, 512},  {128, 1024}, {128, 2048}};
+constexpr int kNumTuneCandidates = (int)(sizeof(kTuneCandidates) / sizeof(kTuneCandidates[0]));
 
 struct TuneScratch {
     DevImage<float4> vmap, nmap;
@@ -932,8 +983,28 @@ int sicp_system_set_tuning(sicp_system* s, const sicp_tuning* t) {
     return SICP_OK;
 }
 
+int sicp_system_autotune_reset(sicp_system* s) {
+    if (!s) return invalid("system_autotune_reset: null system");
+    s->tune_sum.clear();
+    s->tune_calls = 0;
+    return SICP_OK;
+}
+
+int sicp_system_autotune_results(const sicp_system* s, sicp_tuning* shapes, float* mean_us,
+                                 int32_t cap, int32_t* count) {
+    if (!s || !count) return invalid("system_autotune_results: null argument");
+    *count = s->tune_calls ? kNumTuneCandidates : 0;
+    for (int c = 0; c < *count && c < cap; ++c) {
+        if (shapes)
+            shapes[c] = sicp_tuning{kTuneCandidates[c].first_pass_samples,
+                                    kTuneCandidates[c].second_pass_workgroups};
+        if (mean_us) mean_us[c] = (float)(1000.0 * s->tune_sum[c] / s->tune_calls);
+    }
+    return SICP_OK;
+}
+
 int sicp_system_autotune(sicp_system* s, int32_t repeats, sicp_tuning* chosen, float* chosen_us,
-                         float* previous_us, void* stream) {
+                         float* default_us, void* stream) {
     if (!s) return invalid("system_autotune: null system");
     if (repeats < 1 || repeats > 64) return invalid("system_autotune: repeats outside 1..64");
     if (s->frame_index == 0) return invalid("system_autotune: no frame processed yet");
@@ -946,13 +1017,7 @@ int sicp_system_autotune(sicp_system* s, int32_t repeats, sicp_tuning* chosen, f
     SICP_HIP(hipStreamSynchronize(st));
     if (!t.tracked || tiles == 0)
         return invalid("system_autotune: the last frame raycast nothing (lost, or no visible block)");
-    // Candidate 0 is the tuning in effect.
-    std::vector<RayTuning> cand{s->tuning};
-    for (int smp : {16, 24, 32, 48, 64, 96, 128})
-        for (int wg : {256, 512, 1024, 2048})
-            if (smp != s->tuning.first_pass_samples || wg != s->tuning.second_pass_workgroups)
-                cand.push_back(RayTuning{smp, wg});
-    const int nc = (int)cand.size();
+    const int nc = kNumTuneCandidates;
     TuneScratch ts;
     int rc;
     if ((rc = ts.vmap.create(s->cfg.cols, s->cfg.rows)) ||
@@ -973,29 +1038,31 @@ int sicp_system_autotune(sicp_system* s, int32_t repeats, sicp_tuning* chosen, f
         for (int c = 0; c < nc; ++c) {
             hipEvent_t* e = &ts.events[(size_t)2 * (r * nc + c)];
             SICP_HIP(hipEventRecord(e[0], st));
-            if ((rc = run(cand[c]))) return rc;
+            if ((rc = run(kTuneCandidates[c]))) return rc;
             SICP_HIP(hipEventRecord(e[1], st));
         }
     SICP_HIP(hipStreamSynchronize(st));
-    std::vector<float> med(nc), x(repeats);
+    std::vector<float> x(repeats);
+    if (s->tune_sum.size() != (size_t)nc) s->tune_sum.assign(nc, 0.0);
     for (int c = 0; c < nc; ++c) {
         for (int r = 0; r < repeats; ++r) {
             const hipEvent_t* e = &ts.events[(size_t)2 * (r * nc + c)];
             SICP_HIP(hipEventElapsedTime(&x[r], e[0], e[1]));
         }
         std::sort(x.begin(), x.end());
-        med[c] = x[repeats / 2];
+        s->tune_sum[c] += x[repeats / 2];
     }
+    s->tune_calls++;
     int best = 0;
     for (int c = 1; c < nc; ++c)
-        if (med[c] < med[best]) best = c;
+        if (s->tune_sum[c] < s->tune_sum[best]) best = c;
     // Leave the hand-over queue count as the chosen launch shape leaves it.
-    if ((rc = run(cand[best]))) return rc;
+    if ((rc = run(kTuneCandidates[best]))) return rc;
     SICP_HIP(hipStreamSynchronize(st));
-    set_tuning(s, cand[best]);
+    set_tuning(s, kTuneCandidates[best]);
     if (chosen) (void)sicp_system_get_tuning(s, chosen);
-    if (chosen_us) *chosen_us = 1000.0f * med[best];
-    if (previous_us) *previous_us = 1000.0f * med[0];
+    if (chosen_us) *chosen_us = (float)(1000.0 * s->tune_sum[best] / s->tune_calls);
+    if (default_us) *default_us = (float)(1000.0 * s->tune_sum[0] / s->tune_calls);
     return SICP_OK;
 }
 

@@ -40,6 +40,32 @@ def test_system_vs_oracle(dev, c, frames, half, k32):
             assert na == nb and rg["visible_blocks"] == ro["visible_blocks"]
 
 
+@pytest.mark.parametrize("jump,tracked", [(9, 0), (8, 1)])
+def test_pose_jump_motion_gate(dev, jump, tracked):
+    """Frame 0 then frame `jump` (true motion 0.104 m / 0.093 m): tracking
+    converges either way; the 0.1 m translation gate (Tracking.cc:505-516)
+    loses the first frame, restoring the pose and skipping its fusion, and
+    keeps the second, as the oracle does."""
+    c = CFG_SMALL
+    cfg = sys_cfg(c)
+    g = B.System(cfg)
+    o = O.OracleSystem(cfg)
+    s = synth(c)
+    out = []
+    for i in (0, jump):
+        d, rgb, lab = s.render(i)
+        g.grab(to_dev(d, dev), to_dev(rgb, dev), to_dev(lab, dev))
+        out.append((g.result(), o.grab(d, rgb, lab)))
+    (g0, _), (rg, ro) = out
+    assert rg["tracked"] == ro["tracked"] == tracked, (rg, ro)
+    assert 0 < rg["icp_error"] < cfg.icp_thresh
+    assert rg["allocated_blocks"] == ro["allocated_blocks"]
+    assert np.abs(rg["pose"] - ro["pose"]).max() < 1e-4
+    if not tracked:
+        assert rg["iterations"] == 18 and np.array_equal(rg["pose"], g0["pose"])
+        assert rg["allocated_blocks"] == g0["allocated_blocks"]
+
+
 def test_system_first_frame_model_bit_exact(dev):
     """Frame 0 has no tracking: its fused map and raycast model must match the
     oracle exactly."""
