@@ -649,6 +649,11 @@ def main():
                     help="skip the pass with the photometric tracking term")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the graph-form pass (counter collection runs)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the raycast's default launch shape (counter runs)")
+    ap.add_argument("--ray-lds-table", type=int, default=1, choices=(0, 1),
+                    help="0: the raycast's first pass without its LDS block table (counter "
+                         "runs that isolate the hash probes' L2 traffic)")
     ap.add_argument("--workload", default="vga", choices=("vga", "batched8", "adjoint720"),
                     help="vga: the 640x480 / 5 mm sequence (default); batched8: eight "
                          "independent sequences at once; adjoint720: the raycast adjoint at "
@@ -705,11 +710,12 @@ def main():
 
     # The autotuner on a separate pass over the sequence, called after every
     # 30th frame; its sums over those frames pick the raycast launch shape.
-    tune_frames = list(range(29, n, 30))
+    tune_frames = [] if args.no_autotune else list(range(29, n, 30))
     log(f"[bench] autotuning the raycast launch shape at {len(tune_frames)} frames")
     tuner = B.System(cfg)
+    tuner.set_tuning(48, 1024, args.ray_lds_table)
     tuning, used = None, []
-    for i in range(n):
+    for i in range(tune_frames[-1] + 1 if tune_frames else 0):
         tuner.grab(depth[i], rgb[i], lab[i], stream)
         if i in tune_frames:
             try:
@@ -728,7 +734,8 @@ def main():
         timed frames; with `graph`, every frame is copied into the system's
         staged inputs and runs as one hipGraph launch."""
         sysg = B.System(system_cfg or cfg)
-        sysg.set_tuning(tuning["ray_first_pass_samples"], tuning["ray_second_pass_workgroups"])
+        sysg.set_tuning(tuning["ray_first_pass_samples"], tuning["ray_second_pass_workgroups"],
+                        tuning["ray_lds_block_table"])
 
         def grab(i):
             if graph:

@@ -504,14 +504,18 @@ int sicp_batch_grab_staged(sicp_batch* batch, const int32_t* num_detection, int 
 
 /* Launch shape of the system's raycast (Mapping::RayTrace, Mapping.cc:142-161;
  * the reference's RayCastKernel launch is fixed at 4 x 8 threads per block,
- * RenderScene.cu:528-533): the samples one
+ * RenderScene.cu:528-533) and its block lookup cache: the samples one
  * first-pass lane marches before handing its ray to the second pass, and the
  * second pass's 256-thread workgroups. Every valid choice gives bit-identical
- * model maps. Defaults 48 and 1024; set_tuning takes 1..4096 and 1..65536.
+ * model maps. Defaults 48, 1024 and 1; set_tuning takes 1..4096, 1..65536, 0/1.
  * A change re-records the system's (and its batches') graphs on next use. */
 typedef struct sicp_tuning {
     int32_t ray_first_pass_samples;
     int32_t ray_second_pass_workgroups;
+    int32_t ray_lds_block_table; /* 1 (default): the first pass's block lookups go
+                                    through a per-wavefront LDS table; 0: only each
+                                    lane's one-entry cache (DeviceMap.cu:115-133).
+                                    The autotuner keeps the value set here. */
 } sicp_tuning;
 int sicp_system_get_tuning(const sicp_system* sys, sicp_tuning* out);
 int sicp_system_set_tuning(sicp_system* sys, const sicp_tuning* tuning);

@@ -58,7 +58,8 @@ class MapStats(C.Structure):
 
 class Tuning(C.Structure):
     """sicp_tuning: the raycast's launch shape."""
-    _fields_ = [("ray_first_pass_samples", C.c_int32), ("ray_second_pass_workgroups", C.c_int32)]
+    _fields_ = [("ray_first_pass_samples", C.c_int32), ("ray_second_pass_workgroups", C.c_int32),
+                ("ray_lds_block_table", C.c_int32)]
 
 
 class SystemConfig(C.Structure):
@@ -485,10 +486,12 @@ class System:
         t = Tuning()
         check(lib().sicp_system_get_tuning(self.h, C.byref(t)), "system_get_tuning")
         return dict(ray_first_pass_samples=t.ray_first_pass_samples,
-                    ray_second_pass_workgroups=t.ray_second_pass_workgroups)
+                    ray_second_pass_workgroups=t.ray_second_pass_workgroups,
+                    ray_lds_block_table=t.ray_lds_block_table)
 
-    def set_tuning(self, ray_first_pass_samples, ray_second_pass_workgroups):
-        t = Tuning(ray_first_pass_samples, ray_second_pass_workgroups)
+    def set_tuning(self, ray_first_pass_samples, ray_second_pass_workgroups,
+                   ray_lds_block_table=1):
+        t = Tuning(ray_first_pass_samples, ray_second_pass_workgroups, ray_lds_block_table)
         check(lib().sicp_system_set_tuning(self.h, C.byref(t)), "system_set_tuning")
 
     def autotune(self, repeats=5, stream=None):
@@ -502,6 +505,7 @@ class System:
                                          stream_ptr(stream)), "system_autotune")
         return dict(ray_first_pass_samples=t.ray_first_pass_samples,
                     ray_second_pass_workgroups=t.ray_second_pass_workgroups,
+                    ray_lds_block_table=t.ray_lds_block_table,
                     raycast_us=round(us.value, 2), default_us=round(dflt.value, 2))
 
     def autotune_results(self):

@@ -392,6 +392,7 @@ struct RayArgs {
     float invfx, invfy, cx, cy;
     int cols, rows;
     int first_pass_samples;  // the first pass's hand-over point (raycast_kernel only)
+    int lds_block_table;     // raycast_kernel: block lookups through its LDS table
 };
 
 // The ray of pixel (x, y) (RenderScene.cu:393-421): start point and unit
@@ -456,7 +457,8 @@ __global__ __launch_bounds__(64) void raycast_kernel(MapDev m, View<float4> vmap
         bool valid_sdf = false, back_face = false;
         float step, prev_sdf;
         BlockCache cache{0, 0, 0, -1,
-                         m.num_blocks < (1 << 21) - 2 ? lookup_table : nullptr};
+                         g.lds_block_table && m.num_blocks < (1 << 21) - 2 ? lookup_table
+                                                                            : nullptr};
         int iters = 0;
         while (dist_s < ray.dist_e) {
             if (iters == g.first_pass_samples) {
@@ -991,7 +993,7 @@ int launch_raycast_adjoint(MapDev m, View<float> zmin, View<float> zmax, PoseArg
                            const int32_t* pixels, int n, const float* grad_v,
                            const float* grad_n, float* grad_sdf, float* grad_px,
                            double* partials, double* grad_pose, hipStream_t s) {
-    AdjointArgs a{RayArgs{invfx, invfy, cx, cy, cols, rows, 0}, pixels, n, grad_v, grad_n,
+    AdjointArgs a{RayArgs{invfx, invfy, cx, cy, cols, rows, 0, 0}, pixels, n, grad_v, grad_n,
                   grad_sdf, grad_px, partials};
     int nb = (int)div_up(n > 0 ? n : 1, kAdjThreads);
     if (nb > kAdjMaxBlocks) nb = kAdjMaxBlocks;
@@ -1032,9 +1034,11 @@ int launch_raycast_gated(MapDev m, View<float4> vmap, View<float4> nmap, View<fl
                          bool queue_cleared, RayTuning tune) {
     if ((size_t)vmap.cols * vmap.rows > m.rq_cap) return invalid("raycast: image above the map's max_cols x max_rows");
     if (tune.first_pass_samples < 1 || tune.first_pass_samples > kRayTuneMaxSamples ||
-        tune.second_pass_workgroups < 1 || tune.second_pass_workgroups > kRayTuneMaxWorkgroups)
-        return invalid("raycast: tuning outside 1..4096 samples, 1..65536 workgroups");
-    RayArgs g{invfx, invfy, cx, cy, vmap.cols, vmap.rows, tune.first_pass_samples};
+        tune.second_pass_workgroups < 1 || tune.second_pass_workgroups > kRayTuneMaxWorkgroups ||
+        (tune.lds_block_table != 0 && tune.lds_block_table != 1))
+        return invalid("raycast: tuning outside 1..4096 samples, 1..65536 workgroups, table 0/1");
+    RayArgs g{invfx, invfy, cx, cy, vmap.cols, vmap.rows, tune.first_pass_samples,
+              tune.lds_block_table};
     if (!queue_cleared) {
         ray_queue_reset_kernel<<<1, 1, 0, s>>>(m.rq_count);
         SICP_LAUNCH_CHECK("ray_queue_reset_kernel");

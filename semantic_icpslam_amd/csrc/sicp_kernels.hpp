@@ -290,6 +290,7 @@ int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible,
 struct RayTuning {
     int first_pass_samples = 48;     // samples a first-pass lane marches before the hand-over
     int second_pass_workgroups = 1024;  // 256-thread workgroups draining the hand-over queue
+    int lds_block_table = 1;  // first pass: block lookups through the wavefront's LDS table
 };
 constexpr int kRayTuneMaxSamples = 4096;
 constexpr int kRayTuneMaxWorkgroups = 65536;

@@ -953,12 +953,22 @@ struct TuneScratch {
 
 static bool tuning_ok(const sicp_tuning& t) {
     return t.ray_first_pass_samples >= 1 && t.ray_first_pass_samples <= kRayTuneMaxSamples &&
-           t.ray_second_pass_workgroups >= 1 && t.ray_second_pass_workgroups <= kRayTuneMaxWorkgroups;
+           t.ray_second_pass_workgroups >= 1 &&
+           t.ray_second_pass_workgroups <= kRayTuneMaxWorkgroups &&
+           (t.ray_lds_block_table == 0 || t.ray_lds_block_table == 1);
+}
+
+// Candidate c with the system's own block-table setting.
+static RayTuning tune_candidate(const sicp_system* s, int c) {
+    RayTuning t = kTuneCandidates[c];
+    t.lds_block_table = s->tuning.lds_block_table;
+    return t;
 }
 
 static void set_tuning(sicp_system* s, const RayTuning& t) {
     if (t.first_pass_samples == s->tuning.first_pass_samples &&
-        t.second_pass_workgroups == s->tuning.second_pass_workgroups)
+        t.second_pass_workgroups == s->tuning.second_pass_workgroups &&
+        t.lds_block_table == s->tuning.lds_block_table)
         return;
     s->tuning = t;
     s->tuning_epoch++;
@@ -972,14 +982,17 @@ int sicp_system_get_tuning(const sicp_system* s, sicp_tuning* out) {
     if (!s || !out) return invalid("system_get_tuning: null argument");
     out->ray_first_pass_samples = s->tuning.first_pass_samples;
     out->ray_second_pass_workgroups = s->tuning.second_pass_workgroups;
+    out->ray_lds_block_table = s->tuning.lds_block_table;
     return SICP_OK;
 }
 
 int sicp_system_set_tuning(sicp_system* s, const sicp_tuning* t) {
     if (!s || !t) return invalid("system_set_tuning: null argument");
     if (!tuning_ok(*t))
-        return invalid("system_set_tuning: samples outside 1..4096 or workgroups outside 1..65536");
-    set_tuning(s, RayTuning{t->ray_first_pass_samples, t->ray_second_pass_workgroups});
+        return invalid("system_set_tuning: samples outside 1..4096, workgroups outside "
+                       "1..65536 or block table not 0/1");
+    set_tuning(s, RayTuning{t->ray_first_pass_samples, t->ray_second_pass_workgroups,
+                            t->ray_lds_block_table});
     return SICP_OK;
 }
 
@@ -995,9 +1008,11 @@ int sicp_system_autotune_results(const sicp_system* s, sicp_tuning* shapes, floa
     if (!s || !count) return invalid("system_autotune_results: null argument");
     *count = s->tune_calls ? kNumTuneCandidates : 0;
     for (int c = 0; c < *count && c < cap; ++c) {
-        if (shapes)
-            shapes[c] = sicp_tuning{kTuneCandidates[c].first_pass_samples,
-                                    kTuneCandidates[c].second_pass_workgroups};
+        if (shapes) {
+            const RayTuning t = tune_candidate(s, c);
+            shapes[c] = sicp_tuning{t.first_pass_samples, t.second_pass_workgroups,
+                                    t.lds_block_table};
+        }
         if (mean_us) mean_us[c] = (float)(1000.0 * s->tune_sum[c] / s->tune_calls);
     }
     return SICP_OK;
@@ -1038,7 +1053,7 @@ int sicp_system_autotune(sicp_system* s, int32_t repeats, sicp_tuning* chosen, f
         for (int c = 0; c < nc; ++c) {
             hipEvent_t* e = &ts.events[(size_t)2 * (r * nc + c)];
             SICP_HIP(hipEventRecord(e[0], st));
-            if ((rc = run(kTuneCandidates[c]))) return rc;
+            if ((rc = run(tune_candidate(s, c)))) return rc;
             SICP_HIP(hipEventRecord(e[1], st));
         }
     SICP_HIP(hipStreamSynchronize(st));
@@ -1057,9 +1072,9 @@ int sicp_system_autotune(sicp_system* s, int32_t repeats, sicp_tuning* chosen, f
     for (int c = 1; c < nc; ++c)
         if (s->tune_sum[c] < s->tune_sum[best]) best = c;
     // Leave the hand-over queue count as the chosen launch shape leaves it.
-    if ((rc = run(kTuneCandidates[best]))) return rc;
+    if ((rc = run(tune_candidate(s, best)))) return rc;
     SICP_HIP(hipStreamSynchronize(st));
-    set_tuning(s, kTuneCandidates[best]);
+    set_tuning(s, tune_candidate(s, best));
     if (chosen) (void)sicp_system_get_tuning(s, chosen);
     if (chosen_us) *chosen_us = (float)(1000.0 * s->tune_sum[best] / s->tune_calls);
     if (default_us) *default_us = (float)(1000.0 * s->tune_sum[0] / s->tune_calls);

@@ -12,7 +12,8 @@ from helpers import B, CFG_VGA, download, sys_cfg
 pytestmark = pytest.mark.gpu
 
 FRAMES = 8
-SHAPES = [(48, 1024), (4, 64), (4096, 1), (17, 300)]
+# (first-pass samples, second-pass workgroups, LDS block table)
+SHAPES = [(48, 1024, 1), (4, 64, 1), (4096, 1, 1), (17, 300, 1), (48, 1024, 0), (4, 64, 0)]
 
 
 @pytest.fixture(scope="module")
@@ -100,16 +101,24 @@ def test_autotune_in_graph_form_equals_default(frames, default_run):
     assert 0 < tuned["raycast_us"] == min(means) and tuned["default_us"] == means[0]
     # the second call's means average both calls
     assert means != [t[2] for t in table1]
+    assert tuned["ray_lds_block_table"] == 1
     # the graphs were recorded again with the new first-pass samples (never 4)
     handover_4 = _run(frames, (4, 64))[2]
     assert got[2][:3] == handover_4[:3]
     assert got[2][3:] != handover_4[3:]
 
 
+def test_autotune_keeps_the_block_table_setting(frames, default_run):
+    got = _run(frames, (4, 64, 0), autotune_at=(2,))
+    _same(got, default_run)
+    assert got[3][0][0]["ray_lds_block_table"] == 0
+
+
 def test_tuning_arguments(dev):
     g = B.System(sys_cfg(CFG_VGA))
-    assert g.tuning() == dict(ray_first_pass_samples=48, ray_second_pass_workgroups=1024)
-    for bad in ((0, 1024), (4097, 1024), (48, 0), (48, 65537)):
+    assert g.tuning() == dict(ray_first_pass_samples=48, ray_second_pass_workgroups=1024,
+                              ray_lds_block_table=1)
+    for bad in ((0, 1024, 1), (4097, 1024, 1), (48, 0, 1), (48, 65537, 1), (48, 1024, 2)):
         t = B.Tuning(*bad)
         assert B.lib().sicp_system_set_tuning(g.h, B.C.byref(t)) == B.SICP_ERR_INVALID_ARGUMENT
     assert g.tuning()["ray_first_pass_samples"] == 48
