@@ -801,6 +801,13 @@ def main():
         photometric = run_pass(False, system_cfg=cfg_rgb)
     log("[bench] instrumented pass (HIP events per stage and kernel)")
     inst = run_pass(True)
+    # The map at the configured heap size: the same frames at 2 mm voxels and
+    # 6 mm truncation fill about half of the 400k-block heap.
+    cfg_scale = B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"], 0.002,
+                                0.006, w["buckets"], w["blocks"])
+    log("[bench] configured-scale passes (2 mm voxels, the same frames)")
+    scale_clean = run_pass(False, system_cfg=cfg_scale)
+    scale_inst = run_pass(True, system_cfg=cfg_scale)
     st0, st1, res = inst["st0"], inst["st1"], clean["res"]
     tot, cnt, fm = inst["tot"], inst["cnt"], inst["frame_ms"]
     gt = syn.pose(n - 1)
@@ -962,6 +969,22 @@ def main():
                 float(np.linalg.norm(precise["res"]["pose"][:3, 3] - gt[:3, 3])), 5),
             "vs_fp64": prec},
         "pose_precision": pose_check,
+        "at_scale": {
+            "what": "the same frames with 2 mm voxels and 6 mm truncation (2^20 x 2 buckets, "
+                    "400k-block heap), plain FP32, eager launches: the heap-slot visibility "
+                    "scan and the table at the configured heap scale",
+            "ms_per_step": round(1000.0 * scale_clean["elapsed"] / args.steps, 4),
+            "allocated_blocks": scale_inst["st1"]["allocated_blocks"],
+            "heap_blocks": w["blocks"],
+            "visible_blocks_per_frame": int(
+                (scale_inst["st1"]["visible_total"] - scale_inst["st0"]["visible_total"]) /
+                max(scale_inst["st1"]["fusions"] - scale_inst["st0"]["fusions"], 1)),
+            "stage_ms_per_frame": {
+                s_: round(float(scale_inst["tot"][k] / max(scale_inst["cnt"][k], 1)), 4)
+                for k, s_ in enumerate(STAGES) if s_ not in ("icp_iteration", "unify")},
+            "last_frame_tracked": scale_clean["res"]["tracked"],
+            "last_frame_translation_vs_ground_truth_m": round(
+                float(np.linalg.norm(scale_clean["res"]["pose"][:3, 3] - gt[:3, 3])), 5)},
         "photometric_tracking": None if photometric is None else {
             "ms_per_step": round(1000.0 * photometric["elapsed"] / args.steps, 4),
             "what": "the same frames with the reference's default tracking, ComputeSE3(false): "

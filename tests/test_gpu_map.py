@@ -109,6 +109,24 @@ def test_fusion_vga_sequence_bit_exact(dev, frames):
     assert (sem["label2"] > 0).sum() > 0
 
 
+def test_fusion_at_configured_heap_scale_bit_exact(dev):
+    """The map at the configured heap size (400k blocks): the 640x480
+    sequence fused at 2 mm voxels (every 10th frame of 300, at ground-truth
+    poses) allocates about 199k blocks into a 2^16-bucket table (131k
+    entries, so about 68k blocks live in overflow chains), with about 100k
+    visible blocks per frame. Table, chains, block list, both voxel planes
+    and every frame's visible count equal the oracle's."""
+    c = dict(CFG_VGA, voxel=0.002, trunc=0.006)
+    cfg = map_cfg(c, num_buckets=1 << 16, num_blocks=400000)
+    gm, om = B.Map(cfg), O.OracleMap(cfg)
+    _, vis = _fuse_both(dev, c, range(0, 300, 10), gm, om)
+    n = _compare_maps(gm, om)
+    print("allocated", n, "max visible", max(vis), flush=True)
+    assert n > 180000 and max(vis) > 90000
+    assert (om.export()["excess"]["slot"] >= 0).sum() > 50000
+    assert om.stats()["dropped_heap"] == 0
+
+
 def test_fusion_overflow_chains_and_heap_exhaustion(dev):
     """A 2^8-bucket table forces long overflow chains; a 700-block heap runs
     out, and both sides must drop the same requests and count them."""
