@@ -139,9 +139,10 @@ __global__ __launch_bounds__(256) void prepass_kernel(MapDev m, MapCounters* ctr
 }
 
 // ------------------------------------------------------------------ raycast --
-// The march is bound by instruction issue (about five wavefronts share each
-// SIMD for the whole kernel), so voxel addressing is written for few
-// instructions: floor once per distinct float coordinate, block = v >> 3 and
+// Every march iteration is a dependent chain (address, lookup, voxel load,
+// decision), run by all of the launch's wavefronts at once (about five share
+// each SIMD), so voxel addressing is written for few instructions: floor
+// once per distinct float coordinate, block = v >> 3 and
 // local = v & 7 (the same values as voxelPosToBlockPos / voxelPosToLocalIdx,
 // DeviceMap.cu:169-201, for every int), and one cache compare per block.
 struct BlockCache {
