@@ -649,6 +649,8 @@ def main():
                     help="skip the pass with the photometric tracking term")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the graph-form pass (counter collection runs)")
+    ap.add_argument("--no-scale", action="store_true",
+                    help="skip the configured-scale passes (counter runs of the 5 mm workload)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the raycast's default launch shape (counter runs)")
     ap.add_argument("--ray-lds-table", type=int, default=1, choices=(0, 1),
@@ -807,9 +809,11 @@ def main():
     # the oracle does too.
     cfg_scale = B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"], 0.002,
                                 0.010, w["buckets"], w["blocks"])
-    log("[bench] configured-scale passes (2 mm voxels, the same frames)")
-    scale_clean = run_pass(False, system_cfg=cfg_scale)
-    scale_inst = run_pass(True, system_cfg=cfg_scale)
+    scale_clean = scale_inst = None
+    if not args.no_scale:
+        log("[bench] configured-scale passes (2 mm voxels, the same frames)")
+        scale_clean = run_pass(False, system_cfg=cfg_scale)
+        scale_inst = run_pass(True, system_cfg=cfg_scale)
     st0, st1, res = inst["st0"], inst["st1"], clean["res"]
     tot, cnt, fm = inst["tot"], inst["cnt"], inst["frame_ms"]
     gt = syn.pose(n - 1)
@@ -878,8 +882,9 @@ def main():
     roofline = dict(kernels[dom])
     roofline["traffic_source"] = (
         "profiles/pmc_traffic_vga_5mm_sequence.json: rocprofv3 FETCH_SIZE and WRITE_SIZE "
-        "passes over bench.py --no-graph on this 300-frame workload, each scaled by the same "
-        "counter on a plain device copy")
+        "passes over bench.py --no-graph --no-autotune --no-scale on this 300-frame workload "
+        "(the same kernels, default raycast shape), each scaled by the same counter on a plain "
+        "device copy")
 
     # Every stage of the frame: algorithmic bytes per frame over the stage's
     # event time (the instrumented pass, timed frames).
@@ -971,7 +976,7 @@ def main():
                 float(np.linalg.norm(precise["res"]["pose"][:3, 3] - gt[:3, 3])), 5),
             "vs_fp64": prec},
         "pose_precision": pose_check,
-        "at_scale": {
+        "at_scale": None if scale_inst is None else {
             "what": "the same frames with 2 mm voxels and 10 mm truncation (2^20 x 2 buckets, "
                     "400k-block heap), plain FP32, eager launches: the heap-slot visibility "
                     "scan and the table at the configured heap scale",
