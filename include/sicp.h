@@ -529,8 +529,9 @@ int sicp_system_set_tuning(sicp_system* sys, const sicp_tuning* tuning);
  * mean time per call (microseconds; either pointer may be NULL). The
  * frame's own maps are untouched. Needs a processed frame that was tracked
  * and rendered; synchronises `stream`; allocates scratch for the call.
- * autotune_reset clears the sums; autotune_results lists every candidate
- * with its mean (up to `cap`; `count` = 28, or 0 before any call). */
+ * autotune_reset clears the sums, as set_tuning does when it changes the
+ * block-table setting; autotune_results lists every candidate with its mean
+ * (up to `cap`; `count` = 28, or 0 before any call). */
 int sicp_system_autotune(sicp_system* sys, int32_t repeats, sicp_tuning* chosen,
                          float* chosen_us, float* default_us, void* stream);
 int sicp_system_autotune_reset(sicp_system* sys);

@@ -991,6 +991,9 @@ int sicp_system_set_tuning(sicp_system* s, const sicp_tuning* t) {
     if (!tuning_ok(*t))
         return invalid("system_set_tuning: samples outside 1..4096, workgroups outside "
                        "1..65536 or block table not 0/1");
+    // The autotuner's sums were measured with the block-table setting in
+    // effect; a new setting starts them again.
+    if (t->ray_lds_block_table != s->tuning.lds_block_table) (void)sicp_system_autotune_reset(s);
     set_tuning(s, RayTuning{t->ray_first_pass_samples, t->ray_second_pass_workgroups,
                             t->ray_lds_block_table});
     return SICP_OK;

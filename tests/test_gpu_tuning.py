@@ -111,7 +111,21 @@ def test_autotune_in_graph_form_equals_default(frames, default_run):
 def test_autotune_keeps_the_block_table_setting(frames, default_run):
     got = _run(frames, (4, 64, 0), autotune_at=(2,))
     _same(got, default_run)
-    assert got[3][0][0]["ray_lds_block_table"] == 0
+    (tuned, table), = got[3]
+    assert tuned["ray_lds_block_table"] == 0 and len(table) == 28
+
+
+def test_block_table_change_clears_the_autotuner_sums(frames):
+    g = B.System(sys_cfg(CFG_VGA))
+    for d, c, lab in frames[:3]:
+        g.grab(d, c, lab)
+    g.autotune(repeats=1)
+    assert len(g.autotune_results()) == 28
+    g.set_tuning(32, 512, 1)  # same table setting: the sums stay
+    assert len(g.autotune_results()) == 28
+    g.set_tuning(32, 512, 0)
+    assert g.autotune_results() == []
+    g.close()
 
 
 def test_tuning_arguments(dev):
