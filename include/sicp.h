@@ -501,6 +501,12 @@ int sicp_batch_create(sicp_system* const* systems, int32_t n, sicp_batch** out);
 int sicp_batch_destroy(sicp_batch* batch);
 int sicp_batch_grab_staged(sicp_batch* batch, const int32_t* num_detection, int use_graph,
                            void* stream);
+/* shared = 1 (default): the tracking iterations of the batch's systems share
+ * launches, as above. shared = 0: each system's whole frame, its own
+ * tracking launches included, runs on a stream of its own, so a round is n
+ * independent chains (parallel branches of the round's graph) joined at its
+ * end. Results are the same either way; the graphs are recorded again. */
+int sicp_batch_set_shared_tracking(sicp_batch* batch, int shared);
 
 /* Launch shape of the system's raycast (Mapping::RayTrace, Mapping.cc:142-161;
  * the reference's RayCastKernel launch is fixed at 4 x 8 threads per block,

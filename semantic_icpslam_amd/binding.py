@@ -179,6 +179,7 @@ _SIGS = {
     "sicp_batch_create": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
     "sicp_batch_destroy": (C.c_int, [_P]),
     "sicp_batch_grab_staged": (C.c_int, [_P, _P, C.c_int, _P]),
+    "sicp_batch_set_shared_tracking": (C.c_int, [_P, C.c_int]),
     "sicp_system_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "sicp_system_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "sicp_system_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(Tuning), C.POINTER(C.c_float),
@@ -553,6 +554,12 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def set_shared_tracking(self, shared):
+        """True: the systems' tracking iterations share launches; False: each
+        system's frame is an independent chain of the round."""
+        check(lib().sicp_batch_set_shared_tracking(self.h, 1 if shared else 0),
+              "sicp_batch_set_shared_tracking")
 
     def grab_staged(self, frames, use_graph=True, stream=None):
         """Copies every system's frame into its staged inputs on `stream` and

@@ -543,14 +543,27 @@ static int join_from(sicp_system* s, hipStream_t st) {
 // stages run on its side stream, so the systems' pyramids overlap, and so do
 // their fusions and raycasts; in a captured round these are the graph's
 // parallel branches.
+// With `shared` false, every system's whole frame (its own tracking
+// launches included) runs on its side stream instead: the round is n
+// independent chains, joined at its end.
 static int grab_batch(sicp_system* const* sys, int n, const sicp_image* depth,
                       const sicp_image* rgb, const sicp_image* labels, const int32_t* nd,
-                      hipStream_t st) {
+                      hipStream_t st, bool shared = true) {
     int rc;
     sicp_system* tracked[kMaxBatch];
     int nt = 0;
     for (int q = 0; q < n; ++q)
         if ((rc = side_init(sys[q])) || (rc = fork_to(sys[q], st))) return rc;
+    if (!shared) {
+        for (int q = 0; q < n; ++q) {
+            sicp_system* s = sys[q];
+            const sicp_image lab = labels ? labels[q] : sicp_image{nullptr, 0, 0, 0};
+            if ((rc = grab_frame(s, depth[q], rgb[q], lab, nd ? nd[q] : 0, s->side)) ||
+                (rc = join_from(s, st)))
+                return rc;
+        }
+        return SICP_OK;
+    }
     for (int q = 0; q < n; ++q) {
         sicp_system* s = sys[q];
         const sicp_image lab = labels ? labels[q] : sicp_image{nullptr, 0, 0, 0};
@@ -751,6 +764,7 @@ struct sicp_batch {
     std::vector<int32_t> nd;  // num_detection the graphs were captured with
     std::vector<uint64_t> epochs;  // the systems' tuning epochs at capture
     hipStream_t capture_stream = nullptr;
+    bool shared = true;  // tracking iterations in shared launches (else per-system chains)
 };
 
 namespace sicp {
@@ -820,7 +834,7 @@ static int batch_capture(sicp_batch* b, const int32_t* nd) {
             break;
         }
         rc = grab_batch(b->sys.data(), n, depth.data(), rgb.data(), lab.data(), nd,
-                        b->capture_stream);
+                        b->capture_stream, b->shared);
         e = hipStreamEndCapture(b->capture_stream, &g);
         if (rc == SICP_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
         if (rc == SICP_OK) {
@@ -861,6 +875,15 @@ int sicp_batch_create(sicp_system* const* systems, int32_t n, sicp_batch** out) 
     return SICP_OK;
 }
 
+int sicp_batch_set_shared_tracking(sicp_batch* b, int shared) {
+    if (!b) return invalid("batch_set_shared_tracking: null batch");
+    if ((shared != 0) != b->shared) {
+        b->shared = shared != 0;
+        batch_drop_graphs(b);
+    }
+    return SICP_OK;
+}
+
 int sicp_batch_destroy(sicp_batch* b) {
     if (!b) return SICP_OK;
     batch_drop_graphs(b);
@@ -887,7 +910,7 @@ int sicp_batch_grab_staged(sicp_batch* b, const int32_t* num_detection, int use_
     }
     if (eager)
         return grab_batch(b->sys.data(), n, depth.data(), rgb.data(), lab.data(), nd.data(),
-                          (hipStream_t)stream);
+                          (hipStream_t)stream, b->shared);
     const uint64_t cur = batch_pattern(b);
     int p = (b->graph[0] && b->pattern[0] == cur) ? 0 : ((b->graph[1] && b->pattern[1] == cur) ? 1 : -1);
     if (p < 0 || b->nd != nd || b->epochs != batch_epochs(b)) {

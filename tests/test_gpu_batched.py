@@ -100,14 +100,17 @@ def test_eight_sequences_concurrent_equal_alone(alone):
     assert tracked >= len(SEQS) * (FRAMES - 1) * 3 // 4, tracked
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_eight_sequences_one_launch_per_iteration_equal_alone(alone, graph):
+@pytest.mark.parametrize("graph,shared", [(False, True), (True, True), (True, False)])
+def test_eight_sequences_one_launch_per_iteration_equal_alone(alone, graph, shared):
     """The eight sequences advanced as one batch: their tracking iterations
     share launches (sicp_system_grab_batch), eagerly or as one hipGraph per
-    round (sicp_batch_grab_staged)."""
+    round (sicp_batch_grab_staged); or, unshared, each system's frame as an
+    independent chain of the round's graph."""
     frames, want = alone
     systems = [B.System(_config(w)) for w in SEQS]
     batch = B.Batch(systems) if graph else None
+    if batch and not shared:
+        batch.set_shared_tracking(False)
     res = [[] for _ in SEQS]
     for i in range(FRAMES):
         fr = [frames[k][i] for k in range(len(SEQS))]
@@ -124,7 +127,7 @@ def test_eight_sequences_one_launch_per_iteration_equal_alone(alone, graph):
     for g in systems:
         g.close()
     for k in range(len(SEQS)):
-        print(f"sequence {k} ({'graph' if graph else 'eager'})", flush=True)
+        print(f"sequence {k} ({'graph' if graph else 'eager'}, shared {shared})", flush=True)
         _same(k, got[k], want[k])
 
 
