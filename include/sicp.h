@@ -476,6 +476,12 @@ int sicp_system_read_timing_detail(sicp_system* sys, int stage, double* ms_out, 
 int sicp_system_staging(sicp_system* sys, sicp_image* depth_u16, sicp_image* rgb_u8x3,
                         sicp_image* labels_u8);
 int sicp_system_grab_staged(sicp_system* sys, int num_detection, int use_graph, void* stream);
+/* Fills the staged inputs from a frame with one copy kernel on `stream` (no
+ * counterpart in the reference): depth (u16) and rgb (u8x3) of the system's
+ * size, and labels (u8) unless labels_u8 or its ptr is NULL. Rows are copied
+ * 16 bytes per lane where their addresses, pitches and length allow. */
+int sicp_system_stage(sicp_system* sys, const sicp_image* depth_u16, const sicp_image* rgb_u8x3,
+                      const sicp_image* labels_u8, void* stream);
 
 /* GrabImage of several independent systems (sequences of any resolutions)
  * as one batch, on one stream (no counterpart in the reference, whose System
@@ -501,6 +507,11 @@ int sicp_batch_create(sicp_system* const* systems, int32_t n, sicp_batch** out);
 int sicp_batch_destroy(sicp_batch* batch);
 int sicp_batch_grab_staged(sicp_batch* batch, const int32_t* num_detection, int use_graph,
                            void* stream);
+/* sicp_system_stage for every system of the batch, one kernel for the round:
+ * system i's frame from entry i of each array (labels_u8 NULL, or an entry
+ * with a NULL ptr: no labels for that system). */
+int sicp_batch_stage(sicp_batch* batch, const sicp_image* depth_u16, const sicp_image* rgb_u8x3,
+                     const sicp_image* labels_u8, void* stream);
 /* shared = 1 (default): the tracking iterations of the batch's systems share
  * launches, as above. shared = 0: each system's whole frame, its own
  * tracking launches included, runs on a stream of its own, so a round is n

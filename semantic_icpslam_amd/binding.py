@@ -172,6 +172,8 @@ _SIGS = {
     "sicp_system_read_timing": (C.c_int, [_P, _P, _P, C.c_int, _P]),
     "sicp_system_staging": (C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image)]),
     "sicp_system_grab_staged": (C.c_int, [_P, C.c_int, C.c_int, _P]),
+    "sicp_system_stage": (C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image),
+                                    _P]),
     "sicp_system_read_timing_detail": (C.c_int, [_P, C.c_int, _P, C.c_int, C.POINTER(C.c_int),
                                                  _P]),
     "sicp_system_frame_image": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Image)]),
@@ -179,6 +181,7 @@ _SIGS = {
     "sicp_batch_create": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
     "sicp_batch_destroy": (C.c_int, [_P]),
     "sicp_batch_grab_staged": (C.c_int, [_P, _P, C.c_int, _P]),
+    "sicp_batch_stage": (C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     "sicp_batch_set_shared_tracking": (C.c_int, [_P, C.c_int]),
     "sicp_system_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "sicp_system_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
@@ -195,8 +198,6 @@ _SIGS = {
 EXPORTED = tuple(_SIGS.keys())
 
 
-_hip = None
-
 
 def mesh_tables():
     """The marching-cubes tables sicp_mesh_scene uses: edge (256,),
@@ -207,20 +208,6 @@ def mesh_tables():
     check(lib().sicp_mesh_tables(edge.ctypes.data, tri.ctypes.data, nvert.ctypes.data),
           "sicp_mesh_tables")
     return edge, tri, nvert
-
-
-def copy2d(dst, src, stream=None):
-    """Device-to-device copy of a torch image tensor into a pitched image."""
-    global _hip
-    if _hip is None:
-        _hip = C.CDLL("libamdhip64.so")
-        _hip.hipMemcpy2DAsync.argtypes = [_P, C.c_size_t, _P, C.c_size_t, C.c_size_t, C.c_size_t,
-                                          C.c_int, _P]
-    s = image(src)
-    width = src.shape[1] * src.element_size() * (src.shape[2] if src.dim() == 3 else 1)
-    if _hip.hipMemcpy2DAsync(dst.ptr, dst.pitch, s.ptr, s.pitch, width, s.rows, 3,
-                             stream_ptr(stream)) != 0:
-        raise SicpError(SICP_ERR_HIP, "hipMemcpy2DAsync failed")
 
 
 def lib():
@@ -453,14 +440,17 @@ class System:
         check(lib().sicp_system_staging(self.h, *[C.byref(i) for i in ims]), "system_staging")
         return ims
 
+    def stage(self, depth, rgb, labels=None, stream=None):
+        """Copies the frame into the staged inputs on `stream` (one kernel)."""
+        lab = image(labels) if labels is not None else None
+        check(lib().sicp_system_stage(self.h, C.byref(image(depth)), C.byref(image(rgb)),
+                                      C.byref(lab) if lab is not None else None,
+                                      stream_ptr(stream)), "system_stage")
+
     def grab_staged(self, depth, rgb, labels=None, use_graph=True, stream=None):
         """Copies the frame into the staged inputs on `stream` (device to device)
         and runs GrabImage from them; from the second frame on as one graph."""
-        if not hasattr(self, "_staged"):
-            self._staged = self.staging()
-        for dst, src in zip(self._staged, (depth, rgb, labels)):
-            if src is not None:
-                copy2d(dst, src, stream)
+        self.stage(depth, rgb, labels, stream)
         nd = 1 if labels is not None else 0
         check(lib().sicp_system_grab_staged(self.h, nd, 1 if use_graph else 0, stream_ptr(stream)),
               "system_grab_staged")
@@ -564,13 +554,11 @@ class Batch:
     def grab_staged(self, frames, use_graph=True, stream=None):
         """Copies every system's frame into its staged inputs on `stream` and
         runs the round."""
-        for g, f in zip(self.systems, frames):
-            if not hasattr(g, "_staged"):
-                g._staged = g.staging()
-            for dst, src in zip(g._staged, f):
-                if src is not None:
-                    copy2d(dst, src, stream)
         n = len(self.systems)
+        dep, rgb, lab = (Image * n)(), (Image * n)(), (Image * n)()
+        for q, f in enumerate(frames):
+            dep[q], rgb[q], lab[q] = image(f[0]), image(f[1]), image(f[2])
+        check(lib().sicp_batch_stage(self.h, dep, rgb, lab, stream_ptr(stream)), "batch_stage")
         nd = (C.c_int32 * n)(*[1 if f[2] is not None else 0 for f in frames])
         check(lib().sicp_batch_grab_staged(self.h, nd, 1 if use_graph else 0,
                                            stream_ptr(stream)), "sicp_batch_grab_staged")

@@ -195,6 +195,96 @@ static void system_free(sicp_system* s) {
 
 using namespace sicp;
 
+namespace sicp {
+// Staged inputs: the images of a round (three per system, at most eight
+// systems) copied by one kernel. Image i owns workgroups wg0[i] ..
+// wg0[i + 1] - 1, each copying kStageUnits units (16, 4 or 1 bytes, the
+// widest that the rows' addresses, pitches and length allow) per thread.
+constexpr int kStageMax = 3 * kMaxBatch, kStageUnits = 4, kStageThreads = 256;
+struct StageCopy {
+    const unsigned char* src;
+    unsigned char* dst;
+    size_t spitch, dpitch;
+    uint32_t row_units, units, shift, wg0;
+};
+struct StageArgs {
+    StageCopy c[kStageMax];
+    int n;
+};
+__global__ __launch_bounds__(kStageThreads) void stage_copy_kernel(StageArgs a) {
+    int i = 0;
+    while (i + 1 < a.n && blockIdx.x >= a.c[i + 1].wg0) ++i;
+    const StageCopy& c = a.c[i];
+    const uint32_t u0 = (blockIdx.x - c.wg0) * (kStageThreads * kStageUnits) + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kStageUnits; ++j) {
+        const uint32_t u = u0 + j * kStageThreads;
+        if (u >= c.units) break;
+        const uint32_t row = u / c.row_units, col = u - row * c.row_units;
+        const unsigned char* s = c.src + row * c.spitch + ((size_t)col << c.shift);
+        unsigned char* d = c.dst + row * c.dpitch + ((size_t)col << c.shift);
+        if (c.shift == 4)
+            *(uint4*)d = *(const uint4*)s;
+        else if (c.shift == 2)
+            *(uint32_t*)d = *(const uint32_t*)s;
+        else
+            *d = *s;
+    }
+}
+
+// Adds the copy of `src` into `dst` (same shape, `elem` bytes per element).
+static void stage_add(StageArgs& a, uint32_t& wgs, const sicp_image& src, const sicp_image& dst,
+                      size_t elem) {
+    const size_t row_bytes = (size_t)dst.cols * elem;
+    const uintptr_t any = (uintptr_t)src.ptr | (uintptr_t)dst.ptr | src.pitch | dst.pitch | row_bytes;
+    const uint32_t shift = (any & 15u) == 0 ? 4u : ((any & 3u) == 0 ? 2u : 0u);
+    StageCopy& c = a.c[a.n++];
+    c.src = (const unsigned char*)src.ptr;
+    c.dst = (unsigned char*)dst.ptr;
+    c.spitch = src.pitch;
+    c.dpitch = dst.pitch;
+    c.row_units = (uint32_t)(row_bytes >> shift);
+    c.units = c.row_units * (uint32_t)dst.rows;
+    c.shift = shift;
+    c.wg0 = wgs;
+    wgs += div_up(c.units, (uint32_t)(kStageThreads * kStageUnits));
+}
+
+// The system's staged inputs, allocated on first use.
+static int staged_buffers(sicp_system* s) {
+    if (s->in_depth.ptr) return SICP_OK;
+    int rc;
+    const int c = s->cfg.cols, r = s->cfg.rows;
+    if ((rc = s->in_depth.create(c, r)) || (rc = s->in_rgb.create(c, r, 3)) ||
+        (rc = s->in_labels.create(c, r)))
+        return rc;
+    return SICP_OK;
+}
+
+// Checks one system's frame and adds its copies.
+static int stage_frame(sicp_system* s, const sicp_image* depth, const sicp_image* rgb,
+                       const sicp_image* labels, StageArgs& a, uint32_t& wgs) {
+    const int c = s->cfg.cols, r = s->cfg.rows;
+    if (!depth || !rgb || !image_ok(*depth, 2, c, r) || !image_ok(*rgb, 3, c, r))
+        return invalid("stage: depth / rgb shape mismatch");
+    if (labels && labels->ptr && !image_ok(*labels, 1, c, r))
+        return invalid("stage: label image shape mismatch");
+    const int rc = staged_buffers(s);
+    if (rc) return rc;
+    stage_add(a, wgs, *depth, s->in_depth.img(), 2);
+    stage_add(a, wgs, *rgb, s->in_rgb.img(), 3);
+    if (labels && labels->ptr) stage_add(a, wgs, *labels, s->in_labels.img(), 1);
+    return SICP_OK;
+}
+
+static int stage_launch(const StageArgs& a, uint32_t wgs, hipStream_t st) {
+    if (wgs == 0) return SICP_OK;
+    stage_copy_kernel<<<wgs, kStageThreads, 0, st>>>(a);
+    SICP_LAUNCH_CHECK("stage_copy_kernel");
+    return SICP_OK;
+}
+}  // namespace sicp
+
 extern "C" {
 
 void sicp_system_config_default(sicp_system_config* cfg, int cols, int rows, float fx, float fy,
@@ -664,13 +754,8 @@ int sicp_system_last_result(sicp_system* s, sicp_frame_result* out, void* stream
 int sicp_system_staging(sicp_system* s, sicp_image* depth_u16, sicp_image* rgb_u8x3,
                         sicp_image* labels_u8) {
     if (!s) return invalid("system_staging: null system");
-    if (!s->in_depth.ptr) {
-        int rc;
-        const int c = s->cfg.cols, r = s->cfg.rows;
-        if ((rc = s->in_depth.create(c, r)) || (rc = s->in_rgb.create(c, r, 3)) ||
-            (rc = s->in_labels.create(c, r)))
-            return rc;
-    }
+    const int rc = staged_buffers(s);
+    if (rc) return rc;
     if (depth_u16) *depth_u16 = s->in_depth.img();
     if (rgb_u8x3) *rgb_u8x3 = s->in_rgb.img();
     if (labels_u8) *labels_u8 = s->in_labels.img();
@@ -721,6 +806,16 @@ static int capture_graphs(sicp_system* s, int nd) {
     s->graph_nd = rc == SICP_OK ? nd : -1;
     s->graph_epoch = rc == SICP_OK ? s->tuning_epoch : ~0ull;
     return rc;
+}
+
+int sicp_system_stage(sicp_system* s, const sicp_image* depth_u16, const sicp_image* rgb_u8x3,
+                      const sicp_image* labels_u8, void* stream) {
+    if (!s) return invalid("system_stage: null system");
+    StageArgs a{};
+    uint32_t wgs = 0;
+    const int rc = stage_frame(s, depth_u16, rgb_u8x3, labels_u8, a, wgs);
+    if (rc) return rc;
+    return stage_launch(a, wgs, (hipStream_t)stream);
 }
 
 int sicp_system_grab_staged(sicp_system* s, int num_detection, int use_graph, void* stream) {
@@ -890,6 +985,20 @@ int sicp_batch_destroy(sicp_batch* b) {
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
     delete b;
     return SICP_OK;
+}
+
+int sicp_batch_stage(sicp_batch* b, const sicp_image* depth_u16, const sicp_image* rgb_u8x3,
+                     const sicp_image* labels_u8, void* stream) {
+    if (!b) return invalid("batch_stage: null batch");
+    if (!depth_u16 || !rgb_u8x3) return invalid("batch_stage: null image array");
+    StageArgs a{};
+    uint32_t wgs = 0;
+    for (size_t q = 0; q < b->sys.size(); ++q) {
+        const int rc = stage_frame(b->sys[q], depth_u16 + q, rgb_u8x3 + q,
+                                   labels_u8 ? labels_u8 + q : nullptr, a, wgs);
+        if (rc) return rc;
+    }
+    return stage_launch(a, wgs, (hipStream_t)stream);
 }
 
 int sicp_batch_grab_staged(sicp_batch* b, const int32_t* num_detection, int use_graph,
