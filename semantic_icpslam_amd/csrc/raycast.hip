@@ -77,65 +77,165 @@ SICP_DEV bool project_block(const sicp_block& b, const PoseF& pose, const Prepas
 // runs x < ny and does not offset the tiles' lower-right corner, so a box
 // [ul, lr] covers columns ul.x .. min(ul.x+15, lr.x) (+ ul.x+16 when ny >= 2
 // and lr.x >= ul.x+16) times rows ul.y .. min(ul.y+15, lr.y) (+ ul.y+16 on
-// the same condition).
-__global__ __launch_bounds__(256) void prepass_kernel(MapDev m, MapCounters* ctr,
-                                                      const sicp_block* visible, View<float> zmin,
-                                                      View<float> zmax, PoseArg pa,
-                                                      PrepassArgs g, const int* gate,
-                                                      const uint32_t* fuse_counts) {
-    if (fuse_counts && !(gate && *gate == 0)) {
-        // the fusion's sums (fuse_end_kernel's, updated_voxels reset before
-        // the fusion): 256 per-wavefront counts per one-wavefront workgroup
-        // (lane t adds entries t, t + 64, t + 128, t + 192), integer sums
-        uint32_t c = 0;
-        for (int i = blockIdx.x * 256 + threadIdx.x; i < kFuseBlocks * 4; i += gridDim.x * 256)
-#pragma unroll
-            for (int j = 0; j < 256; j += 64) c += fuse_counts[i + j];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-        if (threadIdx.x == 0 && c) {
-            atomicAdd(&ctr->updated_voxels, c);
-            atomicAdd(&ctr->updated_total, (unsigned long long)c);
-        }
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            ctr->visible_total += ctr->visible;
-            ctr->heap_total += ctr->heap_used;
-            ctr->fusions += 1ull;
-        }
-    }
-    if (gate && *gate == 0) return;
-    const uint32_t nvis = ctr->visible;
-    const PoseF pose = load_pose(pa);
-    uint32_t valid_blocks = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nvis;
-         i += gridDim.x * blockDim.x) {
-        const sicp_block b = visible[i];
-        int ulx, uly, lrx, lry;
-        float z0, z1;
-        if (!project_block(b, pose, g, m.voxel, ulx, uly, lrx, lry, z0, z1)) continue;
-        float dy = (float)lry - (float)uly + 1.0f;
-        int ny = (int)ceilf(dy / 16.0f);
-        if (ny <= 0) continue;
-        ++valid_blocks;
-        const int xe = min(ulx + 15, lrx), ye = min(uly + 15, lry);
-        const bool xx = ny >= 2 && lrx >= ulx + 16;
-        const bool yy = ny >= 2 && lry >= uly + 16;
-        const int lo = __float_as_int(z0), hi = __float_as_int(z1);
-        for (int yi = uly; yi <= ye + (yy ? 1 : 0); ++yi) {
-            int Y = (yi > ye) ? uly + 16 : yi;
-            if (Y >= g.zrows) continue;
-            for (int xi = ulx; xi <= xe + (xx ? 1 : 0); ++xi) {
-                int X = (xi > xe) ? ulx + 16 : xi;
-                if (X >= g.zcols) continue;
-                atomicMin((int*)&zmin.at(Y, X), lo);
-                atomicMax((int*)&zmax.at(Y, X), hi);
-            }
+// the same condition). `tile(Y, X, lo, hi)` receives every covered tile with
+// the block's depth range as int bit patterns (positive floats order as
+// their bits do). Returns whether the block produced tiles.
+template <class TileFn>
+SICP_DEV bool block_tiles(const sicp_block& b, const PoseF& pose, const PrepassArgs& g,
+                          float voxel, TileFn tile) {
+    int ulx, uly, lrx, lry;
+    float z0, z1;
+    if (!project_block(b, pose, g, voxel, ulx, uly, lrx, lry, z0, z1)) return false;
+    float dy = (float)lry - (float)uly + 1.0f;
+    int ny = (int)ceilf(dy / 16.0f);
+    if (ny <= 0) return false;
+    const int xe = min(ulx + 15, lrx), ye = min(uly + 15, lry);
+    const bool xx = ny >= 2 && lrx >= ulx + 16;
+    const bool yy = ny >= 2 && lry >= uly + 16;
+    const int lo = __float_as_int(z0), hi = __float_as_int(z1);
+    for (int yi = uly; yi <= ye + (yy ? 1 : 0); ++yi) {
+        int Y = (yi > ye) ? uly + 16 : yi;
+        if (Y >= g.zrows) continue;
+        for (int xi = ulx; xi <= xe + (xx ? 1 : 0); ++xi) {
+            int X = (xi > xe) ? ulx + 16 : xi;
+            if (X >= g.zcols) continue;
+            tile(Y, X, lo, hi);
         }
     }
-    // One atomic per wavefront on the shared counter.
+    return true;
+}
+
+// What a prepass thread reads before it can decide anything, loaded at once:
+// the gate, the counters, the pose, its first visible block (at a clamped
+// index, used only below the visible count) and its share of the fusion's
+// per-wavefront counts. Each is a round trip to memory the previous launches
+// wrote; issued one after another, as early returns would order them, they
+// set the length of a launch this small.
+struct PrepassIn {
+    bool gated;
+    uint32_t nvis, heap_used, fuse_sum;
+    uint64_t visible_total, heap_total, fusions;
+    PoseF pose;
+    sicp_block b0;
+};
+// kThreads, kGrid: the launch's workgroup size and count (constants, so the
+// kernel does not load them from the dispatch packet, one more round trip).
+template <int kThreads, int kGrid>
+SICP_DEV PrepassIn prepass_loads(const MapDev& m, const MapCounters* ctr,
+                                 const sicp_block* visible, const PoseArg& pa, const int* gate,
+                                 const uint32_t* fuse_counts) {
+    PrepassIn in;
+    const uint32_t i0 = blockIdx.x * kThreads + threadIdx.x;
+    // relaxed atomic loads: issued here, not sunk below the gate's test
+    auto ld = [](const auto* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
+    in.nvis = ld(&ctr->visible);
+    in.heap_used = ld(&ctr->heap_used);
+    in.visible_total = ld(&ctr->visible_total);
+    in.heap_total = ld(&ctr->heap_total);
+    in.fusions = ld(&ctr->fusions);
+    in.b0 = visible[i0 < (uint32_t)m.num_blocks ? i0 : 0u];
+    static_assert(kThreads * kGrid >= kFuseBlocks * 4, "one fusion count per thread at most");
+    in.fuse_sum = fuse_counts && i0 < (uint32_t)(kFuseBlocks * 4) ? fuse_counts[i0] : 0u;
+    in.pose = load_pose(pa);
+    in.gated = gate && *gate == 0;  // last: its null test would order the loads after it
+    return in;
+}
+
+// The fusion's sums (fuse_end_kernel's, updated_voxels reset before the
+// fusion) from its kFuseBlocks * 4 per-wavefront counts: integer sums, one
+// atomic per wavefront; workgroup 0 also advances the running totals.
+SICP_DEV void fusion_sums(MapCounters* ctr, const PrepassIn& in) {
+    uint32_t c = in.fuse_sum;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) {
+        atomicAdd(&ctr->updated_voxels, c);
+        atomicAdd(&ctr->updated_total, (unsigned long long)c);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctr->visible_total = in.visible_total + in.nvis;
+        ctr->heap_total = in.heap_total + in.heap_used;
+        ctr->fusions = in.fusions + 1ull;
+    }
+}
+
+SICP_DEV void count_render_blocks(MapCounters* ctr, uint32_t valid_blocks) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) valid_blocks += __shfl_xor(valid_blocks, off, 64);
     if ((threadIdx.x & 63) == 0 && valid_blocks) atomicAdd(&ctr->render_tiles, valid_blocks);
+}
+
+// The visible blocks of this thread (the first one preloaded), each with
+// its tiles passed to `tile`; returns the number that produced tiles.
+template <int kThreads, int kGrid, class TileFn>
+SICP_DEV uint32_t prepass_blocks(const MapDev& m, const sicp_block* visible, const PrepassIn& in,
+                                 const PrepassArgs& g, TileFn tile) {
+    uint32_t valid_blocks = 0;
+    const uint32_t stride = kGrid * kThreads;
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < in.nvis; i += stride)
+        valid_blocks += block_tiles(i < stride ? in.b0 : visible[i], in.pose, g, m.voxel, tile);
+    return valid_blocks;
+}
+
+// The tiles straight into the zRange images: two device-scope atomics per
+// covered tile (the fallback for zRange images above kPrepassLdsTiles).
+// One-wavefront workgroups: the visible blocks (a thread each) spread over
+// more CUs than 256-thread groups would give them.
+constexpr int kPrepassGrid = 2048;
+__global__ __launch_bounds__(64) void prepass_kernel(MapDev m, MapCounters* ctr,
+                                                     const sicp_block* visible, View<float> zmin,
+                                                     View<float> zmax, PoseArg pa,
+                                                     PrepassArgs g, const int* gate,
+                                                     const uint32_t* fuse_counts) {
+    const PrepassIn in = prepass_loads<64, kPrepassGrid>(m, ctr, visible, pa, gate, fuse_counts);
+    if (in.gated) return;
+    if (fuse_counts) fusion_sums(ctr, in);
+    const uint32_t valid_blocks = prepass_blocks<64, kPrepassGrid>(m, visible, in, g, [&](int Y, int X, int lo, int hi) {
+        atomicMin((int*)&zmin.at(Y, X), lo);
+        atomicMax((int*)&zmax.at(Y, X), hi);
+    });
+    count_render_blocks(ctr, valid_blocks);
+}
+
+// The same with the workgroup's tiles gathered in LDS first: its 256
+// consecutive visible blocks (heap-slot order, so mostly neighbours) cover
+// overlapping tiles, and LDS atomics merge them; the workgroup then applies
+// one atomic per touched tile and bound to the zRange images (device-scope
+// atomics from eight XCDs are carried out beyond the L2s, one memory
+// transaction each). min / max are order-independent, so the images are the
+// same as the direct kernel's.
+constexpr int kPrepassLdsTiles = 8000;  // 2 x 32000 bytes of LDS
+constexpr int kPrepassLdsGrid = 512;
+__global__ __launch_bounds__(256) void prepass_lds_kernel(MapDev m, MapCounters* ctr,
+                                                          const sicp_block* visible,
+                                                          View<float> zmin, View<float> zmax,
+                                                          PoseArg pa, PrepassArgs g,
+                                                          const int* gate,
+                                                          const uint32_t* fuse_counts) {
+    __shared__ int tmin[kPrepassLdsTiles], tmax[kPrepassLdsTiles];
+    const PrepassIn in = prepass_loads<256, kPrepassLdsGrid>(m, ctr, visible, pa, gate, fuse_counts);
+    if (in.gated) return;
+    if (fuse_counts) fusion_sums(ctr, in);
+    if (blockIdx.x * 256 >= in.nvis) return;  // no blocks for this workgroup
+    const int ntiles = g.zcols * g.zrows;
+    for (int t = threadIdx.x; t < ntiles; t += 256) {
+        tmin[t] = 0x7fffffff;
+        tmax[t] = 0;
+    }
+    __syncthreads();
+    const uint32_t valid_blocks = prepass_blocks<256, kPrepassLdsGrid>(m, visible, in, g, [&](int Y, int X, int lo, int hi) {
+        atomicMin(&tmin[Y * g.zcols + X], lo);
+        atomicMax(&tmax[Y * g.zcols + X], hi);
+    });
+    count_render_blocks(ctr, valid_blocks);
+    __syncthreads();
+    for (int t = threadIdx.x; t < ntiles; t += 256) {
+        const int lo = tmin[t], hi = tmax[t];
+        if (lo == 0x7fffffff) continue;  // untouched (every touch sets both)
+        const int Y = t / g.zcols, X = t - Y * g.zcols;
+        atomicMin((int*)&zmin.at(Y, X), lo);
+        atomicMax((int*)&zmax.at(Y, X), hi);
+    }
 }
 
 // ------------------------------------------------------------------ raycast --
@@ -1018,12 +1118,15 @@ int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible,
         SICP_LAUNCH_CHECK("zrange_init_kernel");
     }
     PrepassArgs g{K.fx, K.fy, K.cx, K.cy, dmax, dmin, zmin.cols, zmin.rows};
-    // One-wavefront workgroups: the visible blocks (a thread each) spread over
-    // more CUs than 256-thread groups would give them (16.5 against 19.8 µs
-    // per launch on the 640x480 sequence).
-    static_assert(kFuseBlocks * 4 % 256 == 0, "whole 256-count chunks");
-    prepass_kernel<<<2048, 64, 0, s>>>(m, ctr, visible, zmin, zmax, pose, g, gate, fuse_counts);
-    SICP_LAUNCH_CHECK("prepass_kernel");
+    if (n <= kPrepassLdsTiles) {
+        prepass_lds_kernel<<<kPrepassLdsGrid, 256, 0, s>>>(m, ctr, visible, zmin, zmax, pose, g, gate,
+                                               fuse_counts);
+        SICP_LAUNCH_CHECK("prepass_lds_kernel");
+    } else {
+        prepass_kernel<<<kPrepassGrid, 64, 0, s>>>(m, ctr, visible, zmin, zmax, pose, g, gate,
+                                           fuse_counts);
+        SICP_LAUNCH_CHECK("prepass_kernel");
+    }
     return SICP_OK;
 }
 

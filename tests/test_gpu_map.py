@@ -179,8 +179,14 @@ def test_empty_frame_no_visible(dev):
     assert rc == B.SICP_ERR_NO_VISIBLE and vh.value == 0
 
 
+# 1280x720: a 160x90 zRange, above what the prepass gathers in LDS (its
+# direct-atomics kernel); the others use the LDS kernel.
+CFG_HD = dict(cols=1280, rows=720, f=1050.0, cx=639.5, cy=359.5, voxel=0.008, trunc=0.024,
+              buckets=1 << 19, blocks=200000)
+
+
 @pytest.mark.parametrize("half", [False, True])
-@pytest.mark.parametrize("c", [CFG_SMALL, CFG_VGA])
+@pytest.mark.parametrize("c", [CFG_SMALL, CFG_VGA, CFG_HD])
 def test_prepass_and_raycast_bit_exact(dev, c, half):
     cfg = map_cfg(c, **(HALF if half else {}))
     gm, om = B.Map(cfg), O.OracleMap(cfg)
