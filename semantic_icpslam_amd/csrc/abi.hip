@@ -106,6 +106,20 @@ int map_alloc(const sicp_map_config& cfg, sicp_map** out) {
         map_free(m);
         return rc;
     }
+    {
+        int* one = nullptr;
+        if ((rc = dmalloc(m, &one, 1))) {
+            map_free(m);
+            return rc;
+        }
+        const int v = 1;
+        hipError_t e = hipMemcpy(one, &v, sizeof(v), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            map_free(m);
+            return hip_fail(e, "hipMemcpy(one)");
+        }
+        d.one = one;
+    }
     if (cfg.unify_labels) {
         // scratch of the neighbour label pass, here so that no later call allocates
         if ((rc = dmalloc(m, &m->unify_snap, nblk * kBlockVoxels)) ||

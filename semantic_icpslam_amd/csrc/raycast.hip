@@ -126,18 +126,16 @@ SICP_DEV PrepassIn prepass_loads(const MapDev& m, const MapCounters* ctr,
                                  const uint32_t* fuse_counts) {
     PrepassIn in;
     const uint32_t i0 = blockIdx.x * kThreads + threadIdx.x;
-    // relaxed atomic loads: issued here, not sunk below the gate's test
-    auto ld = [](const auto* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
-    in.nvis = ld(&ctr->visible);
-    in.heap_used = ld(&ctr->heap_used);
-    in.visible_total = ld(&ctr->visible_total);
-    in.heap_total = ld(&ctr->heap_total);
-    in.fusions = ld(&ctr->fusions);
+    in.nvis = early_load(&ctr->visible);
+    in.heap_used = early_load(&ctr->heap_used);
+    in.visible_total = early_load(&ctr->visible_total);
+    in.heap_total = early_load(&ctr->heap_total);
+    in.fusions = early_load(&ctr->fusions);
     in.b0 = visible[i0 < (uint32_t)m.num_blocks ? i0 : 0u];
     static_assert(kThreads * kGrid >= kFuseBlocks * 4, "one fusion count per thread at most");
     in.fuse_sum = fuse_counts && i0 < (uint32_t)(kFuseBlocks * 4) ? fuse_counts[i0] : 0u;
     in.pose = load_pose(pa);
-    in.gated = gate && *gate == 0;  // last: its null test would order the loads after it
+    in.gated = early_load(gate) == 0;  // never null: MapDev::one for an absent gate
     return in;
 }
 
@@ -502,10 +500,8 @@ struct Ray {
     v3 start, dir;
     float dist_s, dist_e;
 };
-SICP_DEV bool ray_setup(const MapDev& m, const View<float>& zmin, const View<float>& zmax,
-                        const PoseF& pose, const RayArgs& g, int x, int y, Ray& r) {
-    const float z0 = zmin.at(y >> 3, x >> 3);
-    const float z1 = zmax.at(y >> 3, x >> 3);
+SICP_DEV bool ray_setup(const MapDev& m, float z0, float z1, const PoseF& pose, const RayArgs& g,
+                        int x, int y, Ray& r) {
     if ((double)z1 < 1e-3 || (double)z0 < 1e-3 || isnan_(z0) || isnan_(z1)) return false;
     v3 p;
     p.z = z0;
@@ -522,6 +518,10 @@ SICP_DEV bool ray_setup(const MapDev& m, const View<float>& zmin, const View<flo
     r.start = block_s;
     return true;
 }
+SICP_DEV bool ray_setup(const MapDev& m, const View<float>& zmin, const View<float>& zmax,
+                        const PoseF& pose, const RayArgs& g, int x, int y, Ray& r) {
+    return ray_setup(m, zmin.at(y >> 3, x >> 3), zmax.at(y >> 3, x >> 3), pose, g, x, y, r);
+}
 
 // RayArgs::first_pass_samples is the number of samples one ray marches in
 // the first pass; rays still marching then are handed to the second pass.
@@ -529,14 +529,21 @@ SICP_DEV bool ray_setup(const MapDev& m, const View<float>& zmin, const View<flo
 // one-voxel steps would otherwise set the length of the whole launch.
 
 // First pass: one lane per ray, 8x8-pixel wavefronts (the zRange tile).
-__global__ __launch_bounds__(64) void raycast_kernel(MapDev m, View<float4> vmap,
+__global__ __launch_bounds__(64, 5) void raycast_kernel(MapDev m, View<float4> vmap,
                                                      View<float4> nmap, View<float> zmin,
                                                      View<float> zmax, PoseArg pa, RayArgs g,
                                                      const int* gate, const uint32_t* nonempty) {
-    if (gate && *gate == 0) return;
-    if (nonempty && *nonempty == 0) return;  // CreateRenderingBlocks returned false
     const int x = blockIdx.x * 8 + (threadIdx.x & 7);
     const int y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    // the tile's depth range, the pose and the gates, loaded together
+    const int ty = min(y, g.rows - 1) >> 3, tx = min(x, g.cols - 1) >> 3;
+    const float z0 = early_load(&zmin.at(ty, tx)), z1 = early_load(&zmax.at(ty, tx));
+    // gate and nonempty (CreateRenderingBlocks returned true) are never null
+    // here: the launch passes MapDev::one for an absent one
+    const int open = early_load(gate);
+    const uint32_t any = early_load(nonempty);
+    const PoseF pose = load_pose(pa);
+    if ((open == 0) | (any == 0u)) return;
     // One wavefront per workgroup: the table is cleared before any lane uses
     // it, and its accesses are ordered by the wavefront's own instruction
     // order (no barrier needed).
@@ -547,12 +554,11 @@ __global__ __launch_bounds__(64) void raycast_kernel(MapDev m, View<float4> vmap
     const float nan = qnanf();
     float4 vout = make_float4(nan, nan, nan, nan);
     float4 nout = vout;
-    const PoseF pose = load_pose(pa);
     Ray ray;
     bool deferred = false;
     v3 result{0.f, 0.f, 0.f};
     float dist_s = 0.f, sdf = 1.0f;
-    if (ray_setup(m, zmin, zmax, pose, g, x, y, ray)) {
+    if (ray_setup(m, z0, z1, pose, g, x, y, ray)) {
         result = ray.start;
         dist_s = ray.dist_s;
         bool valid_sdf = false, back_face = false;
@@ -616,7 +622,8 @@ __global__ __launch_bounds__(64) void raycast_kernel(MapDev m, View<float4> vmap
         if (deferred) {
             const uint32_t q = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
             if (q < m.rq_cap)
-                m.rq[q] = RayState{y * g.cols + x, result.x, result.y, result.z, dist_s, sdf, {0, 0}};
+                m.rq[q] = RayState{y * g.cols + x, result.x, result.y, result.z, dist_s, sdf,
+                                   ray.dir.x, ray.dir.y, ray.dir.z, ray.dist_e, {0, 0}};
         }
         if (deferred) return;  // the second pass writes this pixel
     }
@@ -717,25 +724,33 @@ SICP_DEV bool coop_normal(const MapDev& m, v3 pt, int k, BlockCache& c, const Po
     return true;
 }
 
-__global__ __launch_bounds__(256) void raycast_long_kernel(MapDev m, View<float4> vmap,
+constexpr int kRayLongThreads = 256;
+__global__ __launch_bounds__(kRayLongThreads) void raycast_long_kernel(MapDev m, View<float4> vmap,
                                                            View<float4> nmap, View<float> zmin,
                                                            View<float> zmax, PoseArg pa,
                                                            RayArgs g, const int* gate,
                                                            const uint32_t* nonempty) {
-    if (gate && *gate == 0) return;
-    if (nonempty && *nonempty == 0) return;
-    const uint32_t n = min(*m.rq_count, m.rq_cap);
+    constexpr uint32_t kGroupsPerWg = kRayLongThreads / kRayGroup;
+    const uint32_t i0 = blockIdx.x * kGroupsPerWg + (threadIdx.x / kRayGroup);
+    // the queue's count, this group's first entry (at a clamped index, used
+    // only below the count), the pose and the gates, loaded together
+    const uint32_t queued = early_load(m.rq_count);
+    const RayState st0 = m.rq[i0 < m.rq_cap ? i0 : 0u];
+    const int open = early_load(gate);  // gate and nonempty: never null here
+    const uint32_t any = early_load(nonempty);
+    const PoseF pose = load_pose(pa);
+    if ((open == 0) | (any == 0u)) return;
+    const uint32_t n = min(queued, m.rq_cap);
     const int k = threadIdx.x & (kRayGroup - 1);
     const int gbase = (threadIdx.x & 63) & ~(kRayGroup - 1);  // first lane of the group
-    const uint32_t groups = gridDim.x * (blockDim.x / kRayGroup);
-    const PoseF pose = load_pose(pa);
+    const uint32_t groups = gridDim.x * kGroupsPerWg;
     const float nan = qnanf();
-    for (uint32_t i = blockIdx.x * (blockDim.x / kRayGroup) + (threadIdx.x / kRayGroup); i < n;
-         i += groups) {
-        const RayState st = m.rq[i];
+    for (uint32_t i = i0; i < n; i += groups) {
+        const RayState st = i == i0 ? st0 : m.rq[i];
         const int x = st.px % g.cols, y = st.px / g.cols;
-        Ray ray;
-        ray_setup(m, zmin, zmax, pose, g, x, y, ray);  // true: the first pass marched it
+        Ray ray;  // the first pass's ray_setup
+        ray.dir = v3{st.dx, st.dy, st.dz};
+        ray.dist_e = st.dist_e;
         v3 result{st.rx, st.ry, st.rz};
         float dist_s = st.dist_s, sdf = st.sdf;
         bool back_face = false;
@@ -1118,6 +1133,7 @@ int launch_render_prepass(MapDev m, MapCounters* ctr, const sicp_block* visible,
         SICP_LAUNCH_CHECK("zrange_init_kernel");
     }
     PrepassArgs g{K.fx, K.fy, K.cx, K.cy, dmax, dmin, zmin.cols, zmin.rows};
+    if (!gate) gate = m.one;
     if (n <= kPrepassLdsTiles) {
         prepass_lds_kernel<<<kPrepassLdsGrid, 256, 0, s>>>(m, ctr, visible, zmin, zmax, pose, g, gate,
                                                fuse_counts);
@@ -1143,6 +1159,8 @@ int launch_raycast_gated(MapDev m, View<float4> vmap, View<float4> nmap, View<fl
         return invalid("raycast: tuning outside 1..4096 samples, 1..65536 workgroups, table 0/1");
     RayArgs g{invfx, invfy, cx, cy, vmap.cols, vmap.rows, tune.first_pass_samples,
               tune.lds_block_table};
+    if (!gate) gate = m.one;
+    if (!nonempty) nonempty = reinterpret_cast<const uint32_t*>(m.one);
     if (!queue_cleared) {
         ray_queue_reset_kernel<<<1, 1, 0, s>>>(m.rq_count);
         SICP_LAUNCH_CHECK("ray_queue_reset_kernel");
@@ -1150,7 +1168,7 @@ int launch_raycast_gated(MapDev m, View<float4> vmap, View<float4> nmap, View<fl
     dim3 grid(div_up(vmap.cols, 8), div_up(vmap.rows, 8));
     raycast_kernel<<<grid, 64, 0, s>>>(m, vmap, nmap, zmin, zmax, pose, g, gate, nonempty);
     SICP_LAUNCH_CHECK("raycast_kernel");
-    raycast_long_kernel<<<tune.second_pass_workgroups, 256, 0, s>>>(m, vmap, nmap, zmin, zmax,
+    raycast_long_kernel<<<tune.second_pass_workgroups, kRayLongThreads, 0, s>>>(m, vmap, nmap, zmin, zmax,
                                                                     pose, g, gate, nonempty);
     SICP_LAUNCH_CHECK("raycast_long_kernel");
     return SICP_OK;

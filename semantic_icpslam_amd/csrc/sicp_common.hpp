@@ -70,6 +70,20 @@ SICP_DEV i3 floor_i3(v3 a) {
 SICP_DEV float qnanf() { return __builtin_bit_cast(float, 0x7fffffffu); }
 SICP_DEV bool isnan_(float v) { return !(v == v); }
 
+// A load issued where it stands in the source. The compiler does not move
+// relaxed atomic loads past branches, so a kernel can start every
+// independent load of its prologue (gates, counters, poses, first inputs)
+// before the first decision that needs one of them: each is a round trip to
+// memory that an earlier launch wrote, and early returns in between would
+// otherwise order them one after another.
+template <class T> SICP_DEV T early_load(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+// A device flag written by an earlier launch (a gate: 0 skips the launch's
+// work), or 1 when there is none.
+SICP_DEV int flag_or_one(const int* flag) { return flag ? early_load(flag) : 1; }
+SICP_DEV bool gate_closed(const int* gate) { return flag_or_one(gate) == 0; }
+
 // ------------------------------------------------------------------ poses --
 struct PoseF {
     m3 R, Rinv;

@@ -200,9 +200,12 @@ void se3_update_host(const double* A, const double* b, double* delta, const doub
 // ---------------------------------------------------------------- map -----
 // A ray handed from the first raycast pass to the second: its pixel and the
 // march state at the hand-over (the rest is recomputed from the pixel).
+// A ray handed from the raycast's first pass to its second: the pixel, the
+// march's position, distance and last sdf, and the ray's direction and end.
 struct RayState {
     int px;
     float rx, ry, rz, dist_s, sdf;
+    float dx, dy, dz, dist_e;
     int pad[2];
 };
 
@@ -221,6 +224,9 @@ struct MapDev {
     RayState* rq;        // raycast hand-over queue, rq_cap entries
     uint32_t* rq_count;  // entries queued by the first pass
     uint32_t rq_cap;
+    // A device word holding 1, read in place of an absent gate or flag, so
+    // that kernels load their flags without testing the pointers first.
+    const int* one;
 };
 
 struct MapCounters {
