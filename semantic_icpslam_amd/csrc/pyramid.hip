@@ -316,7 +316,7 @@ __global__ __launch_bounds__(256) void resize_map_kernel(View<float4> vsrc, View
 //     pyramid of the last frame (levels 1 and 2, the latter straight from
 //     level 0 through the same 2x2 means);
 //  2. PyrDown to levels 1 and 2 of depth and intensity: a workgroup owns an
-//     8x8 tile of level 2, computes the 20x20 level-1 window it needs into
+//     8x4 tile of level 2, computes the 20x12 level-1 window it needs into
 //     LDS (writing the level-1 pixels it owns), then its level-2 pixels;
 //  3. vertex map, normal map (the two neighbour vertices recomputed from
 //     depth) and derivatives for every level.
@@ -400,44 +400,43 @@ __global__ __launch_bounds__(256) void frame_level0_kernel(View<uint16_t> d16, V
     }
 }
 
-template <class T> struct TileSrc {
+// A row-major LDS window of `W` columns starting at (x0, y0).
+template <class T, int W> struct LdsWin {
     const T* p;
     int x0, y0;
-    SICP_DEV T operator()(int y, int x) const { return p[(y - y0) * 20 + (x - x0)]; }
+    SICP_DEV T operator()(int y, int x) const { return p[(y - y0) * W + (x - x0)]; }
 };
 
-// The level-0 window of a tile: the 44 x 44 pixels the 5x5 taps of its
-// 20 x 20 level-1 window reach (clamped taps stay inside it), staged in LDS
-// with every load in flight at once.
-constexpr int kWin0 = 44;
-template <class T> struct WinSrc {
-    const T* p;
-    int x0, y0;
-    SICP_DEV T operator()(int y, int x) const { return p[(y - y0) * kWin0 + (x - x0)]; }
-};
-
+// A workgroup owns a TX x TY tile of level 2. It computes the level-1 window
+// its 5x5 taps reach (2 TX + 4 by 2 TY + 4 pixels, writing the level-1 pixels
+// it owns) from the level-0 window those reach in turn (twice that plus 4),
+// staged in LDS with every load in flight at once; then its level-2 pixels.
+template <int TX, int TY>
 __global__ __launch_bounds__(256) void pyrdown_levels_kernel(View<float> d0, View<uint8_t> g0,
                                                              View<float> d1, View<uint8_t> g1,
                                                              View<float> d2, View<uint8_t> g2) {
-    __shared__ float w0d[kWin0 * kWin0];
-    __shared__ uint8_t w0g[kWin0 * kWin0];
-    __shared__ float rd[400];
-    __shared__ uint8_t rg[400];
-    const int X0 = blockIdx.x * 8, Y0 = blockIdx.y * 8;  // level-2 tile origin
-    const int rx0 = 2 * X0 - 2, ry0 = 2 * Y0 - 2;        // level-1 window origin
-    const int wx0 = 2 * rx0 - 2, wy0 = 2 * ry0 - 2;      // level-0 window origin
+    constexpr int W1X = 2 * TX + 4, W1Y = 2 * TY + 4;  // level-1 window
+    constexpr int W0X = 2 * W1X + 4, W0Y = 2 * W1Y + 4;  // level-0 window
+    static_assert(TX * TY <= 256, "one level-2 pixel per thread");
+    __shared__ float w0d[W0X * W0Y];
+    __shared__ uint8_t w0g[W0X * W0Y];
+    __shared__ float rd[W1X * W1Y];
+    __shared__ uint8_t rg[W1X * W1Y];
+    const int X0 = blockIdx.x * TX, Y0 = blockIdx.y * TY;  // level-2 tile origin
+    const int rx0 = 2 * X0 - 2, ry0 = 2 * Y0 - 2;          // level-1 window origin
+    const int wx0 = 2 * rx0 - 2, wy0 = 2 * ry0 - 2;        // level-0 window origin
     // Every load of the window is issued before the first LDS store (a loop
     // of load-then-store waits out one memory round trip per element);
     // entries outside the image are loaded from (0, 0) and not stored.
-    constexpr int kWinIters = (kWin0 * kWin0 + 255) / 256;
+    constexpr int kWinIters = (W0X * W0Y + 255) / 256;
     float dv[kWinIters];
     uint8_t gv[kWinIters];
     bool in[kWinIters];
 #pragma unroll
     for (int j = 0; j < kWinIters; ++j) {
         const int i = (int)threadIdx.x + 256 * j;
-        const int wx = wx0 + i % kWin0, wy = wy0 + i / kWin0;
-        in[j] = i < kWin0 * kWin0 && wx >= 0 && wy >= 0 && wx < d0.cols && wy < d0.rows;
+        const int wx = wx0 + i % W0X, wy = wy0 + i / W0X;
+        in[j] = i < W0X * W0Y && wx >= 0 && wy >= 0 && wx < d0.cols && wy < d0.rows;
         const int cx = in[j] ? wx : 0, cy = in[j] ? wy : 0;
         dv[j] = d0.at(cy, cx);
         gv[j] = g0.at(cy, cx);
@@ -451,16 +450,17 @@ __global__ __launch_bounds__(256) void pyrdown_levels_kernel(View<float> d0, Vie
         }
     }
     __syncthreads();
-    // Level-1 pixels owned here: [2*X0, 2*X0 + 16), the last tile column
+    // Level-1 pixels owned here: [2*X0, 2*X0 + 2 TX), the last tile column
     // (row) also the odd remainder up to the image edge.
-    const int ox1 = blockIdx.x == gridDim.x - 1 ? d1.cols : 2 * X0 + 16;
-    const int oy1 = blockIdx.y == gridDim.y - 1 ? d1.rows : 2 * Y0 + 16;
-    for (int i = threadIdx.x; i < 400; i += 256) {
-        const int rx = rx0 + i % 20, ry = ry0 + i / 20;
+    const int ox1 = blockIdx.x == gridDim.x - 1 ? d1.cols : 2 * X0 + 2 * TX;
+    const int oy1 = blockIdx.y == gridDim.y - 1 ? d1.rows : 2 * Y0 + 2 * TY;
+    for (int i = threadIdx.x; i < W1X * W1Y; i += 256) {
+        const int rx = rx0 + i % W1X, ry = ry0 + i / W1X;
         if (rx < 0 || ry < 0 || rx >= d1.cols || ry >= d1.rows) continue;
-        const float v = pyrdown_px<float>(WinSrc<float>{w0d, wx0, wy0}, d0.cols, d0.rows, rx, ry);
+        const float v =
+            pyrdown_px<float>(LdsWin<float, W0X>{w0d, wx0, wy0}, d0.cols, d0.rows, rx, ry);
         const uint8_t u =
-            pyrdown_px<uint8_t>(WinSrc<uint8_t>{w0g, wx0, wy0}, g0.cols, g0.rows, rx, ry);
+            pyrdown_px<uint8_t>(LdsWin<uint8_t, W0X>{w0g, wx0, wy0}, g0.cols, g0.rows, rx, ry);
         rd[i] = v;
         rg[i] = u;
         if (rx >= 2 * X0 && rx < ox1 && ry >= 2 * Y0 && ry < oy1) {
@@ -469,15 +469,21 @@ __global__ __launch_bounds__(256) void pyrdown_levels_kernel(View<float> d0, Vie
         }
     }
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const int x = X0 + (threadIdx.x & 7), y = Y0 + (threadIdx.x >> 3);
+    if (threadIdx.x < TX * TY) {
+        const int x = X0 + (int)threadIdx.x % TX, y = Y0 + (int)threadIdx.x / TX;
         if (x < d2.cols && y < d2.rows) {
-            d2.at(y, x) = pyrdown_px<float>(TileSrc<float>{rd, rx0, ry0}, d1.cols, d1.rows, x, y);
+            d2.at(y, x) =
+                pyrdown_px<float>(LdsWin<float, W1X>{rd, rx0, ry0}, d1.cols, d1.rows, x, y);
             g2.at(y, x) =
-                pyrdown_px<uint8_t>(TileSrc<uint8_t>{rg, rx0, ry0}, g1.cols, g1.rows, x, y);
+                pyrdown_px<uint8_t>(LdsWin<uint8_t, W1X>{rg, rx0, ry0}, g1.cols, g1.rows, x, y);
         }
     }
 }
+// The level-2 tile of pyrdown_levels_kernel: 8x4 gives 600 workgroups at
+// 640x480, so the CUs hold two or three each and one's window loads overlap
+// another's arithmetic (frame pyramid 36.6 us against 39.1 with 8x8 tiles,
+// 38.0 with 10x8, 42.0 with 16x8, 38.3 with 4x4, 38.9 with 8x2; 4x8 as 8x4).
+constexpr int kPyrTX = 8, kPyrTY = 4;
 
 __global__ __launch_bounds__(256) void frame_maps_kernel(FramePyramid P, float cutoff) {
     int b = blockIdx.x, x = 0, y = 0, l = 0;
@@ -576,9 +582,10 @@ int launch_frame_pyramid(View<uint16_t> depth_u16, View<uint8_t> rgb, View<float
     frame_level0_kernel<<<n0, dim3(kTX, kTY), 0, s>>>(depth_u16, rgb, range, P.depth[0],
                                                       P.image[0], scale_inv, cutoff, mp, reset);
     SICP_LAUNCH_CHECK("frame_level0_kernel");
-    const dim3 g2((unsigned)div_up(P.depth[2].cols, 8), (unsigned)div_up(P.depth[2].rows, 8));
-    pyrdown_levels_kernel<<<g2, 256, 0, s>>>(P.depth[0], P.image[0], P.depth[1], P.image[1],
-                                             P.depth[2], P.image[2]);
+    const dim3 g2((unsigned)div_up(P.depth[2].cols, kPyrTX),
+                  (unsigned)div_up(P.depth[2].rows, kPyrTY));
+    pyrdown_levels_kernel<kPyrTX, kPyrTY><<<g2, 256, 0, s>>>(P.depth[0], P.image[0], P.depth[1],
+                                                             P.image[1], P.depth[2], P.image[2]);
     SICP_LAUNCH_CHECK("pyrdown_levels_kernel");
     int n2 = 0;
     for (int l = 0; l < 3; ++l) n2 += tiles(P.depth[l].cols, P.depth[l].rows);

@@ -97,7 +97,8 @@ def test_system_first_frame_model_bit_exact(dev):
     assert st["visible_total"] == w["visible_blocks"] > 0
 
 
-@pytest.mark.parametrize("c", [CFG_SMALL, dict(CFG_SMALL, cols=317, rows=239, cx=158.0, cy=119.0)])
+@pytest.mark.parametrize("c", [CFG_SMALL, dict(CFG_SMALL, cols=317, rows=239, cx=158.0, cy=119.0),
+                               CFG_VGA])
 def test_system_frame_pyramid_bit_exact(dev, c):
     """The three-launch frame pyramid (filter, intensity, PyrDown to levels
     1-2, vertex / normal maps, derivatives) and the model pyramid of the
