@@ -252,7 +252,12 @@ __global__ __launch_bounds__(256) void alloc_emit_kernel(MapDev m, MapCounters* 
         s += c[k];
     }
     const uint2 off = chunk_offset(partials, blockIdx.x, nchunks);
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->new_total = off.y;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctr->new_total = off.y;
+        // the first slot of this frame's new blocks, kept apart from
+        // heap_used, which the insertion advances while others still read it
+        ctr->alloc_base = ctr->heap_used;
+    }
     uint32_t tot;
     uint32_t r = off.x + block_exclusive_scan(s, &tot);
     const PoseF pose = load_pose(pa);
@@ -271,11 +276,12 @@ __global__ __launch_bounds__(256) void alloc_emit_kernel(MapDev m, MapCounters* 
 }
 
 // Clears the voxels of the blocks this frame creates (empty sdf, weight 0,
-// zero semantic record), over `nwg` workgroups starting at `wg`.
+// zero semantic record: Voxel::release, Mapping/DeviceMap.h:76-87), over
+// `nwg` workgroups starting at `wg`.
 SICP_DEV void clear_new_blocks(const MapDev& m, const MapCounters* ctr, uint32_t wg,
                                uint32_t nwg) {
     const uint32_t total = ctr->new_total;
-    const uint32_t base = ctr->heap_used;
+    const uint32_t base = ctr->alloc_base;
     uint32_t created = 0;
     if (base < (uint32_t)m.num_blocks) created = min(total, (uint32_t)m.num_blocks - base);
     const uint2 g = make_uint2(0x7fffffffu, 0u);
@@ -304,7 +310,7 @@ __global__ __launch_bounds__(256) void alloc_link_kernel(MapDev m, const MapCoun
         return;
     }
     const uint32_t total = ctr->new_total;
-    const uint32_t base = ctr->heap_used;
+    const uint32_t base = ctr->alloc_base;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < total;
          r += kLinkBlocks * blockDim.x) {
         uint32_t slot = base + r;
@@ -322,11 +328,24 @@ SICP_DEV void store_entry(sicp_hash_entry* e, uint64_t key, int slot, int next) 
     *(uint4*)e = make_uint4((uint32_t)key, (uint32_t)(key >> 32), (uint32_t)slot, (uint32_t)next);
 }
 
-__global__ __launch_bounds__(256) void alloc_insert_kernel(MapDev m, const MapCounters* ctr,
+// Workgroup 0's first thread also advances the heap by the blocks created
+// (the heap's room bounds them; the rest are counted as dropped), which no
+// thread of this launch reads: the ranks' slots start at alloc_base.
+__global__ __launch_bounds__(256) void alloc_insert_kernel(MapDev m, MapCounters* ctr,
                                                            AllocScratch a, const int* gate) {
-    if (gate && *gate == 0) return;
+    if (gate && *gate == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctr->new_blocks = 0;
+        return;
+    }
     const uint32_t total = ctr->new_total;
-    const uint32_t base = ctr->heap_used;
+    const uint32_t base = ctr->alloc_base;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t room = base < (uint32_t)m.num_blocks ? (uint32_t)m.num_blocks - base : 0u;
+        const uint32_t created = min(total, room);
+        ctr->dropped_heap += total - created;
+        ctr->heap_used = base + created;
+        ctr->new_blocks = created;
+    }
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < total;
          r += gridDim.x * blockDim.x) {
         // Every rank clears its request-set slot for the next frame.
@@ -376,22 +395,6 @@ __global__ __launch_bounds__(256) void alloc_insert_kernel(MapDev m, const MapCo
         a.bucket_leader[b] = 0x7fffffff;
         a.bucket_head[b] = -1;
     }
-}
-
-// New blocks start empty: sdf NaN, weight 0, labels 0 (Voxel::release,
-// Mapping/DeviceMap.h:76-87). One workgroup of 256 per block.
-__global__ void alloc_finish_kernel(MapDev m, MapCounters* ctr, const int* gate) {
-    if (gate && *gate == 0) {
-        ctr->new_blocks = 0;
-        return;
-    }
-    uint32_t total = ctr->new_total;
-    uint32_t base = ctr->heap_used;
-    uint32_t room = base < (uint32_t)m.num_blocks ? (uint32_t)m.num_blocks - base : 0u;
-    uint32_t created = min(total, room);
-    ctr->dropped_heap += total - created;
-    ctr->heap_used = base + created;
-    ctr->new_blocks = created;
 }
 
 // ------------------------------------------------------------------ scan --
@@ -924,8 +927,6 @@ int launch_allocate(MapDev m, MapCounters* ctr, AllocScratch a, View<float> dept
     SICP_LAUNCH_CHECK("alloc_link_kernel");
     alloc_insert_kernel<<<512, 256, 0, s>>>(m, ctr, a, gate);
     SICP_LAUNCH_CHECK("alloc_insert_kernel");
-    alloc_finish_kernel<<<1, 1, 0, s>>>(m, ctr, gate);
-    SICP_LAUNCH_CHECK("alloc_finish_kernel");
     return SICP_OK;
 }
 

@@ -239,7 +239,7 @@ struct MapCounters {
     uint32_t dropped_range;
     uint32_t updated_voxels;
     uint32_t render_tiles;
-    uint32_t pad0;
+    uint32_t alloc_base;     // heap_used when this frame's allocation began (its first slot)
     unsigned long long updated_total;  // voxels written, summed since reset
     unsigned long long visible_total;  // visible blocks, summed since reset
     unsigned long long fusions;        // fusion passes since reset
