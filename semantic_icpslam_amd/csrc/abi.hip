@@ -532,13 +532,12 @@ int sicp_fuse_map_color(sicp_map* map, sicp_image depth_f32, sicp_image rgb_u8x3
         return invalid("fuse_map_color: image larger than the map's max_cols x max_rows");
     hipStream_t s = (hipStream_t)stream;
     PoseArg pa = pose_arg(pose_host, pose_dev);
-    int rc = launch_allocate(map->dev, map->ctr, map->alloc, view<float>(depth_f32), pa, K,
-                             nullptr, s);
-    if (rc) return rc;
     // FuseMapColor checks visibility against the map's own depth range
-    // (FuseMap.cu:1080-1081).
-    rc = launch_visibility(map->dev, map->ctr, map->alloc, map->visible, pa, K, c, r,
-                           map->cfg.depth_max, map->cfg.depth_min, nullptr, s);
+    // (FuseMap.cu:1080-1081); the allocation's last launch flags the blocks.
+    int rc = launch_allocate(map->dev, map->ctr, map->alloc, view<float>(depth_f32), pa, K,
+                             VisFrustum{c, r, map->cfg.depth_max, map->cfg.depth_min}, nullptr, s);
+    if (rc) return rc;
+    rc = launch_visible_compact(map->dev, map->ctr, map->alloc, map->visible, nullptr, s);
     if (rc) return rc;
     sicp_image nm = nmap_f4, mk = mask_u8;
     if (num_detection <= 0) {

@@ -328,26 +328,26 @@ SICP_DEV void store_entry(sicp_hash_entry* e, uint64_t key, int slot, int next) 
     *(uint4*)e = make_uint4((uint32_t)key, (uint32_t)(key >> 32), (uint32_t)slot, (uint32_t)next);
 }
 
-// Workgroup 0's first thread also advances the heap by the blocks created
-// (the heap's room bounds them; the rest are counted as dropped), which no
-// thread of this launch reads: the ranks' slots start at alloc_base.
-__global__ __launch_bounds__(256) void alloc_insert_kernel(MapDev m, MapCounters* ctr,
-                                                           AllocScratch a, const int* gate) {
-    if (gate && *gate == 0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) ctr->new_blocks = 0;
-        return;
-    }
-    const uint32_t total = ctr->new_total;
-    const uint32_t base = ctr->alloc_base;
+// The blocks this frame creates: the heap's room bounds them; the rest are
+// counted as dropped.
+SICP_DEV uint32_t blocks_created(const MapDev& m, uint32_t total, uint32_t base) {
+    const uint32_t room = base < (uint32_t)m.num_blocks ? (uint32_t)m.num_blocks - base : 0u;
+    return min(total, room);
+}
+
+// The insertion of the frame's new blocks into the table, over the launch's
+// threads (grid-stride, 256 per workgroup). Workgroup 0's first thread also
+// advances the heap by the blocks created, which no thread of the launch
+// reads: the ranks' slots start at alloc_base.
+SICP_DEV void insert_ranks(const MapDev& m, MapCounters* ctr, const AllocScratch& a,
+                           uint32_t total, uint32_t base) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const uint32_t room = base < (uint32_t)m.num_blocks ? (uint32_t)m.num_blocks - base : 0u;
-        const uint32_t created = min(total, room);
+        const uint32_t created = blocks_created(m, total, base);
         ctr->dropped_heap += total - created;
         ctr->heap_used = base + created;
         ctr->new_blocks = created;
     }
-    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < total;
-         r += gridDim.x * blockDim.x) {
+    for (uint32_t r = blockIdx.x * 256 + threadIdx.x; r < total; r += gridDim.x * 256) {
         // Every rank clears its request-set slot for the next frame.
         uint32_t rs = a.new_req_slot[r];
         a.req_keys[rs] = kEmptyKey;
@@ -569,14 +569,11 @@ SICP_DEV bool block_visible(const sicp_block& b, float s, const PoseF& pose, con
 // P heap slots per thread (a chunk of 256 P slots per workgroup): one slot
 // per thread spreads the allocated heap over four times the workgroups,
 // when the partial sums of the smaller chunks fit the scan.
+// The visibility flags of workgroup blockIdx.x's chunk of heap slots below
+// n, and the chunk's count.
 template <int P>
-__global__ __launch_bounds__(kScanThreads) void vis_flag_kernel(MapDev m, const MapCounters* ctr,
-                                                                AllocScratch a, PoseArg pa,
-                                                                VisArgs g, uint32_t* partials,
-                                                                const int* gate) {
-    const bool off = gate && *gate == 0;
-    const uint32_t n = off ? 0u : ctr->heap_used;
-    const PoseF pose = load_pose(pa);
+SICP_DEV void flag_chunk(const MapDev& m, const AllocScratch& a, const PoseF& pose,
+                         const VisArgs& g, uint32_t n, uint32_t* partials) {
     const uint32_t base = blockIdx.x * (kScanThreads * P) + threadIdx.x * P;
     uint32_t s = 0;
 #pragma unroll
@@ -591,6 +588,42 @@ __global__ __launch_bounds__(kScanThreads) void vis_flag_kernel(MapDev m, const 
     uint32_t tot;
     block_exclusive_scan(s, &tot);
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+template <int P>
+__global__ __launch_bounds__(kScanThreads) void vis_flag_kernel(MapDev m, const MapCounters* ctr,
+                                                                AllocScratch a, PoseArg pa,
+                                                                VisArgs g, uint32_t* partials,
+                                                                const int* gate) {
+    const bool off = gate && *gate == 0;
+    const uint32_t n = off ? 0u : ctr->heap_used;
+    flag_chunk<P>(m, a, load_pose(pa), g, n, partials);
+}
+
+// The allocation's last launch: the insertion of the new blocks and, in the
+// same launch, the visibility flags and chunk counts of the heap as it will
+// stand (workgroups below nchunks; the heap count is formed from the
+// frame's first slot and the blocks created, the value the insertion
+// stores), so the visibility pass that follows only compacts.
+template <int P>
+__global__ __launch_bounds__(256) void alloc_insert_flag_kernel(MapDev m, MapCounters* ctr,
+                                                                AllocScratch a, const int* gate,
+                                                                PoseArg pa, VisArgs g,
+                                                                uint32_t* partials,
+                                                                uint32_t nchunks) {
+    static_assert(kScanThreads == 256, "one launch shape for both parts");
+    const bool off = gate && *gate == 0;
+    const uint32_t total = ctr->new_total;
+    const uint32_t base = ctr->alloc_base;  // stale when off: not used then
+    if (off) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctr->new_blocks = 0;
+    } else {
+        insert_ranks(m, ctr, a, total, base);
+    }
+    if (blockIdx.x < nchunks) {
+        const uint32_t n = off ? 0u : base + blocks_created(m, total, base);
+        flag_chunk<P>(m, a, load_pose(pa), g, n, partials);
+    }
 }
 
 // Also stores the visible count (the sum of every chunk's count; 0 on a lost
@@ -873,6 +906,18 @@ __global__ __launch_bounds__(1024) void fuse_end_kernel(MapCounters* ctr,
 }  // namespace
 
 // ------------------------------------------------------------- launchers --
+// The visibility pass's chunks of the heap: one slot per thread while the
+// chunk sums fit the scan (returns true), else kScanPerThread.
+static bool vis_chunks(const MapDev& m, uint32_t& nchunks) {
+    const uint32_t nsmall = div_up(m.num_blocks, kScanThreads);
+    if (nsmall <= (uint32_t)kScanMaxChunks) {
+        nchunks = nsmall > 0 ? nsmall : 1;
+        return true;
+    }
+    nchunks = div_up(m.num_blocks, kScanChunk);
+    if (nchunks == 0) nchunks = 1;
+    return false;
+}
 int launch_scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t cap,
                     uint32_t* partials, uint32_t* total_out, hipStream_t s) {
     uint32_t nchunks = div_up(cap, kScanChunk);
@@ -896,9 +941,12 @@ int launch_map_reset(MapDev m, MapCounters* ctr, AllocScratch a, hipStream_t s) 
 }
 
 int launch_allocate(MapDev m, MapCounters* ctr, AllocScratch a, View<float> depth, PoseArg pose,
-                    sicp_intrinsics K, const int* gate, hipStream_t s) {
+                    sicp_intrinsics K, VisFrustum vf, const int* gate, hipStream_t s) {
     const int cols = depth.cols, rows = depth.rows;
     if ((uint32_t)(cols * rows) > a.cap_pixels) return invalid("image larger than the map's max_cols x max_rows");
+    uint32_t vchunks;
+    const bool vone = vis_chunks(m, vchunks);
+    if (vchunks > (uint32_t)kScanMaxChunks) return invalid("heap larger than 4M blocks");
     AllocArgs g{(float)(1.0 / (double)K.fx), (float)(1.0 / (double)K.fy), K.cx, K.cy, cols, rows};
     dim3 grid(div_up(cols, 64), div_up(rows, 4));
     alloc_mark_kernel<<<grid, 256, 0, s>>>(m, ctr, a, depth, pose, g, gate);
@@ -925,20 +973,26 @@ int launch_allocate(MapDev m, MapCounters* ctr, AllocScratch a, View<float> dept
     SICP_LAUNCH_CHECK("alloc_emit_kernel");
     alloc_link_kernel<<<kLinkBlocks + kClearBlocks, 256, 0, s>>>(m, ctr, a, gate);
     SICP_LAUNCH_CHECK("alloc_link_kernel");
-    alloc_insert_kernel<<<512, 256, 0, s>>>(m, ctr, a, gate);
-    SICP_LAUNCH_CHECK("alloc_insert_kernel");
+    // the insertion with the visibility flags (chunks as launch_visibility
+    // forms them)
+    const VisArgs vg{K.fx, K.fy, K.cx, K.cy, vf.cols, vf.rows, vf.dmin, vf.dmax};
+    const uint32_t nwg = vchunks > 512u ? vchunks : 512u;
+    if (vone)
+        alloc_insert_flag_kernel<1><<<nwg, 256, 0, s>>>(m, ctr, a, gate, pose, vg, a.scan_partials,
+                                                        vchunks);
+    else
+        alloc_insert_flag_kernel<kScanPerThread><<<nwg, 256, 0, s>>>(
+            m, ctr, a, gate, pose, vg, a.scan_partials, vchunks);
+    SICP_LAUNCH_CHECK("alloc_insert_flag_kernel");
     return SICP_OK;
 }
 
 int launch_visibility(MapDev m, MapCounters* ctr, AllocScratch a, sicp_block* visible,
                       PoseArg pose, sicp_intrinsics K, int cols, int rows, float dmax,
                       float dmin, const int* gate, hipStream_t s) {
-    uint32_t nchunks = div_up(m.num_blocks, kScanChunk);
-    if (nchunks == 0) nchunks = 1;
+    uint32_t nchunks;
+    const bool one = vis_chunks(m, nchunks);
     if (nchunks > kScanMaxChunks) return invalid("heap larger than 4M blocks");
-    const uint32_t nsmall = div_up(m.num_blocks, kScanThreads);
-    const bool one = nsmall <= (uint32_t)kScanMaxChunks;  // one slot per thread
-    if (one) nchunks = nsmall > 0 ? nsmall : 1;
     VisArgs g{K.fx, K.fy, K.cx, K.cy, cols, rows, dmin, dmax};
     if (one)
         vis_flag_kernel<1><<<nchunks, kScanThreads, 0, s>>>(m, ctr, a, pose, g, a.scan_partials,
@@ -947,6 +1001,14 @@ int launch_visibility(MapDev m, MapCounters* ctr, AllocScratch a, sicp_block* vi
         vis_flag_kernel<kScanPerThread><<<nchunks, kScanThreads, 0, s>>>(m, ctr, a, pose, g,
                                                                          a.scan_partials, gate);
     SICP_LAUNCH_CHECK("vis_flag_kernel");
+    return launch_visible_compact(m, ctr, a, visible, gate, s);
+}
+
+int launch_visible_compact(MapDev m, MapCounters* ctr, AllocScratch a, sicp_block* visible,
+                           const int* gate, hipStream_t s) {
+    uint32_t nchunks;
+    const bool one = vis_chunks(m, nchunks);
+    if (nchunks > kScanMaxChunks) return invalid("heap larger than 4M blocks");
     if (one)
         vis_compact_kernel<1><<<nchunks, kScanThreads, 0, s>>>(m, ctr, a, visible,
                                                                a.scan_partials, nchunks, gate);

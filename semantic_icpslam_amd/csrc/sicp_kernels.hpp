@@ -266,8 +266,20 @@ struct AllocScratch {
 constexpr int kFuseBlocks = 2048;  // fusion workgroups (4 wavefronts each), grid-stride
 
 int launch_map_reset(MapDev m, MapCounters* ctr, AllocScratch a, hipStream_t s);
+// The camera frustum of a visibility pass: image size and depth range.
+struct VisFrustum {
+    int cols, rows;
+    float dmax, dmin;
+};
+// The allocation (Fusion::CreateBlocks) and, in its last launch, the
+// visibility flags of the heap it leaves (CheckBlockVisibility) for the
+// camera `K` and `vf`; launch_visible_compact then completes the
+// visibility pass (the visible list and count).
 int launch_allocate(MapDev m, MapCounters* ctr, AllocScratch a, View<float> depth, PoseArg pose,
-                    sicp_intrinsics K, const int* gate, hipStream_t s);
+                    sicp_intrinsics K, VisFrustum vf, const int* gate, hipStream_t s);
+int launch_visible_compact(MapDev m, MapCounters* ctr, AllocScratch a, sicp_block* visible,
+                           const int* gate, hipStream_t s);
+// A whole visibility pass (flags, then the visible list).
 int launch_visibility(MapDev m, MapCounters* ctr, AllocScratch a, sicp_block* visible,
                       PoseArg pose, sicp_intrinsics K, int cols, int rows, float dmax,
                       float dmin, const int* gate, hipStream_t s);

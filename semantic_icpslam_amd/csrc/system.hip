@@ -496,12 +496,13 @@ static int grab_end(sicp_system* s, const sicp_image& rgb_u8x3, const sicp_image
     sicp_map* m = s->map;
     // Mapping::FuseColor (Mapping.cc:76-97) -> FuseMapColor (FuseMap.cu:1039-1162)
     ev = stage_begin(s, st);
-    if ((rc = launch_allocate(m->dev, m->ctr, m->alloc, f.range.v(), pose, s->K[0], gate, st)))
+    // the allocation's last launch also flags the visible blocks
+    if ((rc = launch_allocate(m->dev, m->ctr, m->alloc, f.range.v(), pose, s->K[0],
+                              VisFrustum{c, r, m->cfg.depth_max, m->cfg.depth_min}, gate, st)))
         return rc;
     stage_end(s, kStageAlloc, ev, st);
     ev = stage_begin(s, st);
-    if ((rc = launch_visibility(m->dev, m->ctr, m->alloc, m->visible, pose, s->K[0], c, r,
-                               m->cfg.depth_max, m->cfg.depth_min, gate, st)))
+    if ((rc = launch_visible_compact(m->dev, m->ctr, m->alloc, m->visible, gate, st)))
         return rc;
     stage_end(s, kStageVisible, ev, st);
     View<uint8_t> mask = labels ? view<uint8_t>(labels_u8) : View<uint8_t>{};
