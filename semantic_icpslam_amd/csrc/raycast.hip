@@ -188,10 +188,11 @@ __global__ __launch_bounds__(64) void prepass_kernel(MapDev m, MapCounters* ctr,
     const PrepassIn in = prepass_loads<64, kPrepassGrid>(m, ctr, visible, pa, gate, fuse_counts);
     if (in.gated) return;
     if (fuse_counts) fusion_sums(ctr, in);
-    const uint32_t valid_blocks = prepass_blocks<64, kPrepassGrid>(m, visible, in, g, [&](int Y, int X, int lo, int hi) {
+    auto to_images = [&](int Y, int X, int lo, int hi) {
         atomicMin((int*)&zmin.at(Y, X), lo);
         atomicMax((int*)&zmax.at(Y, X), hi);
-    });
+    };
+    const uint32_t valid_blocks = prepass_blocks<64, kPrepassGrid>(m, visible, in, g, to_images);
     count_render_blocks(ctr, valid_blocks);
 }
 
@@ -211,7 +212,8 @@ __global__ __launch_bounds__(256) void prepass_lds_kernel(MapDev m, MapCounters*
                                                           const int* gate,
                                                           const uint32_t* fuse_counts) {
     __shared__ int tmin[kPrepassLdsTiles], tmax[kPrepassLdsTiles];
-    const PrepassIn in = prepass_loads<256, kPrepassLdsGrid>(m, ctr, visible, pa, gate, fuse_counts);
+    const PrepassIn in =
+        prepass_loads<256, kPrepassLdsGrid>(m, ctr, visible, pa, gate, fuse_counts);
     if (in.gated) return;
     if (fuse_counts) fusion_sums(ctr, in);
     if (blockIdx.x * 256 >= in.nvis) return;  // no blocks for this workgroup
@@ -221,10 +223,12 @@ __global__ __launch_bounds__(256) void prepass_lds_kernel(MapDev m, MapCounters*
         tmax[t] = 0;
     }
     __syncthreads();
-    const uint32_t valid_blocks = prepass_blocks<256, kPrepassLdsGrid>(m, visible, in, g, [&](int Y, int X, int lo, int hi) {
+    auto to_lds = [&](int Y, int X, int lo, int hi) {
         atomicMin(&tmin[Y * g.zcols + X], lo);
         atomicMax(&tmax[Y * g.zcols + X], hi);
-    });
+    };
+    const uint32_t valid_blocks =
+        prepass_blocks<256, kPrepassLdsGrid>(m, visible, in, g, to_lds);
     count_render_blocks(ctr, valid_blocks);
     __syncthreads();
     for (int t = threadIdx.x; t < ntiles; t += 256) {
@@ -725,11 +729,9 @@ SICP_DEV bool coop_normal(const MapDev& m, v3 pt, int k, BlockCache& c, const Po
 }
 
 constexpr int kRayLongThreads = 256;
-__global__ __launch_bounds__(kRayLongThreads) void raycast_long_kernel(MapDev m, View<float4> vmap,
-                                                           View<float4> nmap, View<float> zmin,
-                                                           View<float> zmax, PoseArg pa,
-                                                           RayArgs g, const int* gate,
-                                                           const uint32_t* nonempty) {
+__global__ __launch_bounds__(kRayLongThreads) void raycast_long_kernel(
+    MapDev m, View<float4> vmap, View<float4> nmap, View<float> zmin, View<float> zmax, PoseArg pa,
+    RayArgs g, const int* gate, const uint32_t* nonempty) {
     constexpr uint32_t kGroupsPerWg = kRayLongThreads / kRayGroup;
     const uint32_t i0 = blockIdx.x * kGroupsPerWg + (threadIdx.x / kRayGroup);
     // the queue's count, this group's first entry (at a clamped index, used
@@ -1168,8 +1170,8 @@ int launch_raycast_gated(MapDev m, View<float4> vmap, View<float4> nmap, View<fl
     dim3 grid(div_up(vmap.cols, 8), div_up(vmap.rows, 8));
     raycast_kernel<<<grid, 64, 0, s>>>(m, vmap, nmap, zmin, zmax, pose, g, gate, nonempty);
     SICP_LAUNCH_CHECK("raycast_kernel");
-    raycast_long_kernel<<<tune.second_pass_workgroups, kRayLongThreads, 0, s>>>(m, vmap, nmap, zmin, zmax,
-                                                                    pose, g, gate, nonempty);
+    raycast_long_kernel<<<tune.second_pass_workgroups, kRayLongThreads, 0, s>>>(
+        m, vmap, nmap, zmin, zmax, pose, g, gate, nonempty);
     SICP_LAUNCH_CHECK("raycast_long_kernel");
     return SICP_OK;
 }
