@@ -252,7 +252,8 @@ int sicp_mesh_write_ply(const char* path, const float* vertices, const float* no
 /* ICPStep (Tracking/Reduction.cu:328-385). Computes the 29 sums (27 upper
  * A|b entries, sum r^2, count) of the point-to-plane Gauss-Newton system into
  * `sums_dev` (29 floats, device). `partials_dev` is device scratch of
- * `sicp_icp_partials_size()` bytes (per-workgroup partial sums, FP64). The
+ * `sicp_icp_partials_size()` bytes, 16-byte aligned (per-workgroup partial
+ * sums, FP64). The
  * sums are bitwise reproducible for a given image size. When `host_sums` is non-NULL the call
  * synchronises `stream` and copies the 29 sums to the host; when `A_host`,
  * `b_host`, `residual_host` are non-NULL it also expands them exactly like
@@ -301,7 +302,7 @@ int sicp_icp_step_fp32(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sic
  * sicp_icp_step. Images: intensities (u8) and derivatives (int16) of the next
  * frame, the last frame's intensity, both frames' vertex maps; one shape.
  * `scratch_dev` holds `scratch_bytes` >= sicp_rgb_scratch_size(cols, rows)
- * bytes, 8-byte aligned. Host outputs may be NULL; any non-NULL one
+ * bytes, 16-byte aligned. Host outputs may be NULL; any non-NULL one
  * synchronises `stream`. */
 int sicp_rgb_step(sicp_image next_image, sicp_image last_image, sicp_image next_vmap,
                   sicp_image last_vmap, sicp_image didx, sicp_image didy,
@@ -317,8 +318,8 @@ size_t sicp_rgb_scratch_size(int32_t cols, int32_t rows);
  * into the current camera (curr->Rinv * last->R) and matched by rounded
  * projection; the 11 sums (A 3x3 upper and b interleaved, sum r^2, count) go
  * to `sums_dev` (11 floats, device). `partials_dev` is device scratch of
- * sicp_icp_partials_size() bytes. Host outputs as for sicp_icp_step, with
- * CreateMatrix<3,4>: A row-major 3x3 (mirrored), b[3]. */
+ * sicp_icp_partials_size() bytes, 16-byte aligned. Host outputs as for
+ * sicp_icp_step, with CreateMatrix<3,4>: A row-major 3x3 (mirrored), b[3]. */
 int sicp_so3_step(sicp_image next_image, sicp_image last_image, sicp_image didx,
                   sicp_image didy, const sicp_pose_f* curr, const sicp_pose_f* last,
                   sicp_intrinsics K, void* partials_dev, float* sums_dev, float* host_sums,

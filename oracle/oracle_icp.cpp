@@ -229,15 +229,17 @@ template void rgb_sums<double>(const uint8_t*, const uint8_t*, const double*, co
 // order is that of the GPU reduction this build runs (a decision of this
 // build, not the reference's), so the result is bit-exact:
 // - 256 threads per workgroup, nb = clamp(ceil(n / 1024), 1, 512)
-//   workgroups, stride = 256 nb; thread g adds the contributing pixels
+//   workgroups (ceil(n / 256) for n <= 320 * 256: one pixel per thread),
+//   stride = 256 nb; thread g adds the contributing pixels
 //   g, g + stride, g + 2 stride, ... in that order (28 products, then 1 to
 //   the count);
 // - in each workgroup, for each sum, eight partial sums k = 0..7 each meet
 //   the threads k, k + 8, ..., k + 248 in turn (from (0, 0)), then a
 //   butterfly over the eight (distance 1, 2, 4);
-// - over the workgroups likewise: partial k meets the workgroup rows
-//   k, k + 8, ..., k + 504 (rows past nb count as (0, 0)), then the same
-//   butterfly; the sum is s + e rounded to FP32.
+// - over the workgroups likewise, by pairs of rows: partial k meets the
+//   workgroup rows 2k, 2k + 1, 2k + 16, 2k + 17, ..., 2k + 497 (rows past nb
+//   count as (0, 0)), then the same butterfly; the sum is s + e rounded to
+//   FP32.
 // `compensate` = false keeps every step but drops the error terms: plain
 // FP32 in the same order (used by the tests to show what compensation buys).
 struct K32 {
@@ -273,7 +275,8 @@ static inline K32 k32_fold8(K32 lane[8], bool comp) {
 }
 
 void sums_k32(const float* rows29, int n, bool compensate, float* sums) {
-    constexpr int kThreads = 256, kMaxWg = 512, kPix = 4;
+    constexpr int kThreads = 256, kMaxWg = 512;
+    const int kPix = n <= 320 * kThreads ? 1 : 4;
     int nb = (n + kThreads * kPix - 1) / (kThreads * kPix);
     nb = nb < 1 ? 1 : (nb > kMaxWg ? kMaxWg : nb);
     const long stride = (long)nb * kThreads;
@@ -305,10 +308,12 @@ void sums_k32(const float* rows29, int n, bool compensate, float* sums) {
         K32 lane[8];
         for (int k = 0; k < 8; ++k) {
             K32 s{0.f, 0.f};
-            for (int r = 0; r < kMaxWg / 8; ++r) {
-                const int bi = k + 8 * r;
-                s = k32_merge(s, bi < nb ? part[(size_t)bi * 29 + j] : K32{0.f, 0.f},
-                              compensate);
+            for (int r = 0; r < kMaxWg / 16; ++r) {
+                for (int h = 0; h < 2; ++h) {
+                    const int bi = 2 * (k + 8 * r) + h;
+                    s = k32_merge(s, bi < nb ? part[(size_t)bi * 29 + j] : K32{0.f, 0.f},
+                                  compensate);
+                }
             }
             lane[k] = s;
         }

@@ -299,6 +299,8 @@ int icp_step(sicp_image vcurr, sicp_image vlast, sicp_image ncurr, sicp_image nl
         !image_ok(nlast, 16, c, r))
         return invalid("icp_step: the four maps must share one shape");
     if (!curr || !last || !partials_dev || !sums_dev) return invalid("icp_step: null argument");
+    if (((uintptr_t)partials_dev & 15) != 0)
+        return invalid("icp_step: partials not 16-byte aligned");
     IcpScratch sc{(double*)partials_dev};
     hipStream_t s = (hipStream_t)stream;
     int rc = launch_icp_sums(view<float4>(vcurr), view<float4>(vlast), view<float4>(ncurr),
@@ -350,6 +352,8 @@ int sicp_so3_step(sicp_image next_image, sicp_image last_image, sicp_image didx,
         !image_ok(didx, 2, c, r) || !image_ok(didy, 2, c, r))
         return invalid("so3_step: the four images must share one shape");
     if (!curr || !last || !partials_dev || !sums_dev) return invalid("so3_step: null argument");
+    if (((uintptr_t)partials_dev & 15) != 0)
+        return invalid("so3_step: partials not 16-byte aligned");
     IcpScratch sc{(double*)partials_dev};
     hipStream_t s = (hipStream_t)stream;
     int rc = launch_so3_sums(view<uint8_t>(next_image), view<uint8_t>(last_image),
@@ -409,7 +413,7 @@ int sicp_rgb_step(sicp_image next_image, sicp_image last_image, sicp_image next_
     const size_t n = (size_t)c * r;
     if (scratch_bytes < rgb_scratch_bytes(n))
         return invalid("rgb_step: scratch smaller than sicp_rgb_scratch_size(cols, rows)");
-    if (((uintptr_t)scratch_dev & 7) != 0) return invalid("rgb_step: scratch not 8-byte aligned");
+    if (((uintptr_t)scratch_dev & 15) != 0) return invalid("rgb_step: scratch not 16-byte aligned");
     char* b = (char*)scratch_dev;
     const size_t pr = (size_t)kIcpSums * kIcpMaxBlocks * sizeof(double);
     const size_t ip = 2 * kIcpMaxBlocks * sizeof(uint32_t);

@@ -28,6 +28,15 @@ __device__ unsigned long long g_icp_phase[kPhaseLaunches][kIcpMaxBlocks][kPhaseS
 namespace {
 
 constexpr int kPixPerThread = 4;
+// Images of at most this many pixels take one pixel per thread instead (four
+// times the workgroups, each lane a quarter of the arithmetic).
+constexpr int kIcpSpreadPixels = 320 * kIcpThreads;
+
+// Pixel slot k of every thread in a launch lies past the image: the launch
+// gives fewer than kPixPerThread pixels per thread. A uniform test, so the
+// slot's work is skipped by a scalar branch. (A second pass of a thread's
+// loop needs 4 strides below n, so the test is false there.)
+SICP_DEV bool slot_empty(int k, int stride, int n) { return k > 0 && k * stride >= n; }
 // Every kernel here runs kIcpThreads threads per workgroup; the constant is
 // used instead of blockDim, whose value the kernel would otherwise load from
 // the dispatch packet (a memory round trip before the first address).
@@ -215,6 +224,7 @@ SICP_DEV void icp_load_batch(IcpBatch& bt, int base, int stride, const View<floa
                              const View<float4>& ncurr, const RgbPass& rp, const IcpGeom& g) {
 #pragma unroll
     for (int k = 0; k < kPixPerThread; ++k) {
+        if (slot_empty(k, stride, g.n)) continue;
         const int i = base + k * stride;
         const int ii = i < g.n ? i : 0;
         const int y = ii / g.cols, x = ii - y * g.cols;
@@ -365,6 +375,7 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
         bool rok[kPixPerThread];
 #pragma unroll
         for (int k = 0; k < kPixPerThread; ++k) {
+            if (slot_empty(k, stride, g.n)) continue;
             pr[k] = icp_project(bt.vc[k], cur, last, g);
             const int u = pr[k].ok ? pr[k].u : 0, v = pr[k].ok ? pr[k].v : 0;
             vl[k] = vlast.at(v, u);
@@ -378,6 +389,7 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
         }
 #pragma unroll
         for (int k = 0; k < kPixPerThread; ++k) {
+            if (slot_empty(k, stride, g.n)) continue;
             float row[7];
             if (pr[k].ok && icp_finish(pr[k], bt.nc[k], vl[k], nl[k], cur, last, row))
                 add_row(acc, row);
@@ -426,7 +438,8 @@ SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
 #pragma unroll
         for (int k = 0; k < kPixPerThread; ++k) {
             const int i = base + k * stride;
-            rc[k] = i < g.n ? rs.rec[i] : make_int2(-1, 0);
+            rc[k] = make_int2(-1, 0);
+            if (!slot_empty(k, stride, g.n) && i < g.n) rc[k] = rs.rec[i];
         }
     };
     load_records();
@@ -453,6 +466,7 @@ SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
         int gx[kPixPerThread], gy[kPixPerThread];
 #pragma unroll
         for (int k = 0; k < kPixPerThread; ++k) {
+            if (slot_empty(k, stride, g.n)) continue;
             const int li = rc[k].x >= 0 ? rc[k].x : 0;
             const int ly = li / g.cols, lx = li - ly * g.cols;
             vl[k] = vlast.at(ly, lx);
@@ -463,7 +477,7 @@ SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
         }
 #pragma unroll
         for (int k = 0; k < kPixPerThread; ++k) {
-            if (rc[k].x >= 0) {
+            if (!slot_empty(k, stride, g.n) && rc[k].x >= 0) {
                 float row[7];
                 rgb_row(rc[k].y, vl[k], gx[k], gy[k], sigma, cur, last, g, row);
                 add_row(acc, row);
@@ -476,39 +490,54 @@ SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
 }
 
 // Sum of the partial rows of `nb` workgroups into totals (LDS), in a fixed
-// order: thread (j, k) adds rows k, k+8, ... (below nb) of column j, then
-// eight lanes fold. NS sums per row.
+// order: thread (j, k) adds the row pairs (2p, 2p + 1), p = k, k+8, ...
+// (rows below nb) of column j, then eight lanes fold. NS sums per row. A
+// pair is one 16-byte load (column j starts 16-byte aligned).
 template <int NS = kIcpSums, class Acc = SumF64>
 SICP_DEV void icp_totals_issue(const double* partials, int nb, Acc& s) {
     const int t = threadIdx.x;
     s.zero();
     if (t < NS * 8) {
         const int j = t >> 3, k = t & 7;
-        const Acc* col = reinterpret_cast<const Acc*>(partials) + j * kIcpMaxBlocks;
-        // Rows below nb only: sixteen loads in flight per step while sixteen
-        // complete rows remain, then eight with predicates. A fixed 64-row
-        // unroll with predicated loads cost every launch the full sequence
-        // of loads and merges (3.2 against 1.5 us at 19 rows).
+        struct alignas(16) Pair {
+            Acc a, b;
+        };
+        const Pair* col = reinterpret_cast<const Pair*>(partials + j * kIcpMaxBlocks);
+        const int full = nb >> 1, pairs = (nb + 1) >> 1;
+        // Rows below nb only: sixteen pairs in flight per step while sixteen
+        // complete pairs remain, then eight with predicates (the second row
+        // of a last, odd pair counts as zero). A fixed 64-row unroll with
+        // predicated loads cost every launch the full sequence of loads and
+        // merges (3.2 against 1.5 us at 19 rows).
         int r0 = 0;
-        for (; k + 8 * (r0 + 15) < nb; r0 += 16) {
-            Acc v[16];
+        for (; k + 8 * (r0 + 15) < full; r0 += 16) {
+            Pair v[16];
 #pragma unroll
             for (int r = 0; r < 16; ++r) v[r] = col[k + 8 * (r0 + r)];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) s.merge(v[r]);
+            for (int r = 0; r < 16; ++r) {
+                s.merge(v[r].a);
+                s.merge(v[r].b);
+            }
         }
-        for (; k + 8 * r0 < nb; r0 += 8) {
-            Acc v[8];
+        for (; k + 8 * r0 < pairs; r0 += 8) {
+            Pair v[8];
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
-                const int bi = k + 8 * (r0 + r);
-                if (bi < nb)
-                    v[r] = col[bi];
-                else
-                    v[r].zero();
+                const int p = k + 8 * (r0 + r);
+                if (p < pairs) {
+                    v[r] = col[p];
+                    if (p >= full) v[r].b.zero();
+                } else {
+                    v[r].a.zero();
+                    v[r].b.zero();
+                }
             }
 #pragma unroll
-            for (int r = 0; r < 8; ++r) s.merge(v[r]);
+            for (int r = 0; r < 8; ++r) {
+                s.merge(v[r].a);
+                s.merge(v[r].b);
+            }
         }
     }
 }
@@ -1059,9 +1088,15 @@ __global__ void track_init_kernel(FrameMeta* next, FrameMeta* last, TrackState* 
 // Workgroups of the tracking finish.
 constexpr int kFinishBlocks = 64;
 
-// Four pixels per thread, at most kIcpMaxBlocks workgroups.
+// Four pixels per thread (one up to kIcpSpreadPixels), at most
+// kIcpMaxBlocks workgroups. The pixel pass of a lane is a chain of about 360
+// arithmetic instructions per pixel after one dependent gather; at 640x480's
+// levels 1 and 2 (76.8 k and 19.2 k pixels) four pixels per thread keep 75
+// and 19 of the 256 compute units busy, and one per thread cuts the launch by
+// more than the fold of four times the partial rows adds to the next one.
 inline int icp_blocks(int n) {
-    int b = (int)div_up(n, kIcpThreads * kPixPerThread);
+    const int per_thread = n <= kIcpSpreadPixels ? 1 : kPixPerThread;
+    int b = (int)div_up(n, kIcpThreads * per_thread);
     return b < 1 ? 1 : (b > kIcpMaxBlocks ? kIcpMaxBlocks : b);
 }
 

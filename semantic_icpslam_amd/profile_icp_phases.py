@@ -59,7 +59,9 @@ def main():
         cfg.map.tsdf_half, cfg.map.max_weight, cfg.icp_sums_fp32 = 1, 128, 1
     iters = [cfg.icp_iters[2], cfg.icp_iters[1], cfg.icp_iters[0]]
     levels = [2] * iters[0] + [1] * iters[1] + [0] * iters[2]
-    nwg = [min(MAX_WG, -(-((cols >> l) * (rows >> l)) // 1024)) for l in levels]
+    def workgroups(n):  # the launch's grid: 4 pixels per thread, 1 up to 320 * 256 pixels
+        return min(MAX_WG, -(-n // (256 if n <= 320 * 256 else 1024)))
+    nwg = [workgroups((cols >> l) * (rows >> l)) for l in levels]
     dev = torch.device("cuda:0")
     g = B.System(cfg)
     buf = np.zeros((LAUNCHES, MAX_WG, STAMPS), np.uint64)

@@ -103,6 +103,23 @@ def test_icp_sums_fp32_compensated_bit_exact(dev, c, level):
     _check_sums(hs, hd.astype(np.float64))
 
 
+def test_icp_step_refuses_misaligned_partials(dev):
+    """The partial rows are folded in 16-byte pairs: a scratch pointer off a
+    16-byte boundary is refused before any launch."""
+    c = CFG_SMALL
+    v, n = _maps(c, 0)
+    t = [to_dev(a, dev) for a in (v, v, n, n)]
+    partials = torch.zeros(B.lib().sicp_icp_partials_size() + 16, dtype=torch.uint8, device=dev)
+    sums = torch.zeros(29, dtype=torch.float32, device=dev)
+    pc = B.pose_f(np.eye(4))
+    for fn in (B.lib().sicp_icp_step, B.lib().sicp_icp_step_fp32):
+        rc = fn(B.image(t[0]), B.image(t[1]), B.image(t[2]), B.image(t[3]), B.C.byref(pc),
+                B.C.byref(pc), K(c), partials.data_ptr() + 8, sums.data_ptr(), None, None, None,
+                None, B.stream_ptr())
+        assert rc != 0
+        assert "16-byte aligned" in B.lib().sicp_last_error().decode()
+
+
 def test_icp_identical_frames(dev):
     """Zero motion on identical maps: r = 0 for every match, so b = 0, r^2 = 0
     and every valid vertex with a valid normal is a correspondence."""
