@@ -425,9 +425,10 @@ def run_batched(args):
         shared = torch.cuda.Stream()
         for k, q in enumerate(seqs):
             w = q["w"]
-            systems.append(B.System(B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"],
-                                                    w["cy"], w["voxel"], w["trunc"], w["buckets"],
-                                                    w["blocks"])))
+            cfg = B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"],
+                                  w["voxel"], w["trunc"], w["buckets"], w["blocks"])
+            cfg.icp_spread = 0  # eight sequences fill the GPU without spreading small levels
+            systems.append(B.System(cfg))
             if tuning:
                 systems[-1].set_tuning(tuning[k]["ray_first_pass_samples"],
                                        tuning[k]["ray_second_pass_workgroups"])
@@ -507,7 +508,8 @@ def run_batched(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded procedural rooms, seeds 0-7)",
-        "config": {"workload": "batched8", "sequences": len(seqs), "frames_per_sequence": nfr},
+        "config": {"workload": "batched8", "sequences": len(seqs), "frames_per_sequence": nfr,
+                   "icp_spread": 0},
         "value_form": ("each sequence's frame an independent chain of the round's graph"
                        if best == "chains" else
                        "each Gauss-Newton iteration of all eight sequences in one launch"),

@@ -423,6 +423,15 @@ typedef struct sicp_system_config {
                                 addition, as sicp_icp_step_fp32). The reference
                                 sums in plain FP32 (BlockReduce / Reduce<float,
                                 29>, Reduction.cu:29-60, 315, 374). */
+    int32_t icp_spread;      /* 1 (default): pyramid levels of at most 320 x 256
+                                pixels take one pixel per thread in the ICP
+                                reduction (four times the workgroups on a GPU
+                                the level alone would leave mostly idle); 0:
+                                four pixels per thread at every level, for
+                                systems run together (sicp_batch_*) whose
+                                launches already fill the GPU. The sums' order
+                                depends on it, so runs compared bit for bit
+                                use the same value. */
 } sicp_system_config;
 
 typedef struct sicp_system sicp_system;

@@ -51,8 +51,9 @@ void oracle_icp_sums(const float* vcurr, const float* vlast, const float* ncurr,
                      float* rows_out);
 /* The precision form's sums of per-pixel rows (29 floats a pixel as
  * oracle_icp_sums writes them, slot 28 = 1 where the pixel contributes): FP32
- * with compensation, in the GPU reduction's order; compensate = 0: plain
- * FP32 in the same order. */
+ * with compensation, in the GPU reduction's order (the single-step entry
+ * points' partition: small images spread one pixel per thread); compensate =
+ * 0: plain FP32 in the same order. */
 void oracle_sums_k32(const float* rows29, int n, int compensate, float* sums);
 /* The same computed entirely in double from double poses (4x4 row-major). */
 void oracle_icp_sums_fp64(const double* vcurr, const double* vlast, const double* ncurr,

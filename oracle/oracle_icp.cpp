@@ -229,8 +229,8 @@ template void rgb_sums<double>(const uint8_t*, const uint8_t*, const double*, co
 // order is that of the GPU reduction this build runs (a decision of this
 // build, not the reference's), so the result is bit-exact:
 // - 256 threads per workgroup, nb = clamp(ceil(n / 1024), 1, 512)
-//   workgroups (ceil(n / 256) for n <= 320 * 256: one pixel per thread),
-//   stride = 256 nb; thread g adds the contributing pixels
+//   workgroups (ceil(n / 256) for n <= 320 * 256 when the system spreads
+//   small levels, icp_spread: one pixel per thread), stride = 256 nb; thread g adds the contributing pixels
 //   g, g + stride, g + 2 stride, ... in that order (28 products, then 1 to
 //   the count);
 // - in each workgroup, for each sum, eight partial sums k = 0..7 each meet
@@ -274,9 +274,9 @@ static inline K32 k32_fold8(K32 lane[8], bool comp) {
     return lane[0];
 }
 
-void sums_k32(const float* rows29, int n, bool compensate, float* sums) {
+void sums_k32(const float* rows29, int n, bool compensate, bool spread, float* sums) {
     constexpr int kThreads = 256, kMaxWg = 512;
-    const int kPix = n <= 320 * kThreads ? 1 : 4;
+    const int kPix = spread && n <= 320 * kThreads ? 1 : 4;
     int nb = (n + kThreads * kPix - 1) / (kThreads * kPix);
     nb = nb < 1 ? 1 : (nb > kMaxWg ? kMaxWg : nb);
     const long stride = (long)nb * kThreads;
@@ -526,7 +526,7 @@ void oracle_icp_sums(const float* vcurr, const float* vlast, const float* ncurr,
 }
 
 void oracle_sums_k32(const float* rows29, int n, int compensate, float* sums) {
-    sums_k32(rows29, n, compensate != 0, sums);
+    sums_k32(rows29, n, compensate != 0, true, sums);
 }
 
 void oracle_icp_sums_fp64(const double* vcurr, const double* vlast, const double* ncurr,

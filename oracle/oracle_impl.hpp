@@ -52,7 +52,7 @@ void rgb_sums(const uint8_t* nimg, const uint8_t* limg, const R* nvmap, const R*
 // The precision form's FP32 compensated sums of per-pixel rows (29 floats a
 // pixel, slot 28 = 1 where the pixel contributes) in the GPU reduction's
 // order; `compensate` = false: plain FP32 in the same order.
-void sums_k32(const float* rows29, int n, bool compensate, float* sums);
+void sums_k32(const float* rows29, int n, bool compensate, bool spread, float* sums);
 void create_matrix(const double* sums29, double* A, double* b);
 bool cholesky_solve6(const double* A, const double* b, double* x);
 void se3_exp(const double* xi, double* T);

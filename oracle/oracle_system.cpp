@@ -97,9 +97,9 @@ template <> void round_sums<float>(double* s) {
 }
 template <> void round_sums<double>(double*) {}
 
-static void compensated_sums(const std::vector<float>& rows, int n, double* sums) {
+static void compensated_sums(const std::vector<float>& rows, int n, bool spread, double* sums) {
     float f[29];
-    sums_k32(rows.data(), n, true, f);
+    sums_k32(rows.data(), n, true, spread, f);
     for (int j = 0; j < 29; ++j) sums[j] = (double)f[j];
 }
 
@@ -138,7 +138,7 @@ static bool compute_se3(const sicp_system_config& cfg, const Intr<R>* K, OFrame<
                             nf.rows[i], pc, pl, K[i], rs, res2, nullptr,
                             k32 ? rows.data() : nullptr);
                 if (k32)
-                    compensated_sums(rows, nf.cols[i] * nf.rows[i], rs);
+                    compensated_sums(rows, nf.cols[i] * nf.rows[i], cfg.icp_spread != 0, rs);
                 else
                     round_sums<R>(rs);
                 create_matrix(rs, A_rgb, b_rgb);
@@ -159,7 +159,7 @@ static bool compute_se3(const sicp_system_config& cfg, const Intr<R>* K, OFrame<
                         lf.nmap[i].data(), nf.cols[i], nf.rows[i], pc, pl, K[i], sums,
                         k32 ? rows.data() : nullptr);
             if (k32)
-                compensated_sums(rows, nf.cols[i] * nf.rows[i], sums);
+                compensated_sums(rows, nf.cols[i] * nf.rows[i], cfg.icp_spread != 0, sums);
             else
                 round_sums<R>(sums);
             create_matrix(sums, A, b);

@@ -69,6 +69,7 @@ class SystemConfig(C.Structure):
         ("icp_iters", C.c_int32 * 3), ("icp_min_count", C.c_int32 * 3),
         ("icp_thresh", C.c_float), ("map", MapConfig),
         ("icp_only", C.c_int32), ("rgb_weight", C.c_float), ("icp_sums_fp32", C.c_int32),
+        ("icp_spread", C.c_int32),
     ]
 
 

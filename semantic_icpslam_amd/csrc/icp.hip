@@ -17,6 +17,8 @@
 #include "se3_math.hpp"
 #include "sicp_kernels.hpp"
 
+#include <type_traits>
+
 namespace sicp {
 
 #ifdef SICP_ICP_PHASES
@@ -219,12 +221,11 @@ struct IcpBatch {
     float4 vc[kPixPerThread], nc[kPixPerThread];
     int key[kPixPerThread];
 };
-template <bool kRgb>
+template <bool kRgb, int NP>
 SICP_DEV void icp_load_batch(IcpBatch& bt, int base, int stride, const View<float4>& vcurr,
                              const View<float4>& ncurr, const RgbPass& rp, const IcpGeom& g) {
 #pragma unroll
-    for (int k = 0; k < kPixPerThread; ++k) {
-        if (slot_empty(k, stride, g.n)) continue;
+    for (int k = 0; k < NP; ++k) {
         const int i = base + k * stride;
         const int ii = i < g.n ? i : 0;
         const int y = ii / g.cols, x = ii - y * g.cols;
@@ -357,8 +358,15 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
                              PoseFn get_pose) {
     const int stride = wg.n * kIcpThreads;
     int base = wg.id * kIcpThreads + threadIdx.x;
+    // A launch of one pixel per thread runs a one-slot copy of the pass (a
+    // uniform choice), so neither copy carries per-slot branches that would
+    // split the slots' loads and arithmetic into separate blocks.
+    const bool single = stride >= g.n;
     IcpBatch bt;
-    icp_load_batch<kRgb>(bt, base, stride, vcurr, ncurr, rp, g);
+    if (single)
+        icp_load_batch<kRgb, 1>(bt, base, stride, vcurr, ncurr, rp, g);
+    else
+        icp_load_batch<kRgb, kPixPerThread>(bt, base, stride, vcurr, ncurr, rp, g);
     PoseF cur, last;
     if (!get_pose(cur, last)) {
         phase_mark(4);
@@ -368,48 +376,53 @@ SICP_DEV void icp_accumulate(const View<float4>& vcurr, const View<float4>& vlas
 #pragma unroll
     for (int j = 0; j < kIcpSums; ++j) acc[j].zero();
     uint32_t cnt = 0, sq = 0;
-    while (base < g.n) {
-        float4 vl[kPixPerThread], nl[kPixPerThread], rvl[kPixPerThread];
-        IcpProj pr[kPixPerThread];
-        int ru[kPixPerThread], rv[kPixPerThread], rli[kPixPerThread];
-        bool rok[kPixPerThread];
+    auto pass = [&](auto slots) {
+        constexpr int NP = decltype(slots)::value;
+        while (base < g.n) {
+            float4 vl[NP], nl[NP], rvl[NP];
+            IcpProj pr[NP];
+            int ru[NP], rv[NP], rli[NP];
+            bool rok[NP];
 #pragma unroll
-        for (int k = 0; k < kPixPerThread; ++k) {
-            if (slot_empty(k, stride, g.n)) continue;
-            pr[k] = icp_project(bt.vc[k], cur, last, g);
-            const int u = pr[k].ok ? pr[k].u : 0, v = pr[k].ok ? pr[k].v : 0;
-            vl[k] = vlast.at(v, u);
-            nl[k] = nlast.at(v, u);
-            if (kRgb) {
-                rok[k] = rgb_project(pr[k].vcurr_p, g, ru[k], rv[k]) && bt.key[k] >= 0;
-                const int uu = rok[k] ? ru[k] : 0, vv = rok[k] ? rv[k] : 0;
-                rvl[k] = vlast.at(vv, uu);
-                rli[k] = rp.ilast.at(vv, uu);
+            for (int k = 0; k < NP; ++k) {
+                pr[k] = icp_project(bt.vc[k], cur, last, g);
+                const int u = pr[k].ok ? pr[k].u : 0, v = pr[k].ok ? pr[k].v : 0;
+                vl[k] = vlast.at(v, u);
+                nl[k] = nlast.at(v, u);
+                if (kRgb) {
+                    rok[k] = rgb_project(pr[k].vcurr_p, g, ru[k], rv[k]) && bt.key[k] >= 0;
+                    const int uu = rok[k] ? ru[k] : 0, vv = rok[k] ? rv[k] : 0;
+                    rvl[k] = vlast.at(vv, uu);
+                    rli[k] = rp.ilast.at(vv, uu);
+                }
             }
-        }
 #pragma unroll
-        for (int k = 0; k < kPixPerThread; ++k) {
-            if (slot_empty(k, stride, g.n)) continue;
-            float row[7];
-            if (pr[k].ok && icp_finish(pr[k], bt.nc[k], vl[k], nl[k], cur, last, row))
-                add_row(acc, row);
-            if (kRgb) {
-                const int i = base + k * stride;
-                if (i < g.n) {
-                    const int2 rc = rok[k] ? rgb_record(bt.key[k], bt.vc[k], rvl[k], rli[k],
-                                                        ru[k], rv[k], g.cols)
-                                           : make_int2(-1, 0);
-                    rp.rec[i] = rc;
-                    if (rc.x >= 0) {
-                        cnt += 1u;
-                        sq += (uint32_t)(rc.y * rc.y);
+            for (int k = 0; k < NP; ++k) {
+                float row[7];
+                if (pr[k].ok && icp_finish(pr[k], bt.nc[k], vl[k], nl[k], cur, last, row))
+                    add_row(acc, row);
+                if (kRgb) {
+                    const int i = base + k * stride;
+                    if (i < g.n) {
+                        const int2 rc = rok[k] ? rgb_record(bt.key[k], bt.vc[k], rvl[k], rli[k],
+                                                            ru[k], rv[k], g.cols)
+                                               : make_int2(-1, 0);
+                        rp.rec[i] = rc;
+                        if (rc.x >= 0) {
+                            cnt += 1u;
+                            sq += (uint32_t)(rc.y * rc.y);
+                        }
                     }
                 }
             }
+            base += stride * NP;
+            if (base < g.n) icp_load_batch<kRgb, NP>(bt, base, stride, vcurr, ncurr, rp, g);
         }
-        base += stride * kPixPerThread;
-        if (base < g.n) icp_load_batch<kRgb>(bt, base, stride, vcurr, ncurr, rp, g);
-    }
+    };
+    if (single)
+        pass(std::integral_constant<int, 1>{});
+    else
+        pass(std::integral_constant<int, kPixPerThread>{});
     phase_mark(4);
     if (kRgb) {
         const uint2 r = block_sum_u32x2(cnt, sq);
@@ -1088,14 +1101,17 @@ __global__ void track_init_kernel(FrameMeta* next, FrameMeta* last, TrackState* 
 // Workgroups of the tracking finish.
 constexpr int kFinishBlocks = 64;
 
-// Four pixels per thread (one up to kIcpSpreadPixels), at most
-// kIcpMaxBlocks workgroups. The pixel pass of a lane is a chain of about 360
+// Four pixels per thread (one up to kIcpSpreadPixels when `spread`), at
+// most kIcpMaxBlocks workgroups. The pixel pass of a lane is a chain of about 360
 // arithmetic instructions per pixel after one dependent gather; at 640x480's
 // levels 1 and 2 (76.8 k and 19.2 k pixels) four pixels per thread keep 75
 // and 19 of the 256 compute units busy, and one per thread cuts the launch by
 // more than the fold of four times the partial rows adds to the next one.
-inline int icp_blocks(int n) {
-    const int per_thread = n <= kIcpSpreadPixels ? 1 : kPixPerThread;
+// Systems run together fill the GPU already and leave it off: there the
+// extra workgroups only queue (batched eight sequences: 3390 against 3568
+// frames/s).
+inline int icp_blocks(int n, bool spread) {
+    const int per_thread = spread && n <= kIcpSpreadPixels ? 1 : kPixPerThread;
     int b = (int)div_up(n, kIcpThreads * per_thread);
     return b < 1 ? 1 : (b > kIcpMaxBlocks ? kIcpMaxBlocks : b);
 }
@@ -1106,7 +1122,7 @@ int launch_icp_sums(View<float4> vcurr, View<float4> vlast, View<float4> ncurr,
                     View<float4> nlast, PoseArg curr, PoseArg last, sicp_intrinsics K,
                     IcpScratch scratch, float* sums_out, bool k32, hipStream_t s) {
     IcpGeom g{vcurr.cols, vcurr.rows, vcurr.cols * vcurr.rows, K.fx, K.fy, K.cx, K.cy};
-    const int nb = icp_blocks(g.n);
+    const int nb = icp_blocks(g.n, true);
     if (k32) {
         icp_partials_kernel<SumK32><<<nb, kIcpThreads, 0, s>>>(vcurr, vlast, ncurr, nlast, curr,
                                                                last, g, scratch.partials);
@@ -1137,8 +1153,8 @@ static void icp_iteration_launch(const IcpLevel& cur, const IcpLevel* prev,
                                  TrackState* st_out, const double* partials_prev,
                                  double* partials, const RgbScratch* rgb, hipStream_t s) {
     const IcpGeom g = geom_of(cur);
-    const int nb = icp_blocks(g.n);
-    const int nb_prev = prev ? icp_blocks(prev->vcurr.cols * prev->vcurr.rows) : 0;
+    const int nb = icp_blocks(g.n, tp.spread);
+    const int nb_prev = prev ? icp_blocks(prev->vcurr.cols * prev->vcurr.rows, tp.spread) : 0;
     const int min_prev = prev ? prev->min_count : 0;
     RgbPass rp{};
     const double* rgb_prev = nullptr;
@@ -1190,7 +1206,7 @@ int launch_icp_iteration_batch(const IcpBatchEntry* e, int n, hipStream_t s) {
             return invalid("icp batch: sequences at different iterations or forms");
         const IcpGeom g = geom_of(e[q].cur);
         a.wg0[q] = wg;
-        wg += icp_blocks(g.n);
+        wg += icp_blocks(g.n, e[q].tp.spread);
         const IcpLevel* p = e[q].prev;
         a.seq[q] = IcpSeqArgs{e[q].cur.vcurr,
                               e[q].cur.vlast,
@@ -1203,7 +1219,7 @@ int launch_icp_iteration_batch(const IcpBatchEntry* e, int n, hipStream_t s) {
                               e[q].st_out,
                               e[q].partials_prev,
                               e[q].partials,
-                              p ? icp_blocks(p->vcurr.cols * p->vcurr.rows) : 0,
+                              p ? icp_blocks(p->vcurr.cols * p->vcurr.rows, e[q].tp.spread) : 0,
                               p ? p->min_count : 0};
     }
     a.wg0[n] = wg;
@@ -1235,7 +1251,8 @@ int launch_track_finish_batch(const FinishBatchEntry* e, int n, hipStream_t s) {
         a.seq[q] = FinishSeqArgs{e[q].tp,       e[q].st_in,  e[q].last,  e[q].partials_prev,
                                  e[q].st_final, e[q].next,   e[q].vlast, e[q].nlast,
                                  e[q].vnext,    e[q].nnext,
-                                 icp_blocks(p->vcurr.cols * p->vcurr.rows), p->min_count};
+                                 icp_blocks(p->vcurr.cols * p->vcurr.rows, e[q].tp.spread),
+                                 p->min_count};
     }
     a.wg0[n] = wg;
     if (k32)
@@ -1247,9 +1264,9 @@ int launch_track_finish_batch(const FinishBatchEntry* e, int n, hipStream_t s) {
 }
 
 int launch_rgb_rows(const IcpLevel& cur, const TrackState* st, RgbScratch rgb, bool k32,
-                    hipStream_t s) {
+                    bool spread, hipStream_t s) {
     const IcpGeom g = geom_of(cur);
-    const int nb = icp_blocks(g.n);
+    const int nb = icp_blocks(g.n, spread);
     if (k32)
         rgb_rows_kernel<SumK32><<<nb, kIcpThreads, 0, s>>>(cur.vlast, cur.rgb.dx, cur.rgb.dy, g,
                                                            rgb, nb, st, PoseArg{}, PoseArg{});
@@ -1272,7 +1289,7 @@ int launch_track_finish(const IcpLevel* prev, const FrameMeta* last, TrackParams
     FrameImages im{};
     if (images) im = *images;
     if (prev) {
-        const int nb_prev = icp_blocks(prev->vcurr.cols * prev->vcurr.rows);
+        const int nb_prev = icp_blocks(prev->vcurr.cols * prev->vcurr.rows, tp.spread);
         if (tp.sums_k32)
             track_finish_kernel<false, SumK32><<<blocks, kIcpThreads, 0, s>>>(
                 st_in, last, tp, partials_prev, rgb ? rgb->partials : nullptr, nb_prev,
@@ -1304,7 +1321,7 @@ int launch_rgb_sums(View<uint8_t> inext, View<uint8_t> ilast, View<float4> vnext
     lv.K = K;
     lv.rgb = RgbLevel{kv, dx, dy, ilast};
     const IcpGeom g = geom_of(lv);
-    const int nb = icp_blocks(g.n);
+    const int nb = icp_blocks(g.n, true);
     rgb_match_kernel<<<nb, kIcpThreads, 0, s>>>(vnext, vlast, g, curr, last,
                                                  RgbPass{kv, ilast, rgb.rec, rgb.ipart});
     SICP_LAUNCH_CHECK("rgb_match_kernel");
@@ -1320,7 +1337,7 @@ int launch_so3_sums(View<uint8_t> inext, View<uint8_t> ilast, View<int16_t> dx,
                     View<int16_t> dy, PoseArg curr, PoseArg last, sicp_intrinsics K,
                     IcpScratch scratch, float* sums_out, hipStream_t s) {
     const IcpGeom g{inext.cols, inext.rows, inext.cols * inext.rows, K.fx, K.fy, K.cx, K.cy};
-    const int nb = icp_blocks(g.n);
+    const int nb = icp_blocks(g.n, true);
     so3_partials_kernel<<<nb, kIcpThreads, 0, s>>>(inext, ilast, dx, dy, curr, last, g,
                                                      scratch.partials);
     SICP_LAUNCH_CHECK("so3_partials_kernel");

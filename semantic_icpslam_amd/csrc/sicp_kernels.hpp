@@ -82,6 +82,7 @@ struct TrackParams {
     float rgb_weight;  // w (Tracking.cc:392)
     int rgb;           // !icpOnly: RGBStep runs before ICPStep in every iteration
     int sums_k32;      // the precision form: FP32 compensated sums (else FP64)
+    int spread;        // one pixel per thread on small levels (the ICP partition)
     int launch;        // the launch's index in the frame's tracking loop (phase stamps)
 };
 
@@ -135,7 +136,7 @@ int launch_icp_iteration(const IcpLevel& cur, const IcpLevel* prev, const FrameM
 // correspondence counts, then the 29 weighted photometric sums into
 // `rgb.partials`.
 int launch_rgb_rows(const IcpLevel& cur, const TrackState* st, RgbScratch rgb, bool k32,
-                    hipStream_t s);
+                    bool spread, hipStream_t s);
 // The last iteration's update, the final gate, the new frame's pose and,
 // when the frame is not tracked, the copy of the last frame's level-0 model
 // maps (and, with the photometric term, its intensity pyramid) into the new

@@ -301,6 +301,7 @@ void sicp_system_config_default(sicp_system_config* cfg, int cols, int rows, flo
     cfg->icp_iters[2] = 3;
     cfg->icp_only = 1;             // the survey's tracking decision: ICP only
     cfg->rgb_weight = 1e-4f;       // Tracking.cc:392
+    cfg->icp_spread = 1;
     cfg->icp_min_count[0] = 2000;  // MIN_ICP_COUNT (Tracking.h:112)
     cfg->icp_min_count[1] = 1000;
     cfg->icp_min_count[2] = 100;
@@ -321,6 +322,8 @@ int sicp_system_create(const sicp_system_config* cfg, sicp_system** out) {
         return invalid("system_create: rgb_weight must be finite");
     if (cfg->icp_sums_fp32 != 0 && cfg->icp_sums_fp32 != 1)
         return invalid("system_create: icp_sums_fp32 must be 0 or 1");
+    if (cfg->icp_spread != 0 && cfg->icp_spread != 1)
+        return invalid("system_create: icp_spread must be 0 or 1");
     const bool rgb = !cfg->icp_only;
     sicp_system* s = new sicp_system();
     s->cfg = *cfg;
@@ -409,7 +412,7 @@ static int grab_begin(sicp_system* s, const sicp_image& depth_u16, const sicp_im
 
 static TrackParams track_params(const sicp_system* s, int launch = 0) {
     return TrackParams{s->cfg.icp_thresh, s->cfg.rgb_weight, s->cfg.icp_only ? 0 : 1,
-                       s->cfg.icp_sums_fp32, launch};
+                       s->cfg.icp_sums_fp32, s->cfg.icp_spread, launch};
 }
 
 // The level-l maps of the tracking pair (next frame against the last).
@@ -456,7 +459,7 @@ static int track_single(sicp_system* s, hipStream_t st) {
                                           st);
                 if (rc) return rc;
                 if (rgb && (rc = launch_rgb_rows(cur, iter + (k & 1), s->rgb,
-                                                 tp.sums_k32 != 0, st)))
+                                                 tp.sums_k32 != 0, tp.spread != 0, st)))
                     return rc;
                 stage_end(s, kStageIcpIter, ek, st);
                 prev = cur;

@@ -21,8 +21,13 @@ FRAMES = 5
 
 
 def _config(w):
-    return B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"], w["voxel"],
-                           3 * w["voxel"], 1 << 18, 60000)
+    """The 320x240 sequences keep one pixel per thread on their small levels
+    (icp_spread, the default), the others four pixels everywhere, as the
+    batched bench runs them: both partitions in one batch."""
+    cfg = B.system_config(w["cols"], w["rows"], w["f"], w["f"], w["cx"], w["cy"], w["voxel"],
+                          3 * w["voxel"], 1 << 18, 60000)
+    cfg.icp_spread = 1 if w["cols"] == 320 else 0
+    return cfg
 
 
 def _frames(w, dev):

@@ -16,13 +16,15 @@ pytestmark = pytest.mark.gpu
 
 
 # half = 1: the precision form's map (sdf as half, weights up to 128);
-# k32 = 1: its FP32 compensated Gauss-Newton sums, on both sides.
-@pytest.mark.parametrize("c,frames,half,k32", [(CFG_SMALL, 12, 0, 0), (CFG_VGA, 4, 0, 0),
-                                               (CFG_SMALL, 8, 1, 0), (CFG_SMALL, 8, 1, 1),
-                                               (CFG_VGA, 4, 1, 1)])
-def test_system_vs_oracle(dev, c, frames, half, k32):
+# k32 = 1: its FP32 compensated Gauss-Newton sums, on both sides; spread = 0:
+# four pixels per thread at every level (the oracle follows the partition).
+@pytest.mark.parametrize("c,frames,half,k32,spread", [
+    (CFG_SMALL, 12, 0, 0, 1), (CFG_VGA, 4, 0, 0, 1), (CFG_SMALL, 8, 1, 0, 1),
+    (CFG_SMALL, 8, 1, 1, 1), (CFG_VGA, 4, 1, 1, 1), (CFG_SMALL, 8, 1, 1, 0)])
+def test_system_vs_oracle(dev, c, frames, half, k32, spread):
     cfg = sys_cfg(c, **(dict(tsdf_half=1, max_weight=128) if half else {}))
     cfg.icp_sums_fp32 = k32
+    cfg.icp_spread = spread
     g = B.System(cfg)
     o = O.OracleSystem(cfg)
     s = synth(c)
@@ -210,3 +212,13 @@ def test_graph_form_matches_eager(dev, c):
     ma, mb = B.map_stats(ge.map_handle()), B.map_stats(gg.map_handle())
     for k in ("allocated_blocks", "updated_total", "visible_total", "fusions"):
         assert ma[k] == mb[k], k
+
+
+def test_system_rejects_bad_flags(dev):
+    """The 0/1 switches of the configuration refuse other values."""
+    for field in ("icp_only", "icp_sums_fp32", "icp_spread"):
+        cfg = sys_cfg(CFG_SMALL)
+        setattr(cfg, field, 2)
+        h = B.C.c_void_p()
+        assert B.lib().sicp_system_create(B.C.byref(cfg), B.C.byref(h)) != 0, field
+        assert field in B.lib().sicp_last_error().decode(), field
