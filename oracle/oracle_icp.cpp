@@ -236,10 +236,10 @@ template void rgb_sums<double>(const uint8_t*, const uint8_t*, const double*, co
 // - in each workgroup, for each sum, eight partial sums k = 0..7 each meet
 //   the threads k, k + 8, ..., k + 248 in turn (from (0, 0)), then a
 //   butterfly over the eight (distance 1, 2, 4);
-// - over the workgroups likewise, by pairs of rows: partial k meets the
-//   workgroup rows 2k, 2k + 1, 2k + 16, 2k + 17, ..., 2k + 497 (rows past nb
-//   count as (0, 0)), then the same butterfly; the sum is s + e rounded to
-//   FP32.
+// - over the workgroups likewise: partial k meets the workgroup rows
+//   k, k + 8, ..., k + 504, or for nb > 128 by pairs of rows, 2k, 2k + 1,
+//   2k + 16, 2k + 17, ..., 2k + 497 (rows past nb count as (0, 0)), then the
+//   same butterfly; the sum is s + e rounded to FP32.
 // `compensate` = false keeps every step but drops the error terms: plain
 // FP32 in the same order (used by the tests to show what compensation buys).
 struct K32 {
@@ -308,12 +308,11 @@ void sums_k32(const float* rows29, int n, bool compensate, bool spread, float* s
         K32 lane[8];
         for (int k = 0; k < 8; ++k) {
             K32 s{0.f, 0.f};
-            for (int r = 0; r < kMaxWg / 16; ++r) {
-                for (int h = 0; h < 2; ++h) {
-                    const int bi = 2 * (k + 8 * r) + h;
-                    s = k32_merge(s, bi < nb ? part[(size_t)bi * 29 + j] : K32{0.f, 0.f},
-                                  compensate);
-                }
+            for (int r = 0; r < kMaxWg / 8; ++r) {
+                // row r of thread k's sequence
+                const int bi = nb > 128 ? 2 * (k + 8 * (r >> 1)) + (r & 1) : k + 8 * r;
+                s = k32_merge(s, bi < nb ? part[(size_t)bi * 29 + j] : K32{0.f, 0.f},
+                              compensate);
             }
             lane[k] = s;
         }

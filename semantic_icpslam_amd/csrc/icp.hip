@@ -33,12 +33,6 @@ constexpr int kPixPerThread = 4;
 // Images of at most this many pixels take one pixel per thread instead (four
 // times the workgroups, each lane a quarter of the arithmetic).
 constexpr int kIcpSpreadPixels = 320 * kIcpThreads;
-
-// Pixel slot k of every thread in a launch lies past the image: the launch
-// gives fewer than kPixPerThread pixels per thread. A uniform test, so the
-// slot's work is skipped by a scalar branch. (A second pass of a thread's
-// loop needs 4 strides below n, so the test is false there.)
-SICP_DEV bool slot_empty(int k, int stride, int n) { return k > 0 && k * stride >= n; }
 // Every kernel here runs kIcpThreads threads per workgroup; the constant is
 // used instead of blockDim, whose value the kernel would otherwise load from
 // the dispatch packet (a memory round trip before the first address).
@@ -446,16 +440,20 @@ SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
     const Wg wg = grid_wg();
     const int stride = wg.n * kIcpThreads;
     int base = wg.id * kIcpThreads + threadIdx.x;
+    // One-slot or four-slot copy of the pass, as in icp_accumulate.
+    const bool single = stride >= g.n;
     int2 rc[kPixPerThread];
-    auto load_records = [&]() {
+    auto load_records = [&](auto slots) {
 #pragma unroll
-        for (int k = 0; k < kPixPerThread; ++k) {
+        for (int k = 0; k < decltype(slots)::value; ++k) {
             const int i = base + k * stride;
-            rc[k] = make_int2(-1, 0);
-            if (!slot_empty(k, stride, g.n) && i < g.n) rc[k] = rs.rec[i];
+            rc[k] = i < g.n ? rs.rec[i] : make_int2(-1, 0);
         }
     };
-    load_records();
+    if (single)
+        load_records(std::integral_constant<int, 1>{});
+    else
+        load_records(std::integral_constant<int, kPixPerThread>{});
     uint32_t c = 0, q = 0;
     for (int b = threadIdx.x; b < nb_match; b += kIcpThreads) {
         c += rs.ipart[b];
@@ -474,54 +472,88 @@ SICP_DEV void rgb_rows(const View<float4>& vlast, const View<int16_t>& dx,
     Acc acc[kIcpSums];
 #pragma unroll
     for (int j = 0; j < kIcpSums; ++j) acc[j].zero();
-    while (base < g.n) {
-        float4 vl[kPixPerThread];
-        int gx[kPixPerThread], gy[kPixPerThread];
+    auto pass = [&](auto slots) {
+        constexpr int NP = decltype(slots)::value;
+        while (base < g.n) {
+            float4 vl[NP];
+            int gx[NP], gy[NP];
 #pragma unroll
-        for (int k = 0; k < kPixPerThread; ++k) {
-            if (slot_empty(k, stride, g.n)) continue;
-            const int li = rc[k].x >= 0 ? rc[k].x : 0;
-            const int ly = li / g.cols, lx = li - ly * g.cols;
-            vl[k] = vlast.at(ly, lx);
-            const int i = rc[k].x >= 0 ? base + k * stride : 0;
-            const int y = i / g.cols, x = i - y * g.cols;
-            gx[k] = dx.at(y, x);
-            gy[k] = dy.at(y, x);
-        }
-#pragma unroll
-        for (int k = 0; k < kPixPerThread; ++k) {
-            if (!slot_empty(k, stride, g.n) && rc[k].x >= 0) {
-                float row[7];
-                rgb_row(rc[k].y, vl[k], gx[k], gy[k], sigma, cur, last, g, row);
-                add_row(acc, row);
+            for (int k = 0; k < NP; ++k) {
+                const int li = rc[k].x >= 0 ? rc[k].x : 0;
+                const int ly = li / g.cols, lx = li - ly * g.cols;
+                vl[k] = vlast.at(ly, lx);
+                const int i = rc[k].x >= 0 ? base + k * stride : 0;
+                const int y = i / g.cols, x = i - y * g.cols;
+                gx[k] = dx.at(y, x);
+                gy[k] = dy.at(y, x);
             }
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                if (rc[k].x >= 0) {
+                    float row[7];
+                    rgb_row(rc[k].y, vl[k], gx[k], gy[k], sigma, cur, last, g, row);
+                    add_row(acc, row);
+                }
+            }
+            base += stride * NP;
+            if (base < g.n) load_records(slots);
         }
-        base += stride * kPixPerThread;
-        if (base < g.n) load_records();
-    }
+    };
+    if (single)
+        pass(std::integral_constant<int, 1>{});
+    else
+        pass(std::integral_constant<int, kPixPerThread>{});
     block_rows_to_partials(acc, rs.partials, wg);
 }
 
 // Sum of the partial rows of `nb` workgroups into totals (LDS), in a fixed
-// order: thread (j, k) adds the row pairs (2p, 2p + 1), p = k, k+8, ...
-// (rows below nb) of column j, then eight lanes fold. NS sums per row. A
-// pair is one 16-byte load (column j starts 16-byte aligned).
+// order: thread (j, k) adds rows k, k+8, ... (below nb) of column j, then
+// eight lanes fold. NS sums per row. Above kFoldPairRows rows the thread
+// adds the row pairs (2p, 2p + 1), p = k, k+8, ... instead, a pair being one
+// 16-byte load (column j starts 16-byte aligned). Measured per launch at
+// 640x480 (launch span, single rows against pairs): 75 rows 6.9-8.0 against
+// 7.6-8.7 us, 300 rows 11.9 against 11.6 us.
+constexpr int kFoldPairRows = 128;
 template <int NS = kIcpSums, class Acc = SumF64>
 SICP_DEV void icp_totals_issue(const double* partials, int nb, Acc& s) {
     const int t = threadIdx.x;
     s.zero();
-    if (t < NS * 8) {
+    if (t < NS * 8 && nb <= kFoldPairRows) {
+        const int j = t >> 3, k = t & 7;
+        const Acc* col = reinterpret_cast<const Acc*>(partials) + j * kIcpMaxBlocks;
+        // Rows below nb only: sixteen loads in flight per step while sixteen
+        // complete rows remain, then eight with predicates. A fixed 64-row
+        // unroll with predicated loads cost every launch the full sequence
+        // of loads and merges (3.2 against 1.5 us at 19 rows).
+        int r0 = 0;
+        for (; k + 8 * (r0 + 15) < nb; r0 += 16) {
+            Acc v[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[r] = col[k + 8 * (r0 + r)];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) s.merge(v[r]);
+        }
+        for (; k + 8 * r0 < nb; r0 += 8) {
+            Acc v[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int bi = k + 8 * (r0 + r);
+                if (bi < nb)
+                    v[r] = col[bi];
+                else
+                    v[r].zero();
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) s.merge(v[r]);
+        }
+    } else if (t < NS * 8) {
         const int j = t >> 3, k = t & 7;
         struct alignas(16) Pair {
             Acc a, b;
         };
         const Pair* col = reinterpret_cast<const Pair*>(partials + j * kIcpMaxBlocks);
         const int full = nb >> 1, pairs = (nb + 1) >> 1;
-        // Rows below nb only: sixteen pairs in flight per step while sixteen
-        // complete pairs remain, then eight with predicates (the second row
-        // of a last, odd pair counts as zero). A fixed 64-row unroll with
-        // predicated loads cost every launch the full sequence of loads and
-        // merges (3.2 against 1.5 us at 19 rows).
+        // As above by pairs; the second row of a last, odd pair counts as zero.
         int r0 = 0;
         for (; k + 8 * (r0 + 15) < full; r0 += 16) {
             Pair v[16];
