@@ -43,14 +43,17 @@ SICP_DEV void filter_px(const View<uint16_t>& depth, const View<float>& raw,
 #pragma unroll
         for (int dx = 0; dx < 5; ++dx) {
             const int cx = x - 2 + dx;
-            if (cy >= 0 && cy < ty && cx >= 0 && cx < tx) {
-                float val = (float)tap[dy * 5 + dx] * scale_inv;
-                float space2 = (float)((x - cx) * (x - cx) + (y - cy) * (y - cy));
-                float color2 = (center - val) * (center - val);
-                float weight = expf_cw(-(space2 * sig_space + color2 * sig_range));
-                sum1 += val * weight;
-                sum2 += weight;
-            }
+            // Every tap is computed; one outside the clipped window adds +0
+            // through a select (the sums are never -0, so their bits do not
+            // change). One straight block lets the compiler interleave the
+            // 25 taps' exponentials; a branch per tap kept them apart.
+            const bool in = cy >= 0 && cy < ty && cx >= 0 && cx < tx;
+            float val = (float)tap[dy * 5 + dx] * scale_inv;
+            float space2 = (float)((x - cx) * (x - cx) + (y - cy) * (y - cy));
+            float color2 = (center - val) * (center - val);
+            float weight = expf_cw(-(space2 * sig_space + color2 * sig_range));
+            sum1 += in ? val * weight : 0.f;
+            sum2 += in ? weight : 0.f;
         }
     }
     float fin = sum1 / sum2;
@@ -148,14 +151,12 @@ SICP_DEV T pyrdown_px(const Src& src, int scols, int srows, int x, int y) {
 #pragma unroll
         for (int dx = 0; dx < 5; ++dx) {
             const int cx = 2 * x - 2 + dx;
-            if (cy >= 0 && cy < ty && cx >= 0 && cx < tx) {
-                float s = (float)tap[dy * 5 + dx];
-                if (!isnan_(s)) {
-                    float g = binom5(ty - cy - 1) * binom5(tx - cx - 1);
-                    sum += s * g;
-                    count = (int)((float)count + g);
-                }
-            }
+            // Selects instead of branches, as in filter_px.
+            const float s = (float)tap[dy * 5 + dx];
+            const bool use = cy >= 0 && cy < ty && cx >= 0 && cx < tx && !isnan_(s);
+            const float g = binom5(ty - cy - 1) * binom5(tx - cx - 1);
+            sum += use ? s * g : 0.f;
+            count = use ? (int)((float)count + g) : count;
         }
     }
     return out_cast<T>(sum / (float)count);
