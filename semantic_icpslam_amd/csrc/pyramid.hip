@@ -209,14 +209,17 @@ SICP_DEV int2 derivative_px(const View<uint8_t>& img, const View<int16_t>& dx,
     for (int j = 0; j < 3; ++j) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
+            // Selects instead of branches, as in filter_px; k counts the
+            // taps inside the image, as the reference's index does.
             const int cy = y - 1 + j, cx = x - 1 + i;
-            if (cy >= 0 && cy < img.rows && cx >= 0 && cx < img.cols) {
-                float p = (float)tap[j * 3 + i];
-                const int r = k / 3, c = k - 3 * r;
-                dxv += p * ((float)(1 - c) * (r == 1 ? 2.f : 1.f));
-                dyv += p * ((float)(1 - r) * (c == 1 ? 2.f : 1.f));
-                --k;
-            }
+            const bool in = cy >= 0 && cy < img.rows && cx >= 0 && cx < img.cols;
+            const float p = (float)tap[j * 3 + i];
+            const int r = k / 3, c = k - 3 * r;
+            const float wx = p * ((float)(1 - c) * (r == 1 ? 2.f : 1.f));
+            const float wy = p * ((float)(1 - r) * (c == 1 ? 2.f : 1.f));
+            dxv += in ? wx : 0.f;
+            dyv += in ? wy : 0.f;
+            k -= in ? 1 : 0;
         }
     }
     const int16_t gx = (int16_t)(int)dxv, gy = (int16_t)(int)dyv;
